@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Throughput of the batched Metropolis VMC hot path on one MI355X.
+
+Workload: L=16 staggered-flux ground state, N=128 particles per flavour, 16384 walkers
+on an 8-point (phi, neel) grid (phi in {0.10, 0.1667, 0.2333, 0.30} x neel in {0, 0.1},
+walker w -> point w mod 8).  One bench step = one sweep: L^2 = 256 Metropolis steps on
+every walker followed by one ProjHeis + StagMagnZ measurement (meas_interv = L^2, the
+vmc.cpp:121 default), with an FP64 Gauss-Jordan refresh of the inverses every
+`--refresh-every` sweeps in mixed precision.
+
+Prints ONE JSON line: value = MC steps/s of the mixed-precision path (whole job, inputs
+resident in HBM), plus E_loc samples/s, the plain-FP64 rate on the same workload, the
+energy difference between the FP32 and FP64 estimators on identical configurations,
+the roofline of the sweep kernel (HIP events on the library's stream) and a CPU
+baseline (the reference's own MetroMC::Step built from its sources, on host cores).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MC steps/s + ProjHeis E_loc samples/s at L=16; mixed-FP32 vs plain FP64"
+PHIS = (0.10, 0.1667, 0.2333, 0.30)
+NEELS = (0.00, 0.10)
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def setup(b, L, W, seed):
+    from gpvmc_amd import model
+    grid = [(p, n) for p in PHIS for n in NEELS]
+    for gi, (p, n) in enumerate(grid):
+        b.set_orbitals(gi, model.ground_occ(model.orbital_table(L, p * np.pi, n)))
+    b.set_groups(np.arange(W, dtype=np.int32) % len(grid))
+    spins = model.random_spins(L, W, seed)
+    ok = b.init(spins)
+    tries = 1
+    while not ok.all() and tries < 10:  # vmc.cpp:230-241: redraw singular starts
+        bad = ~ok
+        spins[bad] = model.random_spins(L, W, seed + 1000 * tries)[bad]
+        ok = b.init(spins)
+        tries += 1
+    if not ok.all():
+        raise RuntimeError("could not find regular starting configurations")
+    return spins
+
+
+def sweep(b, L, mask):
+    b.walk(L * L, meas=L * L, mask=mask)
+
+
+def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None):
+    from gpvmc_amd import binding
+    mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ
+    b = binding.Batch(L, W, precision, n_groups=len(PHIS) * len(NEELS), seed=seed)
+    if spins is None:
+        spins = setup(b, L, W, seed)
+    else:
+        from gpvmc_amd import model
+        grid = [(p, n) for p in PHIS for n in NEELS]
+        for gi, (p, n) in enumerate(grid):
+            b.set_orbitals(gi, model.ground_occ(model.orbital_table(L, p * np.pi, n)))
+        b.set_groups(np.arange(W, dtype=np.int32) % len(grid))
+        assert b.init(spins).all()
+    for i in range(warmup):
+        sweep(b, L, mask)
+        if precision == binding.MIXED and (i + 1) % refresh_every == 0:
+            b.refresh()
+    b.reset_accumulators()
+    b.reset_kernel_time()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sweep(b, L, mask)
+        if precision == binding.MIXED and (i + 1) % refresh_every == 0:
+            b.refresh()
+    t1 = time.perf_counter()
+    return b, t1 - t0
+
+
+def cpu_baseline(L, budget_s):
+    """The reference's own MetroMC::Step (oracle/_ref/ref_harness, built from the
+    reference sources) on one host core for a bounded sample; falls back to the oracle's
+    C restatement when the reference build is absent."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    if os.path.exists(harness):
+        try:
+            probe = json.loads(subprocess.run([harness, "time", str(L), "0.1667", "0.1", "1", "5000"],
+                                              check=True, capture_output=True, text=True, env=env,
+                                              timeout=120).stdout)
+            n = int(max(5000, min(2e6, probe["steps_per_s"] * budget_s)))
+            r = json.loads(subprocess.run([harness, "time", str(L), "0.1667", "0.1", "1", str(n)],
+                                          check=True, capture_output=True, text=True, env=env,
+                                          timeout=600).stdout)
+            return {"value": round(r["steps_per_s"], 1), "unit": "MC steps/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": "reference MetroMC::Step (LatticeStepper + SlaterDeterminant, MKL "
+                              "sequential) on one chain, L=%d phi=0.1667 neel=0.1, %d steps in %.1f s"
+                              % (L, n, r["seconds"])}
+        except Exception as e:  # noqa: BLE001
+            print("reference harness unavailable: %s" % e, file=sys.stderr)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref as O
+    from gpvmc_amd import model
+    occ = O.ground_occ(L, O.orbitals(L, 0.1667 * np.pi, 0.1))
+    w = O.Walker(L, occ, model.random_spins(L, 1, 1)[0])
+    n = 2000
+    t0 = time.perf_counter()
+    w.walk(model.philox_uniforms(1, [0], 0, n).reshape(-1))
+    dt = time.perf_counter() - t0
+    n = int(max(n, min(2e6, n / dt * budget_s)))
+    u = model.philox_uniforms(1, [0], n, n).reshape(-1)
+    t0 = time.perf_counter()
+    w.walk(u)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "MC steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle C restatement, one chain at L=%d, %d steps in %.1f s" % (L, n, dt)}
+
+
+def cpu_eloc_rate(L, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref as O
+    from gpvmc_amd import model
+    occ = O.ground_occ(L, O.orbitals(L, 0.1667 * np.pi, 0.1))
+    w = O.Walker(L, occ, model.random_spins(L, 1, 2)[0])
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        w.eloc()
+        n += 1
+    return n / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed sweeps")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed thermalisation sweeps")
+    ap.add_argument("--L", type=int, default=16)
+    ap.add_argument("--walkers", type=int, default=16384)
+    ap.add_argument("--refresh-every", type=int, default=1, help="sweeps between FP64 refreshes")
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed sweeps of the FP64 path")
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; walkers are independent chains and the "
+              "multi-GPU layout (walker shards per GPU, one FP64 reduction per save) is a "
+              "design note only.  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    from gpvmc_amd import binding
+    L, W = args.L, args.walkers
+    N = L * L // 2
+    print("bench: mixed path L=%d W=%d warmup=%d steps=%d" % (L, W, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    bm, tm = run_path(binding.MIXED, L, W, args.seed, args.warmup, args.steps, args.refresh_every)
+    acc = bm.accumulators()
+    kt = bm.kernel_time()
+    mc_steps = W * L * L * args.steps
+    value = mc_steps / tm
+    eloc_rate = W * args.steps / tm
+    steps_done = acc[:, 7].sum()
+    p_ap = 1.0 - acc[:, 10].sum() / steps_done
+    a_ap = acc[:, 8].sum() / steps_done
+    E = acc[:, 0].sum() / acc[:, 3].sum()
+    ms2 = acc[:, 6].sum() / acc[:, 3].sum()
+
+    # identical-configuration energy check: FP32 estimator of the drifted mixed state vs
+    # the FP64 estimator after an FP64 LU of the same configurations
+    print("bench: identical-configuration FP32 vs FP64 E_loc", file=sys.stderr, flush=True)
+    m32 = bm.measure()
+    spins = bm.state()[0]
+    b64 = binding.Batch(L, W, binding.FP64, n_groups=len(PHIS) * len(NEELS), seed=args.seed)
+    from gpvmc_amd import model
+    grid = [(p, n) for p in PHIS for n in NEELS]
+    for gi, (p, n) in enumerate(grid):
+        b64.set_orbitals(gi, model.ground_occ(model.orbital_table(L, p * np.pi, n)))
+    b64.set_groups(np.arange(W, dtype=np.int32) % len(grid))
+    b64.init(spins)
+    m64 = b64.measure()
+    dE = np.abs(m32["eloc"].real - m64["eloc"].real)
+    b64.close()
+    bm.close()
+
+    print("bench: plain FP64 path", file=sys.stderr, flush=True)
+    b2, t64 = run_path(binding.FP64, L, W, args.seed, 0, args.fp64_steps, args.refresh_every,
+                       spins=spins)
+    kt64 = b2.kernel_time()
+    fp64_rate = W * L * L * args.fp64_steps / t64
+    b2.close()
+
+    # roofline of the mixed sweep kernel: algorithmic work per launch (per MC step
+    # p_ap*4*N*s bytes + a_ap*4*N^2*s bytes, 16*p_ap*N + 32*a_ap*N^2 flops; per E_loc
+    # sample 2*N^2*s bytes, 16*n_ap*N flops with n_ap ~ p_ap * 2 Ns antiparallel bonds)
+    s = 8
+    steps_per_launch = W * L * L
+    bytes_launch = steps_per_launch * (p_ap * 4 * N * s + a_ap * 4 * N * N * s) + W * 2 * N * N * s
+    flops_launch = steps_per_launch * (16 * p_ap * N + 32 * a_ap * N * N) + W * 16 * (p_ap * 2 * L * L) * N
+    avg_ms = kt["sweep_ms"] / max(kt["sweep_launches"], 1)
+    ach_gbs = bytes_launch / (avg_ms * 1e-3) / 1e9
+    ach_tf = flops_launch / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "sweep_L16_mixed_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(ach_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "sweep_kernel<float,128,8,RES>",
+                "avg_launch_ms": round(avg_ms, 3),
+                "note": "algorithmic bytes of the survey's per-step model; the inverses are "
+                        "register-resident during a launch, so this is VGPR/LDS-level traffic "
+                        "compared against the HBM peak",
+                "flops": {"achieved": round(ach_tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(ach_tf / FP32_PEAK_TFLOPS, 5)}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(L, args.cpu_seconds)
+        cpu["eloc_samples_per_s_port"] = round(cpu_eloc_rate(L, min(3.0, args.cpu_seconds)), 1)
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "MC steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tm / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "L=16 staggered-flux groundstate, N=128, %d walkers, 8-point "
+                               "(phi,neel) grid, 1 sweep = %d steps + 1 ProjHeis/StagMagnZ "
+                               "measurement, FP64 LU refresh every %d sweep(s); therm shortened "
+                               "to %d sweeps" % (W, L * L, args.refresh_every, args.warmup),
+                   "L": L, "walkers": W, "parallelism": "1 GPU, 1 workgroup per walker"},
+        "eloc_samples_per_s": round(eloc_rate, 1),
+        "fp64_steps_per_s": round(fp64_rate, 1),
+        "mixed_over_fp64": round(value / fp64_rate, 2),
+        "fp64_sweep_avg_ms": round(kt64["sweep_ms"] / max(kt64["sweep_launches"], 1), 3),
+        "refresh_avg_ms": round(kt["refresh_ms"] / max(kt["refresh_launches"], 1), 3),
+        "dE_per_site_fp32_vs_fp64": {"max": float(dE.max()), "mean": float(dE.mean())},
+        "energy_per_site": round(float(E), 6), "stag_magn_sq": round(float(ms2), 6),
+        "p_antiparallel": round(float(p_ap), 4), "p_accept_move": round(float(a_ap), 4),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

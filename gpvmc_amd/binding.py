@@ -1,0 +1,195 @@
+"""ctypes binding of libgpvmc_hip.so (C ABI: include/gpvmc.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+FP64 = 0
+MIXED = 1
+MEAS_PROJHEIS = 1
+MEAS_STAGMAGNZ = 2
+MEAS_OVERLAP = 4
+MEAS_STATSPIN = 8
+ACC_STRIDE = 12
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# exported entry points and their ctypes signatures (every one returns int status
+# except gpvmc_last_error / gpvmc_version)
+_vp = ctypes.c_void_p
+SIGNATURES = {
+    "gpvmc_last_error": (ctypes.c_char_p, []),
+    "gpvmc_version": (ctypes.c_int, []),
+    "gpvmc_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "gpvmc_destroy": (ctypes.c_int, [_vp]),
+    "gpvmc_set_orbitals": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_set_groups": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_init_walkers": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gpvmc_refresh": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_walk": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32,
+                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gpvmc_measure": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
+    "gpvmc_read_accumulators": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_reset_accumulators": (ctypes.c_int, [_vp]),
+    "gpvmc_read_statspin": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_read_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "gpvmc_read_inverse": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "gpvmc_reset_kernel_time": (ctypes.c_int, [_vp]),
+}
+
+
+class GpvmcError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int32), ("n_walkers", ctypes.c_int32),
+                ("n_groups", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
+
+
+def library_path():
+    return os.path.join(_PKG, "lib", "libgpvmc_hip.so")
+
+
+def load_library():
+    """Loads the in-tree libgpvmc_hip.so; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise GpvmcError("libgpvmc_hip.so is not built (run __graft_entry__.build()); "
+                             "there is no CPU fallback")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Batch:
+    """A batch of independent walkers on one device (one gpvmc_ctx)."""
+
+    def __init__(self, L, n_walkers, precision=FP64, n_groups=1, seed=0, device=0,
+                 lanes_per_row=0):
+        self.lib = load_library()
+        self.L, self.Ns, self.N = L, L * L, L * L // 2
+        self.W, self.G = n_walkers, n_groups
+        self.precision = precision
+        cfg = _Config(L, n_walkers, n_groups, precision, device, lanes_per_row, seed)
+        h = ctypes.c_void_p()
+        self._check(self.lib.gpvmc_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GpvmcError("gpvmc error %d: %s" % (rc, self.lib.gpvmc_last_error().decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._check(self.lib.gpvmc_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_orbitals(self, group, occ):
+        occ = np.ascontiguousarray(occ, np.complex128)
+        assert occ.shape == (2, self.Ns, self.N)
+        self._check(self.lib.gpvmc_set_orbitals(self.h, group, _ptr(occ)))
+
+    def set_groups(self, groups):
+        g = np.ascontiguousarray(groups, np.int32)
+        assert g.shape == (self.W,)
+        self._check(self.lib.gpvmc_set_groups(self.h, _ptr(g)))
+
+    def init(self, spins):
+        s = np.ascontiguousarray(spins, np.int32).reshape(self.W, self.Ns)
+        ok = np.zeros(self.W, np.int32)
+        self._check(self.lib.gpvmc_init_walkers(self.h, _ptr(s), _ptr(ok)))
+        return ok.astype(bool)
+
+    def refresh(self, drift=False):
+        d = np.zeros(self.W) if drift else None
+        self._check(self.lib.gpvmc_refresh(self.h, _ptr(d)))
+        return d
+
+    def walk(self, n_steps, meas=0, mask=MEAS_PROJHEIS | MEAS_STAGMAGNZ, uniforms=None,
+             trace=False):
+        u = None
+        if uniforms is not None:
+            u = np.ascontiguousarray(uniforms, np.float64).reshape(self.W, n_steps, 4)
+        out = {}
+        if trace:
+            nm = 0 if meas <= 0 else (n_steps if meas == 1 else (n_steps + meas - 1) // meas)
+            out = dict(noop=np.zeros((self.W, n_steps), np.int8),
+                       accept=np.zeros((self.W, n_steps), np.int8),
+                       ratio=np.zeros((self.W, n_steps)),
+                       eloc=np.zeros((self.W, max(nm, 1)), np.complex128),
+                       mz=np.zeros((self.W, max(nm, 1)), np.complex128),
+                       logw=np.zeros((self.W, max(nm, 1))))
+        g = out.get
+        self._check(self.lib.gpvmc_walk(self.h, n_steps, meas, mask, _ptr(u), _ptr(g("noop")),
+                                        _ptr(g("accept")), _ptr(g("ratio")), _ptr(g("eloc")),
+                                        _ptr(g("mz")), _ptr(g("logw"))))
+        return out
+
+    def measure(self, mask=MEAS_PROJHEIS | MEAS_STAGMAGNZ):
+        """Estimators at the current configurations (no step, accumulators untouched)."""
+        e = np.zeros(self.W, np.complex128)
+        m = np.zeros(self.W, np.complex128)
+        lw = np.zeros(self.W)
+        self._check(self.lib.gpvmc_measure(self.h, mask, _ptr(e), _ptr(m), _ptr(lw)))
+        return dict(eloc=e, mz=m, logw=lw)
+
+    def accumulators(self):
+        a = np.zeros((self.W, ACC_STRIDE))
+        self._check(self.lib.gpvmc_read_accumulators(self.h, _ptr(a)))
+        return a
+
+    def reset_accumulators(self):
+        self._check(self.lib.gpvmc_reset_accumulators(self.h))
+
+    def state(self):
+        spins = np.zeros((self.W, self.Ns), np.int32)
+        part = np.zeros((self.W, 2, self.N), np.int32)
+        la = np.zeros(self.W)
+        ph = np.zeros(self.W)
+        self._check(self.lib.gpvmc_read_state(self.h, _ptr(spins), _ptr(part), _ptr(la), _ptr(ph)))
+        return spins, part, la, ph
+
+    def inverse(self, w):
+        ai = np.zeros((2, self.N, self.N), np.complex128)
+        self._check(self.lib.gpvmc_read_inverse(self.h, w, _ptr(ai)))
+        return ai
+
+    def kernel_time(self):
+        sm = ctypes.c_double()
+        sl = ctypes.c_int64()
+        rm = ctypes.c_double()
+        rl = ctypes.c_int64()
+        self._check(self.lib.gpvmc_kernel_time(self.h, ctypes.byref(sm), ctypes.byref(sl),
+                                               ctypes.byref(rm), ctypes.byref(rl)))
+        return dict(sweep_ms=sm.value, sweep_launches=sl.value, refresh_ms=rm.value,
+                    refresh_launches=rl.value)
+
+    def reset_kernel_time(self):
+        self._check(self.lib.gpvmc_reset_kernel_time(self.h))

@@ -1,0 +1,525 @@
+// C ABI of libgpvmc_hip.so (declared in include/gpvmc.h): context, device buffers,
+// argument validation and kernel dispatch.  The kernels live in vmc_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpvmc.h"
+#include "vmc_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(GPVMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Lattice tables of SquareLattice (SquareLattice.cpp:22-27): edges with x outer, y inner,
+// first (x,y)->(x,y+1) then (x,y)->(x+1,y); each vertex lists its edges in creation order
+// (Lattice.h:79-84).
+void build_lattice(int L, std::vector<int32_t> &nbr, std::vector<int32_t> &edge4)
+{
+    const int Ns = L * L;
+    std::vector<int> ea, eb;
+    for (int x = 0; x < L; ++x)
+        for (int y = 0; y < L; ++y) {
+            ea.push_back(y * L + x);
+            eb.push_back(((y + 1) % L) * L + x);
+            ea.push_back(y * L + x);
+            eb.push_back(y * L + (x + 1) % L);
+        }
+    nbr.assign(4 * Ns, -1);
+    std::vector<int> cnt(Ns, 0);
+    for (size_t e = 0; e < ea.size(); ++e) {
+        nbr[4 * ea[e] + cnt[ea[e]]++] = eb[e];
+        nbr[4 * eb[e] + cnt[eb[e]]++] = ea[e];
+    }
+    edge4.assign(4 * ea.size(), 0);
+    for (size_t e = 0; e < ea.size(); ++e) {
+        int a = ea[e], b = eb[e], da = -1, db = -1;
+        for (int d = 0; d < 4; ++d) {
+            if (nbr[4 * a + d] == b && da < 0) da = d;
+            if (nbr[4 * b + d] == a && db < 0) db = d;
+        }
+        edge4[4 * e] = a;
+        edge4[4 * e + 1] = da;
+        edge4[4 * e + 2] = b;
+        edge4[4 * e + 3] = db;
+    }
+}
+
+int default_lpr(int prec, int N)
+{
+    if (N == 8) return 8;
+    if (N == 18) return 2;
+    if (N == 32) return 4;
+    if (N == 128) return 8;
+    return 0;
+}
+
+}  // namespace
+
+struct gpvmc_ctx {
+    gpvmc_config cfg;
+    int L = 0, Ns = 0, N = 0, W = 0, G = 0, prec = 0, lpr = 0, res = 0, nthreads = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t esize = 0;  // bytes per complex element of the inverse
+    void *d_Ai = nullptr;
+    int32_t *d_site = nullptr, *d_part = nullptr, *d_group = nullptr, *d_order = nullptr;
+    int32_t *d_nbr = nullptr, *d_status = nullptr;
+    int4 *d_edge = nullptr;
+    double *d_logabs = nullptr, *d_phase = nullptr, *d_acc = nullptr, *d_drift = nullptr;
+    uint64_t *d_counter = nullptr;
+    double2 *d_occ64 = nullptr;
+    float2 *d_occ32 = nullptr;
+    std::vector<int> have_orb;
+    std::vector<int32_t> groups;
+    bool initialised = false;
+    double sweep_ms = 0, refresh_ms = 0;
+    int64_t n_sweep = 0, n_refresh = 0;
+    // scratch for walk traces / replay
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+static void free_ctx(gpvmc_ctx *c)
+{
+    if (!c) return;
+    void *ptrs[] = {c->d_Ai, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
+                    c->d_status, c->d_edge, c->d_logabs, c->d_phase, c->d_acc, c->d_drift,
+                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int upload_order(gpvmc_ctx *c)
+{
+    // Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b % 8).  Walkers are
+    // sorted by parameter group and XCD x receives a contiguous run of that order, so
+    // walkers sharing an orbital table share an XCD's L2.
+    const int W = c->W, X = 8;
+    std::vector<int32_t> byg(W);
+    for (int i = 0; i < W; ++i) byg[i] = i;
+    std::stable_sort(byg.begin(), byg.end(),
+                     [&](int a, int b) { return c->groups[a] < c->groups[b]; });
+    std::vector<int32_t> order(W);
+    int pos = 0;
+    for (int x = 0; x < X; ++x)
+        for (int b = x; b < W; b += X) order[b] = byg[pos++];
+    HIPCHK(hipMemcpy(c->d_order, order.data(), sizeof(int32_t) * W, hipMemcpyHostToDevice));
+    return GPVMC_OK;
+}
+
+static int ensure_scratch(gpvmc_ctx *c, size_t bytes)
+{
+    if (bytes <= c->scratch_bytes) return GPVMC_OK;
+    if (c->d_scratch) HIPCHK(hipFree(c->d_scratch));
+    c->d_scratch = nullptr;
+    c->scratch_bytes = 0;
+    HIPCHK(hipMalloc(&c->d_scratch, bytes));
+    c->scratch_bytes = bytes;
+    return GPVMC_OK;
+}
+
+static int run_gj(gpvmc_ctx *c, double *drift_dev)
+{
+    gv::GJArgs a{};
+    a.Ai = c->d_Ai;
+    a.part = c->d_part;
+    a.occ = c->d_occ64;
+    a.group = c->d_group;
+    a.logabs = c->d_logabs;
+    a.phase = c->d_phase;
+    a.status = c->d_status;
+    a.drift = drift_dev;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(gv_launch_gj(c->prec, c->N, &a, c->W, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->refresh_ms += ms;
+    c->n_refresh += 1;
+    return GPVMC_OK;
+}
+
+extern "C" {
+
+const char *gpvmc_last_error(void) { return g_err.c_str(); }
+
+int gpvmc_version(void) { return 1; }
+
+int gpvmc_create(const gpvmc_config *cfg, gpvmc_ctx **out)
+{
+    if (!cfg || !out) return fail(GPVMC_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int L = cfg->L;
+    if (L < 4 || (L * L) % 2) return fail(GPVMC_ERR_ARG, "L must be >= 4 with L*L even");
+    if (cfg->n_walkers < 1) return fail(GPVMC_ERR_ARG, "n_walkers must be >= 1");
+    if (cfg->n_groups < 1) return fail(GPVMC_ERR_ARG, "n_groups must be >= 1");
+    if (cfg->precision != GPVMC_FP64 && cfg->precision != GPVMC_MIXED)
+        return fail(GPVMC_ERR_ARG, "precision must be GPVMC_FP64 or GPVMC_MIXED");
+    const int N = L * L / 2;
+    int lpr = cfg->lanes_per_row ? cfg->lanes_per_row : default_lpr(cfg->precision, N);
+    int res = 0, nt = 0;
+    if (!lpr || gv_sweep_supported(cfg->precision, N, lpr, &res, &nt) != lpr)
+        return fail(GPVMC_ERR_UNSUPPORTED, "no sweep kernel for N=" + std::to_string(N) +
+                                               " lanes_per_row=" + std::to_string(lpr));
+    if (!gv_gj_supported(cfg->precision, N))
+        return fail(GPVMC_ERR_UNSUPPORTED, "no LU kernel for N=" + std::to_string(N));
+
+    gpvmc_ctx *c = new gpvmc_ctx();
+    c->cfg = *cfg;
+    c->L = L;
+    c->Ns = L * L;
+    c->N = N;
+    c->W = cfg->n_walkers;
+    c->G = cfg->n_groups;
+    c->prec = cfg->precision;
+    c->lpr = lpr;
+    c->res = res;
+    c->nthreads = nt;
+    c->esize = c->prec == GPVMC_FP64 ? sizeof(double2) : sizeof(float2);
+    c->have_orb.assign(c->G, 0);
+    c->groups.assign(c->W, 0);
+    auto bail = [&](int rc) {
+        free_ctx(c);
+        return rc;
+    };
+#define CK(expr)                                                                           \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return bail(fail(GPVMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    CK(hipSetDevice(cfg->device));
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipEventCreate(&c->ev0));
+    CK(hipEventCreate(&c->ev1));
+    const size_t W = c->W, Ns = c->Ns;
+    CK(hipMalloc(&c->d_Ai, W * 2 * N * N * c->esize));
+    CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_part, W * 2 * N * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_group, W * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_order, W * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_status, W * 2 * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_logabs, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_phase, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_drift, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_acc, W * gv::ACC_STRIDE * sizeof(double)));
+    CK(hipMalloc(&c->d_counter, W * sizeof(uint64_t)));
+    CK(hipMalloc(&c->d_occ64, (size_t)c->G * 2 * Ns * N * sizeof(double2)));
+    if (c->prec == GPVMC_MIXED) CK(hipMalloc(&c->d_occ32, (size_t)c->G * 2 * Ns * N * sizeof(float2)));
+    CK(hipMemset(c->d_group, 0, W * sizeof(int32_t)));
+    CK(hipMemset(c->d_acc, 0, W * gv::ACC_STRIDE * sizeof(double)));
+    CK(hipMemset(c->d_counter, 0, W * sizeof(uint64_t)));
+    std::vector<int32_t> nbr, edge4;
+    build_lattice(L, nbr, edge4);
+    CK(hipMalloc(&c->d_nbr, nbr.size() * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_edge, edge4.size() * sizeof(int32_t)));
+    CK(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->d_edge, edge4.data(), edge4.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+#undef CK
+    int rc = upload_order(c);
+    if (rc) return bail(rc);
+    *out = c;
+    return GPVMC_OK;
+}
+
+int gpvmc_destroy(gpvmc_ctx *ctx)
+{
+    if (!ctx) return fail(GPVMC_ERR_ARG, "null context");
+    (void)hipSetDevice(ctx->cfg.device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+    return GPVMC_OK;
+}
+
+int gpvmc_set_orbitals(gpvmc_ctx *c, int32_t group, const double *occ)
+{
+    if (!c || !occ) return fail(GPVMC_ERR_ARG, "null argument");
+    if (group < 0 || group >= c->G) return fail(GPVMC_ERR_ARG, "group out of range");
+    const size_t n = (size_t)2 * c->Ns * c->N;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(occ[i])) return fail(GPVMC_ERR_ARG, "non-finite orbital value");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipMemcpy(c->d_occ64 + (size_t)group * n, occ, n * sizeof(double2), hipMemcpyHostToDevice));
+    if (c->prec == GPVMC_MIXED) {
+        std::vector<float2> f(n);
+        for (size_t i = 0; i < n; ++i) f[i] = make_float2((float)occ[2 * i], (float)occ[2 * i + 1]);
+        HIPCHK(hipMemcpy(c->d_occ32 + (size_t)group * n, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    c->have_orb[group] = 1;
+    return GPVMC_OK;
+}
+
+int gpvmc_set_groups(gpvmc_ctx *c, const int32_t *g)
+{
+    if (!c || !g) return fail(GPVMC_ERR_ARG, "null argument");
+    for (int w = 0; w < c->W; ++w)
+        if (g[w] < 0 || g[w] >= c->G) return fail(GPVMC_ERR_ARG, "group index out of range");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    c->groups.assign(g, g + c->W);
+    HIPCHK(hipMemcpy(c->d_group, g, sizeof(int32_t) * c->W, hipMemcpyHostToDevice));
+    return upload_order(c);
+}
+
+int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
+{
+    if (!c || !spins) return fail(GPVMC_ERR_ARG, "null argument");
+    for (int w = 0; w < c->W; ++w)
+        if (!c->have_orb[c->groups[w]])
+            return fail(GPVMC_ERR_STATE, "orbitals of group " + std::to_string(c->groups[w]) + " not set");
+    const int Ns = c->Ns, N = c->N, W = c->W;
+    std::vector<int32_t> site((size_t)W * Ns), part((size_t)W * 2 * N);
+    for (int w = 0; w < W; ++w) {
+        int cnt[2] = {0, 0};
+        for (int v = 0; v < Ns; ++v) {
+            int fl = spins[(size_t)w * Ns + v];
+            if (fl != 0 && fl != 1) return fail(GPVMC_ERR_ARG, "spins must be 0 or 1");
+            if (cnt[fl] >= N) return fail(GPVMC_ERR_ARG, "walker " + std::to_string(w) + " is not at Sz=0");
+            // State::InitFock (State.cpp:29-41): labels in increasing site order
+            site[(size_t)w * Ns + v] = (fl << 16) | cnt[fl];
+            part[(size_t)w * 2 * N + (size_t)fl * N + cnt[fl]] = v;
+            cnt[fl]++;
+        }
+    }
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipMemcpy(c->d_site, site.data(), site.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
+    int rc = run_gj(c, nullptr);
+    if (rc) return rc;
+    std::vector<int32_t> st((size_t)W * 2);
+    HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ok)
+        for (int w = 0; w < W; ++w) ok[w] = (st[2 * w] == 0 && st[2 * w + 1] == 0) ? 1 : 0;
+    c->initialised = true;
+    return GPVMC_OK;
+}
+
+int gpvmc_refresh(gpvmc_ctx *c, double *drift)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = run_gj(c, drift ? c->d_drift : nullptr);
+    if (rc) return rc;
+    if (drift) {
+        std::vector<double> d((size_t)c->W * 2);
+        HIPCHK(hipMemcpy(d.data(), c->d_drift, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int w = 0; w < c->W; ++w) drift[w] = std::max(d[2 * w], d[2 * w + 1]);
+    }
+    return GPVMC_OK;
+}
+
+static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
+                     const double *uniforms, int8_t *noop, int8_t *accept, double *ratio,
+                     double *eloc, double *mz, double *logw, int measure_only)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
+    if (n_steps < 0 || meas < 0) return fail(GPVMC_ERR_ARG, "negative n_steps or meas");
+    if (mask & ~(uint32_t)(GPVMC_MEAS_PROJHEIS | GPVMC_MEAS_STAGMAGNZ | GPVMC_MEAS_OVERLAP))
+        return fail(GPVMC_ERR_UNSUPPORTED, "estimator not available in this build");
+    if (n_steps == 0 && !measure_only) return GPVMC_OK;
+    const size_t W = c->W;
+    const int64_t nmeas = measure_only ? 1
+                          : meas > 0 ? (meas == 1 ? n_steps : (n_steps + meas - 1) / meas) : 0;
+    if (uniforms) {
+        const size_t n = W * (size_t)n_steps * 4;
+        for (size_t i = 0; i < n; ++i)
+            if (!(uniforms[i] >= 0.0 && uniforms[i] < 1.0))
+                return fail(GPVMC_ERR_ARG, "replay uniforms must lie in [0, 1)");
+    }
+    // scratch layout: uniforms | noop | accept | ratio | eloc | mz | logw
+    size_t off = 0, o_u = 0, o_no = 0, o_ac = 0, o_ra = 0, o_el = 0, o_mz = 0, o_lw = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    if (uniforms) o_u = take(W * n_steps * 4 * sizeof(double));
+    if (noop) o_no = take(W * n_steps);
+    if (accept) o_ac = take(W * n_steps);
+    if (ratio) o_ra = take(W * n_steps * sizeof(double));
+    if (eloc && nmeas) o_el = take(W * nmeas * 2 * sizeof(double));
+    if (mz && nmeas) o_mz = take(W * nmeas * 2 * sizeof(double));
+    if (logw && nmeas) o_lw = take(W * nmeas * sizeof(double));
+    HIPCHK(hipSetDevice(c->cfg.device));
+    if (off) {
+        int rc = ensure_scratch(c, off);
+        if (rc) return rc;
+    }
+    char *sb = static_cast<char *>(c->d_scratch);
+    gv::SweepArgs a{};
+    a.Ai = c->d_Ai;
+    a.site = c->d_site;
+    a.part = c->d_part;
+    a.logabs = c->d_logabs;
+    a.phase = c->d_phase;
+    a.counter = c->d_counter;
+    a.acc = c->d_acc;
+    a.occ = c->prec == GPVMC_FP64 ? (const void *)c->d_occ64 : (const void *)c->d_occ32;
+    a.group = c->d_group;
+    a.order = c->d_order;
+    a.nbr = c->d_nbr;
+    a.edge = c->d_edge;
+    a.n_steps = n_steps;
+    a.meas = meas;
+    a.n_meas_slots = nmeas;
+    a.mask = mask;
+    a.key0 = (uint32_t)c->cfg.seed;
+    a.key1 = (uint32_t)(c->cfg.seed >> 32);
+    a.L = c->L;
+    a.measure_only = measure_only;
+    if (uniforms) {
+        a.uniforms = reinterpret_cast<const double *>(sb + o_u);
+        HIPCHK(hipMemcpyAsync(sb + o_u, uniforms, W * n_steps * 4 * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    if (noop) a.tr_noop = reinterpret_cast<int8_t *>(sb + o_no);
+    if (accept) a.tr_accept = reinterpret_cast<int8_t *>(sb + o_ac);
+    if (ratio) a.tr_ratio = reinterpret_cast<double *>(sb + o_ra);
+    if (eloc && nmeas) a.tr_eloc = reinterpret_cast<double *>(sb + o_el);
+    if (mz && nmeas) a.tr_mz = reinterpret_cast<double *>(sb + o_mz);
+    if (logw && nmeas) a.tr_logw = reinterpret_cast<double *>(sb + o_lw);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(gv_launch_sweep(c->prec, c->N, c->lpr, &a, c->W, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (!measure_only) {
+        c->sweep_ms += ms;
+        c->n_sweep += 1;
+    }
+    if (noop) HIPCHK(hipMemcpy(noop, a.tr_noop, W * n_steps, hipMemcpyDeviceToHost));
+    if (accept) HIPCHK(hipMemcpy(accept, a.tr_accept, W * n_steps, hipMemcpyDeviceToHost));
+    if (ratio) HIPCHK(hipMemcpy(ratio, a.tr_ratio, W * n_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (a.tr_eloc) HIPCHK(hipMemcpy(eloc, a.tr_eloc, W * nmeas * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (a.tr_mz) HIPCHK(hipMemcpy(mz, a.tr_mz, W * nmeas * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (a.tr_logw) HIPCHK(hipMemcpy(logw, a.tr_logw, W * nmeas * sizeof(double), hipMemcpyDeviceToHost));
+    return GPVMC_OK;
+}
+
+int gpvmc_walk(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask, const double *uniforms,
+               int8_t *noop, int8_t *accept, double *ratio, double *eloc, double *mz, double *logw)
+{
+    return walk_impl(c, n_steps, meas, mask, uniforms, noop, accept, ratio, eloc, mz, logw, 0);
+}
+
+int gpvmc_measure(gpvmc_ctx *c, uint32_t mask, double *eloc, double *mz, double *logw)
+{
+    return walk_impl(c, 0, 0, mask, nullptr, nullptr, nullptr, nullptr, eloc, mz, logw, 1);
+}
+
+int gpvmc_read_accumulators(gpvmc_ctx *c, double *acc)
+{
+    if (!c || !acc) return fail(GPVMC_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(acc, c->d_acc, (size_t)c->W * gv::ACC_STRIDE * sizeof(double), hipMemcpyDeviceToHost));
+    return GPVMC_OK;
+}
+
+int gpvmc_reset_accumulators(gpvmc_ctx *c)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)c->W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GPVMC_OK;
+}
+
+int gpvmc_read_statspin(gpvmc_ctx *c, double *acc)
+{
+    (void)acc;
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    return fail(GPVMC_ERR_UNSUPPORTED, "StatSpinStruct is not available in this build");
+}
+
+int gpvmc_read_state(gpvmc_ctx *c, int32_t *spins, int32_t *part, double *logabs, double *phase)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t W = c->W, Ns = c->Ns, N = c->N;
+    if (spins) {
+        std::vector<int32_t> site(W * Ns);
+        HIPCHK(hipMemcpy(site.data(), c->d_site, site.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < site.size(); ++i) spins[i] = site[i] >> 16;
+    }
+    if (part) HIPCHK(hipMemcpy(part, c->d_part, W * 2 * N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> la(W * 2), ph(W * 2);
+    HIPCHK(hipMemcpy(la.data(), c->d_logabs, la.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ph.data(), c->d_phase, ph.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t w = 0; w < W; ++w) {
+        if (logabs) logabs[w] = la[2 * w] + la[2 * w + 1];
+        if (phase) phase[w] = std::remainder(ph[2 * w] + ph[2 * w + 1], 2 * M_PI);
+    }
+    return GPVMC_OK;
+}
+
+int gpvmc_read_inverse(gpvmc_ctx *c, int32_t walker, double *ai)
+{
+    if (!c || !ai) return fail(GPVMC_ERR_ARG, "null argument");
+    if (walker < 0 || walker >= c->W) return fail(GPVMC_ERR_ARG, "walker out of range");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)2 * c->N * c->N;
+    const char *src = static_cast<const char *>(c->d_Ai) + (size_t)walker * n * c->esize;
+    if (c->prec == GPVMC_FP64) {
+        HIPCHK(hipMemcpy(ai, src, n * sizeof(double2), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float2> f(n);
+        HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float2), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            ai[2 * i] = f[i].x;
+            ai[2 * i + 1] = f[i].y;
+        }
+    }
+    return GPVMC_OK;
+}
+
+int gpvmc_kernel_time(gpvmc_ctx *c, double *sweep_ms, int64_t *sweep_launches, double *refresh_ms,
+                      int64_t *refresh_launches)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (sweep_ms) *sweep_ms = c->sweep_ms;
+    if (sweep_launches) *sweep_launches = c->n_sweep;
+    if (refresh_ms) *refresh_ms = c->refresh_ms;
+    if (refresh_launches) *refresh_launches = c->n_refresh;
+    return GPVMC_OK;
+}
+
+int gpvmc_reset_kernel_time(gpvmc_ctx *c)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    c->sweep_ms = c->refresh_ms = 0;
+    c->n_sweep = c->n_refresh = 0;
+    return GPVMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Kernel argument blocks shared by vmc_kernels.hip and the C-ABI layer (gpvmc_api.cpp).
+#ifndef GPVMC_VMC_KERNELS_H
+#define GPVMC_VMC_KERNELS_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gv {
+
+enum : uint32_t { MEAS_PROJHEIS = 1u, MEAS_STAGMAGNZ = 2u, MEAS_OVERLAP = 4u, MEAS_STATSPIN = 8u };
+constexpr int ACC_STRIDE = 12;
+
+struct SweepArgs {
+    void *Ai;                 // T2 [W][2][N][N], Ai[fl][label][slot]
+    int32_t *site;            // [W][Ns] (flavour << 16) | label
+    int32_t *part;            // [W][2][N] site of each label
+    double *logabs;           // [W][2] ln|det| per flavour
+    double *phase;            // [W][2] arg det per flavour
+    uint64_t *counter;        // [W] Philox step counter
+    double *acc;              // [W][ACC_STRIDE]
+    const void *occ;          // T2 [G][2][Ns][N]
+    const int32_t *group;     // [W]
+    const int32_t *order;     // [blocks] block -> walker (may be null)
+    const int32_t *nbr;       // [Ns][4]
+    const int4 *edge;         // [2Ns] (a, slot of b in nbr[a], b, slot of a in nbr[b])
+    const double *uniforms;   // [W][n_steps][4] replay stream or null (Philox)
+    int8_t *tr_noop;          // [W][n_steps] or null
+    int8_t *tr_accept;
+    double *tr_ratio;
+    double *tr_eloc;          // [W][n_meas_slots] complex or null
+    double *tr_mz;
+    double *tr_logw;
+    int64_t n_steps;
+    int64_t meas;
+    int64_t n_meas_slots;
+    uint32_t mask;
+    uint32_t key0, key1;
+    int32_t L;
+    int32_t measure_only;     // 1: no step, one measurement into the traces only
+};
+
+struct GJArgs {
+    void *Ai;                 // OUT2 [W][2][N][N]
+    const int32_t *part;      // [W][2][N]
+    const double2 *occ;       // [G][2][Ns][N] FP64
+    const int32_t *group;     // [W]
+    double *logabs;           // [W][2]
+    double *phase;            // [W][2]
+    int32_t *status;          // [W][2]
+    double *drift;            // [W][2] or null
+};
+
+}  // namespace gv
+
+extern "C" {
+int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
+hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
+                           hipStream_t st);
+int gv_gj_supported(int prec, int N);
+hipError_t gv_launch_gj(int prec, int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
+}
+
+#endif

@@ -1,0 +1,660 @@
+// gfx950 kernels of the Metropolis VMC hot path: one workgroup per Markov chain.
+//
+// Thread layout (shared by the sweep and the Gauss-Jordan kernels): thread t owns row
+// `row = t / LPR` of each flavour's N x N matrix and the LPR-strided columns
+// j = c + LPR*k (c = t % LPR, k < CPT = N/LPR).  A row dot product is CPT complex FMAs per
+// lane plus a log2(LPR)-stage xor-shuffle reduction inside the row's lane group, so the
+// ratio of a proposal (one row of the inverse against one orbital column) costs O(N/LPR)
+// on the critical path, and the Sherman-Morrison update is one fused read-modify-write
+// pass over the inverse with no cross-row communication except one LDS broadcast of the
+// replaced row.  Workgroups are padded to whole wavefronts; padding threads own no row.
+//
+// Matrix conventions (reference: SlaterDeterminant.cpp:68-74, linalg.cpp:9-44):
+// A(f, r) = Phi(orbital of wavefunction particle f, site of lattice particle r); the
+// inverse is stored row-major as Ai[label r][slot f] = (A^-1)(r, f); the ratio of moving
+// lattice particle ci to site s is q = sum_f Ai[ci][f] Phi(orb_f, s)  (linalg.cpp:143-154).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vmc_kernels.h"
+
+namespace gv {
+
+template <typename T> struct cpx;
+template <> struct cpx<float> { using t = float2; };
+template <> struct cpx<double> { using t = double2; };
+
+template <typename C> __device__ __forceinline__ C cmk(double x, double y)
+{
+    C r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+
+template <typename C> __device__ __forceinline__ C cfma(C a, C b, C acc)
+{
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+    return acc;
+}
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+// sum over the LPR lanes of one row group (aligned, power-of-two group inside a wavefront)
+template <int LPR, typename C> __device__ __forceinline__ C row_reduce(C v)
+{
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o);
+        v.y += __shfl_xor(v.y, o);
+    }
+    return v;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ double u32_to_open01(uint32_t x)
+{
+    return ((double)x + 0.5) * 2.3283064365386963e-10;  // 2^-32
+}
+
+// block-wide sum of NV doubles; every thread receives the totals.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
+{
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) s_red[wid][i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) t += s_red[q][i];
+        v[i] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Sweep kernel: n_steps of MetroMC::Step (MetroMC.cpp:90-103) for one walker per block.
+// RES = true keeps both inverses in registers for the whole launch; RES = false streams
+// the walker's rows from global memory at every use.
+template <typename T, int N, int LPR, bool RES>
+__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
+sweep_kernel(SweepArgs a)
+{
+    using C = typename cpx<T>::t;
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    constexpr int CPT = N / LPR;
+    constexpr int NS = 2 * N;
+    static_assert(N % LPR == 0, "LPR must divide N");
+    static_assert(NT <= 1024, "workgroup too large");
+
+    __shared__ int s_site[NS];        // (flavour << 16) | label
+    __shared__ int s_part[2][N];      // label -> site
+    __shared__ int s_nbr[NS * 4];
+    __shared__ C s_row[2][2][N];      // [parity][flavour] replaced row of the inverse
+    __shared__ C s_col[2][2][N];      // [parity][flavour] orbital column of the new site
+    __shared__ C s_q[2][2];           // [parity][flavour] single-hop ratios
+    __shared__ C s_qb[NS * 4];        // measurement: ratio of the particle at site s hopping to nbr d
+    __shared__ double s_red[NT / 64][4];
+
+    const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+    const int tid = threadIdx.x;
+    const int row = tid / LPR, c = tid % LPR;
+    const bool has_row = row < N;
+
+    for (int i = tid; i < NS; i += NT) s_site[i] = a.site[(size_t)w * NS + i];
+    for (int i = tid; i < 2 * N; i += NT) s_part[i / N][i % N] = a.part[(size_t)w * 2 * N + i];
+    for (int i = tid; i < NS * 4; i += NT) s_nbr[i] = a.nbr[i];
+
+    const C *occ = reinterpret_cast<const C *>(a.occ) + (size_t)a.group[w] * 2 * NS * N;
+    C *gA = reinterpret_cast<C *>(a.Ai) + (size_t)w * 2 * N * N;
+
+    C m[2][CPT];
+    if (RES && has_row) {
+#pragma unroll
+        for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) m[fl][k] = gA[(size_t)fl * N * N + (size_t)row * N + c + LPR * k];
+    }
+    // per-walker scalars owned by thread 0: [0..1] ln|det| per flavour, [2..3] arg det,
+    // then the ACC_STRIDE estimator sums of this launch
+    __shared__ double s_sc[4 + ACC_STRIDE];
+    if (tid == 0) {
+        s_sc[0] = a.logabs[2 * w];
+        s_sc[1] = a.logabs[2 * w + 1];
+        s_sc[2] = a.phase[2 * w];
+        s_sc[3] = a.phase[2 * w + 1];
+        for (int i = 0; i < ACC_STRIDE; ++i) s_sc[4 + i] = 0.0;
+    }
+    double *const sacc = s_sc + 4;
+    const uint64_t ctr0 = a.counter[w];
+    int par = 0;
+    int64_t mslot = 0;
+    __syncthreads();
+
+    // s = -1 is a measurement-only pass (gpvmc_measure): no step, one measurement
+    for (int64_t s = a.measure_only ? -1 : 0; s < a.n_steps; ++s) {
+      if (s >= 0) {
+        double u0, u1, u3;
+        if (a.uniforms) {
+            const double *up = a.uniforms + ((size_t)w * a.n_steps + s) * 4;
+            u0 = up[0];
+            u1 = up[1];
+            u3 = up[3];
+        } else {
+            uint64_t st = ctr0 + (uint64_t)s;
+            uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
+            philox4x32_10(cc, a.key0, a.key1);
+            u0 = u32_to_open01(cc[0]);
+            u1 = u32_to_open01(cc[1]);
+            u3 = u32_to_open01(cc[3]);  // cc[2] is the internal-state draw (one state per site)
+        }
+        int v1 = (int)(NS * u0);
+        v1 = v1 < NS ? v1 : NS - 1;
+        int ed = (int)(4 * u1);
+        ed = ed < 4 ? ed : 3;
+        const int v2 = s_nbr[v1 * 4 + ed];
+        const int i1 = s_site[v1], i2 = s_site[v2];
+        const bool noop = (i1 >> 16) == (i2 >> 16);
+        double ratio = 1.0;
+        bool acc;
+        if (noop) {
+            acc = u3 < 1.0;  // LatticeStepper.cpp:64-68 returns the current weight: ratio 1
+            if (tid == 0) sacc[10] += 1;
+        } else {
+            int to[2], ci[2];
+            if ((i1 >> 16) == 0) {
+                ci[0] = i1 & 0xFFFF; to[0] = v2;
+                ci[1] = i2 & 0xFFFF; to[1] = v1;
+            } else {
+                ci[0] = i2 & 0xFFFF; to[0] = v1;
+                ci[1] = i1 & 0xFFFF; to[1] = v2;
+            }
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                if (row == ci[fl]) {
+                    const C *col = occ + ((size_t)fl * NS + to[fl]) * N;
+                    C q = cmk<C>(0, 0);
+#pragma unroll
+                    for (int k = 0; k < CPT; ++k) {
+                        const int j = c + LPR * k;
+                        C cv = col[j];
+                        C mv = RES ? m[fl][k] : gA[(size_t)fl * N * N + (size_t)row * N + j];
+                        q = cfma(mv, cv, q);
+                        s_row[par][fl][j] = mv;
+                        s_col[par][fl][j] = cv;
+                    }
+                    q = row_reduce<LPR>(q);
+                    if (c == 0) s_q[par][fl] = q;
+                }
+            }
+            __syncthreads();
+            const C q0 = s_q[par][0], q1 = s_q[par][1];
+            const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
+            const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
+            ratio = n0 * n1;
+            acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo, MetroMC.cpp:27-43
+            if (acc) {
+                // Sherman-Morrison: Ai -= (Ai C - e_ci)(row_ci / q)  (linalg.cpp:402-561, rc=1)
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const C q = fl ? q1 : q0;
+                    const double nq = fl ? n1 : n0;
+                    const C qi = cmk<C>(q.x / nq, -q.y / nq);
+                    if (has_row) {
+                        C d = cmk<C>(0, 0);
+                        if (RES) {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) d = cfma(m[fl][k], s_col[par][fl][c + LPR * k], d);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
+                                const int j = c + LPR * k;
+                                d = cfma(gA[(size_t)fl * N * N + (size_t)row * N + j], s_col[par][fl][j], d);
+                            }
+                        }
+                        d = row_reduce<LPR>(d);
+                        if (row == ci[fl]) d.x -= 1;
+                        const C coef = cmul(d, qi);
+                        const C nc = cmk<C>(-coef.x, -coef.y);
+                        if (RES) {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) m[fl][k] = cfma(nc, s_row[par][fl][c + LPR * k], m[fl][k]);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
+                                const int j = c + LPR * k;
+                                C *p = gA + (size_t)fl * N * N + (size_t)row * N + j;
+                                *p = cfma(nc, s_row[par][fl][j], *p);
+                            }
+                        }
+                    }
+                }
+                // State::Hop (State.cpp:73-84); every wave writes the same values
+                if ((tid & 63) == 0) {
+                    s_site[to[0]] = ci[0];
+                    s_site[to[1]] = (1 << 16) | ci[1];
+                    s_part[0][ci[0]] = to[0];
+                    s_part[1][ci[1]] = to[1];
+                }
+                if (tid == 0) {
+                    s_sc[0] += 0.5 * log(n0);
+                    s_sc[1] += 0.5 * log(n1);
+                    s_sc[2] += atan2((double)q0.y, (double)q0.x);
+                    s_sc[3] += atan2((double)q1.y, (double)q1.x);
+                    sacc[8] += 1;
+                }
+            } else {
+                if (tid == 0) sacc[9] += 1;
+            }
+            par ^= 1;
+        }
+        if (tid == 0) {
+            const size_t ti = (size_t)w * a.n_steps + s;
+            if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
+            if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
+            if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+        }
+      }
+
+        if (s < 0 || (a.meas > 0 && (a.meas == 1 || s % a.meas == 0))) {
+            // Quantity::measure after the step (MetroMC.cpp:61-68)
+            __syncthreads();
+            double ev[2] = {0.0, 0.0};
+            if (a.mask & MEAS_PROJHEIS) {
+                if (has_row) {
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        const int sh = s_part[fl][row];
+                        for (int d = 0; d < 4; ++d) {
+                            const int t = s_nbr[sh * 4 + d];
+                            if ((s_site[t] >> 16) == fl) continue;  // parallel bond
+                            const C *col = occ + ((size_t)fl * NS + t) * N;
+                            C q = cmk<C>(0, 0);
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
+                                const int j = c + LPR * k;
+                                C mv = RES ? m[fl][k] : gA[(size_t)fl * N * N + (size_t)row * N + j];
+                                q = cfma(mv, col[j], q);
+                            }
+                            q = row_reduce<LPR>(q);
+                            if (c == 0) s_qb[sh * 4 + d] = q;
+                        }
+                    }
+                }
+                __syncthreads();
+                // ProjHeis.cpp:131-166: -(1/4 + 1/2 q_up q_dn) per antiparallel edge, +1/4 per parallel
+                for (int e = tid; e < 2 * NS; e += NT) {
+                    const int4 eg = a.edge[e];
+                    if ((s_site[eg.x] >> 16) == (s_site[eg.z] >> 16)) {
+                        ev[0] += 0.25;
+                    } else {
+                        const C qa = s_qb[eg.x * 4 + eg.y], qb = s_qb[eg.z * 4 + eg.w];
+                        const double pr = (double)qa.x * qb.x - (double)qa.y * qb.y;
+                        const double pi = (double)qa.x * qb.y + (double)qa.y * qb.x;
+                        ev[0] += -0.25 - 0.5 * pr;
+                        ev[1] += -0.5 * pi;
+                    }
+                }
+            }
+            double mv2[1] = {0.0};
+            if (a.mask & MEAS_STAGMAGNZ) {
+                // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
+                for (int v = tid; v < NS; v += NT) {
+                    const int x = v % a.L, y = v / a.L;
+                    const double sg = ((x + y) & 1) ? -1.0 : 1.0;
+                    mv2[0] += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
+                }
+            }
+            double red[3] = {ev[0], ev[1], mv2[0]};
+            block_sum<NT, 3>(red, s_red);
+            const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
+            if (tid == 0 && s >= 0) {
+                const double lw = 2.0 * (s_sc[0] + s_sc[1]);
+                if (a.mask & MEAS_PROJHEIS) {
+                    sacc[0] += E_re;
+                    sacc[1] += E_im;
+                    sacc[2] += E_re * E_re + E_im * E_im;
+                }
+                if (a.mask & MEAS_STAGMAGNZ) {
+                    sacc[4] += mz;
+                    sacc[6] += mz * mz;
+                }
+                sacc[11] += lw;
+                sacc[3] += 1;
+            }
+            if (tid == 0) {
+                const double lw = 2.0 * (s_sc[0] + s_sc[1]);
+                const size_t mi = (size_t)w * a.n_meas_slots + mslot;
+                if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
+                if (a.tr_mz) { a.tr_mz[2 * mi] = mz; a.tr_mz[2 * mi + 1] = 0.0; }
+                if (a.tr_logw) a.tr_logw[mi] = lw;
+            }
+            ++mslot;
+            __syncthreads();
+        }
+    }
+
+    __syncthreads();
+    if (RES && has_row) {
+#pragma unroll
+        for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) gA[(size_t)fl * N * N + (size_t)row * N + c + LPR * k] = m[fl][k];
+    }
+    for (int i = tid; i < NS; i += NT) a.site[(size_t)w * NS + i] = s_site[i];
+    for (int i = tid; i < 2 * N; i += NT) a.part[(size_t)w * 2 * N + i] = s_part[i / N][i % N];
+    if (tid == 0) {
+        a.logabs[2 * w] = s_sc[0];
+        a.logabs[2 * w + 1] = s_sc[1];
+        a.phase[2 * w] = remainder(s_sc[2], 2 * M_PI);
+        a.phase[2 * w + 1] = remainder(s_sc[3], 2 * M_PI);
+        a.counter[w] = ctr0 + (uint64_t)a.n_steps;
+        sacc[7] = (double)a.n_steps;
+        if (a.measure_only) sacc[7] = 0.0;
+        double *ac = a.acc + (size_t)w * ACC_STRIDE;
+        for (int i = 0; i < ACC_STRIDE; ++i) ac[i] += sacc[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 Gauss-Jordan inversion with partial pivoting, one block per (walker, flavour).
+// Builds A(f, r) = Phi(orb_f, site(r)) from the configuration, applies N Jordan exchanges
+// with the pivot row of each column chosen as the largest |re|+|im| among unpivoted rows
+// (the zgetrf choice, linalg.cpp:16), and writes Ai[label][slot] = (A^-1)(label, slot)
+// converted to OUT.  ln|det| and arg(det) include the permutation sign (linalg.cpp:17-22);
+// status[w*2+fl] = 1 when |det|^2 < 1e-9 (linalg.cpp:23) or a pivot vanished.  If `drift`
+// is given, max|Ai_old - Ai_new| / max|Ai_new| is stored there before overwriting.
+template <typename OUT, int N, int LPR>
+__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
+gj_kernel(GJArgs a)
+{
+    using C = double2;
+    using CO = typename cpx<OUT>::t;
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    constexpr int NW = NT / 64;
+    constexpr int CPT = N / LPR;
+    constexpr int NS = 2 * N;
+    static_assert(N % LPR == 0, "LPR must divide N");
+
+    __shared__ int s_part[N];
+    __shared__ C s_prow[2][NW][N];     // [parity][wave] best candidate row of that wave
+    __shared__ double s_cval[2][NW];
+    __shared__ int s_cidx[2][NW];
+    __shared__ int s_perm[N];          // pivot row chosen for column k
+    __shared__ int s_invp[N];          // column at which row i was pivoted
+    __shared__ double s_red[NW][4];
+    __shared__ int s_flag;
+    __shared__ unsigned char s_seen[N];
+
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int row = tid / LPR, c = tid % LPR;
+    const bool has_row = row < N;
+
+    for (int i = tid; i < N; i += NT) {
+        s_part[i] = a.part[(size_t)w * 2 * N + (size_t)fl * N + i];
+        s_invp[i] = 0;
+    }
+    if (tid == 0) s_flag = 0;
+    __syncthreads();
+
+    const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
+    C b[CPT];
+    if (has_row) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) b[k] = occ[(size_t)s_part[c + LPR * k] * N + row];
+    } else {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) b[k] = make_double2(0.0, 0.0);
+    }
+    bool pivoted = !has_row;
+    double la = 0.0, ph = 0.0;
+    int par = 0;
+
+    for (int k = 0; k < N; ++k) {
+        const int kk = k / LPR, ck = k % LPR;
+        // my row's element in column k lives in lane ck of the row group, slot kk
+        C mine = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int t = 0; t < CPT; ++t)
+            if (t == kk) mine = b[t];
+        const int src = (tid & ~(LPR - 1)) + ck;
+        C bk;
+        bk.x = __shfl(mine.x, src);
+        bk.y = __shfl(mine.y, src);
+        double val = pivoted ? -1.0 : fabs(bk.x) + fabs(bk.y);
+        int idx = has_row ? row : 0x7fffffff;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(val, o);
+            const int oi = __shfl_xor(idx, o);
+            if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
+        }
+        // the wave's best row writes itself as that wave's candidate
+        if (has_row && row == idx) {
+#pragma unroll
+            for (int t = 0; t < CPT; ++t) s_prow[par][wid][c + LPR * t] = b[t];
+        }
+        if (lane == 0) { s_cval[par][wid] = val; s_cidx[par][wid] = idx; }
+        __syncthreads();
+        int bw = 0;
+        double bv = s_cval[par][0];
+        int bi = s_cidx[par][0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const double v = s_cval[par][q];
+            const int i = s_cidx[par][q];
+            if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; bw = q; }
+        }
+        const int p = bi;
+        const C piv = s_prow[par][bw][k];
+        if (tid == 0) { s_perm[k] = p; s_invp[p] = k; }
+        const double np = piv.x * piv.x + piv.y * piv.y;
+        C ainv;
+        if (np > 0.0) {
+            ainv = make_double2(piv.x / np, -piv.y / np);
+            la += 0.5 * log(np);
+            ph += atan2(piv.y, piv.x);
+        } else {
+            ainv = make_double2(0.0, 0.0);
+            if (tid == 0) s_flag = 1;
+        }
+        if (has_row) {
+            const bool me = (row == p);
+            const C bka = cmul(bk, ainv);
+#pragma unroll
+            for (int t = 0; t < CPT; ++t) {
+                const int j = c + LPR * t;
+                const C rj = s_prow[par][bw][j];
+                if (me) {
+                    b[t] = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
+                } else {
+                    b[t] = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, b[t]);
+                }
+            }
+            if (me) pivoted = true;
+        }
+        par ^= 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // sign of the permutation k -> perm[k] by cycle counting
+        int sgn = 1;
+        for (int i = 0; i < N; ++i) s_seen[i] = 0;
+        for (int i = 0; i < N; ++i) {
+            if (s_seen[i]) continue;
+            int len = 0, j = i;
+            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
+            if ((len & 1) == 0) sgn = -sgn;
+        }
+        if (sgn < 0) ph += M_PI;
+        const int sing = s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
+        a.logabs[2 * w + fl] = la;
+        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
+        a.status[2 * w + fl] = sing;
+    }
+    // A^-1[invp[i]][perm[j]] = B[i][j]  (Jordan exchange with implicit pivoting)
+    CO *out = reinterpret_cast<CO *>(a.Ai) + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    double dmax = 0.0, nmax = 0.0;
+    if (has_row) {
+        int orow = s_invp[row];
+        orow = orow < 0 ? 0 : (orow < N ? orow : N - 1);  // in range even for a NaN matrix
+#pragma unroll
+        for (int t = 0; t < CPT; ++t) {
+            int oc = s_perm[c + LPR * t];
+            oc = oc < 0 ? 0 : (oc < N ? oc : N - 1);
+            const size_t o = (size_t)orow * N + oc;
+            if (a.drift) {
+                const CO old = out[o];
+                const double dx = (double)old.x - b[t].x, dy = (double)old.y - b[t].y;
+                dmax = fmax(dmax, sqrt(dx * dx + dy * dy));
+                nmax = fmax(nmax, sqrt(b[t].x * b[t].x + b[t].y * b[t].y));
+            }
+            CO v;
+            v.x = (OUT)b[t].x;
+            v.y = (OUT)b[t].y;
+            out[o] = v;
+        }
+    }
+    if (a.drift) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, o));
+            nmax = fmax(nmax, __shfl_xor(nmax, o));
+        }
+        if (lane == 0) { s_red[wid][0] = dmax; s_red[wid][1] = nmax; }
+        __syncthreads();
+        if (tid == 0) {
+            double d = 0.0, n = 0.0;
+            for (int q = 0; q < NW; ++q) { d = fmax(d, s_red[q][0]); n = fmax(n, s_red[q][1]); }
+            a.drift[2 * w + fl] = n > 0.0 ? d / n : 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host-side dispatch tables
+template <typename T, int N, int LPR, bool RES>
+static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st)
+{
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES>), dim3(nblocks), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename OUT, int N, int LPR>
+static hipError_t launch_gj_t(const GJArgs &a, int nwalkers, hipStream_t st)
+{
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    hipLaunchKernelGGL((gj_kernel<OUT, N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+struct SweepEntry {
+    int prec, N, lpr, res;
+    hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
+};
+struct GJEntry {
+    int prec, N, lpr;
+    hipError_t (*fn)(const GJArgs &, int, hipStream_t);
+};
+
+static const SweepEntry kSweep[] = {
+    {0, 8, 8, 1, launch_sweep_t<double, 8, 8, true>},
+    {0, 8, 4, 1, launch_sweep_t<double, 8, 4, true>},
+    {0, 18, 2, 1, launch_sweep_t<double, 18, 2, true>},
+    {0, 32, 2, 1, launch_sweep_t<double, 32, 2, true>},
+    {0, 32, 4, 1, launch_sweep_t<double, 32, 4, true>},
+    {0, 32, 8, 1, launch_sweep_t<double, 32, 8, true>},
+    {0, 128, 8, 0, launch_sweep_t<double, 128, 8, false>},
+    {1, 8, 8, 1, launch_sweep_t<float, 8, 8, true>},
+    {1, 18, 2, 1, launch_sweep_t<float, 18, 2, true>},
+    {1, 32, 2, 1, launch_sweep_t<float, 32, 2, true>},
+    {1, 32, 4, 1, launch_sweep_t<float, 32, 4, true>},
+    {1, 128, 8, 1, launch_sweep_t<float, 128, 8, true>},
+};
+
+static const GJEntry kGJ[] = {
+    {0, 8, 8, launch_gj_t<double, 8, 8>},
+    {0, 18, 2, launch_gj_t<double, 18, 2>},
+    {0, 32, 4, launch_gj_t<double, 32, 4>},
+    {0, 128, 8, launch_gj_t<double, 128, 8>},
+    {1, 8, 8, launch_gj_t<float, 8, 8>},
+    {1, 18, 2, launch_gj_t<float, 18, 2>},
+    {1, 32, 4, launch_gj_t<float, 32, 4>},
+    {1, 128, 8, launch_gj_t<float, 128, 8>},
+};
+
+}  // namespace gv
+
+extern "C" int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads)
+{
+    for (const auto &e : gv::kSweep)
+        if (e.prec == prec && e.N == N && (lpr == 0 || e.lpr == lpr)) {
+            if (res) *res = e.res;
+            if (nthreads) *nthreads = ((e.N * e.lpr + 63) / 64) * 64;
+            return e.lpr;
+        }
+    return 0;
+}
+
+extern "C" hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
+                                      hipStream_t st)
+{
+    for (const auto &e : gv::kSweep)
+        if (e.prec == prec && e.N == N && e.lpr == lpr) return e.fn(*a, nblocks, st);
+    return hipErrorInvalidValue;
+}
+
+extern "C" int gv_gj_supported(int prec, int N)
+{
+    for (const auto &e : gv::kGJ)
+        if (e.prec == prec && e.N == N) return e.lpr;
+    return 0;
+}
+
+extern "C" hipError_t gv_launch_gj(int prec, int N, const gv::GJArgs *a, int nwalkers, hipStream_t st)
+{
+    for (const auto &e : gv::kGJ)
+        if (e.prec == prec && e.N == N) return e.fn(*a, nwalkers, st);
+    return hipErrorInvalidValue;
+}

@@ -1,0 +1,130 @@
+/* gpvmc.h - C ABI of libgpvmc_hip.so, the MI355X (gfx950) Metropolis-VMC hot path.
+ *
+ * Plain C types only; every call returns GPVMC_OK (0) or a negative error code, with a
+ * message from gpvmc_last_error().  No exceptions cross the ABI.  The context owns all
+ * device memory; host buffers passed in are caller-owned and copied.  One host thread
+ * per context; several contexts may coexist.
+ *
+ * A context holds a batch of independent Markov chains ("walkers") of one lattice size
+ * and channel.  Each walker is the state of one reference process: LatticeState
+ * occupations, the two SlaterDeterminant inverse matrices, the log-magnitude/phase of
+ * the amplitude (the BigComplex m_amp role), a Philox counter and FP64 estimator sums.
+ *
+ * Reference interfaces each entry point replaces (file:line in EPFL-LQM/gpvmc):
+ *   gpvmc_set_orbitals    WaveFunction::init_matrices / MatEl (WaveFunction.cpp:27-37,
+ *                         WaveFunction.h:61-66), restricted to occupied orbitals
+ *   gpvmc_init_walkers    LatticeState::FockInit + SlaterDeterminant::Init
+ *                         (LatticeState.cpp:84-113, SlaterDeterminant.cpp:60-92)
+ *   gpvmc_walk            MetroMC::Walk (MetroMC.cpp:45-88) driving MetroMC::Step,
+ *                         LatticeStepper::trystep/step (LatticeStepper.cpp:36-147),
+ *                         SlaterDeterminant::VirtUpdate/Update (SlaterDeterminant.cpp:96-326)
+ *                         and Quantity::measure of ProjHeis (ProjHeis.cpp:40-166),
+ *                         StagMagnZ (StagMagnZ.cpp:12-43) and OverlapTrack (OverlapTrack.cpp:26-29)
+ *   gpvmc_measure         Quantity::measure of the same estimators at the current state
+ *   gpvmc_refresh         SlaterDeterminant::Init on the current configuration (FP64 LU
+ *                         recomputation of the inverse; the reference never refreshes)
+ *   gpvmc_read_*          Quantity::save (ScalarQuantity.cpp:10-14, MatrixQuantity.cpp:12-17),
+ *                         LatticeState::Getfs/Getpt (State.h:93-97), SlaterDeterminant::Amp
+ */
+#ifndef GPVMC_H
+#define GPVMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPVMC_OK 0
+#define GPVMC_ERR_ARG -1
+#define GPVMC_ERR_HIP -2
+#define GPVMC_ERR_STATE -3
+#define GPVMC_ERR_UNSUPPORTED -4
+
+/* arithmetic of the inverse matrices and ratios */
+#define GPVMC_FP64 0   /* plain FP64 (the parity mode) */
+#define GPVMC_MIXED 1  /* FP32 inverse and ratios; FP64 LU refresh, log-det, sums */
+
+/* estimator mask for gpvmc_walk */
+#define GPVMC_MEAS_PROJHEIS 1u    /* ProjHeis local energy per site (ground state) */
+#define GPVMC_MEAS_STAGMAGNZ 2u   /* StagMagnZ staggered magnetisation */
+#define GPVMC_MEAS_OVERLAP 4u     /* OverlapTrack weight |amp|^2 (as log) */
+#define GPVMC_MEAS_STATSPIN 8u    /* StatSpinStruct 5 x Ns^2 correlators */
+
+typedef struct gpvmc_ctx gpvmc_ctx;
+
+typedef struct {
+    int32_t L;             /* lattice side; Ns = L*L sites, N = Ns/2 particles per flavour */
+    int32_t n_walkers;     /* independent chains */
+    int32_t n_groups;      /* parameter groups (distinct orbital tables), >= 1 */
+    int32_t precision;     /* GPVMC_FP64 or GPVMC_MIXED */
+    int32_t device;        /* HIP device ordinal */
+    int32_t lanes_per_row; /* 0 = library default; else 1,2,4,8,16 (tuning) */
+    uint64_t seed;         /* Philox4x32-10 key */
+} gpvmc_config;
+
+const char *gpvmc_last_error(void);
+int gpvmc_version(void);
+
+int gpvmc_create(const gpvmc_config *cfg, gpvmc_ctx **out);
+int gpvmc_destroy(gpvmc_ctx *ctx);
+
+/* Occupied-orbital table of one parameter group: occ[fl][site][p] = Phi_fl(orbital of
+ * wavefunction particle p, site), complex interleaved, 2*Ns*N complex values. */
+int gpvmc_set_orbitals(gpvmc_ctx *ctx, int32_t group, const double *occ);
+/* walker -> parameter group map (n_walkers entries); default all 0 */
+int gpvmc_set_groups(gpvmc_ctx *ctx, const int32_t *group_of_walker);
+
+/* Initial occupations spins[w][site] in {0 = flavour 0 (up), 1 = flavour 1 (down)},
+ * N of each.  Labels are assigned in increasing site order (State::InitFock), the inverse
+ * is formed by FP64 LU with partial pivoting.  ok[w] = 1 when the amplitude is non-zero
+ * (SlaterDeterminant::m_amp_ok), else 0.  `ok` may be NULL.  Resets the accumulators and
+ * the Philox counters. */
+int gpvmc_init_walkers(gpvmc_ctx *ctx, const int32_t *spins, int32_t *ok);
+
+/* Recompute every walker's inverse and log-amplitude from its configuration in FP64
+ * (drift refresh).  drift[w] (may be NULL) receives max|Ai_old - Ai_new| / max|Ai_new|. */
+int gpvmc_refresh(gpvmc_ctx *ctx, double *drift);
+
+/* Runs n_steps Metropolis steps on every walker (MetroMC::Walk(len=n_steps, meas)).
+ * After step s (0-based within this call) the estimators in `mask` are measured when
+ * meas > 0 and (meas == 1 or s % meas == 0).  Uniforms: Philox4x32-10 (key = seed,
+ * counter = (step, walker)) when `uniforms` is NULL, else the replay stream
+ * uniforms[w][s][4] (host array, values in [0,1)).  Optional traces (host arrays, NULL to
+ * skip): noop/accept[w][s] (int8), ratio[w][s] (double), and per measurement
+ * eloc[w][m] (complex), mz[w][m] (complex), logw[w][m] (double, ln |amp|^2). */
+int gpvmc_walk(gpvmc_ctx *ctx, int64_t n_steps, int64_t meas, uint32_t mask,
+               const double *uniforms, int8_t *noop, int8_t *accept, double *ratio,
+               double *eloc, double *mz, double *logw);
+
+/* Measures the estimators in `mask` once at every walker's current configuration,
+ * without stepping and without touching the accumulators (Quantity::measure outside a
+ * walk).  Outputs (host, NULL to skip): eloc[w] complex, mz[w] complex, logw[w]. */
+int gpvmc_measure(gpvmc_ctx *ctx, uint32_t mask, double *eloc, double *mz, double *logw);
+
+/* Per-walker FP64 sums since the last gpvmc_reset_accumulators / init, 12 doubles per
+ * walker: [0,1] sum E_loc (complex), [2] sum |E_loc|^2, [3] n_meas, [4,5] sum m_z,
+ * [6] sum |m_z|^2, [7] steps, [8] accepted moves, [9] rejected proposals, [10] no-op
+ * proposals, [11] sum ln|amp|^2. */
+#define GPVMC_ACC_STRIDE 12
+int gpvmc_read_accumulators(gpvmc_ctx *ctx, double *acc);
+int gpvmc_reset_accumulators(gpvmc_ctx *ctx);
+/* StatSpinStruct sums per walker [w][5][Ns][Ns] complex (szz, spm, smp, spp, smm). */
+int gpvmc_read_statspin(gpvmc_ctx *ctx, double *acc);
+
+/* Current state: spins[w][site], labels part[w][2][N] (site of each particle),
+ * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */
+int gpvmc_read_state(gpvmc_ctx *ctx, int32_t *spins, int32_t *part, double *logabs,
+                     double *phase);
+/* Inverse matrices of walker w as FP64 complex [2][N][N], Ai[fl][label][slot]. */
+int gpvmc_read_inverse(gpvmc_ctx *ctx, int32_t walker, double *ai);
+
+/* Cumulative device time of the sweep / refresh kernels (HIP events on the context's
+ * stream) and the number of launches since creation or the last reset. */
+int gpvmc_kernel_time(gpvmc_ctx *ctx, double *sweep_ms, int64_t *sweep_launches,
+                      double *refresh_ms, int64_t *refresh_launches);
+int gpvmc_reset_kernel_time(gpvmc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -17,7 +17,9 @@
 //   orbitals L phi_in_units_of_pi neel neel_exp bcx bcy
 //   walk L phi_in_units_of_pi neel neel_exp seed nsteps snap_every
 //   excgraph L channel(trans|long) qx qy
+//   time L phi_in_units_of_pi neel seed nsteps   (wall time of nsteps MetroMC::Step)
 #include <cstdio>
+#include <ctime>
 #include <cstdlib>
 #include <cmath>
 #include <complex>
@@ -284,6 +286,32 @@ static int mode_excgraph(int L, const string& channel, size_t qx, size_t qy)
     return 0;
 }
 
+// Times the reference's own hot loop: nsteps of MetroMC::Step on one chain after the
+// vmc.cpp:221-263 start-up (random start with retries).
+static int mode_time(int L, double phi, double neel, int seed, long nsteps)
+{
+    size_t N = L * L / 2;
+    RanGen::srand(seed);
+    SquareLattice slat(L, L);
+    LatticeState sp(nullptr, &slat, {N, N}, {1, 1});
+    StagFluxGroundState wav(nullptr, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0});
+    SlaterDeterminant amp(&sp, &wav);
+    vector<vector<size_t>> pop = {{N}, {N}};
+    int tc = 0;
+    while(amp.Amp() == 0.0 && tc < 10){ sp.RanInit(pop); amp.Init(); tc++; }
+    if(tc == 10){ fprintf(stderr, "no regular start\n"); return 1; }
+    LatticeStepper step(&sp, &wav, &amp);
+    MetroMC varmc(&step, nullptr);
+    timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for(long s=0; s<nsteps; ++s) varmc.Step(true);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    printf("{\"L\":%d,\"nsteps\":%ld,\"seconds\":%.6f,\"steps_per_s\":%.3f,\"rejection\":%.6f}\n",
+           L, nsteps, sec, nsteps / sec, varmc.Rejection());
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if(argc < 2){ fprintf(stderr, "usage: see header\n"); return 2; }
@@ -295,6 +323,8 @@ int main(int argc, char** argv)
     if(mode == "walk" && argc == 9)
         return mode_walk(atoi(argv[2]), atof(argv[3]), atof(argv[4]), atof(argv[5]),
                          atoi(argv[6]), atol(argv[7]), atol(argv[8]));
+    if(mode == "time" && argc == 7)
+        return mode_time(atoi(argv[2]), atof(argv[3]), atof(argv[4]), atoi(argv[5]), atol(argv[6]));
     if(mode == "excgraph" && argc == 6)
         return mode_excgraph(atoi(argv[2]), argv[3], (size_t)atoi(argv[4]), (size_t)atoi(argv[5]));
     fprintf(stderr, "bad arguments\n");

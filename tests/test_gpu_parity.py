@@ -1,0 +1,271 @@
+"""GPU parity of libgpvmc_hip.so against the oracle and the reference fixtures.
+
+Every test calls the HIP path through the C ABI (gpvmc_amd.binding) and compares it
+with the C oracle fed the same uniform stream.  Bars:
+  * FP64 mode: identical no-op/accept sequences and configurations (bit-exact integer
+    state), weight ratios within 1e-10 relative (plus an absolute floor for moves into
+    exactly-zero amplitudes), inverses within 1e-9, ln|amp| within 1e-9, E_loc within
+    1e-10, m_z exact;
+  * mixed FP32 mode: ratios within 2e-4 relative while decisions agree, drift refresh
+    restores the FP64 inverse to FP32 rounding, E_loc on identical configurations within
+    1e-5 of the FP64 value.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+from gpvmc_amd import binding, model
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def occ_for(L, phi, neel, ne=0.0):
+    return model.ground_occ(model.orbital_table(L, phi * np.pi, neel, ne))
+
+
+def ratio_ok(a, b, rtol):
+    return np.abs(a - b) <= rtol * np.abs(b) + 1e-30
+
+
+def relabel(inv, part):
+    """GPU rows are indexed by the walker's own particle labels (kept as particles move);
+    an oracle walker built from the configuration labels particles in increasing site
+    order.  Returns the GPU inverse with rows reordered to the oracle's labels."""
+    out = np.empty_like(inv)
+    for fl in range(2):
+        rank = np.argsort(np.argsort(part[fl]))  # label -> rank of its site
+        out[fl][rank] = inv[fl]
+    return out
+
+
+def oracle_walk(L, occ, spins, u, meas=0):
+    w = O.Walker(L, occ, spins)
+    assert w.ok
+    out = w.walk(u.reshape(-1), meas=meas)
+    return w, out
+
+
+# ----------------------------------------------------------------------------- FP64
+@pytest.mark.parametrize("name", ["walk_L4_s12345.npz", "walk_L6_s4242.npz", "walk_L8_s777.npz"])
+def test_fp64_replays_reference_trajectory(name):
+    g = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    L, nsteps = int(g["L"]), int(g["nsteps"])
+    N = L * L // 2
+    say("case fp64 reference replay", name)
+    u = O.glibc_uniforms(int(g["seed"]), 4 * nsteps + 20000)
+    spins, used = O.raninit(L, u)
+    ws = u[used:used + 4 * nsteps].reshape(nsteps, 4)
+    occ = occ_for(L, float(g["phi"]), float(g["neel"]), float(g["neel_exp"]))
+    with binding.Batch(L, 1, binding.FP64) as b:
+        b.set_orbitals(0, occ)
+        assert b.init(spins[None, :]).all()
+        ai0 = b.inverse(0)
+        # reference layout Ai[r + f*N]  ->  ours Ai[r][f]
+        np.testing.assert_allclose(ai0.reshape(2, N, N),
+                                   g["init_inv"].reshape(2, N, N).transpose(0, 2, 1), atol=1e-11)
+        out = b.walk(nsteps, uniforms=ws[None], trace=True)
+        moved = np.unpackbits(g["moved"])[:nsteps].astype(bool)
+        mine = out["accept"][0].astype(bool) & ~out["noop"][0].astype(bool)
+        assert np.array_equal(mine, moved)
+        spins_f, part, la, ph = b.state()
+        assert np.array_equal(spins_f[0], g["snaps"][-1])
+        assert np.array_equal(part[0], g["final_labels"])
+        assert abs(2 * la[0] / np.log(10) - g["snap_log10w"][-1]) < 1e-8
+        np.testing.assert_allclose(b.inverse(0), g["final_inv"].reshape(2, N, N).transpose(0, 2, 1),
+                                   atol=1e-9)
+    # the oracle on the same stream: ratios to 1e-10
+    w, o = oracle_walk(L, occ, spins, ws)
+    live = ~o["noop"].astype(bool)
+    assert np.array_equal(o["noop"], out["noop"][0])
+    assert ratio_ok(out["ratio"][0][live], o["ratio"][live], 1e-10).all()
+
+
+@pytest.mark.parametrize("L,W,nsteps,meas", [(4, 16, 2000, 16), (8, 32, 3000, 64), (6, 8, 1500, 36)])
+def test_fp64_philox_batch_matches_oracle(L, W, nsteps, meas):
+    say("case fp64 philox batch", L, W, nsteps, meas)
+    seed = 20261203 + L
+    occ = occ_for(L, 0.25, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.FP64, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        ok = b.init(spins)
+        out = b.walk(nsteps, meas=meas, trace=True)
+        s_f, part, la, ph = b.state()
+        acc = b.accumulators()
+        inv = [b.inverse(w) for w in range(W)]
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo, o = oracle_walk(L, occ, spins[w], u[w], meas=meas)
+        assert np.array_equal(o["noop"], out["noop"][w])
+        assert np.array_equal(o["accept"], out["accept"][w])
+        live = ~o["noop"].astype(bool)
+        assert ratio_ok(out["ratio"][w][live], o["ratio"][live], 1e-10).all()
+        assert np.array_equal(s_f[w], wo.spins())
+        assert np.array_equal(part[w].reshape(-1), wo.part)
+        assert abs(la[w] - wo.logabs) < 1e-9
+        assert abs(np.angle(np.exp(1j * (ph[w] - wo.phase)))) < 1e-8
+        np.testing.assert_allclose(inv[w], wo.Ai.transpose(0, 2, 1), atol=1e-9)
+        nm = len(o["eloc"])
+        np.testing.assert_allclose(out["eloc"][w][:nm], o["eloc"], atol=1e-10, rtol=0)
+        np.testing.assert_allclose(out["mz"][w][:nm].real, o["mz"].real, atol=1e-12)
+        np.testing.assert_allclose(out["logw"][w][:nm], o["logw"], atol=1e-8)
+        # accumulators: sums of the same measurements
+        assert acc[w, 3] == nm
+        assert abs(acc[w, 0] - o["eloc"].real.sum()) < 1e-9
+        assert acc[w, 7] == nsteps
+        assert acc[w, 8] == (o["accept"].astype(bool) & live).sum()
+        assert acc[w, 10] == (~live).sum()
+
+
+def test_fp64_init_inverse_matches_oracle_L16():
+    L, W = 16, 4
+    say("case fp64 init L16")
+    occ = occ_for(L, 0.1667, 0.1)
+    spins = model.random_spins(L, W, 5)
+    with binding.Batch(L, W, binding.FP64) as b:
+        b.set_orbitals(0, occ)
+        ok = b.init(spins)
+        _, _, la, ph = b.state()
+        inv = [b.inverse(w) for w in range(W)]
+    for w in range(W):
+        wo = O.Walker(L, occ, spins[w])
+        assert ok[w] == wo.ok
+        assert abs(la[w] - wo.logabs) < 1e-8
+        assert abs(np.angle(np.exp(1j * (ph[w] - wo.phase)))) < 1e-8
+        np.testing.assert_allclose(inv[w], wo.Ai.transpose(0, 2, 1), atol=1e-9)
+
+
+def test_fp64_streaming_L16_matches_oracle():
+    L, W, nsteps, meas = 16, 2, 600, 256
+    say("case fp64 streaming L16")
+    seed = 77
+    occ = occ_for(L, 0.2333, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.FP64, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        assert b.init(spins).all()
+        out = b.walk(nsteps, meas=meas, trace=True)
+        s_f, _, la, _ = b.state()
+        inv = [b.inverse(w) for w in range(W)]
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    for w in range(W):
+        wo, o = oracle_walk(L, occ, spins[w], u[w], meas=meas)
+        assert np.array_equal(o["accept"], out["accept"][w])
+        live = ~o["noop"].astype(bool)
+        assert ratio_ok(out["ratio"][w][live], o["ratio"][live], 1e-10).all()
+        assert np.array_equal(s_f[w], wo.spins())
+        assert abs(la[w] - wo.logabs) < 1e-9
+        np.testing.assert_allclose(inv[w], wo.Ai.transpose(0, 2, 1), atol=1e-9)
+        nm = len(o["eloc"])
+        np.testing.assert_allclose(out["eloc"][w][:nm], o["eloc"], atol=1e-10, rtol=0)
+
+
+def test_groups_and_walker_order():
+    # walkers on different parameter groups each follow their own orbitals
+    L, W = 8, 24
+    say("case groups")
+    occs = [occ_for(L, p, n) for p, n in ((0.1, 0.0), (0.25, 0.1), (0.3, 0.05))]
+    groups = np.arange(W, dtype=np.int32) % 3
+    spins = model.random_spins(L, W, 11)
+    with binding.Batch(L, W, binding.FP64, n_groups=3, seed=11) as b:
+        for gi, oc in enumerate(occs):
+            b.set_orbitals(gi, oc)
+        b.set_groups(groups)
+        ok = b.init(spins)
+        out = b.walk(400, meas=64, trace=True)
+        s_f = b.state()[0]
+    u = model.philox_uniforms(11, range(W), 0, 400)
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo, o = oracle_walk(L, occs[groups[w]], spins[w], u[w], meas=64)
+        assert np.array_equal(o["accept"], out["accept"][w])
+        assert np.array_equal(s_f[w], wo.spins())
+        np.testing.assert_allclose(out["eloc"][w][:len(o["eloc"])], o["eloc"], atol=1e-10, rtol=0)
+
+
+def test_errors_are_reported():
+    say("case errors")
+    L = 4
+    with binding.Batch(L, 2, binding.FP64) as b:
+        with pytest.raises(binding.GpvmcError, match="not set"):
+            b.init(np.zeros((2, 16), np.int32))
+        b.set_orbitals(0, occ_for(L, 0.25, 0.1))
+        with pytest.raises(binding.GpvmcError, match="Sz=0"):
+            b.init(np.zeros((2, 16), np.int32))
+        with pytest.raises(binding.GpvmcError, match="not initialised"):
+            b.walk(10)
+        b.init(model.random_spins(L, 2, 1))
+        bad = np.full((2, 5, 4), 0.5)
+        bad[1, 3, 0] = 1.0
+        with pytest.raises(binding.GpvmcError, match=r"\[0, 1\)"):
+            b.walk(5, uniforms=bad)
+
+
+# ----------------------------------------------------------------------------- mixed FP32
+@pytest.mark.parametrize("L,W,nsteps", [(8, 16, 600), (16, 4, 400)])
+def test_mixed_ratios_track_oracle(L, W, nsteps):
+    say("case mixed ratios", L, W, nsteps)
+    seed = 314 + L
+    occ = occ_for(L, 0.25, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        ok = b.init(spins)
+        out = b.walk(nsteps, trace=True)
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    agree_steps = 0
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo, o = oracle_walk(L, occ, spins[w], u[w])
+        diff = np.nonzero(o["accept"] != out["accept"][w])[0]
+        upto = diff[0] if len(diff) else nsteps
+        if len(diff):
+            # a decision may only flip on a near-tie between u and the ratio
+            s = diff[0]
+            assert abs(min(o["ratio"][s], 1.0) - u[w, s, 3]) < 1e-3
+        live = ~o["noop"][:upto].astype(bool)
+        assert ratio_ok(out["ratio"][w][:upto][live], o["ratio"][:upto][live], 2e-4).all()
+        agree_steps += upto
+    assert agree_steps > 0.5 * W * nsteps
+
+
+@pytest.mark.parametrize("L,W", [(8, 8), (16, 4)])
+def test_mixed_refresh_and_energy_on_identical_configurations(L, W):
+    say("case mixed refresh", L, W)
+    seed = 99 + L
+    occ = occ_for(L, 0.1667, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        assert b.init(spins).all()
+        b.walk(3 * L * L)
+        s_f, part, la, _ = b.state()
+        drift = b.refresh(drift=True)
+        assert (drift < 1e-3).all()
+        inv = [b.inverse(w) for w in range(W)]
+        _, _, la2, _ = b.state()
+        # one measurement on the refreshed state: FP32 E_loc vs the FP64 oracle on the same
+        # configuration
+        out = b.walk(1, meas=1, trace=True)
+        s_after = b.state()[0]
+    for w in range(W):
+        wo = O.Walker(L, occ, s_f[w])
+        np.testing.assert_allclose(relabel(inv[w], part[w]), wo.Ai.transpose(0, 2, 1),
+                                   atol=5e-6 * np.abs(wo.Ai).max())
+        assert abs(la2[w] - wo.logabs) < 1e-8
+        assert abs(la[w] - wo.logabs) < 1e-3  # FP32 running log-amplitude before the refresh
+        wo2 = O.Walker(L, occ, s_after[w])
+        assert abs(out["eloc"][w][0] - wo2.eloc()) < 1e-5
