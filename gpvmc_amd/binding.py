@@ -35,6 +35,7 @@ SIGNATURES = {
     "gpvmc_read_statspin": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_read_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gpvmc_read_inverse": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_read_ratio_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gpvmc_reset_kernel_time": (ctypes.c_int, [_vp]),
 }
@@ -180,6 +181,11 @@ class Batch:
         ai = np.zeros((2, self.N, self.N), np.complex128)
         self._check(self.lib.gpvmc_read_inverse(self.h, w, _ptr(ai)))
         return ai
+
+    def ratio_table(self, w):
+        g = np.zeros((2, self.N, self.N), np.complex128)
+        self._check(self.lib.gpvmc_read_ratio_table(self.h, w, _ptr(g)))
+        return g
 
     def kernel_time(self):
         sm = ctypes.c_double()

@@ -65,11 +65,9 @@ void build_lattice(int L, std::vector<int32_t> &nbr, std::vector<int32_t> &edge4
 
 int default_lpr(int prec, int N)
 {
-    if (N == 8) return 8;
-    if (N == 18) return 2;
-    if (N == 32) return 4;
-    if (N == 128) return 8;
-    return 0;
+    // the first registered launch shape of (precision, N) is the default
+    int res = 0, nt = 0;
+    return gv_sweep_supported(prec, N, 0, &res, &nt);
 }
 
 }  // namespace
@@ -79,8 +77,9 @@ struct gpvmc_ctx {
     int L = 0, Ns = 0, N = 0, W = 0, G = 0, prec = 0, lpr = 0, res = 0, nthreads = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t esize = 0;  // bytes per complex element of the inverse
-    void *d_Ai = nullptr;
+    size_t esize = 0;  // bytes per complex element of the ratio tables
+    void *d_G = nullptr;          // ratio tables, working precision
+    double2 *d_Ai64 = nullptr;    // FP64 inverse of the last init/refresh
     int32_t *d_site = nullptr, *d_part = nullptr, *d_group = nullptr, *d_order = nullptr;
     int32_t *d_nbr = nullptr, *d_status = nullptr;
     int4 *d_edge = nullptr;
@@ -101,7 +100,7 @@ struct gpvmc_ctx {
 static void free_ctx(gpvmc_ctx *c)
 {
     if (!c) return;
-    void *ptrs[] = {c->d_Ai, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
+    void *ptrs[] = {c->d_G, c->d_Ai64, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
                     c->d_status, c->d_edge, c->d_logabs, c->d_phase, c->d_acc, c->d_drift,
                     c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch};
     for (void *p : ptrs)
@@ -141,19 +140,28 @@ static int ensure_scratch(gpvmc_ctx *c, size_t bytes)
     return GPVMC_OK;
 }
 
-static int run_gj(gpvmc_ctx *c, double *drift_dev)
+// FP64 Gauss-Jordan inverse of every walker's Slater matrices, then the ratio tables
+// G = Ai Phi_other in FP64, stored in the working precision.
+static int run_refresh(gpvmc_ctx *c, double *drift_dev)
 {
     gv::GJArgs a{};
-    a.Ai = c->d_Ai;
+    a.Ai = c->d_Ai64;
     a.part = c->d_part;
     a.occ = c->d_occ64;
     a.group = c->d_group;
     a.logabs = c->d_logabs;
     a.phase = c->d_phase;
     a.status = c->d_status;
-    a.drift = drift_dev;
+    gv::GBArgs b{};
+    b.G = c->d_G;
+    b.Ai = c->d_Ai64;
+    b.part = c->d_part;
+    b.occ = c->d_occ64;
+    b.group = c->d_group;
+    b.drift = drift_dev;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(gv_launch_gj(c->prec, c->N, &a, c->W, c->stream));
+    HIPCHK(gv_launch_gj(c->N, &a, c->W, c->stream));
+    HIPCHK(gv_launch_gbuild(c->prec, c->N, &b, c->W, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -185,7 +193,7 @@ int gpvmc_create(const gpvmc_config *cfg, gpvmc_ctx **out)
     if (!lpr || gv_sweep_supported(cfg->precision, N, lpr, &res, &nt) != lpr)
         return fail(GPVMC_ERR_UNSUPPORTED, "no sweep kernel for N=" + std::to_string(N) +
                                                " lanes_per_row=" + std::to_string(lpr));
-    if (!gv_gj_supported(cfg->precision, N))
+    if (!gv_refresh_supported(cfg->precision, N))
         return fail(GPVMC_ERR_UNSUPPORTED, "no LU kernel for N=" + std::to_string(N));
 
     gpvmc_ctx *c = new gpvmc_ctx();
@@ -217,7 +225,8 @@ int gpvmc_create(const gpvmc_config *cfg, gpvmc_ctx **out)
     CK(hipEventCreate(&c->ev0));
     CK(hipEventCreate(&c->ev1));
     const size_t W = c->W, Ns = c->Ns;
-    CK(hipMalloc(&c->d_Ai, W * 2 * N * N * c->esize));
+    CK(hipMalloc(&c->d_G, W * 2 * N * N * c->esize));
+    CK(hipMalloc(&c->d_Ai64, W * 2 * N * N * sizeof(double2)));
     CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));
     CK(hipMalloc(&c->d_part, W * 2 * N * sizeof(int32_t)));
     CK(hipMalloc(&c->d_group, W * sizeof(int32_t)));
@@ -309,7 +318,7 @@ int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
-    int rc = run_gj(c, nullptr);
+    int rc = run_refresh(c, nullptr);
     if (rc) return rc;
     std::vector<int32_t> st((size_t)W * 2);
     HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -324,7 +333,7 @@ int gpvmc_refresh(gpvmc_ctx *c, double *drift)
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
     HIPCHK(hipSetDevice(c->cfg.device));
-    int rc = run_gj(c, drift ? c->d_drift : nullptr);
+    int rc = run_refresh(c, drift ? c->d_drift : nullptr);
     if (rc) return rc;
     if (drift) {
         std::vector<double> d((size_t)c->W * 2);
@@ -374,7 +383,7 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
     }
     char *sb = static_cast<char *>(c->d_scratch);
     gv::SweepArgs a{};
-    a.Ai = c->d_Ai;
+    a.G = c->d_G;
     a.site = c->d_site;
     a.part = c->d_part;
     a.logabs = c->d_logabs;
@@ -489,15 +498,26 @@ int gpvmc_read_inverse(gpvmc_ctx *c, int32_t walker, double *ai)
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipStreamSynchronize(c->stream));
     const size_t n = (size_t)2 * c->N * c->N;
-    const char *src = static_cast<const char *>(c->d_Ai) + (size_t)walker * n * c->esize;
+    HIPCHK(hipMemcpy(ai, c->d_Ai64 + (size_t)walker * n, n * sizeof(double2), hipMemcpyDeviceToHost));
+    return GPVMC_OK;
+}
+
+int gpvmc_read_ratio_table(gpvmc_ctx *c, int32_t walker, double *g)
+{
+    if (!c || !g) return fail(GPVMC_ERR_ARG, "null argument");
+    if (walker < 0 || walker >= c->W) return fail(GPVMC_ERR_ARG, "walker out of range");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)2 * c->N * c->N;
+    const char *src = static_cast<const char *>(c->d_G) + (size_t)walker * n * c->esize;
     if (c->prec == GPVMC_FP64) {
-        HIPCHK(hipMemcpy(ai, src, n * sizeof(double2), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(g, src, n * sizeof(double2), hipMemcpyDeviceToHost));
     } else {
         std::vector<float2> f(n);
         HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float2), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) {
-            ai[2 * i] = f[i].x;
-            ai[2 * i + 1] = f[i].y;
+            g[2 * i] = f[i].x;
+            g[2 * i + 1] = f[i].y;
         }
     }
     return GPVMC_OK;

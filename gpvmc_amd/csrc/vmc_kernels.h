@@ -10,7 +10,7 @@ enum : uint32_t { MEAS_PROJHEIS = 1u, MEAS_STAGMAGNZ = 2u, MEAS_OVERLAP = 4u, ME
 constexpr int ACC_STRIDE = 12;
 
 struct SweepArgs {
-    void *Ai;                 // T2 [W][2][N][N], Ai[fl][label][slot]
+    void *G;                  // T2 [W][2][N][N], ratio tables G[fl][label][other label]
     int32_t *site;            // [W][Ns] (flavour << 16) | label
     int32_t *part;            // [W][2][N] site of each label
     double *logabs;           // [W][2] ln|det| per flavour
@@ -39,13 +39,21 @@ struct SweepArgs {
 };
 
 struct GJArgs {
-    void *Ai;                 // OUT2 [W][2][N][N]
+    double2 *Ai;              // [W][2][N][N] FP64 inverse, Ai[fl][label][slot]
     const int32_t *part;      // [W][2][N]
     const double2 *occ;       // [G][2][Ns][N] FP64
     const int32_t *group;     // [W]
     double *logabs;           // [W][2]
     double *phase;            // [W][2]
     int32_t *status;          // [W][2]
+};
+
+struct GBArgs {
+    void *G;                  // OUT2 [W][2][N][N]
+    const double2 *Ai;        // [W][2][N][N] FP64
+    const int32_t *part;      // [W][2][N]
+    const double2 *occ;       // [G][2][Ns][N] FP64
+    const int32_t *group;     // [W]
     double *drift;            // [W][2] or null
 };
 
@@ -55,8 +63,9 @@ extern "C" {
 int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
 hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
                            hipStream_t st);
-int gv_gj_supported(int prec, int N);
-hipError_t gv_launch_gj(int prec, int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
+int gv_refresh_supported(int prec, int N);
+hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
+hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
 }
 
 #endif

@@ -113,8 +113,17 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 
 // ---------------------------------------------------------------------------------
 // Sweep kernel: n_steps of MetroMC::Step (MetroMC.cpp:90-103) for one walker per block.
-// RES = true keeps both inverses in registers for the whole launch; RES = false streams
-// the walker's rows from global memory at every use.
+//
+// State: the ratio tables G_fl[i][j] = sum_f Ai_fl[i][f] Phi_fl(orb_f, site of particle j
+// of the other flavour) = det(A_fl with particle i moved to that site) / det(A_fl).  For
+// the swap of up particle ci with down particle cj (LatticeStepper.cpp:70-73) the two
+// single-hop ratios of linalg::DetUpdate (linalg.cpp:143-154) are table entries,
+// q_up = G_up[ci][cj] and q_dn = G_dn[cj][ci]; the Sherman-Morrison update of the inverse
+// (linalg.cpp:402-561) becomes the rank-1 update G -= (G[:,cj] - e_ci)(G[ci,:] + e_cj)/q
+// of the table (column cj then describes the site the moved-away particle left).  The
+// ProjHeis swap ratios are table products G_up[i][j] G_dn[j][i].
+// RES = true keeps both tables in registers for the whole launch; RES = false reads and
+// writes the walker's rows in global memory at every use.
 template <typename T, int N, int LPR, bool RES>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
@@ -129,42 +138,47 @@ sweep_kernel(SweepArgs a)
     __shared__ int s_site[NS];        // (flavour << 16) | label
     __shared__ int s_part[2][N];      // label -> site
     __shared__ int s_nbr[NS * 4];
-    __shared__ C s_row[2][2][N];      // [parity][flavour] replaced row of the inverse
-    __shared__ C s_col[2][2][N];      // [parity][flavour] orbital column of the new site
-    __shared__ C s_q[2][2];           // [parity][flavour] single-hop ratios
-    __shared__ C s_qb[NS * 4];        // measurement: ratio of the particle at site s hopping to nbr d
+    __shared__ C s_row[2][2][N];      // [parity][flavour] row of G of the moving particle
+    __shared__ C s_qb[NS * 4];        // measurement: G entry of the particle at site s for neighbour d
     __shared__ double s_red[NT / 64][4];
+    // thread-0 scalars: [0..1] ln|det| and [2..3] arg det per flavour at launch start,
+    // [4..7] complex mantissa of the amplitude change per flavour, then the sums
+    __shared__ double s_sc[8 + ACC_STRIDE];
+    __shared__ int s_ex[2];           // binary exponent of the amplitude change per flavour
 
     const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int row = tid / LPR, c = tid % LPR;
+    const int gbase = tid & ~(LPR - 1);
     const bool has_row = row < N;
 
     for (int i = tid; i < NS; i += NT) s_site[i] = a.site[(size_t)w * NS + i];
     for (int i = tid; i < 2 * N; i += NT) s_part[i / N][i % N] = a.part[(size_t)w * 2 * N + i];
     for (int i = tid; i < NS * 4; i += NT) s_nbr[i] = a.nbr[i];
-
-    const C *occ = reinterpret_cast<const C *>(a.occ) + (size_t)a.group[w] * 2 * NS * N;
-    C *gA = reinterpret_cast<C *>(a.Ai) + (size_t)w * 2 * N * N;
-
-    C m[2][CPT];
-    if (RES && has_row) {
-#pragma unroll
-        for (int fl = 0; fl < 2; ++fl)
-#pragma unroll
-            for (int k = 0; k < CPT; ++k) m[fl][k] = gA[(size_t)fl * N * N + (size_t)row * N + c + LPR * k];
-    }
-    // per-walker scalars owned by thread 0: [0..1] ln|det| per flavour, [2..3] arg det,
-    // then the ACC_STRIDE estimator sums of this launch
-    __shared__ double s_sc[4 + ACC_STRIDE];
     if (tid == 0) {
         s_sc[0] = a.logabs[2 * w];
         s_sc[1] = a.logabs[2 * w + 1];
         s_sc[2] = a.phase[2 * w];
         s_sc[3] = a.phase[2 * w + 1];
-        for (int i = 0; i < ACC_STRIDE; ++i) s_sc[4 + i] = 0.0;
+        s_sc[4] = 1.0; s_sc[5] = 0.0; s_sc[6] = 1.0; s_sc[7] = 0.0;
+        s_ex[0] = 0;
+        s_ex[1] = 0;
+        for (int i = 0; i < ACC_STRIDE; ++i) s_sc[8 + i] = 0.0;
     }
-    double *const sacc = s_sc + 4;
+    double *const sacc = s_sc + 8;
+
+    C *gG = reinterpret_cast<C *>(a.G) + (size_t)w * 2 * N * N;
+    auto gat = [&](int fl, int i, int j) -> C & { return gG[(size_t)fl * N * N + (size_t)i * N + j]; };
+
+    C g[2][CPT];
+    if constexpr (RES) {
+        if (has_row) {
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) g[fl][k] = gat(fl, row, c + LPR * k);
+        }
+    }
     const uint64_t ctr0 = a.counter[w];
     int par = 0;
     int64_t mslot = 0;
@@ -200,86 +214,95 @@ sweep_kernel(SweepArgs a)
             acc = u3 < 1.0;  // LatticeStepper.cpp:64-68 returns the current weight: ratio 1
             if (tid == 0) sacc[10] += 1;
         } else {
-            int to[2], ci[2];
-            if ((i1 >> 16) == 0) {
-                ci[0] = i1 & 0xFFFF; to[0] = v2;
-                ci[1] = i2 & 0xFFFF; to[1] = v1;
-            } else {
-                ci[0] = i2 & 0xFFFF; to[0] = v1;
-                ci[1] = i1 & 0xFFFF; to[1] = v2;
-            }
+            // up particle ci on site sa, down particle cj on site sb
+            int ci, cj, sa, sb;
+            if ((i1 >> 16) == 0) { ci = i1 & 0xFFFF; sa = v1; cj = i2 & 0xFFFF; sb = v2; }
+            else { ci = i2 & 0xFFFF; sa = v2; cj = i1 & 0xFFFF; sb = v1; }
+            const int rs[2] = {ci, cj}, cs[2] = {cj, ci};
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl) {
-                if (row == ci[fl]) {
-                    const C *col = occ + ((size_t)fl * NS + to[fl]) * N;
-                    C q = cmk<C>(0, 0);
+                // the moving particle's row of G goes to LDS; its entry in the column of
+                // the swap partner is the single-hop ratio
+                if (row == rs[fl]) {
+                    if constexpr (RES) {
 #pragma unroll
-                    for (int k = 0; k < CPT; ++k) {
-                        const int j = c + LPR * k;
-                        C cv = col[j];
-                        C mv = RES ? m[fl][k] : gA[(size_t)fl * N * N + (size_t)row * N + j];
-                        q = cfma(mv, cv, q);
-                        s_row[par][fl][j] = mv;
-                        s_col[par][fl][j] = cv;
+                        for (int k = 0; k < CPT; ++k) s_row[par][fl][c + LPR * k] = g[fl][k];
+                    } else {
+                        for (int k = 0; k < CPT; ++k) s_row[par][fl][c + LPR * k] = gat(fl, row, c + LPR * k);
                     }
-                    q = row_reduce<LPR>(q);
-                    if (c == 0) s_q[par][fl] = q;
                 }
             }
             __syncthreads();
-            const C q0 = s_q[par][0], q1 = s_q[par][1];
+            const C q0 = s_row[par][0][cj], q1 = s_row[par][1][ci];
             const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
             const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
             ratio = n0 * n1;
             acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo, MetroMC.cpp:27-43
             if (acc) {
-                // Sherman-Morrison: Ai -= (Ai C - e_ci)(row_ci / q)  (linalg.cpp:402-561, rc=1)
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     const C q = fl ? q1 : q0;
                     const double nq = fl ? n1 : n0;
-                    const C qi = cmk<C>(q.x / nq, -q.y / nq);
-                    if (has_row) {
-                        C d = cmk<C>(0, 0);
-                        if (RES) {
+                    const int rsel = rs[fl], csel = cs[fl];
+                    const int ks = csel / LPR, ls = csel % LPR;
+                    // u_i = G[i][csel] - delta(i, rsel), held by lane ls of row i's group
+                    C x;
+                    if constexpr (RES) {
+                        // masked sum over the slots: every index stays a compile-time
+                        // constant, so the row never leaves registers
+                        x = cmk<C>(0, 0);
 #pragma unroll
-                            for (int k = 0; k < CPT; ++k) d = cfma(m[fl][k], s_col[par][fl][c + LPR * k], d);
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < CPT; ++k) {
-                                const int j = c + LPR * k;
-                                d = cfma(gA[(size_t)fl * N * N + (size_t)row * N + j], s_col[par][fl][j], d);
-                            }
+                        for (int k = 0; k < CPT; ++k) {
+                            const T m = (k == ks) ? (T)1 : (T)0;
+                            x.x = fma(m, g[fl][k].x, x.x);
+                            x.y = fma(m, g[fl][k].y, x.y);
                         }
-                        d = row_reduce<LPR>(d);
-                        if (row == ci[fl]) d.x -= 1;
-                        const C coef = cmul(d, qi);
-                        const C nc = cmk<C>(-coef.x, -coef.y);
-                        if (RES) {
+                    }
+                    else x = has_row ? gat(fl, row, csel) : cmk<C>(0, 0);
+                    C uu;
+                    uu.x = __shfl(x.x, gbase + ls);
+                    uu.y = __shfl(x.y, gbase + ls);
+                    if (row == rsel) uu.x -= (T)1;
+                    const C qi = cmk<C>(q.x / nq, -q.y / nq);
+                    const C coef = cmul(uu, qi);
+                    const C nc = cmk<C>(-coef.x, -coef.y);
+                    if (has_row) {
 #pragma unroll
-                            for (int k = 0; k < CPT; ++k) m[fl][k] = cfma(nc, s_row[par][fl][c + LPR * k], m[fl][k]);
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < CPT; ++k) {
-                                const int j = c + LPR * k;
-                                C *p = gA + (size_t)fl * N * N + (size_t)row * N + j;
-                                *p = cfma(nc, s_row[par][fl][j], *p);
+                        for (int k = 0; k < CPT; ++k) {
+                            const int j = c + LPR * k;
+                            C v = s_row[par][fl][j];
+                            if (j == csel) v.x += (T)1;
+                            if constexpr (RES) {
+                                g[fl][k] = cfma(nc, v, g[fl][k]);
+                            } else {
+                                C &e = gat(fl, row, j);
+                                e = cfma(nc, v, e);
                             }
                         }
                     }
                 }
                 // State::Hop (State.cpp:73-84); every wave writes the same values
                 if ((tid & 63) == 0) {
-                    s_site[to[0]] = ci[0];
-                    s_site[to[1]] = (1 << 16) | ci[1];
-                    s_part[0][ci[0]] = to[0];
-                    s_part[1][ci[1]] = to[1];
+                    s_site[sb] = ci;
+                    s_site[sa] = (1 << 16) | cj;
+                    s_part[0][ci] = sb;
+                    s_part[1][cj] = sa;
                 }
                 if (tid == 0) {
-                    s_sc[0] += 0.5 * log(n0);
-                    s_sc[1] += 0.5 * log(n1);
-                    s_sc[2] += atan2((double)q0.y, (double)q0.x);
-                    s_sc[3] += atan2((double)q1.y, (double)q1.x);
+                    // amplitude *= q per flavour (SlaterDeterminant.cpp:321), kept as a
+                    // binary mantissa/exponent so no logarithm is taken per step
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        const C q = fl ? q1 : q0;
+                        const double mr = s_sc[4 + 2 * fl], mi = s_sc[5 + 2 * fl];
+                        double re = mr * (double)q.x - mi * (double)q.y;
+                        double im = mr * (double)q.y + mi * (double)q.x;
+                        int e;
+                        (void)frexp(fmax(fabs(re), fabs(im)), &e);
+                        s_sc[4 + 2 * fl] = ldexp(re, -e);
+                        s_sc[5 + 2 * fl] = ldexp(im, -e);
+                        s_ex[fl] += e;
+                    }
                     sacc[8] += 1;
                 }
             } else {
@@ -306,17 +329,19 @@ sweep_kernel(SweepArgs a)
                         const int sh = s_part[fl][row];
                         for (int d = 0; d < 4; ++d) {
                             const int t = s_nbr[sh * 4 + d];
-                            if ((s_site[t] >> 16) == fl) continue;  // parallel bond
-                            const C *col = occ + ((size_t)fl * NS + t) * N;
-                            C q = cmk<C>(0, 0);
+                            const int info = s_site[t];
+                            if ((info >> 16) == fl) continue;  // parallel bond
+                            const int j = info & 0xFFFF;
+                            if (c == j % LPR) {
+                                if constexpr (RES) {
+                                    // static scan of the slots: no run-time register index
 #pragma unroll
-                            for (int k = 0; k < CPT; ++k) {
-                                const int j = c + LPR * k;
-                                C mv = RES ? m[fl][k] : gA[(size_t)fl * N * N + (size_t)row * N + j];
-                                q = cfma(mv, col[j], q);
+                                    for (int k = 0; k < CPT; ++k)
+                                        if (c + LPR * k == j) s_qb[sh * 4 + d] = g[fl][k];
+                                } else {
+                                    s_qb[sh * 4 + d] = gat(fl, row, j);
+                                }
                             }
-                            q = row_reduce<LPR>(q);
-                            if (c == 0) s_qb[sh * 4 + d] = q;
                         }
                     }
                 }
@@ -335,34 +360,36 @@ sweep_kernel(SweepArgs a)
                     }
                 }
             }
-            double mv2[1] = {0.0};
+            double mv2 = 0.0;
             if (a.mask & MEAS_STAGMAGNZ) {
                 // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
                 for (int v = tid; v < NS; v += NT) {
                     const int x = v % a.L, y = v / a.L;
                     const double sg = ((x + y) & 1) ? -1.0 : 1.0;
-                    mv2[0] += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
+                    mv2 += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
                 }
             }
-            double red[3] = {ev[0], ev[1], mv2[0]};
+            double red[3] = {ev[0], ev[1], mv2};
             block_sum<NT, 3>(red, s_red);
             const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
-            if (tid == 0 && s >= 0) {
-                const double lw = 2.0 * (s_sc[0] + s_sc[1]);
-                if (a.mask & MEAS_PROJHEIS) {
-                    sacc[0] += E_re;
-                    sacc[1] += E_im;
-                    sacc[2] += E_re * E_re + E_im * E_im;
-                }
-                if (a.mask & MEAS_STAGMAGNZ) {
-                    sacc[4] += mz;
-                    sacc[6] += mz * mz;
-                }
-                sacc[11] += lw;
-                sacc[3] += 1;
-            }
             if (tid == 0) {
-                const double lw = 2.0 * (s_sc[0] + s_sc[1]);
+                const double la = s_sc[0] + s_sc[1] +
+                                  log(hypot(s_sc[4], s_sc[5])) + log(hypot(s_sc[6], s_sc[7])) +
+                                  (s_ex[0] + s_ex[1]) * 0.69314718055994531;
+                const double lw = 2.0 * la;
+                if (s >= 0) {
+                    if (a.mask & MEAS_PROJHEIS) {
+                        sacc[0] += E_re;
+                        sacc[1] += E_im;
+                        sacc[2] += E_re * E_re + E_im * E_im;
+                    }
+                    if (a.mask & MEAS_STAGMAGNZ) {
+                        sacc[4] += mz;
+                        sacc[6] += mz * mz;
+                    }
+                    sacc[11] += lw;
+                    sacc[3] += 1;
+                }
                 const size_t mi = (size_t)w * a.n_meas_slots + mslot;
                 if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
                 if (a.tr_mz) { a.tr_mz[2 * mi] = mz; a.tr_mz[2 * mi + 1] = 0.0; }
@@ -374,22 +401,25 @@ sweep_kernel(SweepArgs a)
     }
 
     __syncthreads();
-    if (RES && has_row) {
+    if constexpr (RES) {
+        if (has_row) {
 #pragma unroll
-        for (int fl = 0; fl < 2; ++fl)
+            for (int fl = 0; fl < 2; ++fl)
 #pragma unroll
-            for (int k = 0; k < CPT; ++k) gA[(size_t)fl * N * N + (size_t)row * N + c + LPR * k] = m[fl][k];
+                for (int k = 0; k < CPT; ++k) gat(fl, row, c + LPR * k) = g[fl][k];
+        }
     }
     for (int i = tid; i < NS; i += NT) a.site[(size_t)w * NS + i] = s_site[i];
     for (int i = tid; i < 2 * N; i += NT) a.part[(size_t)w * 2 * N + i] = s_part[i / N][i % N];
     if (tid == 0) {
-        a.logabs[2 * w] = s_sc[0];
-        a.logabs[2 * w + 1] = s_sc[1];
-        a.phase[2 * w] = remainder(s_sc[2], 2 * M_PI);
-        a.phase[2 * w + 1] = remainder(s_sc[3], 2 * M_PI);
+#pragma unroll
+        for (int fl = 0; fl < 2; ++fl) {
+            const double mr = s_sc[4 + 2 * fl], mi = s_sc[5 + 2 * fl];
+            a.logabs[2 * w + fl] = s_sc[fl] + log(hypot(mr, mi)) + s_ex[fl] * 0.69314718055994531;
+            a.phase[2 * w + fl] = remainder(s_sc[2 + fl] + atan2(mi, mr), 2 * M_PI);
+        }
         a.counter[w] = ctr0 + (uint64_t)a.n_steps;
-        sacc[7] = (double)a.n_steps;
-        if (a.measure_only) sacc[7] = 0.0;
+        sacc[7] = a.measure_only ? 0.0 : (double)a.n_steps;
         double *ac = a.acc + (size_t)w * ACC_STRIDE;
         for (int i = 0; i < ACC_STRIDE; ++i) ac[i] += sacc[i];
     }
@@ -399,16 +429,14 @@ sweep_kernel(SweepArgs a)
 // FP64 Gauss-Jordan inversion with partial pivoting, one block per (walker, flavour).
 // Builds A(f, r) = Phi(orb_f, site(r)) from the configuration, applies N Jordan exchanges
 // with the pivot row of each column chosen as the largest |re|+|im| among unpivoted rows
-// (the zgetrf choice, linalg.cpp:16), and writes Ai[label][slot] = (A^-1)(label, slot)
-// converted to OUT.  ln|det| and arg(det) include the permutation sign (linalg.cpp:17-22);
-// status[w*2+fl] = 1 when |det|^2 < 1e-9 (linalg.cpp:23) or a pivot vanished.  If `drift`
-// is given, max|Ai_old - Ai_new| / max|Ai_new| is stored there before overwriting.
-template <typename OUT, int N, int LPR>
+// (the zgetrf choice, linalg.cpp:16), and writes Ai[label][slot] = (A^-1)(label, slot) in
+// FP64.  ln|det| and arg(det) include the permutation sign (linalg.cpp:17-22);
+// status[w*2+fl] = 1 when |det|^2 < 1e-9 (linalg.cpp:23) or a pivot vanished.
+template <int N, int LPR>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 gj_kernel(GJArgs a)
 {
     using C = double2;
-    using CO = typename cpx<OUT>::t;
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
     constexpr int NW = NT / 64;
     constexpr int CPT = N / LPR;
@@ -421,7 +449,6 @@ gj_kernel(GJArgs a)
     __shared__ int s_cidx[2][NW];
     __shared__ int s_perm[N];          // pivot row chosen for column k
     __shared__ int s_invp[N];          // column at which row i was pivoted
-    __shared__ double s_red[NW][4];
     __shared__ int s_flag;
     __shared__ unsigned char s_seen[N];
 
@@ -533,8 +560,7 @@ gj_kernel(GJArgs a)
         a.status[2 * w + fl] = sing;
     }
     // A^-1[invp[i]][perm[j]] = B[i][j]  (Jordan exchange with implicit pivoting)
-    CO *out = reinterpret_cast<CO *>(a.Ai) + (size_t)w * 2 * N * N + (size_t)fl * N * N;
-    double dmax = 0.0, nmax = 0.0;
+    C *out = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
     if (has_row) {
         int orow = s_invp[row];
         orow = orow < 0 ? 0 : (orow < N ? orow : N - 1);  // in range even for a NaN matrix
@@ -542,16 +568,78 @@ gj_kernel(GJArgs a)
         for (int t = 0; t < CPT; ++t) {
             int oc = s_perm[c + LPR * t];
             oc = oc < 0 ? 0 : (oc < N ? oc : N - 1);
-            const size_t o = (size_t)orow * N + oc;
+            out[(size_t)orow * N + oc] = b[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Ratio tables from the FP64 inverses, one block per (walker, flavour):
+// G_fl[i][j] = sum_f Ai_fl[i][f] Phi_fl(orb_f, site of particle j of the other flavour),
+// accumulated in FP64 over LDS-staged chunks of FB orbitals and written as OUT.  If
+// `drift` is given, max|G_old - G_new| / max|G_new| is stored first (drift refresh).
+template <typename OUT, int N, int LPR>
+__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
+gbuild_kernel(GBArgs a)
+{
+    using C = double2;
+    using CO = typename cpx<OUT>::t;
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    constexpr int NW = NT / 64;
+    constexpr int CPT = N / LPR;
+    constexpr int NS = 2 * N;
+    constexpr int FB = (N % 8 == 0) ? 8 : ((N % 6 == 0) ? 6 : ((N % 2 == 0) ? 2 : 1));
+    static_assert(N % FB == 0, "orbital chunk must divide N");
+
+    __shared__ int s_other[N];        // sites of the other flavour's particles
+    __shared__ C s_ai[N][FB + 1];     // +1 pads the row stride against bank conflicts
+    __shared__ C s_b[FB][N];
+    __shared__ double s_red[NW][2];
+
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int row = tid / LPR, c = tid % LPR;
+    const bool has_row = row < N;
+    for (int i = tid; i < N; i += NT) s_other[i] = a.part[(size_t)w * 2 * N + (size_t)(1 - fl) * N + i];
+    const C *ai = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
+    C acc[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) acc[k] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int f0 = 0; f0 < N; f0 += FB) {
+        for (int i = tid; i < N * FB; i += NT) {
+            const int r = i / FB, ff = i % FB;
+            s_ai[r][ff] = ai[(size_t)r * N + f0 + ff];
+            const int j = i / FB;
+            s_b[ff][j] = occ[(size_t)s_other[j] * N + f0 + ff];
+        }
+        __syncthreads();
+        if (has_row) {
+#pragma unroll
+            for (int ff = 0; ff < FB; ++ff) {
+                const C x = s_ai[row][ff];
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) acc[k] = cfma(x, s_b[ff][c + LPR * k], acc[k]);
+            }
+        }
+        __syncthreads();
+    }
+    CO *out = reinterpret_cast<CO *>(a.G) + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    double dmax = 0.0, nmax = 0.0;
+    if (has_row) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const size_t o = (size_t)row * N + c + LPR * k;
             if (a.drift) {
                 const CO old = out[o];
-                const double dx = (double)old.x - b[t].x, dy = (double)old.y - b[t].y;
+                const double dx = (double)old.x - acc[k].x, dy = (double)old.y - acc[k].y;
                 dmax = fmax(dmax, sqrt(dx * dx + dy * dy));
-                nmax = fmax(nmax, sqrt(b[t].x * b[t].x + b[t].y * b[t].y));
+                nmax = fmax(nmax, sqrt(acc[k].x * acc[k].x + acc[k].y * acc[k].y));
             }
             CO v;
-            v.x = (OUT)b[t].x;
-            v.y = (OUT)b[t].y;
+            v.x = (OUT)acc[k].x;
+            v.y = (OUT)acc[k].y;
             out[o] = v;
         }
     }
@@ -581,11 +669,19 @@ static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st
     return hipGetLastError();
 }
 
-template <typename OUT, int N, int LPR>
+template <int N, int LPR>
 static hipError_t launch_gj_t(const GJArgs &a, int nwalkers, hipStream_t st)
 {
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    hipLaunchKernelGGL((gj_kernel<OUT, N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
+    hipLaunchKernelGGL((gj_kernel<N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename OUT, int N, int LPR>
+static hipError_t launch_gb_t(const GBArgs &a, int nwalkers, hipStream_t st)
+{
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    hipLaunchKernelGGL((gbuild_kernel<OUT, N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
 
@@ -594,34 +690,46 @@ struct SweepEntry {
     hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
 };
 struct GJEntry {
-    int prec, N, lpr;
+    int N, lpr;
     hipError_t (*fn)(const GJArgs &, int, hipStream_t);
 };
+struct GBEntry {
+    int prec, N, lpr;
+    hipError_t (*fn)(const GBArgs &, int, hipStream_t);
+};
 
+// first entry of a (prec, N) pair is the default launch shape
 static const SweepEntry kSweep[] = {
     {0, 8, 8, 1, launch_sweep_t<double, 8, 8, true>},
     {0, 8, 4, 1, launch_sweep_t<double, 8, 4, true>},
     {0, 18, 2, 1, launch_sweep_t<double, 18, 2, true>},
-    {0, 32, 2, 1, launch_sweep_t<double, 32, 2, true>},
     {0, 32, 4, 1, launch_sweep_t<double, 32, 4, true>},
+    {0, 32, 2, 1, launch_sweep_t<double, 32, 2, true>},
     {0, 32, 8, 1, launch_sweep_t<double, 32, 8, true>},
-    {0, 128, 8, 0, launch_sweep_t<double, 128, 8, false>},
+    {0, 128, 2, 0, launch_sweep_t<double, 128, 2, false>},
     {1, 8, 8, 1, launch_sweep_t<float, 8, 8, true>},
     {1, 18, 2, 1, launch_sweep_t<float, 18, 2, true>},
-    {1, 32, 2, 1, launch_sweep_t<float, 32, 2, true>},
     {1, 32, 4, 1, launch_sweep_t<float, 32, 4, true>},
-    {1, 128, 8, 1, launch_sweep_t<float, 128, 8, true>},
+    {1, 32, 2, 1, launch_sweep_t<float, 32, 2, true>},
+    {1, 128, 4, 1, launch_sweep_t<float, 128, 4, true>},
 };
 
 static const GJEntry kGJ[] = {
-    {0, 8, 8, launch_gj_t<double, 8, 8>},
-    {0, 18, 2, launch_gj_t<double, 18, 2>},
-    {0, 32, 4, launch_gj_t<double, 32, 4>},
-    {0, 128, 8, launch_gj_t<double, 128, 8>},
-    {1, 8, 8, launch_gj_t<float, 8, 8>},
-    {1, 18, 2, launch_gj_t<float, 18, 2>},
-    {1, 32, 4, launch_gj_t<float, 32, 4>},
-    {1, 128, 8, launch_gj_t<float, 128, 8>},
+    {8, 8, launch_gj_t<8, 8>},
+    {18, 2, launch_gj_t<18, 2>},
+    {32, 4, launch_gj_t<32, 4>},
+    {128, 4, launch_gj_t<128, 4>},
+};
+
+static const GBEntry kGB[] = {
+    {0, 8, 8, launch_gb_t<double, 8, 8>},
+    {0, 18, 2, launch_gb_t<double, 18, 2>},
+    {0, 32, 4, launch_gb_t<double, 32, 4>},
+    {0, 128, 8, launch_gb_t<double, 128, 8>},
+    {1, 8, 8, launch_gb_t<float, 8, 8>},
+    {1, 18, 2, launch_gb_t<float, 18, 2>},
+    {1, 32, 4, launch_gb_t<float, 32, 4>},
+    {1, 128, 8, launch_gb_t<float, 128, 8>},
 };
 
 }  // namespace gv
@@ -645,16 +753,27 @@ extern "C" hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepA
     return hipErrorInvalidValue;
 }
 
-extern "C" int gv_gj_supported(int prec, int N)
+extern "C" int gv_refresh_supported(int prec, int N)
 {
+    int ok = 0;
     for (const auto &e : gv::kGJ)
-        if (e.prec == prec && e.N == N) return e.lpr;
+        if (e.N == N) ok = 1;
+    if (!ok) return 0;
+    for (const auto &e : gv::kGB)
+        if (e.prec == prec && e.N == N) return 1;
     return 0;
 }
 
-extern "C" hipError_t gv_launch_gj(int prec, int N, const gv::GJArgs *a, int nwalkers, hipStream_t st)
+extern "C" hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st)
 {
     for (const auto &e : gv::kGJ)
+        if (e.N == N) return e.fn(*a, nwalkers, st);
+    return hipErrorInvalidValue;
+}
+
+extern "C" hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st)
+{
+    for (const auto &e : gv::kGB)
         if (e.prec == prec && e.N == N) return e.fn(*a, nwalkers, st);
     return hipErrorInvalidValue;
 }

@@ -23,6 +23,7 @@
  *   gpvmc_measure         Quantity::measure of the same estimators at the current state
  *   gpvmc_refresh         SlaterDeterminant::Init on the current configuration (FP64 LU
  *                         recomputation of the inverse; the reference never refreshes)
+ *   gpvmc_read_ratio_table SlaterDeterminant::VirtUpdate single-swap ratios
  *   gpvmc_read_*          Quantity::save (ScalarQuantity.cpp:10-14, MatrixQuantity.cpp:12-17),
  *                         LatticeState::Getfs/Getpt (State.h:93-97), SlaterDeterminant::Amp
  */
@@ -115,8 +116,15 @@ int gpvmc_read_statspin(gpvmc_ctx *ctx, double *acc);
  * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */
 int gpvmc_read_state(gpvmc_ctx *ctx, int32_t *spins, int32_t *part, double *logabs,
                      double *phase);
-/* Inverse matrices of walker w as FP64 complex [2][N][N], Ai[fl][label][slot]. */
+/* FP64 inverse matrices of walker w formed by the last gpvmc_init_walkers or
+ * gpvmc_refresh, complex [2][N][N], Ai[fl][label][slot] = (A_fl^-1)(label, slot)
+ * (the reference's m_mati, SlaterDeterminant.h:66, transposed to row-major). */
 int gpvmc_read_inverse(gpvmc_ctx *ctx, int32_t walker, double *ai);
+/* Current ratio tables of walker w as FP64 complex [2][N][N]:
+ * G[fl][i][j] = det(A_fl with particle i moved to the site of particle j of the other
+ * flavour) / det(A_fl) = sum_f Ai_fl[i][f] Phi_fl(orb_f, site_other(j)): the ratios
+ * SlaterDeterminant::VirtUpdate computes for every single swap (SlaterDeterminant.cpp:96-268). */
+int gpvmc_read_ratio_table(gpvmc_ctx *ctx, int32_t walker, double *g);
 
 /* Cumulative device time of the sweep / refresh kernels (HIP events on the context's
  * stream) and the number of launches since creation or the last reset. */

@@ -455,11 +455,14 @@ void or_statspinstruct(int L, const double *occin, const int32_t *fock, const in
  *   u4 -> accept iff (ratio > 1 and u4 < 1) or u4 < ratio (MetroMC::YesNo :27-43).
  * After the step, if meas > 0 and (meas == 1 or s % meas == 0), measure E_loc, m_z and
  * the weight ln|amp|^2 into eloc/mz/logw (slot s / meas).  noop/accept/ratio get one
- * entry per step (ratio 1 for no-ops).  Returns the number of measurements. */
+ * entry per step (ratio 1 for no-ops); cond[s] (may be NULL) is the sum over both flavours
+ * of the condition number sum_p |Ai[ci,p]| |col_p| / |q| of the ratio's dot product (0 for
+ * no-ops), which bounds how far two correct FP64 evaluations of the ratio may differ.
+ * Returns the number of measurements. */
 int or_walk(int L, const double *occin, const int32_t *nbr, int32_t *fock, int32_t *part,
             double *Aiin, double *logabs, double *phase, const double *u, int64_t nsteps,
             int64_t meas, int8_t *noop, int8_t *accept, double *ratio, double *eloc,
-            double *mz, double *logw)
+            double *mz, double *logw, double *cond)
 {
     int Ns = L * L, N = Ns / 2;
     const cplx *occ = (const cplx *)occin;
@@ -473,7 +476,7 @@ int or_walk(int L, const double *occin, const int32_t *nbr, int32_t *fock, int32
         int v2 = nbr[4 * v1 + (int)(4 * us[1])];
         int f1 = flavour_at(fock, Ns, N, v1), f2 = flavour_at(fock, Ns, N, v2);
         int isnoop = (f1 == f2);
-        double r = 1.0;
+        double r = 1.0, kappa = 0.0;
         cplx q[2] = {1.0, 1.0};
         int from[2], to[2];
         if (!isnoop) {
@@ -484,12 +487,18 @@ int or_walk(int L, const double *occin, const int32_t *nbr, int32_t *fock, int32
                 const cplx *A = Ai + (size_t)fl * N * N;
                 const cplx *col = occ + ((size_t)fl * Ns + to[fl]) * N;
                 cplx sacc = 0;
-                for (int p = 0; p < N; ++p) sacc += A[ci + (size_t)p * N] * col[p];
+                double mag = 0;
+                for (int p = 0; p < N; ++p) {
+                    sacc += A[ci + (size_t)p * N] * col[p];
+                    mag += cabs(A[ci + (size_t)p * N]) * cabs(col[p]);
+                }
                 q[fl] = sacc;
+                kappa += mag / cabs(sacc);
             }
             double a0 = cabs(q[0]), a1 = cabs(q[1]);
             r = (a0 * a0) * (a1 * a1);
         }
+        if (cond) cond[s] = kappa;
         int acc = r > 1.0 ? (us[3] < 1.0) : (us[3] < r);
         if (noop) noop[s] = (int8_t)isnoop;
         if (accept) accept[s] = (int8_t)acc;

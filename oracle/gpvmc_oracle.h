@@ -41,7 +41,7 @@ int  or_init_walker(int L, const double *occ, const int32_t *spins, int32_t *foc
 int  or_walk(int L, const double *occ, const int32_t *nbr, int32_t *fock, int32_t *part,
              double *Ai, double *logabs, double *phase, const double *u, int64_t nsteps,
              int64_t meas, int8_t *noop, int8_t *accept, double *ratio, double *eloc,
-             double *mz, double *logw);
+             double *mz, double *logw, double *cond);
 
 void or_eloc(int L, const double *occ, const int32_t *edges, const int32_t *fock,
              const int32_t *part, const double *Ai, double *eloc);

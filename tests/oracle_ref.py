@@ -41,7 +41,7 @@ def lib():
                                      ctypes.POINTER(ctypes.c_double)]
         L.or_walk.argtypes = [ctypes.c_int, _c128p, _i32p, _i32p, _i32p, _c128p,
                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                              _f64p, ctypes.c_int64, ctypes.c_int64, dp, dp, dp, dp, dp, dp]
+                              _f64p, ctypes.c_int64, ctypes.c_int64, dp, dp, dp, dp, dp, dp, dp]
         L.or_eloc.argtypes = [ctypes.c_int, _c128p, _i32p, _i32p, _i32p, _c128p, _f64p]
         L.or_stagmagnz.argtypes = [ctypes.c_int, _i32p, _f64p]
         L.or_statspinstruct.argtypes = [ctypes.c_int, _c128p, _i32p, _i32p, _c128p, _c128p]
@@ -142,13 +142,15 @@ class Walker:
         eloc = np.zeros(max(nm, 1), np.complex128)
         mz = np.zeros(max(nm, 1), np.complex128)
         logw = np.zeros(max(nm, 1), np.float64)
+        cond = np.zeros(n, np.float64)
         la, ph = ctypes.c_double(self.logabs), ctypes.c_double(self.phase)
         got = lib().or_walk(self.L, self.occ, self.nbr, self.fock, self.part, self.Ai,
                             ctypes.byref(la), ctypes.byref(ph), u, n, meas,
                             noop.ctypes.data, acc.ctypes.data, rat.ctypes.data,
-                            eloc.ctypes.data, mz.ctypes.data, logw.ctypes.data)
+                            eloc.ctypes.data, mz.ctypes.data, logw.ctypes.data, cond.ctypes.data)
         self.logabs, self.phase = la.value, ph.value
-        return dict(noop=noop, accept=acc, ratio=rat, eloc=eloc[:got], mz=mz[:got], logw=logw[:got])
+        return dict(noop=noop, accept=acc, ratio=rat, eloc=eloc[:got], mz=mz[:got], logw=logw[:got],
+                    cond=cond)
 
     def eloc(self):
         out = np.zeros(2)
