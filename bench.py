@@ -56,8 +56,8 @@ def sweep(b, L, mask):
     b.walk(L * L, meas=L * L, mask=mask)
 
 
-def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None):
-    from gpvmc_amd import binding
+def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None, drift_bound=1e-6):
+    from gpvmc_amd import binding, tune
     mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ
     b = binding.Batch(L, W, precision, n_groups=len(PHIS) * len(NEELS), seed=seed)
     if spins is None:
@@ -71,14 +71,23 @@ def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None):
         assert b.init(spins).all()
     for i in range(warmup):
         sweep(b, L, mask)
-        if precision == binding.MIXED and (i + 1) % refresh_every == 0:
+        if precision == binding.MIXED and (i + 1) % max(refresh_every, 1) == 0:
             b.refresh()
+    tuned = None
+    if precision == binding.MIXED and refresh_every <= 0:
+        # untimed: longest refresh interval whose drift stays under the bound
+        refresh_every, hist = tune.tune_refresh(b, lambda: sweep(b, L, mask), bound=drift_bound)
+        tuned = {"refresh_every_sweeps": refresh_every, "drift_bound": drift_bound,
+                 "trials": [{"k": h[0], "max": h[1], "median": h[2], "q999": h[3]} for h in hist]}
+    b.tuned = tuned
+    b.refresh_every = refresh_every
     b.reset_accumulators()
     b.reset_kernel_time()
     t0 = time.perf_counter()
     for i in range(steps):
         sweep(b, L, mask)
-        if precision == binding.MIXED and (i + 1) % refresh_every == 0:
+        # every k sweeps, and always at the end so the timed work leaves FP64-exact tables
+        if precision == binding.MIXED and ((i + 1) % refresh_every == 0 or i == steps - 1):
             b.refresh()
     t1 = time.perf_counter()
     return b, t1 - t0
@@ -144,7 +153,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3, help="untimed thermalisation sweeps")
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--walkers", type=int, default=16384)
-    ap.add_argument("--refresh-every", type=int, default=1, help="sweeps between FP64 refreshes")
+    ap.add_argument("--refresh-every", type=int, default=0,
+                    help="sweeps between FP64 refreshes (0: autotune against --drift-bound)")
+    ap.add_argument("--drift-bound", type=float, default=1e-6)
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed sweeps of the FP64 path")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -160,7 +171,9 @@ def main():
     N = L * L // 2
     print("bench: mixed path L=%d W=%d warmup=%d steps=%d" % (L, W, args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    bm, tm = run_path(binding.MIXED, L, W, args.seed, args.warmup, args.steps, args.refresh_every)
+    bm, tm = run_path(binding.MIXED, L, W, args.seed, args.warmup, args.steps, args.refresh_every,
+                      drift_bound=args.drift_bound)
+    k_used = bm.refresh_every
     acc = bm.accumulators()
     kt = bm.kernel_time()
     mc_steps = W * L * L * args.steps
@@ -190,8 +203,7 @@ def main():
     bm.close()
 
     print("bench: plain FP64 path", file=sys.stderr, flush=True)
-    b2, t64 = run_path(binding.FP64, L, W, args.seed, 0, args.fp64_steps, args.refresh_every,
-                       spins=spins)
+    b2, t64 = run_path(binding.FP64, L, W, args.seed, 0, args.fp64_steps, 1, spins=spins)
     kt64 = b2.kernel_time()
     fp64_rate = W * L * L * args.fp64_steps / t64
     b2.close()
@@ -212,7 +224,7 @@ def main():
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(ach_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "sweep_kernel<float,128,8,RES>",
+                "kernel": "sweep_kernel<float,128,4,RES>",
                 "avg_launch_ms": round(avg_ms, 3),
                 "note": "algorithmic bytes of the survey's per-step model; the inverses are "
                         "register-resident during a launch, so this is VGPR/LDS-level traffic "
@@ -232,13 +244,14 @@ def main():
         "config": {"workload": "L=16 staggered-flux groundstate, N=128, %d walkers, 8-point "
                                "(phi,neel) grid, 1 sweep = %d steps + 1 ProjHeis/StagMagnZ "
                                "measurement, FP64 LU refresh every %d sweep(s); therm shortened "
-                               "to %d sweeps" % (W, L * L, args.refresh_every, args.warmup),
+                               "to %d sweeps" % (W, L * L, k_used, args.warmup),
                    "L": L, "walkers": W, "parallelism": "1 GPU, 1 workgroup per walker"},
         "eloc_samples_per_s": round(eloc_rate, 1),
         "fp64_steps_per_s": round(fp64_rate, 1),
         "mixed_over_fp64": round(value / fp64_rate, 2),
         "fp64_sweep_avg_ms": round(kt64["sweep_ms"] / max(kt64["sweep_launches"], 1), 3),
         "refresh_avg_ms": round(kt["refresh_ms"] / max(kt["refresh_launches"], 1), 3),
+        "refresh_every_sweeps": k_used, "refresh_tuning": bm.tuned,
         "dE_per_site_fp32_vs_fp64": {"max": float(dE.max()), "mean": float(dE.mean())},
         "energy_per_site": round(float(E), 6), "stag_magn_sq": round(float(ms2), 6),
         "p_antiparallel": round(float(p_ap), 4), "p_accept_move": round(float(a_ap), 4),

@@ -124,7 +124,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 // ProjHeis swap ratios are table products G_up[i][j] G_dn[j][i].
 // RES = true keeps both tables in registers for the whole launch; RES = false reads and
 // writes the walker's rows in global memory at every use.
-template <typename T, int N, int LPR, bool RES>
+template <typename T, int N, int LPR, bool RES, bool MONLY>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
 {
@@ -185,7 +185,7 @@ sweep_kernel(SweepArgs a)
     __syncthreads();
 
     // s = -1 is a measurement-only pass (gpvmc_measure): no step, one measurement
-    for (int64_t s = a.measure_only ? -1 : 0; s < a.n_steps; ++s) {
+    for (int64_t s = MONLY ? -1 : 0; s < (MONLY ? 0 : a.n_steps); ++s) {
       if (s >= 0) {
         double u0, u1, u3;
         if (a.uniforms) {
@@ -419,7 +419,7 @@ sweep_kernel(SweepArgs a)
             a.phase[2 * w + fl] = remainder(s_sc[2 + fl] + atan2(mi, mr), 2 * M_PI);
         }
         a.counter[w] = ctr0 + (uint64_t)a.n_steps;
-        sacc[7] = a.measure_only ? 0.0 : (double)a.n_steps;
+        sacc[7] = MONLY ? 0.0 : (double)a.n_steps;
         double *ac = a.acc + (size_t)w * ACC_STRIDE;
         for (int i = 0; i < ACC_STRIDE; ++i) ac[i] += sacc[i];
     }
@@ -445,8 +445,7 @@ gj_kernel(GJArgs a)
 
     __shared__ int s_part[N];
     __shared__ C s_prow[2][NW][N];     // [parity][wave] best candidate row of that wave
-    __shared__ double s_cval[2][NW];
-    __shared__ int s_cidx[2][NW];
+    __shared__ unsigned long long s_ckey[2][NW];
     __shared__ int s_perm[N];          // pivot row chosen for column k
     __shared__ int s_invp[N];          // column at which row i was pivoted
     __shared__ int s_flag;
@@ -474,73 +473,92 @@ gj_kernel(GJArgs a)
         for (int k = 0; k < CPT; ++k) b[k] = make_double2(0.0, 0.0);
     }
     bool pivoted = !has_row;
-    double la = 0.0, ph = 0.0;
+    // determinant as a binary mantissa/exponent in thread 0 (no logarithm per pivot)
+    C dm = make_double2(1.0, 0.0);
+    int de = 0;
     int par = 0;
+    const int gbase = tid & ~(LPR - 1);
 
-    for (int k = 0; k < N; ++k) {
-        const int kk = k / LPR, ck = k % LPR;
-        // my row's element in column k lives in lane ck of the row group, slot kk
-        C mine = make_double2(0.0, 0.0);
+    // Column k = ck + LPR*kk of the matrix lives in lane ck of each row group, slot kk.
+    // Each row's register array is rotated left by one slot after every LPR columns, so
+    // the current column is always slot 0 (a compile-time register index).
+    for (int kk = 0; kk < CPT; ++kk) {
+        for (int ck = 0; ck < LPR; ++ck) {
+            const int k = ck + LPR * kk;
+            C bk;
+            bk.x = __shfl(b[0].x, gbase + ck);
+            bk.y = __shfl(b[0].y, gbase + ck);
+            // pivot search: largest |re|+|im| among unpivoted rows, lowest row on ties;
+            // key = magnitude bits (non-negative float, order-preserving) : ~row
+            const float mag = (float)(fabs(bk.x) + fabs(bk.y));
+            unsigned long long key = 0ull;
+            if (!pivoted)
+                key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
 #pragma unroll
-        for (int t = 0; t < CPT; ++t)
-            if (t == kk) mine = b[t];
-        const int src = (tid & ~(LPR - 1)) + ck;
-        C bk;
-        bk.x = __shfl(mine.x, src);
-        bk.y = __shfl(mine.y, src);
-        double val = pivoted ? -1.0 : fabs(bk.x) + fabs(bk.y);
-        int idx = has_row ? row : 0x7fffffff;
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long ok2 = __shfl_xor(key, o);
+                key = ok2 > key ? ok2 : key;
+            }
+            const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
+            // the wave's best row publishes itself as that wave's candidate
+            if (has_row && !pivoted && key != 0ull && row == wrow) {
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_xor(val, o);
-            const int oi = __shfl_xor(idx, o);
-            if (ov > val || (ov == val && oi < idx)) { val = ov; idx = oi; }
-        }
-        // the wave's best row writes itself as that wave's candidate
-        if (has_row && row == idx) {
-#pragma unroll
-            for (int t = 0; t < CPT; ++t) s_prow[par][wid][c + LPR * t] = b[t];
-        }
-        if (lane == 0) { s_cval[par][wid] = val; s_cidx[par][wid] = idx; }
-        __syncthreads();
-        int bw = 0;
-        double bv = s_cval[par][0];
-        int bi = s_cidx[par][0];
-#pragma unroll
-        for (int q = 1; q < NW; ++q) {
-            const double v = s_cval[par][q];
-            const int i = s_cidx[par][q];
-            if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; bw = q; }
-        }
-        const int p = bi;
-        const C piv = s_prow[par][bw][k];
-        if (tid == 0) { s_perm[k] = p; s_invp[p] = k; }
-        const double np = piv.x * piv.x + piv.y * piv.y;
-        C ainv;
-        if (np > 0.0) {
-            ainv = make_double2(piv.x / np, -piv.y / np);
-            la += 0.5 * log(np);
-            ph += atan2(piv.y, piv.x);
-        } else {
-            ainv = make_double2(0.0, 0.0);
-            if (tid == 0) s_flag = 1;
-        }
-        if (has_row) {
-            const bool me = (row == p);
-            const C bka = cmul(bk, ainv);
-#pragma unroll
-            for (int t = 0; t < CPT; ++t) {
-                const int j = c + LPR * t;
-                const C rj = s_prow[par][bw][j];
-                if (me) {
-                    b[t] = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
-                } else {
-                    b[t] = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, b[t]);
+                for (int t = 0; t < CPT; ++t) {
+                    const int ot = t + kk < CPT ? t + kk : t + kk - CPT;
+                    s_prow[par][wid][c + LPR * ot] = b[t];
                 }
             }
-            if (me) pivoted = true;
+            if (lane == 0) s_ckey[par][wid] = key;
+            __syncthreads();
+            int bw = 0;
+            unsigned long long bkey = s_ckey[par][0];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) {
+                const unsigned long long v = s_ckey[par][q];
+                if (v > bkey) { bkey = v; bw = q; }
+            }
+            const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
+            const C piv = s_prow[par][bw][k];
+            const double np = piv.x * piv.x + piv.y * piv.y;
+            C ainv;
+            if (bkey != 0ull && np > 0.0) {
+                ainv = make_double2(piv.x / np, -piv.y / np);
+            } else {
+                ainv = make_double2(0.0, 0.0);
+                if (tid == 0) s_flag = 1;
+            }
+            if (tid == 0) {
+                s_perm[k] = (p >= 0 && p < N) ? p : 0;
+                if (p >= 0 && p < N) s_invp[p] = k;
+                const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
+                int e;
+                (void)frexp(fmax(fabs(re), fabs(im)), &e);
+                dm = make_double2(ldexp(re, -e), ldexp(im, -e));
+                de += e;
+            }
+            if (has_row) {
+                const bool me = (row == p);
+                const C bka = cmul(bk, ainv);
+#pragma unroll
+                for (int t = 0; t < CPT; ++t) {
+                    const int ot = t + kk < CPT ? t + kk : t + kk - CPT;
+                    const int j = c + LPR * ot;
+                    const C rj = s_prow[par][bw][j];
+                    if (me) {
+                        b[t] = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
+                    } else {
+                        b[t] = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, b[t]);
+                    }
+                }
+                if (me) pivoted = true;
+            }
+            par ^= 1;
         }
-        par ^= 1;
+        // rotate left by one slot
+        const C first = b[0];
+#pragma unroll
+        for (int t = 0; t + 1 < CPT; ++t) b[t] = b[t + 1];
+        b[CPT - 1] = first;
     }
     __syncthreads();
     if (tid == 0) {
@@ -553,6 +571,8 @@ gj_kernel(GJArgs a)
             while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
             if ((len & 1) == 0) sgn = -sgn;
         }
+        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
+        double ph = atan2(dm.y, dm.x);
         if (sgn < 0) ph += M_PI;
         const int sing = s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
         a.logabs[2 * w + fl] = la;
@@ -665,7 +685,12 @@ template <typename T, int N, int LPR, bool RES>
 static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st)
 {
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES>), dim3(nblocks), dim3(NT), 0, st, a);
+    // the measurement-only pass (gpvmc_measure) is its own kernel symbol, so profiles of
+    // the sweep report sweeps only
+    if (a.measure_only)
+        hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, true>), dim3(nblocks), dim3(NT), 0, st, a);
+    else
+        hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false>), dim3(nblocks), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
 
