@@ -56,7 +56,7 @@ def sweep(b, L, mask):
     b.walk(L * L, meas=L * L, mask=mask)
 
 
-def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None, drift_bound=1e-6):
+def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None, eloc_bound=1e-7):
     from gpvmc_amd import binding, tune
     mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ
     b = binding.Batch(L, W, precision, n_groups=len(PHIS) * len(NEELS), seed=seed)
@@ -75,10 +75,12 @@ def run_path(precision, L, W, seed, warmup, steps, refresh_every, spins=None, dr
             b.refresh()
     tuned = None
     if precision == binding.MIXED and refresh_every <= 0:
-        # untimed: longest refresh interval whose drift stays under the bound
-        refresh_every, hist = tune.tune_refresh(b, lambda: sweep(b, L, mask), bound=drift_bound)
-        tuned = {"refresh_every_sweeps": refresh_every, "drift_bound": drift_bound,
-                 "trials": [{"k": h[0], "max": h[1], "median": h[2], "q999": h[3]} for h in hist]}
+        # untimed: longest refresh interval whose energy error per site (mean over the
+        # batch of |E_loc on drifted tables - E_loc on refreshed tables|) stays under the bound
+        refresh_every, hist = tune.tune_refresh(b, lambda: sweep(b, L, mask), bound=eloc_bound)
+        tuned = {"refresh_every_sweeps": refresh_every, "eloc_bound_per_site": eloc_bound,
+                 "trials": [{k: (v if k == "k" else float("%.3g" % v)) for k, v in h.items()}
+                            for h in hist]}
     b.tuned = tuned
     b.refresh_every = refresh_every
     b.reset_accumulators()
@@ -114,7 +116,8 @@ def cpu_baseline(L, budget_s):
                               "sequential) on one chain, L=%d phi=0.1667 neel=0.1, %d steps in %.1f s"
                               % (L, n, r["seconds"])}
         except Exception as e:  # noqa: BLE001
-            print("reference harness unavailable: %s" % e, file=sys.stderr)
+            err = getattr(e, "stderr", None) or ""
+            print("reference harness unavailable: %s %s" % (e, err.strip()[-500:]), file=sys.stderr)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ref as O
     from gpvmc_amd import model
@@ -154,8 +157,10 @@ def main():
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--walkers", type=int, default=16384)
     ap.add_argument("--refresh-every", type=int, default=0,
-                    help="sweeps between FP64 refreshes (0: autotune against --drift-bound)")
-    ap.add_argument("--drift-bound", type=float, default=1e-6)
+                    help="sweeps between FP64 refreshes (0: autotune against --eloc-bound)")
+    ap.add_argument("--eloc-bound", type=float, default=1e-7,
+                    help="autotune bound on mean |dE_loc| per site between drifted and "
+                         "refreshed FP32 tables")
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed sweeps of the FP64 path")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -172,7 +177,7 @@ def main():
     print("bench: mixed path L=%d W=%d warmup=%d steps=%d" % (L, W, args.warmup, args.steps),
           file=sys.stderr, flush=True)
     bm, tm = run_path(binding.MIXED, L, W, args.seed, args.warmup, args.steps, args.refresh_every,
-                      drift_bound=args.drift_bound)
+                      eloc_bound=args.eloc_bound)
     k_used = bm.refresh_every
     acc = bm.accumulators()
     kt = bm.kernel_time()

@@ -3,12 +3,14 @@
 Every test calls the HIP path through the C ABI (gpvmc_amd.binding) and compares it
 with the C oracle fed the same uniform stream.  Bars:
   * FP64 mode: identical no-op/accept sequences and configurations (bit-exact integer
-    state); weight ratios per `ratios_match` (1e-9 relative with an absolute floor on
-    |q_up q_dn| of 1e-12, median at rounding level, >= 99.5% of steps within 1e-10);
-    inverses within 1e-9; ratio tables within 1e-8 of the oracle's Ai Phi; ln|amp|
-    within 1e-9; E_loc within 1e-10 absolute or relative; m_z exact;
-  * mixed FP32 mode: |q_up q_dn| within 1e-4 relative + 1e-5 absolute while decisions
-    agree; the drift refresh restores the FP64 tables to FP32 rounding; E_loc on
+    state); weight ratios per `ratios_match` (every |q_up q_dn| within 1e-8 relative +
+    1e-12 absolute, median relative error <= 1e-10, and for well-conditioned walks
+    >= 99.9% of ratios above 1e-12 within 1e-10 and all within 1e-9); inverses within
+    1e-9; ratio tables within 1e-8 of the oracle's Ai Phi; ln|amp| within 1e-9; E_loc per
+    `eloc_close` (1e-10 absolute or relative, 1e-6 relative next to a node); m_z exact;
+  * mixed FP32 mode, refreshed every sweep: median relative error of |q_up q_dn| <= 1e-5
+    and 99th percentile <= 1e-3 while decisions agree (a decision may flip only on a
+    near-tie); the drift refresh restores the FP64 tables to FP32 rounding; E_loc on
     identical configurations within 1e-5 of the FP64 value.
 """
 import os
@@ -61,14 +63,15 @@ def ratios_match(gpu, orc, live, well_conditioned=False):
 
 
 def eloc_close(a, b):
-    """E_loc per measurement: within 1e-10 absolute or relative for at least 99% of the
-    measurements and within 1e-7 for all.  A measurement right after a random start can
-    sit next to a node of the wavefunction, where E_loc is large and carries the
-    conditioning of the freshly inverted Slater matrix (Gauss-Jordan and LU inverses
-    differ there at ~1e-9)."""
+    """E_loc per measurement, within 1e-10 absolute or relative.  Physical local energies
+    per site are O(1); |E_loc| > 100 marks a configuration next to a node of the
+    wavefunction (random starts can land there), where the freshly inverted Slater
+    matrix is nearly singular and two correct FP64 inverses (Gauss-Jordan vs LU) differ
+    by its conditioning: those measurements are held to 1e-6 relative."""
     scale = np.maximum(1.0, np.abs(b))
     d = np.abs(a - b)
-    return (d <= 1e-10 * scale).mean() >= 0.99 and (d <= 1e-7 * scale).all()
+    near_node = np.abs(b) > 100.0
+    return bool(np.where(near_node, d <= 1e-6 * scale, d <= 1e-10 * scale).all())
 
 
 def table_from_inverse(ai_math, occ, part):
@@ -270,37 +273,48 @@ def test_errors_are_reported():
 
 
 # ----------------------------------------------------------------------------- mixed FP32
-@pytest.mark.parametrize("L,W,nsteps", [(8, 16, 600), (16, 4, 400)])
-def test_mixed_ratios_track_oracle(L, W, nsteps):
-    say("case mixed ratios", L, W, nsteps)
+@pytest.mark.parametrize("L,W,nsweeps", [(8, 16, 10), (16, 4, 2)])
+def test_mixed_ratios_track_oracle(L, W, nsweeps):
+    say("case mixed ratios", L, W, nsweeps)
     seed = 314 + L
     occ = occ_for(L, 0.25, 0.1)
     spins = model.random_spins(L, W, seed)
+    Ns = L * L
+    nsteps = nsweeps * Ns
+    ratio = np.zeros((W, nsteps))
+    accept = np.zeros((W, nsteps), np.int8)
     with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
         b.set_orbitals(0, occ)
         ok = b.init(spins)
-        out = b.walk(nsteps, trace=True)
+        # as in production: one sweep, then an FP64 refresh of the tables
+        for t in range(nsweeps):
+            out = b.walk(Ns, trace=True)
+            ratio[:, t * Ns:(t + 1) * Ns] = out["ratio"]
+            accept[:, t * Ns:(t + 1) * Ns] = out["accept"]
+            b.refresh()
     u = model.philox_uniforms(seed, range(W), 0, nsteps)
     agree_steps = 0
+    rels = []
     for w in range(W):
         if not ok[w]:
             continue
         wo, o = oracle_walk(L, occ, spins[w], u[w])
-        diff = np.nonzero(o["accept"] != out["accept"][w])[0]
+        diff = np.nonzero(o["accept"] != accept[w])[0]
         upto = diff[0] if len(diff) else nsteps
         if len(diff):
             # a decision may only flip on a near-tie between u and the ratio
             s = diff[0]
             assert abs(min(o["ratio"][s], 1.0) - u[w, s, 3]) < 1e-3
         live = ~o["noop"][:upto].astype(bool)
-        # FP32 tables drift by ~1e-6 relative per ~10 accepted updates without a refresh
-        # (this test never refreshes); compare |q_up q_dn| = sqrt(ratio), where moves into
-        # near-nodes may lose more digits
-        sg, so = np.sqrt(out["ratio"][w][:upto][live]), np.sqrt(o["ratio"][:upto][live])
-        rel = np.abs(sg - so) / np.maximum(so, 1e-300)
-        assert np.median(rel) <= 1e-4
-        assert (np.abs(sg - so) <= 1e-3 * so + 1e-4).all()
+        # compare |q_up q_dn| = sqrt(ratio)
+        sg, so = np.sqrt(ratio[w][:upto][live]), np.sqrt(o["ratio"][:upto][live])
+        rels.append(np.abs(sg - so) / np.maximum(so, 1e-300))
         agree_steps += upto
+    rel = np.concatenate(rels)
+    # FP32 tables refreshed every sweep: typical error ~1e-7 relative; moves into
+    # near-nodes (tiny ratios) lose more digits relative to their own size
+    assert np.median(rel) <= 1e-5
+    assert np.quantile(rel, 0.99) <= 1e-3
     assert agree_steps > 0.5 * W * nsteps
 
 
