@@ -112,6 +112,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 }
 
 // ---------------------------------------------------------------------------------
+// Wave-uniform helpers of the sweep kernel.
+
+// Register vectors: element k of a vector is register (base + k), so an element picked
+// with a wave-uniform index compiles to one indexed move (s_set_gpr_idx_on + v_mov)
+// and the vector never leaves registers.
+template <typename T, int M> using regvec = T __attribute__((ext_vector_type(M)));
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// ---------------------------------------------------------------------------------
 // Sweep kernel: n_steps of MetroMC::Step (MetroMC.cpp:90-103) for one walker per block.
 //
 // State: the ratio tables G_fl[i][j] = sum_f Ai_fl[i][f] Phi_fl(orb_f, site of particle j
@@ -124,6 +140,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 // ProjHeis swap ratios are table products G_up[i][j] G_dn[j][i].
 // RES = true keeps both tables in registers for the whole launch; RES = false reads and
 // writes the walker's rows in global memory at every use.
+//
+// One step, every decision wave-uniform:
+//  * proposals come from a per-wave batch: every 64 steps lane l draws the uniforms of
+//    step s + l (Philox, or the caller's stream), decodes vertex, edge and neighbour, and
+//    each step reads its proposal back with v_readlane;
+//  * each wave keeps its own copy of the site map in registers (lane l holds sites
+//    l + 64 r), so the flavours and labels at the two sites need no LDS round trip;
+//  * every thread picks its row's entries in the two partner columns with a uniform slot
+//    index; the owners of (ci, cj) in G_up and (cj, ci) in G_dn publish q_up and q_dn in
+//    LDS, and one barrier later every thread holds the ratio;
+//  * on acceptance the two moving rows go to LDS (second barrier) and every thread
+//    applies the rank-1 update to its CPT slots of both tables.
+// LDS buffers alternate between consecutive non-trivial steps, so no barrier is needed
+// after the reads.
 template <typename T, int N, int LPR, bool RES, bool MONLY>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
@@ -132,54 +162,64 @@ sweep_kernel(SweepArgs a)
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
     constexpr int CPT = N / LPR;
     constexpr int NS = 2 * N;
+    constexpr int NSR = (NS + 63) / 64;              // site-map registers per lane
+    constexpr int RP = CPT + 16 / (int)sizeof(T);    // LDS row pitch: one 16-byte pad
     static_assert(N % LPR == 0, "LPR must divide N");
+    static_assert(64 % LPR == 0, "row groups must not straddle wavefronts");
     static_assert(NT <= 1024, "workgroup too large");
 
     __shared__ int s_site[NS];        // (flavour << 16) | label
     __shared__ int s_part[2][N];      // label -> site
     __shared__ int s_nbr[NS * 4];
-    __shared__ C s_row[2][2][N];      // [parity][flavour] row of G of the moving particle
+    __shared__ C s_q[2][2];           // [parity][flavour] single-hop ratio of the proposal
+    // [parity][flavour][lane of the row group][re, im][slot]: the moving row, e_cs added
+    __shared__ __align__(16) T s_row[2][2][LPR][2][RP];
     __shared__ C s_qb[NS * 4];        // measurement: G entry of the particle at site s for neighbour d
     __shared__ double s_red[NT / 64][4];
-    // thread-0 scalars: [0..1] ln|det| and [2..3] arg det per flavour at launch start,
-    // [4..7] complex mantissa of the amplitude change per flavour, then the sums
-    __shared__ double s_sc[8 + ACC_STRIDE];
-    __shared__ int s_ex[2];           // binary exponent of the amplitude change per flavour
+    __shared__ double s_acc[ACC_STRIDE];
 
-    const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
-    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(a.order ? a.order[blockIdx.x] : (int)blockIdx.x);
+    const int tid = threadIdx.x, lane = tid & 63;
     const int row = tid / LPR, c = tid % LPR;
-    const int gbase = tid & ~(LPR - 1);
+    const int gbase = lane & ~(LPR - 1);
     const bool has_row = row < N;
 
     for (int i = tid; i < NS; i += NT) s_site[i] = a.site[(size_t)w * NS + i];
     for (int i = tid; i < 2 * N; i += NT) s_part[i / N][i % N] = a.part[(size_t)w * 2 * N + i];
     for (int i = tid; i < NS * 4; i += NT) s_nbr[i] = a.nbr[i];
-    if (tid == 0) {
-        s_sc[0] = a.logabs[2 * w];
-        s_sc[1] = a.logabs[2 * w + 1];
-        s_sc[2] = a.phase[2 * w];
-        s_sc[3] = a.phase[2 * w + 1];
-        s_sc[4] = 1.0; s_sc[5] = 0.0; s_sc[6] = 1.0; s_sc[7] = 0.0;
-        s_ex[0] = 0;
-        s_ex[1] = 0;
-        for (int i = 0; i < ACC_STRIDE; ++i) s_sc[8 + i] = 0.0;
+    if (tid < ACC_STRIDE) s_acc[tid] = 0.0;
+    regvec<int, NSR> smap;
+#pragma unroll
+    for (int r = 0; r < NSR; ++r) {
+        const int v = lane + 64 * r;
+        smap[r] = v < NS ? a.site[(size_t)w * NS + v] : 0;
     }
-    double *const sacc = s_sc + 8;
+    // amplitude change per flavour as a complex binary mantissa and exponent, so no
+    // logarithm is taken per step (thread 0's copy is the one used)
+    double mre[2] = {1.0, 1.0}, mim[2] = {0.0, 0.0};
+    int mex[2] = {0, 0};
+    int64_t n_noop = 0, n_acc = 0, n_rej = 0;
 
     C *gG = reinterpret_cast<C *>(a.G) + (size_t)w * 2 * N * N;
     auto gat = [&](int fl, int i, int j) -> C & { return gG[(size_t)fl * N * N + (size_t)i * N + j]; };
 
-    C g[2][CPT];
+    // RES: real and imaginary parts of the thread's CPT slots of each table
+    regvec<T, CPT> gr[2], gi[2];
     if constexpr (RES) {
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
 #pragma unroll
-                for (int k = 0; k < CPT; ++k) g[fl][k] = gat(fl, row, c + LPR * k);
+                for (int k = 0; k < CPT; ++k) {
+                    const C e = gat(fl, row, c + LPR * k);
+                    gr[fl][k] = e.x;
+                    gi[fl][k] = e.y;
+                }
         }
     }
     const uint64_t ctr0 = a.counter[w];
+    int bpv = 0;          // batch: v1 | (v2 << 16) of step (s & ~63) + lane
+    double bu3 = 1.0;     // batch: acceptance uniform of that step
     int par = 0;
     int64_t mslot = 0;
     __syncthreads();
@@ -187,78 +227,111 @@ sweep_kernel(SweepArgs a)
     // s = -1 is a measurement-only pass (gpvmc_measure): no step, one measurement
     for (int64_t s = MONLY ? -1 : 0; s < (MONLY ? 0 : a.n_steps); ++s) {
       if (s >= 0) {
-        double u0, u1, u3;
-        if (a.uniforms) {
-            const double *up = a.uniforms + ((size_t)w * a.n_steps + s) * 4;
-            u0 = up[0];
-            u1 = up[1];
-            u3 = up[3];
-        } else {
-            uint64_t st = ctr0 + (uint64_t)s;
-            uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
-            philox4x32_10(cc, a.key0, a.key1);
-            u0 = u32_to_open01(cc[0]);
-            u1 = u32_to_open01(cc[1]);
-            u3 = u32_to_open01(cc[3]);  // cc[2] is the internal-state draw (one state per site)
+        const int j64 = (int)(s & 63);
+        if (j64 == 0) {
+            const int64_t sl = s + lane;
+            int v1 = 0, ed = 0;
+            double u3 = 1.0;
+            if (a.uniforms) {
+                if (sl < a.n_steps) {
+                    const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
+                    v1 = (int)(NS * up[0]);
+                    v1 = v1 < NS ? v1 : NS - 1;
+                    ed = (int)(4 * up[1]);
+                    ed = ed < 4 ? ed : 3;
+                    u3 = up[3];
+                }
+            } else {
+                const uint64_t st = ctr0 + (uint64_t)sl;
+                uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
+                philox4x32_10(cc, a.key0, a.key1);
+                // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers (exact:
+                // NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused here
+                v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
+                ed = (int)(cc[1] >> 30);
+                u3 = u32_to_open01(cc[3]);
+            }
+            bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
+            bu3 = u3;
         }
-        int v1 = (int)(NS * u0);
-        v1 = v1 < NS ? v1 : NS - 1;
-        int ed = (int)(4 * u1);
-        ed = ed < 4 ? ed : 3;
-        const int v2 = s_nbr[v1 * 4 + ed];
-        const int i1 = s_site[v1], i2 = s_site[v2];
-        const bool noop = (i1 >> 16) == (i2 >> 16);
+        const int pv = __builtin_amdgcn_readlane(bpv, j64);
+        const double u3 = readlane_f64(bu3, j64);
+        const int v1 = pv & 0xFFFF, v2 = pv >> 16;
+        int i1 = __builtin_amdgcn_readlane(smap[v1 >> 6], v1 & 63);
+        int i2 = __builtin_amdgcn_readlane(smap[v2 >> 6], v2 & 63);
+        // The step's branches are wave-uniform, but taken as such the compiler gives the
+        // tables separate registers on either side of every join and spills them; moved
+        // to VGPRs through an opaque copy the branches are compiled as exec-masked
+        // regions, which skip the untaken side just the same (s_cbranch_execz).
+        asm volatile("v_mov_b32 %0, %1" : "=v"(i1) : "s"(i1));
+        asm volatile("v_mov_b32 %0, %1" : "=v"(i2) : "s"(i2));
+        const bool noop = ((i1 ^ i2) >> 16) == 0;
         double ratio = 1.0;
         bool acc;
         if (noop) {
             acc = u3 < 1.0;  // LatticeStepper.cpp:64-68 returns the current weight: ratio 1
-            if (tid == 0) sacc[10] += 1;
+            ++n_noop;
         } else {
             // up particle ci on site sa, down particle cj on site sb
             int ci, cj, sa, sb;
             if ((i1 >> 16) == 0) { ci = i1 & 0xFFFF; sa = v1; cj = i2 & 0xFFFF; sb = v2; }
             else { ci = i2 & 0xFFFF; sa = v2; cj = i1 & 0xFFFF; sb = v1; }
-            const int rs[2] = {ci, cj}, cs[2] = {cj, ci};
-#pragma unroll
-            for (int fl = 0; fl < 2; ++fl) {
-                // the moving particle's row of G goes to LDS; its entry in the column of
-                // the swap partner is the single-hop ratio
-                if (row == rs[fl]) {
-                    if constexpr (RES) {
-#pragma unroll
-                        for (int k = 0; k < CPT; ++k) s_row[par][fl][c + LPR * k] = g[fl][k];
-                    } else {
-                        for (int k = 0; k < CPT; ++k) s_row[par][fl][c + LPR * k] = gat(fl, row, c + LPR * k);
-                    }
-                }
+            // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
+            const int ks0 = cj / LPR, ls0 = cj % LPR, ks1 = ci / LPR, ls1 = ci % LPR;
+            C x0, x1;
+            if constexpr (RES) {
+                const int k0 = __builtin_amdgcn_readfirstlane(ks0);
+                const int k1 = __builtin_amdgcn_readfirstlane(ks1);
+                x0 = cmk<C>(gr[0][k0], gi[0][k0]);
+                x1 = cmk<C>(gr[1][k1], gi[1][k1]);
+                if (row == ci && c == ls0) s_q[par][0] = x0;
+                if (row == cj && c == ls1) s_q[par][1] = x1;
+            } else {
+                if (row == ci && c == ls0) s_q[par][0] = gat(0, ci, cj);
+                if (row == cj && c == ls1) s_q[par][1] = gat(1, cj, ci);
             }
             __syncthreads();
-            const C q0 = s_row[par][0][cj], q1 = s_row[par][1][ci];
+            const C q0 = s_q[par][0], q1 = s_q[par][1];
             const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
             const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
             ratio = n0 * n1;
             acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo, MetroMC.cpp:27-43
             if (acc) {
+                if constexpr (!RES) {
+                    x0 = has_row ? gat(0, row, cj) : cmk<C>(0, 0);
+                    x1 = has_row ? gat(1, row, ci) : cmk<C>(0, 0);
+                }
+                // the moving rows G_up[ci,:] + e_cj and G_dn[cj,:] + e_ci go to LDS
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int rsel = fl ? cj : ci;
+                    if (row == rsel) {
+                        T *dr = &s_row[par][fl][c][0][0], *di = &s_row[par][fl][c][1][0];
+#pragma unroll
+                        for (int k = 0; k < CPT; ++k) {
+                            if constexpr (RES) {
+                                dr[k] = gr[fl][k];
+                                di[k] = gi[fl][k];
+                            } else {
+                                const C e = gat(fl, row, c + LPR * k);
+                                dr[k] = e.x;
+                                di[k] = e.y;
+                            }
+                        }
+                        if (c == (fl ? ls1 : ls0)) {
+                            const C x = fl ? x1 : x0;
+                            dr[fl ? ks1 : ks0] = x.x + (T)1;
+                        }
+                    }
+                }
+                __syncthreads();
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     const C q = fl ? q1 : q0;
                     const double nq = fl ? n1 : n0;
-                    const int rsel = rs[fl], csel = cs[fl];
-                    const int ks = csel / LPR, ls = csel % LPR;
-                    // u_i = G[i][csel] - delta(i, rsel), held by lane ls of row i's group
-                    C x;
-                    if constexpr (RES) {
-                        // masked sum over the slots: every index stays a compile-time
-                        // constant, so the row never leaves registers
-                        x = cmk<C>(0, 0);
-#pragma unroll
-                        for (int k = 0; k < CPT; ++k) {
-                            const T m = (k == ks) ? (T)1 : (T)0;
-                            x.x = fma(m, g[fl][k].x, x.x);
-                            x.y = fma(m, g[fl][k].y, x.y);
-                        }
-                    }
-                    else x = has_row ? gat(fl, row, csel) : cmk<C>(0, 0);
+                    const C x = fl ? x1 : x0;
+                    const int ls = fl ? ls1 : ls0, rsel = fl ? cj : ci;
+                    // u_i = G[i][cs] - delta(i, rs), held by lane ls of row i's group
                     C uu;
                     uu.x = __shfl(x.x, gbase + ls);
                     uu.y = __shfl(x.y, gbase + ls);
@@ -267,46 +340,53 @@ sweep_kernel(SweepArgs a)
                     const C coef = cmul(uu, qi);
                     const C nc = cmk<C>(-coef.x, -coef.y);
                     if (has_row) {
+                        const T *sr = &s_row[par][fl][c][0][0], *si = &s_row[par][fl][c][1][0];
 #pragma unroll
                         for (int k = 0; k < CPT; ++k) {
-                            const int j = c + LPR * k;
-                            C v = s_row[par][fl][j];
-                            if (j == csel) v.x += (T)1;
+                            const T vr = sr[k], vi = si[k];
                             if constexpr (RES) {
-                                g[fl][k] = cfma(nc, v, g[fl][k]);
+                                T er = gr[fl][k], ei = gi[fl][k];
+                                er = fma(nc.x, vr, er);
+                                er = fma(-nc.y, vi, er);
+                                ei = fma(nc.x, vi, ei);
+                                ei = fma(nc.y, vr, ei);
+                                gr[fl][k] = er;
+                                gi[fl][k] = ei;
                             } else {
-                                C &e = gat(fl, row, j);
-                                e = cfma(nc, v, e);
+                                C &e = gat(fl, row, c + LPR * k);
+                                e = cfma(nc, cmk<C>(vr, vi), e);
                             }
                         }
                     }
                 }
-                // State::Hop (State.cpp:73-84); every wave writes the same values
-                if ((tid & 63) == 0) {
+                // State::Hop (State.cpp:73-84): every wave updates its own site map;
+                // thread 0 keeps the LDS maps used by the measurement
+#pragma unroll
+                for (int r = 0; r < NSR; ++r) {
+                    const int v = lane + 64 * r;
+                    smap[r] = v == sb ? ci : (v == sa ? ((1 << 16) | cj) : smap[r]);
+                }
+                if (tid == 0) {
                     s_site[sb] = ci;
                     s_site[sa] = (1 << 16) | cj;
                     s_part[0][ci] = sb;
                     s_part[1][cj] = sa;
-                }
-                if (tid == 0) {
-                    // amplitude *= q per flavour (SlaterDeterminant.cpp:321), kept as a
-                    // binary mantissa/exponent so no logarithm is taken per step
+                    // amplitude *= q per flavour (SlaterDeterminant.cpp:321)
 #pragma unroll
                     for (int fl = 0; fl < 2; ++fl) {
                         const C q = fl ? q1 : q0;
-                        const double mr = s_sc[4 + 2 * fl], mi = s_sc[5 + 2 * fl];
-                        double re = mr * (double)q.x - mi * (double)q.y;
-                        double im = mr * (double)q.y + mi * (double)q.x;
+                        const double re = mre[fl] * (double)q.x - mim[fl] * (double)q.y;
+                        const double im = mre[fl] * (double)q.y + mim[fl] * (double)q.x;
                         int e;
                         (void)frexp(fmax(fabs(re), fabs(im)), &e);
-                        s_sc[4 + 2 * fl] = ldexp(re, -e);
-                        s_sc[5 + 2 * fl] = ldexp(im, -e);
-                        s_ex[fl] += e;
+                        mre[fl] = ldexp(re, -e);
+                        mim[fl] = ldexp(im, -e);
+                        mex[fl] += e;
                     }
-                    sacc[8] += 1;
                 }
+                ++n_acc;
             } else {
-                if (tid == 0) sacc[9] += 1;
+                ++n_rej;
             }
             par ^= 1;
         }
@@ -334,10 +414,17 @@ sweep_kernel(SweepArgs a)
                             const int j = info & 0xFFFF;
                             if (c == j % LPR) {
                                 if constexpr (RES) {
-                                    // static scan of the slots: no run-time register index
+                                    // masked sum over the slots (the slot index differs
+                                    // between lanes): no run-time register index
+                                    const int kj = j / LPR;
+                                    C val = cmk<C>(0, 0);
 #pragma unroll
-                                    for (int k = 0; k < CPT; ++k)
-                                        if (c + LPR * k == j) s_qb[sh * 4 + d] = g[fl][k];
+                                    for (int k = 0; k < CPT; ++k) {
+                                        const T m = (k == kj) ? (T)1 : (T)0;
+                                        val.x = fma(m, gr[fl][k], val.x);
+                                        val.y = fma(m, gi[fl][k], val.y);
+                                    }
+                                    s_qb[sh * 4 + d] = val;
                                 } else {
                                     s_qb[sh * 4 + d] = gat(fl, row, j);
                                 }
@@ -373,22 +460,22 @@ sweep_kernel(SweepArgs a)
             block_sum<NT, 3>(red, s_red);
             const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
             if (tid == 0) {
-                const double la = s_sc[0] + s_sc[1] +
-                                  log(hypot(s_sc[4], s_sc[5])) + log(hypot(s_sc[6], s_sc[7])) +
-                                  (s_ex[0] + s_ex[1]) * 0.69314718055994531;
+                const double la = a.logabs[2 * w] + a.logabs[2 * w + 1] +
+                                  log(hypot(mre[0], mim[0])) + log(hypot(mre[1], mim[1])) +
+                                  (mex[0] + mex[1]) * 0.69314718055994531;
                 const double lw = 2.0 * la;
                 if (s >= 0) {
                     if (a.mask & MEAS_PROJHEIS) {
-                        sacc[0] += E_re;
-                        sacc[1] += E_im;
-                        sacc[2] += E_re * E_re + E_im * E_im;
+                        s_acc[0] += E_re;
+                        s_acc[1] += E_im;
+                        s_acc[2] += E_re * E_re + E_im * E_im;
                     }
                     if (a.mask & MEAS_STAGMAGNZ) {
-                        sacc[4] += mz;
-                        sacc[6] += mz * mz;
+                        s_acc[4] += mz;
+                        s_acc[6] += mz * mz;
                     }
-                    sacc[11] += lw;
-                    sacc[3] += 1;
+                    s_acc[11] += lw;
+                    s_acc[3] += 1;
                 }
                 const size_t mi = (size_t)w * a.n_meas_slots + mslot;
                 if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
@@ -406,7 +493,7 @@ sweep_kernel(SweepArgs a)
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
 #pragma unroll
-                for (int k = 0; k < CPT; ++k) gat(fl, row, c + LPR * k) = g[fl][k];
+                for (int k = 0; k < CPT; ++k) gat(fl, row, c + LPR * k) = cmk<C>(gr[fl][k], gi[fl][k]);
         }
     }
     for (int i = tid; i < NS; i += NT) a.site[(size_t)w * NS + i] = s_site[i];
@@ -414,14 +501,16 @@ sweep_kernel(SweepArgs a)
     if (tid == 0) {
 #pragma unroll
         for (int fl = 0; fl < 2; ++fl) {
-            const double mr = s_sc[4 + 2 * fl], mi = s_sc[5 + 2 * fl];
-            a.logabs[2 * w + fl] = s_sc[fl] + log(hypot(mr, mi)) + s_ex[fl] * 0.69314718055994531;
-            a.phase[2 * w + fl] = remainder(s_sc[2 + fl] + atan2(mi, mr), 2 * M_PI);
+            a.logabs[2 * w + fl] += log(hypot(mre[fl], mim[fl])) + mex[fl] * 0.69314718055994531;
+            a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(mim[fl], mre[fl]), 2 * M_PI);
         }
         a.counter[w] = ctr0 + (uint64_t)a.n_steps;
-        sacc[7] = MONLY ? 0.0 : (double)a.n_steps;
+        s_acc[7] = MONLY ? 0.0 : (double)a.n_steps;
+        s_acc[8] = (double)n_acc;
+        s_acc[9] = (double)n_rej;
+        s_acc[10] = (double)n_noop;
         double *ac = a.acc + (size_t)w * ACC_STRIDE;
-        for (int i = 0; i < ACC_STRIDE; ++i) ac[i] += sacc[i];
+        for (int i = 0; i < ACC_STRIDE; ++i) ac[i] += s_acc[i];
     }
 }
 
