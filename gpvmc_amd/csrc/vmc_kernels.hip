@@ -119,7 +119,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 // and the vector never leaves registers.
 template <typename T, int M> using regvec = T __attribute__((ext_vector_type(M)));
 
-__device__ __forceinline__ double readlane_f64(double v, int l)
+// v_readlane of a float or double (wave-uniform lane index)
+__device__ __forceinline__ float readlane_real(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_real(double v, int l)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
@@ -141,19 +146,21 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
 // RES = true keeps both tables in registers for the whole launch; RES = false reads and
 // writes the walker's rows in global memory at every use.
 //
-// One step, every decision wave-uniform:
+// Windows.  A rejected or trivial step leaves configuration and tables unchanged, so the
+// proposals of the following steps can all be evaluated against the current tables.  The
+// kernel therefore advances by windows of up to KW steps:
 //  * proposals come from a per-wave batch: every 64 steps lane l draws the uniforms of
-//    step s + l (Philox, or the caller's stream), decodes vertex, edge and neighbour, and
-//    each step reads its proposal back with v_readlane;
-//  * each wave keeps its own copy of the site map in registers (lane l holds sites
-//    l + 64 r), so the flavours and labels at the two sites need no LDS round trip;
-//  * every thread picks its row's entries in the two partner columns with a uniform slot
-//    index; the owners of (ci, cj) in G_up and (cj, ci) in G_dn publish q_up and q_dn in
-//    LDS, and one barrier later every thread holds the ratio;
-//  * on acceptance the two moving rows go to LDS (second barrier) and every thread
-//    applies the rank-1 update to its CPT slots of both tables.
-// LDS buffers alternate between consecutive non-trivial steps, so no barrier is needed
-// after the reads.
+//    step s + l (Philox, or the caller's stream) and decodes vertex, edge and neighbour;
+//    a window never crosses a batch, so the lane holding a step in the batch evaluates it;
+//  * each lane reads the flavours and labels at its two sites from the LDS site map;
+//  * RES: every wavefront takes the steps whose moving row it owns (a ballot mask), picks
+//    q_up or q_dn out of its register tables (indexed moves) and publishes it in LDS; one
+//    barrier later each lane holds its step's ratio (RES = false: the lane loads them);
+//  * every lane applies MetroMC::YesNo to its step; a ballot gives the first accepted
+//    move.  The steps before it are finished (rejected or trivial), the accepted step
+//    applies the rank-1 update (its rows go through LDS, one more barrier), and the next
+//    window starts after it.  Without an accepted move the window simply advances.
+// Every decision is the one MetroMC::Step takes on the same uniforms, in the same order.
 template <typename T, int N, int LPR, bool RES, bool MONLY>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
@@ -162,7 +169,8 @@ sweep_kernel(SweepArgs a)
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
     constexpr int CPT = N / LPR;
     constexpr int NS = 2 * N;
-    constexpr int NSR = (NS + 63) / 64;              // site-map registers per lane
+    constexpr int RPW = 64 / LPR;                    // table rows per wavefront
+    constexpr int KW = 16;                           // steps evaluated per window
     constexpr int RP = CPT + 16 / (int)sizeof(T);    // LDS row pitch: one 16-byte pad
     static_assert(N % LPR == 0, "LPR must divide N");
     static_assert(64 % LPR == 0, "row groups must not straddle wavefronts");
@@ -171,15 +179,18 @@ sweep_kernel(SweepArgs a)
     __shared__ int s_site[NS];        // (flavour << 16) | label
     __shared__ int s_part[2][N];      // label -> site
     __shared__ int s_nbr[NS * 4];
-    __shared__ C s_q[2][2];           // [parity][flavour] single-hop ratio of the proposal
-    // [parity][flavour][lane of the row group][re, im][slot]: the moving row, e_cs added
-    __shared__ __align__(16) T s_row[2][2][LPR][2][RP];
+    __shared__ C s_q[2][64][2];       // [parity][lane of the step][flavour] single-hop ratios
+    // [flavour][lane of the row group][re, im][slot]: the moving row, e_cs added
+    __shared__ __align__(16) T s_row[2][LPR][2][RP];
     __shared__ C s_qb[NS * 4];        // measurement: G entry of the particle at site s for neighbour d
     __shared__ double s_red[NT / 64][4];
     __shared__ double s_acc[ACC_STRIDE];
+    // thread 0: amplitude change per flavour as complex mantissa [0..3] and binary
+    // exponent [4..5], so no logarithm is taken per step
+    __shared__ double s_amp[6];
 
     const int w = __builtin_amdgcn_readfirstlane(a.order ? a.order[blockIdx.x] : (int)blockIdx.x);
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int row = tid / LPR, c = tid % LPR;
     const int gbase = lane & ~(LPR - 1);
     const bool has_row = row < N;
@@ -188,16 +199,10 @@ sweep_kernel(SweepArgs a)
     for (int i = tid; i < 2 * N; i += NT) s_part[i / N][i % N] = a.part[(size_t)w * 2 * N + i];
     for (int i = tid; i < NS * 4; i += NT) s_nbr[i] = a.nbr[i];
     if (tid < ACC_STRIDE) s_acc[tid] = 0.0;
-    regvec<int, NSR> smap;
-#pragma unroll
-    for (int r = 0; r < NSR; ++r) {
-        const int v = lane + 64 * r;
-        smap[r] = v < NS ? a.site[(size_t)w * NS + v] : 0;
+    if (tid == 0) {
+        s_amp[0] = 1.0; s_amp[1] = 0.0; s_amp[2] = 1.0; s_amp[3] = 0.0;
+        s_amp[4] = 0.0; s_amp[5] = 0.0;
     }
-    // amplitude change per flavour as a complex binary mantissa and exponent, so no
-    // logarithm is taken per step (thread 0's copy is the one used)
-    double mre[2] = {1.0, 1.0}, mim[2] = {0.0, 0.0};
-    int mex[2] = {0, 0};
     int64_t n_noop = 0, n_acc = 0, n_rej = 0;
 
     C *gG = reinterpret_cast<C *>(a.G) + (size_t)w * 2 * N * N;
@@ -218,95 +223,239 @@ sweep_kernel(SweepArgs a)
         }
     }
     const uint64_t ctr0 = a.counter[w];
-    int bpv = 0;          // batch: v1 | (v2 << 16) of step (s & ~63) + lane
-    double bu3 = 1.0;     // batch: acceptance uniform of that step
-    int par = 0;
     int64_t mslot = 0;
     __syncthreads();
 
-    // s = -1 is a measurement-only pass (gpvmc_measure): no step, one measurement
-    for (int64_t s = MONLY ? -1 : 0; s < (MONLY ? 0 : a.n_steps); ++s) {
-      if (s >= 0) {
-        const int j64 = (int)(s & 63);
-        if (j64 == 0) {
-            const int64_t sl = s + lane;
-            int v1 = 0, ed = 0;
-            double u3 = 1.0;
-            if (a.uniforms) {
-                if (sl < a.n_steps) {
-                    const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
-                    v1 = (int)(NS * up[0]);
-                    v1 = v1 < NS ? v1 : NS - 1;
-                    ed = (int)(4 * up[1]);
-                    ed = ed < 4 ? ed : 3;
-                    u3 = up[3];
+    // Quantity::measure (MetroMC.cpp:61-68); s < 0 is the measurement-only pass
+    auto measure = [&](int64_t s) {
+        __syncthreads();
+        double ev[2] = {0.0, 0.0};
+        if (a.mask & MEAS_PROJHEIS) {
+            if (has_row) {
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int sh = s_part[fl][row];
+                    for (int d = 0; d < 4; ++d) {
+                        const int t = s_nbr[sh * 4 + d];
+                        const int info = s_site[t];
+                        if ((info >> 16) == fl) continue;  // parallel bond
+                        const int j = info & 0xFFFF;
+                        if (c == j % LPR) {
+                            if constexpr (RES) {
+                                // masked sum over the slots (the slot index differs
+                                // between lanes): no run-time register index
+                                const int kj = j / LPR;
+                                C val = cmk<C>(0, 0);
+#pragma unroll
+                                for (int k = 0; k < CPT; ++k) {
+                                    const T m = (k == kj) ? (T)1 : (T)0;
+                                    val.x = fma(m, gr[fl][k], val.x);
+                                    val.y = fma(m, gi[fl][k], val.y);
+                                }
+                                s_qb[sh * 4 + d] = val;
+                            } else {
+                                s_qb[sh * 4 + d] = gat(fl, row, j);
+                            }
+                        }
+                    }
                 }
-            } else {
-                const uint64_t st = ctr0 + (uint64_t)sl;
-                uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
-                philox4x32_10(cc, a.key0, a.key1);
-                // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers (exact:
-                // NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused here
-                v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
-                ed = (int)(cc[1] >> 30);
-                u3 = u32_to_open01(cc[3]);
-            }
-            bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
-            bu3 = u3;
-        }
-        const int pv = __builtin_amdgcn_readlane(bpv, j64);
-        const double u3 = readlane_f64(bu3, j64);
-        const int v1 = pv & 0xFFFF, v2 = pv >> 16;
-        int i1 = __builtin_amdgcn_readlane(smap[v1 >> 6], v1 & 63);
-        int i2 = __builtin_amdgcn_readlane(smap[v2 >> 6], v2 & 63);
-        // The step's branches are wave-uniform, but taken as such the compiler gives the
-        // tables separate registers on either side of every join and spills them; moved
-        // to VGPRs through an opaque copy the branches are compiled as exec-masked
-        // regions, which skip the untaken side just the same (s_cbranch_execz).
-        asm volatile("v_mov_b32 %0, %1" : "=v"(i1) : "s"(i1));
-        asm volatile("v_mov_b32 %0, %1" : "=v"(i2) : "s"(i2));
-        const bool noop = ((i1 ^ i2) >> 16) == 0;
-        double ratio = 1.0;
-        bool acc;
-        if (noop) {
-            acc = u3 < 1.0;  // LatticeStepper.cpp:64-68 returns the current weight: ratio 1
-            ++n_noop;
-        } else {
-            // up particle ci on site sa, down particle cj on site sb
-            int ci, cj, sa, sb;
-            if ((i1 >> 16) == 0) { ci = i1 & 0xFFFF; sa = v1; cj = i2 & 0xFFFF; sb = v2; }
-            else { ci = i2 & 0xFFFF; sa = v2; cj = i1 & 0xFFFF; sb = v1; }
-            // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
-            const int ks0 = cj / LPR, ls0 = cj % LPR, ks1 = ci / LPR, ls1 = ci % LPR;
-            C x0, x1;
-            if constexpr (RES) {
-                const int k0 = __builtin_amdgcn_readfirstlane(ks0);
-                const int k1 = __builtin_amdgcn_readfirstlane(ks1);
-                x0 = cmk<C>(gr[0][k0], gi[0][k0]);
-                x1 = cmk<C>(gr[1][k1], gi[1][k1]);
-                if (row == ci && c == ls0) s_q[par][0] = x0;
-                if (row == cj && c == ls1) s_q[par][1] = x1;
-            } else {
-                if (row == ci && c == ls0) s_q[par][0] = gat(0, ci, cj);
-                if (row == cj && c == ls1) s_q[par][1] = gat(1, cj, ci);
             }
             __syncthreads();
-            const C q0 = s_q[par][0], q1 = s_q[par][1];
-            const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
-            const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
-            ratio = n0 * n1;
-            acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo, MetroMC.cpp:27-43
-            if (acc) {
-                if constexpr (!RES) {
-                    x0 = has_row ? gat(0, row, cj) : cmk<C>(0, 0);
-                    x1 = has_row ? gat(1, row, ci) : cmk<C>(0, 0);
+            // ProjHeis.cpp:131-166: -(1/4 + 1/2 q_up q_dn) per antiparallel edge, +1/4 per parallel
+            for (int e = tid; e < 2 * NS; e += NT) {
+                const int4 eg = a.edge[e];
+                if ((s_site[eg.x] >> 16) == (s_site[eg.z] >> 16)) {
+                    ev[0] += 0.25;
+                } else {
+                    const C qa = s_qb[eg.x * 4 + eg.y], qb = s_qb[eg.z * 4 + eg.w];
+                    const double pr = (double)qa.x * qb.x - (double)qa.y * qb.y;
+                    const double pi = (double)qa.x * qb.y + (double)qa.y * qb.x;
+                    ev[0] += -0.25 - 0.5 * pr;
+                    ev[1] += -0.5 * pi;
+                }
+            }
+        }
+        double mv2 = 0.0;
+        if (a.mask & MEAS_STAGMAGNZ) {
+            // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
+            for (int v = tid; v < NS; v += NT) {
+                const int x = v % a.L, y = v / a.L;
+                const double sg = ((x + y) & 1) ? -1.0 : 1.0;
+                mv2 += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
+            }
+        }
+        double red[3] = {ev[0], ev[1], mv2};
+        block_sum<NT, 3>(red, s_red);
+        const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
+        if (tid == 0) {
+            const double la = a.logabs[2 * w] + a.logabs[2 * w + 1] +
+                              log(hypot(s_amp[0], s_amp[1])) + log(hypot(s_amp[2], s_amp[3])) +
+                              (s_amp[4] + s_amp[5]) * 0.69314718055994531;
+            const double lw = 2.0 * la;
+            if (s >= 0) {
+                if (a.mask & MEAS_PROJHEIS) {
+                    s_acc[0] += E_re;
+                    s_acc[1] += E_im;
+                    s_acc[2] += E_re * E_re + E_im * E_im;
+                }
+                if (a.mask & MEAS_STAGMAGNZ) {
+                    s_acc[4] += mz;
+                    s_acc[6] += mz * mz;
+                }
+                s_acc[11] += lw;
+                s_acc[3] += 1;
+            }
+            const size_t mi = (size_t)w * a.n_meas_slots + mslot;
+            if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
+            if (a.tr_mz) { a.tr_mz[2 * mi] = mz; a.tr_mz[2 * mi + 1] = 0.0; }
+            if (a.tr_logw) a.tr_logw[mi] = lw;
+        }
+        ++mslot;
+        __syncthreads();
+    };
+
+    if constexpr (MONLY) {
+        measure(-1);
+    } else {
+        int bpv = 0;          // batch: v1 | (v2 << 16) of step (s0 & ~63) + lane
+        double bu3 = 1.0;     // batch: acceptance uniform of that step
+        int par = 0;
+        int64_t s0 = 0;
+        while (s0 < a.n_steps) {
+            const int b0 = (int)(s0 & 63);
+            if (b0 == 0) {
+                const int64_t sl = s0 + lane;
+                int v1 = 0, ed = 0;
+                double u3 = 1.0;
+                if (a.uniforms) {
+                    if (sl < a.n_steps) {
+                        const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
+                        v1 = (int)(NS * up[0]);
+                        v1 = v1 < NS ? v1 : NS - 1;
+                        ed = (int)(4 * up[1]);
+                        ed = ed < 4 ? ed : 3;
+                        u3 = up[3];
+                    }
+                } else {
+                    const uint64_t st = ctr0 + (uint64_t)sl;
+                    uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
+                    philox4x32_10(cc, a.key0, a.key1);
+                    // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers
+                    // (exact: NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused
+                    v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
+                    ed = (int)(cc[1] >> 30);
+                    u3 = u32_to_open01(cc[3]);
+                }
+                bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
+                bu3 = u3;
+            }
+            // window: up to KW steps, within the batch, ending at the next measured step
+            int64_t lim = a.n_steps - s0;
+            if (lim > 64 - b0) lim = 64 - b0;
+            if (a.meas > 0) {
+                const int64_t nm = (a.meas == 1) ? s0 : ((s0 + a.meas - 1) / a.meas) * a.meas;
+                if (nm - s0 + 1 < lim) lim = nm - s0 + 1;
+            }
+            const int len = (int)(lim < KW ? lim : KW);
+            // lane b0 + i holds step s0 + i of the window, as it holds it in the batch
+            const bool inw = lane >= b0 && lane < b0 + len;
+            const int v1 = bpv & 0xFFFF, v2 = bpv >> 16;
+            const double u3 = bu3;
+            const int i1 = s_site[v1], i2 = s_site[v2];
+            const bool noop = ((i1 ^ i2) >> 16) == 0;
+            // up particle ci on site sa, down particle cj on site sb
+            const bool up1 = (i1 >> 16) == 0;
+            const int ci = (up1 ? i1 : i2) & 0xFFFF, cj = (up1 ? i2 : i1) & 0xFFFF;
+            const int sa = up1 ? v1 : v2, sb = up1 ? v2 : v1;
+            C q0l = cmk<C>(0, 0), q1l = cmk<C>(0, 0);
+            if constexpr (RES) {
+                // each wavefront publishes the ratios whose rows it owns: q_up of step i
+                // sits in row ci of G_up, q_dn in row cj of G_dn
+                uint64_t own = __ballot(inw && !noop && ci / RPW == wid);
+                while (own) {
+                    const int i = (int)__builtin_ctzll(own);
+                    own &= own - 1;
+                    const int ci_i = __builtin_amdgcn_readlane(ci, i);
+                    const int cj_i = __builtin_amdgcn_readlane(cj, i);
+                    const int k = cj_i / LPR;
+                    const C x = cmk<C>(gr[0][k], gi[0][k]);
+                    if (row == ci_i && c == cj_i % LPR) s_q[par][i][0] = x;
+                }
+                own = __ballot(inw && !noop && cj / RPW == wid);
+                while (own) {
+                    const int i = (int)__builtin_ctzll(own);
+                    own &= own - 1;
+                    const int ci_i = __builtin_amdgcn_readlane(ci, i);
+                    const int cj_i = __builtin_amdgcn_readlane(cj, i);
+                    const int k = ci_i / LPR;
+                    const C x = cmk<C>(gr[1][k], gi[1][k]);
+                    if (row == cj_i && c == ci_i % LPR) s_q[par][i][1] = x;
+                }
+                __syncthreads();
+                if (inw && !noop) {
+                    q0l = s_q[par][lane][0];
+                    q1l = s_q[par][lane][1];
+                }
+            } else {
+                if (inw && !noop) {
+                    q0l = gat(0, ci, cj);
+                    q1l = gat(1, cj, ci);
+                }
+            }
+            double ratio = 1.0;
+            bool accl;
+            if (noop) {
+                accl = u3 < 1.0;  // LatticeStepper.cpp:64-68 returns the current weight: ratio 1
+            } else {
+                const double n0 = (double)q0l.x * q0l.x + (double)q0l.y * q0l.y;
+                const double n1 = (double)q1l.x * q1l.x + (double)q1l.y * q1l.y;
+                ratio = n0 * n1;
+                accl = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo, MetroMC.cpp:27-43
+            }
+            // first accepted move of the window: lane j, step s0 + j - b0
+            const uint64_t moved = __ballot(inw && !noop && accl);
+            const int j = moved ? (int)__builtin_ctzll(moved) : 64;
+            const int nproc = moved ? j - b0 + 1 : len;
+            const uint64_t pmask = (nproc >= 64 ? ~0ull : ((1ull << nproc) - 1ull)) << b0;
+            const int nnoop = __popcll(__ballot(noop) & pmask);
+            const int nmove = moved ? 1 : 0;
+            n_noop += nnoop;
+            n_acc += nmove;
+            n_rej += nproc - nnoop - nmove;
+            if (wid == 0 && ((pmask >> lane) & 1ull)) {
+                const size_t ti = (size_t)w * a.n_steps + s0 + (lane - b0);
+                if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
+                if (a.tr_accept) a.tr_accept[ti] = accl ? 1 : 0;
+                if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+            }
+            // The accepted move is wave-uniform; taken through an opaque copy it is
+            // compiled as an exec-masked region (skipped by s_cbranch_execz like a scalar
+            // branch), which keeps the tables in the same registers on both paths.
+            int jv;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(jv) : "s"(j));
+            if (jv < 64) {
+                const int ci_j = __builtin_amdgcn_readlane(ci, j), cj_j = __builtin_amdgcn_readlane(cj, j);
+                const int sa_j = __builtin_amdgcn_readlane(sa, j), sb_j = __builtin_amdgcn_readlane(sb, j);
+                const C q0 = cmk<C>(readlane_real(q0l.x, j), readlane_real(q0l.y, j));
+                const C q1 = cmk<C>(readlane_real(q1l.x, j), readlane_real(q1l.y, j));
+                const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
+                const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
+                // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
+                const int ks0 = cj_j / LPR, ls0 = cj_j % LPR, ks1 = ci_j / LPR, ls1 = ci_j % LPR;
+                C x0, x1;
+                if constexpr (RES) {
+                    x0 = cmk<C>(gr[0][ks0], gi[0][ks0]);
+                    x1 = cmk<C>(gr[1][ks1], gi[1][ks1]);
+                } else {
+                    x0 = has_row ? gat(0, row, cj_j) : cmk<C>(0, 0);
+                    x1 = has_row ? gat(1, row, ci_j) : cmk<C>(0, 0);
                 }
                 // the moving rows G_up[ci,:] + e_cj and G_dn[cj,:] + e_ci go to LDS
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int rsel = fl ? cj : ci;
+                    const int rsel = fl ? cj_j : ci_j;
                     if (row == rsel) {
-                        T *dr = &s_row[par][fl][c][0][0], *di = &s_row[par][fl][c][1][0];
+                        T *dr = &s_row[fl][c][0][0], *di = &s_row[fl][c][1][0];
 #pragma unroll
                         for (int k = 0; k < CPT; ++k) {
                             if constexpr (RES) {
@@ -330,7 +479,7 @@ sweep_kernel(SweepArgs a)
                     const C q = fl ? q1 : q0;
                     const double nq = fl ? n1 : n0;
                     const C x = fl ? x1 : x0;
-                    const int ls = fl ? ls1 : ls0, rsel = fl ? cj : ci;
+                    const int ls = fl ? ls1 : ls0, rsel = fl ? cj_j : ci_j;
                     // u_i = G[i][cs] - delta(i, rs), held by lane ls of row i's group
                     C uu;
                     uu.x = __shfl(x.x, gbase + ls);
@@ -340,7 +489,7 @@ sweep_kernel(SweepArgs a)
                     const C coef = cmul(uu, qi);
                     const C nc = cmk<C>(-coef.x, -coef.y);
                     if (has_row) {
-                        const T *sr = &s_row[par][fl][c][0][0], *si = &s_row[par][fl][c][1][0];
+                        const T *sr = &s_row[fl][c][0][0], *si = &s_row[fl][c][1][0];
 #pragma unroll
                         for (int k = 0; k < CPT; ++k) {
                             const T vr = sr[k], vi = si[k];
@@ -359,131 +508,36 @@ sweep_kernel(SweepArgs a)
                         }
                     }
                 }
-                // State::Hop (State.cpp:73-84): every wave updates its own site map;
-                // thread 0 keeps the LDS maps used by the measurement
-#pragma unroll
-                for (int r = 0; r < NSR; ++r) {
-                    const int v = lane + 64 * r;
-                    smap[r] = v == sb ? ci : (v == sa ? ((1 << 16) | cj) : smap[r]);
+                // State::Hop (State.cpp:73-84): every wavefront writes the same values,
+                // so each one reads back its own writes in the next window
+                if (lane == 0) {
+                    s_site[sb_j] = ci_j;
+                    s_site[sa_j] = (1 << 16) | cj_j;
+                    s_part[0][ci_j] = sb_j;
+                    s_part[1][cj_j] = sa_j;
                 }
                 if (tid == 0) {
-                    s_site[sb] = ci;
-                    s_site[sa] = (1 << 16) | cj;
-                    s_part[0][ci] = sb;
-                    s_part[1][cj] = sa;
                     // amplitude *= q per flavour (SlaterDeterminant.cpp:321)
 #pragma unroll
                     for (int fl = 0; fl < 2; ++fl) {
                         const C q = fl ? q1 : q0;
-                        const double re = mre[fl] * (double)q.x - mim[fl] * (double)q.y;
-                        const double im = mre[fl] * (double)q.y + mim[fl] * (double)q.x;
+                        const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
+                        const double re = mr * (double)q.x - mi * (double)q.y;
+                        const double im = mr * (double)q.y + mi * (double)q.x;
                         int e;
                         (void)frexp(fmax(fabs(re), fabs(im)), &e);
-                        mre[fl] = ldexp(re, -e);
-                        mim[fl] = ldexp(im, -e);
-                        mex[fl] += e;
+                        s_amp[2 * fl] = ldexp(re, -e);
+                        s_amp[2 * fl + 1] = ldexp(im, -e);
+                        s_amp[4 + fl] += e;
                     }
                 }
-                ++n_acc;
-            } else {
-                ++n_rej;
+                // streamed tables: every row is rewritten before the next window reads them
+                if constexpr (!RES) __syncthreads();
             }
+            s0 += nproc;
             par ^= 1;
-        }
-        if (tid == 0) {
-            const size_t ti = (size_t)w * a.n_steps + s;
-            if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
-            if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
-            if (a.tr_ratio) a.tr_ratio[ti] = ratio;
-        }
-      }
-
-        if (s < 0 || (a.meas > 0 && (a.meas == 1 || s % a.meas == 0))) {
-            // Quantity::measure after the step (MetroMC.cpp:61-68)
-            __syncthreads();
-            double ev[2] = {0.0, 0.0};
-            if (a.mask & MEAS_PROJHEIS) {
-                if (has_row) {
-#pragma unroll
-                    for (int fl = 0; fl < 2; ++fl) {
-                        const int sh = s_part[fl][row];
-                        for (int d = 0; d < 4; ++d) {
-                            const int t = s_nbr[sh * 4 + d];
-                            const int info = s_site[t];
-                            if ((info >> 16) == fl) continue;  // parallel bond
-                            const int j = info & 0xFFFF;
-                            if (c == j % LPR) {
-                                if constexpr (RES) {
-                                    // masked sum over the slots (the slot index differs
-                                    // between lanes): no run-time register index
-                                    const int kj = j / LPR;
-                                    C val = cmk<C>(0, 0);
-#pragma unroll
-                                    for (int k = 0; k < CPT; ++k) {
-                                        const T m = (k == kj) ? (T)1 : (T)0;
-                                        val.x = fma(m, gr[fl][k], val.x);
-                                        val.y = fma(m, gi[fl][k], val.y);
-                                    }
-                                    s_qb[sh * 4 + d] = val;
-                                } else {
-                                    s_qb[sh * 4 + d] = gat(fl, row, j);
-                                }
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-                // ProjHeis.cpp:131-166: -(1/4 + 1/2 q_up q_dn) per antiparallel edge, +1/4 per parallel
-                for (int e = tid; e < 2 * NS; e += NT) {
-                    const int4 eg = a.edge[e];
-                    if ((s_site[eg.x] >> 16) == (s_site[eg.z] >> 16)) {
-                        ev[0] += 0.25;
-                    } else {
-                        const C qa = s_qb[eg.x * 4 + eg.y], qb = s_qb[eg.z * 4 + eg.w];
-                        const double pr = (double)qa.x * qb.x - (double)qa.y * qb.y;
-                        const double pi = (double)qa.x * qb.y + (double)qa.y * qb.x;
-                        ev[0] += -0.25 - 0.5 * pr;
-                        ev[1] += -0.5 * pi;
-                    }
-                }
-            }
-            double mv2 = 0.0;
-            if (a.mask & MEAS_STAGMAGNZ) {
-                // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
-                for (int v = tid; v < NS; v += NT) {
-                    const int x = v % a.L, y = v / a.L;
-                    const double sg = ((x + y) & 1) ? -1.0 : 1.0;
-                    mv2 += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
-                }
-            }
-            double red[3] = {ev[0], ev[1], mv2};
-            block_sum<NT, 3>(red, s_red);
-            const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
-            if (tid == 0) {
-                const double la = a.logabs[2 * w] + a.logabs[2 * w + 1] +
-                                  log(hypot(mre[0], mim[0])) + log(hypot(mre[1], mim[1])) +
-                                  (mex[0] + mex[1]) * 0.69314718055994531;
-                const double lw = 2.0 * la;
-                if (s >= 0) {
-                    if (a.mask & MEAS_PROJHEIS) {
-                        s_acc[0] += E_re;
-                        s_acc[1] += E_im;
-                        s_acc[2] += E_re * E_re + E_im * E_im;
-                    }
-                    if (a.mask & MEAS_STAGMAGNZ) {
-                        s_acc[4] += mz;
-                        s_acc[6] += mz * mz;
-                    }
-                    s_acc[11] += lw;
-                    s_acc[3] += 1;
-                }
-                const size_t mi = (size_t)w * a.n_meas_slots + mslot;
-                if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
-                if (a.tr_mz) { a.tr_mz[2 * mi] = mz; a.tr_mz[2 * mi + 1] = 0.0; }
-                if (a.tr_logw) a.tr_logw[mi] = lw;
-            }
-            ++mslot;
-            __syncthreads();
+            const int64_t last = s0 - 1;
+            if (a.meas > 0 && (a.meas == 1 || last % a.meas == 0)) measure(last);
         }
     }
 
@@ -501,8 +555,9 @@ sweep_kernel(SweepArgs a)
     if (tid == 0) {
 #pragma unroll
         for (int fl = 0; fl < 2; ++fl) {
-            a.logabs[2 * w + fl] += log(hypot(mre[fl], mim[fl])) + mex[fl] * 0.69314718055994531;
-            a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(mim[fl], mre[fl]), 2 * M_PI);
+            const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
+            a.logabs[2 * w + fl] += log(hypot(mr, mi)) + s_amp[4 + fl] * 0.69314718055994531;
+            a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(mi, mr), 2 * M_PI);
         }
         a.counter[w] = ctr0 + (uint64_t)a.n_steps;
         s_acc[7] = MONLY ? 0.0 : (double)a.n_steps;
