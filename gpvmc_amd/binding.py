@@ -32,7 +32,8 @@ SIGNATURES = {
     "gpvmc_measure": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "gpvmc_read_accumulators": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_reset_accumulators": (ctypes.c_int, [_vp]),
-    "gpvmc_read_statspin": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_read_statspin": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
+    "gpvmc_read_statspin_groups": (ctypes.c_int, [_vp, _vp, _vp]),
     "gpvmc_read_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gpvmc_read_inverse": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_ratio_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
@@ -176,6 +177,22 @@ class Batch:
         ph = np.zeros(self.W)
         self._check(self.lib.gpvmc_read_state(self.h, _ptr(spins), _ptr(part), _ptr(la), _ptr(ph)))
         return spins, part, la, ph
+
+    def statspin(self, w0=0, nw=None):
+        """StatSpinStruct sums of walkers w0 .. w0+nw-1: (acc[nw][5][Ns][Ns] complex, with
+        rows szz, spm, smp, spp, smm; n_meas[nw])."""
+        nw = self.W - w0 if nw is None else nw
+        acc = np.zeros((nw, 5, self.Ns, self.Ns), np.complex128)
+        n = np.zeros(nw)
+        self._check(self.lib.gpvmc_read_statspin(self.h, w0, nw, _ptr(acc), _ptr(n)))
+        return acc, n
+
+    def statspin_groups(self):
+        """StatSpinStruct sums added over the walkers of each parameter group."""
+        acc = np.zeros((self.G, 5, self.Ns, self.Ns), np.complex128)
+        n = np.zeros(self.G)
+        self._check(self.lib.gpvmc_read_statspin_groups(self.h, _ptr(acc), _ptr(n)))
+        return acc, n
 
     def inverse(self, w):
         ai = np.zeros((2, self.N, self.N), np.complex128)

@@ -95,6 +95,8 @@ struct gpvmc_ctx {
     // scratch for walk traces / replay
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    // StatSpinStruct sums [W][ss_block(Ns)], allocated on first request
+    double *d_ss = nullptr;
 };
 
 static void free_ctx(gpvmc_ctx *c)
@@ -102,7 +104,7 @@ static void free_ctx(gpvmc_ctx *c)
     if (!c) return;
     void *ptrs[] = {c->d_G, c->d_Ai64, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
                     c->d_status, c->d_edge, c->d_logabs, c->d_phase, c->d_acc, c->d_drift,
-                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch};
+                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -137,6 +139,23 @@ static int ensure_scratch(gpvmc_ctx *c, size_t bytes)
     c->scratch_bytes = 0;
     HIPCHK(hipMalloc(&c->d_scratch, bytes));
     c->scratch_bytes = bytes;
+    return GPVMC_OK;
+}
+
+static int ensure_statspin(gpvmc_ctx *c)
+{
+    if (c->d_ss) return GPVMC_OK;
+    const size_t bytes = (size_t)c->W * gv::ss_block((size_t)c->Ns) * sizeof(double);
+    HIPCHK(hipMalloc(&c->d_ss, bytes));
+    HIPCHK(hipMemsetAsync(c->d_ss, 0, bytes, c->stream));
+    return GPVMC_OK;
+}
+
+static int clear_statspin(gpvmc_ctx *c)
+{
+    if (c->d_ss)
+        HIPCHK(hipMemsetAsync(c->d_ss, 0, (size_t)c->W * gv::ss_block((size_t)c->Ns) * sizeof(double),
+                              c->stream));
     return GPVMC_OK;
 }
 
@@ -318,7 +337,9 @@ int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
-    int rc = run_refresh(c, nullptr);
+    int rc = clear_statspin(c);
+    if (rc) return rc;
+    rc = run_refresh(c, nullptr);
     if (rc) return rc;
     std::vector<int32_t> st((size_t)W * 2);
     HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -350,8 +371,9 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
     if (n_steps < 0 || meas < 0) return fail(GPVMC_ERR_ARG, "negative n_steps or meas");
-    if (mask & ~(uint32_t)(GPVMC_MEAS_PROJHEIS | GPVMC_MEAS_STAGMAGNZ | GPVMC_MEAS_OVERLAP))
-        return fail(GPVMC_ERR_UNSUPPORTED, "estimator not available in this build");
+    if (mask & ~(uint32_t)(GPVMC_MEAS_PROJHEIS | GPVMC_MEAS_STAGMAGNZ | GPVMC_MEAS_OVERLAP |
+                           GPVMC_MEAS_STATSPIN))
+        return fail(GPVMC_ERR_ARG, "unknown estimator bit in mask");
     if (n_steps == 0 && !measure_only) return GPVMC_OK;
     const size_t W = c->W;
     const int64_t nmeas = measure_only ? 1
@@ -381,6 +403,10 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
         int rc = ensure_scratch(c, off);
         if (rc) return rc;
     }
+    if ((mask & GPVMC_MEAS_STATSPIN) && !measure_only && nmeas) {
+        int rc = ensure_statspin(c);
+        if (rc) return rc;
+    }
     char *sb = static_cast<char *>(c->d_scratch);
     gv::SweepArgs a{};
     a.G = c->d_G;
@@ -403,6 +429,7 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
     a.key1 = (uint32_t)(c->cfg.seed >> 32);
     a.L = c->L;
     a.measure_only = measure_only;
+    a.ss = c->d_ss;
     if (uniforms) {
         a.uniforms = reinterpret_cast<const double *>(sb + o_u);
         HIPCHK(hipMemcpyAsync(sb + o_u, uniforms, W * n_steps * 4 * sizeof(double),
@@ -458,15 +485,94 @@ int gpvmc_reset_accumulators(gpvmc_ctx *c)
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)c->W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    int rc = clear_statspin(c);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return GPVMC_OK;
 }
 
-int gpvmc_read_statspin(gpvmc_ctx *c, double *acc)
+// Expands one raw StatSpinStruct block (szz | smp | spm diagonal | count) into the
+// reference's five Ns x Ns complex matrices (szz, spm, smp, spp, smm), StatSpinStruct.cpp:77-81.
+static void expand_statspin(const double *raw, size_t Ns, double *out)
 {
-    (void)acc;
+    const size_t S = Ns * Ns;
+    const double *zz = raw, *mp = raw + S, *pd = raw + 3 * S;
+    double *szz = out, *spm = out + 2 * S, *smp = out + 4 * S, *spp = out + 6 * S, *smm = out + 8 * S;
+    for (size_t e = 0; e < S; ++e) {
+        szz[2 * e] = zz[e];
+        szz[2 * e + 1] = 0.0;
+        smp[2 * e] = mp[2 * e];
+        smp[2 * e + 1] = mp[2 * e + 1];
+        spp[2 * e] = spp[2 * e + 1] = 0.0;
+        smm[2 * e] = smm[2 * e + 1] = 0.0;
+    }
+    for (size_t x = 0; x < Ns; ++x)
+        for (size_t y = 0; y < Ns; ++y) {
+            const size_t e = x * Ns + y;
+            if (x == y) {
+                spm[2 * e] = pd[x];
+                spm[2 * e + 1] = 0.0;
+            } else {
+                spm[2 * e] = mp[2 * (y * Ns + x)];
+                spm[2 * e + 1] = mp[2 * (y * Ns + x) + 1];
+            }
+        }
+}
+
+int gpvmc_read_statspin(gpvmc_ctx *c, int32_t w0, int32_t nw, double *acc, double *n_meas)
+{
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
-    return fail(GPVMC_ERR_UNSUPPORTED, "StatSpinStruct is not available in this build");
+    if (w0 < 0 || nw < 0 || (int64_t)w0 + nw > c->W) return fail(GPVMC_ERR_ARG, "walker range out of bounds");
+    const size_t Ns = c->Ns, per = gv::ss_block(Ns), S = Ns * Ns;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> raw(per, 0.0);
+    for (int32_t k = 0; k < nw; ++k) {
+        if (c->d_ss)
+            HIPCHK(hipMemcpy(raw.data(), c->d_ss + (size_t)(w0 + k) * per, per * sizeof(double),
+                             hipMemcpyDeviceToHost));
+        if (acc) expand_statspin(raw.data(), Ns, acc + (size_t)k * 10 * S);
+        if (n_meas) n_meas[k] = raw[per - 1];
+    }
+    return GPVMC_OK;
+}
+
+int gpvmc_read_statspin_groups(gpvmc_ctx *c, double *acc, double *n_meas)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    const size_t Ns = c->Ns, per = gv::ss_block(Ns), S = Ns * Ns;
+    const int G = c->G;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    std::vector<double> raw((size_t)G * per, 0.0);
+    if (c->d_ss) {
+        // walkers listed group by group
+        std::vector<int32_t> idx, off(G + 1, 0);
+        for (int g = 0; g < G; ++g) {
+            off[g] = (int32_t)idx.size();
+            for (int w = 0; w < c->W; ++w)
+                if (c->groups[w] == g) idx.push_back(w);
+        }
+        off[G] = (int32_t)idx.size();
+        const size_t ib = (idx.size() + off.size()) * sizeof(int32_t);
+        const size_t ob = ((ib + 255) & ~(size_t)255);
+        int rc = ensure_scratch(c, ob + raw.size() * sizeof(double));
+        if (rc) return rc;
+        char *sb = static_cast<char *>(c->d_scratch);
+        int32_t *d_idx = reinterpret_cast<int32_t *>(sb);
+        int32_t *d_off = d_idx + idx.size();
+        double *d_out = reinterpret_cast<double *>(sb + ob);
+        if (!idx.empty())
+            HIPCHK(hipMemcpyAsync(d_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(gv_launch_ss_reduce(c->d_ss, d_idx, d_off, G, per, d_out, c->stream));
+        HIPCHK(hipMemcpyAsync(raw.data(), d_out, raw.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    for (int g = 0; g < G; ++g) {
+        if (acc) expand_statspin(raw.data() + (size_t)g * per, Ns, acc + (size_t)g * 10 * S);
+        if (n_meas) n_meas[g] = raw[(size_t)g * per + per - 1];
+    }
+    return GPVMC_OK;
 }
 
 int gpvmc_read_state(gpvmc_ctx *c, int32_t *spins, int32_t *part, double *logabs, double *phase)

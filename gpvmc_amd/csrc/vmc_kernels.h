@@ -36,7 +36,13 @@ struct SweepArgs {
     uint32_t key0, key1;
     int32_t L;
     int32_t measure_only;     // 1: no step, one measurement into the traces only
+    double *ss;               // StatSpinStruct sums [W][SS_BLOCK(Ns)] or null
 };
+
+// StatSpinStruct sums of one walker (all FP64): szz[Ns][Ns] | smp[Ns][Ns] complex |
+// spm diagonal [Ns] | measurement count.  spm off the diagonal is the transpose of smp
+// (both receive -ratio of the same swap), spp and smm vanish for two flavours.
+__host__ __device__ constexpr size_t ss_block(size_t Ns) { return 3 * Ns * Ns + Ns + 1; }
 
 struct GJArgs {
     double2 *Ai;              // [W][2][N][N] FP64 inverse, Ai[fl][label][slot]
@@ -66,6 +72,9 @@ hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int
 int gv_refresh_supported(int prec, int N);
 hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
+// out[g][e] = sum over the walkers idx[off[g] .. off[g+1]) of ss[walker][e], e < per
+hipError_t gv_launch_ss_reduce(const double *ss, const int32_t *idx, const int32_t *off, int G,
+                               size_t per, double *out, hipStream_t st);
 }
 
 #endif

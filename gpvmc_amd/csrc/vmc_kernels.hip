@@ -119,6 +119,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
 // and the vector never leaves registers.
 template <typename T, int M> using regvec = T __attribute__((ext_vector_type(M)));
 
+// largest divisor d of n with d * unit <= budget (at least 1)
+constexpr int divisor_within(int n, int unit, int budget)
+{
+    int best = 1;
+    for (int d = 1; d <= n; ++d)
+        if (n % d == 0 && d * unit <= budget) best = d;
+    return best;
+}
+
 // v_readlane of a float or double (wave-uniform lane index)
 __device__ __forceinline__ float readlane_real(float v, int l)
 {
@@ -188,6 +197,10 @@ sweep_kernel(SweepArgs a)
     // thread 0: amplitude change per flavour as complex mantissa [0..3] and binary
     // exponent [4..5], so no logarithm is taken per step
     __shared__ double s_amp[6];
+    // StatSpinStruct: a chunk of the transposed G_dn (RES kernels)
+    constexpr int TRK = LPR * N * 2 * (int)sizeof(T);         // bytes per slot k of G_dn^T
+    constexpr int CH = divisor_within(CPT, TRK, 32768);         // slots per chunk
+    __shared__ double s_tr[RES ? (CH * TRK + 7) / 8 : 1];
 
     const int w = __builtin_amdgcn_readfirstlane(a.order ? a.order[blockIdx.x] : (int)blockIdx.x);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -225,6 +238,69 @@ sweep_kernel(SweepArgs a)
     const uint64_t ctr0 = a.counter[w];
     int64_t mslot = 0;
     __syncthreads();
+
+    // StatSpinStruct::measure (StatSpinStruct.cpp:16-135) into this walker's FP64 sums:
+    // szz[x][y] += sz_x sz_y (1/4 each); spm or smp diagonal += 1 for up or down at x; for
+    // every up particle i (site a) and down particle j (site b) smp[a][b] -= R_ij with the
+    // swap ratio R_ij = G_up[i][j] G_dn[j][i] (conj(amp) amp_swap / w of the reference).
+    // G_dn goes through LDS transposed, in chunks of CH slots of every row group.
+    auto statspin = [&]() {
+        double *ss = a.ss + (size_t)w * ss_block(NS);
+        double *zz = ss;
+        double *mp = ss + (size_t)NS * NS;
+        double *pd = ss + (size_t)3 * NS * NS;
+        for (int e = tid; e < NS * NS; e += NT) {
+            const int x = e / NS, y = e - x * NS;
+            zz[e] += ((s_site[x] >> 16) == (s_site[y] >> 16)) ? 0.25 : -0.25;
+        }
+        for (int v = tid; v < NS; v += NT) {
+            if ((s_site[v] >> 16) == 0) pd[v] += 1.0;
+            else mp[2 * ((size_t)v * NS + v)] += 1.0;
+        }
+        if (tid == 0) pd[NS] += 1.0;
+        const int sa = has_row ? s_part[0][row] : 0;
+#pragma unroll
+        for (int k0 = 0; k0 < CPT; k0 += CH) {
+            // rows j = c' + LPR k of G_dn, k in [k0, k0 + CH), i.e. j in [LPR k0, LPR (k0 + CH))
+            if constexpr (RES) {
+                T *sT = reinterpret_cast<T *>(s_tr);
+                __syncthreads();
+                if (has_row && row >= LPR * k0 && row < LPR * (k0 + CH)) {
+#pragma unroll
+                    for (int k = 0; k < CPT; ++k) {
+                        const int i = c + LPR * k;
+                        const size_t o = ((size_t)(row - LPR * k0) * N + i) * 2;
+                        sT[o] = gr[1][k];
+                        sT[o + 1] = gi[1][k];
+                    }
+                }
+                __syncthreads();
+                if (has_row) {
+#pragma unroll
+                    for (int k = k0; k < k0 + CH; ++k) {
+                        const int j = c + LPR * k;
+                        const size_t o = ((size_t)(j - LPR * k0) * N + row) * 2;
+                        const double br = sT[o], bi = sT[o + 1];
+                        const double ar = gr[0][k], ai = gi[0][k];
+                        double *m = mp + 2 * ((size_t)sa * NS + s_part[1][j]);
+                        m[0] -= ar * br - ai * bi;
+                        m[1] -= ar * bi + ai * br;
+                    }
+                }
+            } else {
+                if (has_row) {
+#pragma unroll
+                    for (int k = k0; k < k0 + CH; ++k) {
+                        const int j = c + LPR * k;
+                        const C ga = gat(0, row, j), gb = gat(1, j, row);
+                        double *m = mp + 2 * ((size_t)sa * NS + s_part[1][j]);
+                        m[0] -= (double)ga.x * gb.x - (double)ga.y * gb.y;
+                        m[1] -= (double)ga.x * gb.y + (double)ga.y * gb.x;
+                    }
+                }
+            }
+        }
+    };
 
     // Quantity::measure (MetroMC.cpp:61-68); s < 0 is the measurement-only pass
     auto measure = [&](int64_t s) {
@@ -275,6 +351,7 @@ sweep_kernel(SweepArgs a)
                 }
             }
         }
+        if ((a.mask & MEAS_STATSPIN) && s >= 0) statspin();
         double mv2 = 0.0;
         if (a.mask & MEAS_STAGMAGNZ) {
             // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
@@ -824,6 +901,20 @@ gbuild_kernel(GBArgs a)
 }
 
 // ---------------------------------------------------------------------------------
+// Sum of per-walker StatSpinStruct blocks over the walkers of each parameter group:
+// blockIdx.y = group, one thread per entry, walkers read in order (coalesced over e).
+__global__ void __launch_bounds__(256) ss_reduce_kernel(const double *ss, const int32_t *idx,
+                                                        const int32_t *off, size_t per, double *out)
+{
+    const int g = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= per) return;
+    double t = 0.0;
+    for (int k = off[g]; k < off[g + 1]; ++k) t += ss[(size_t)idx[k] * per + e];
+    out[(size_t)g * per + e] = t;
+}
+
+// ---------------------------------------------------------------------------------
 // host-side dispatch tables
 template <typename T, int N, int LPR, bool RES>
 static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st)
@@ -902,6 +993,14 @@ static const GBEntry kGB[] = {
 };
 
 }  // namespace gv
+
+extern "C" hipError_t gv_launch_ss_reduce(const double *ss, const int32_t *idx, const int32_t *off,
+                                          int G, size_t per, double *out, hipStream_t st)
+{
+    const dim3 grid((unsigned)((per + 255) / 256), (unsigned)G);
+    hipLaunchKernelGGL(gv::ss_reduce_kernel, grid, dim3(256), 0, st, ss, idx, off, per, out);
+    return hipGetLastError();
+}
 
 extern "C" int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads)
 {

@@ -109,8 +109,16 @@ int gpvmc_measure(gpvmc_ctx *ctx, uint32_t mask, double *eloc, double *mz, doubl
 #define GPVMC_ACC_STRIDE 12
 int gpvmc_read_accumulators(gpvmc_ctx *ctx, double *acc);
 int gpvmc_reset_accumulators(gpvmc_ctx *ctx);
-/* StatSpinStruct sums per walker [w][5][Ns][Ns] complex (szz, spm, smp, spp, smm). */
-int gpvmc_read_statspin(gpvmc_ctx *ctx, double *acc);
+/* StatSpinStruct sums (GPVMC_MEAS_STATSPIN) since the last reset, as the reference's
+ * MatrixQuantity m_vals (StatSpinStruct.cpp:12-14,77-81) before Quantity::save divides
+ * them by the count: five Ns x Ns complex matrices szz, spm, smp, spp, smm per walker,
+ * acc[k][5][Ns][Ns] complex for walkers w0 .. w0+nw-1, and n_meas[k] measurements.
+ * spp and smm are zero for two flavours.  Either output may be NULL. */
+int gpvmc_read_statspin(gpvmc_ctx *ctx, int32_t w0, int32_t nw, double *acc, double *n_meas);
+/* The same sums added over the walkers of each parameter group: acc[g][5][Ns][Ns]
+ * complex, n_meas[g] (the reduction over chains that the reference leaves to its
+ * post-processing, which averages the per-rank files). */
+int gpvmc_read_statspin_groups(gpvmc_ctx *ctx, double *acc, double *n_meas);
 
 /* Current state: spins[w][site], labels part[w][2][N] (site of each particle),
  * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */

@@ -345,3 +345,87 @@ def test_mixed_refresh_and_energy_on_identical_configurations(L, W):
         assert abs(la[w] - wo.logabs) < 1e-3  # FP32 running log-amplitude before the refresh
         wo2 = O.Walker(L, occ, s_after[w])
         assert abs(out["eloc"][w][0] - wo2.eloc()) < 1e-5
+
+
+def oracle_statspin_walk(L, occ, spins, u, nsteps, meas):
+    """The oracle chain with StatSpinStruct measured after step s when s % meas == 0
+    (MetroMC::Walk cadence), summed as the reference's m_vals."""
+    wo = O.Walker(L, occ, spins)
+    assert wo.ok
+    acc = np.zeros((5, L * L, L * L), np.complex128)
+    n = 0
+    s = 0
+    while s < nsteps:
+        # walk up to and including the next measured step
+        nxt = min(nsteps - 1, ((s + meas - 1) // meas) * meas)
+        wo.walk(u[s:nxt + 1].reshape(-1))
+        s = nxt + 1
+        if nxt % meas == 0:
+            wo.statspinstruct(acc)
+            n += 1
+    return wo, acc, n
+
+
+@pytest.mark.parametrize("L,W", [(4, 8), (8, 6), (16, 2)])
+def test_fp64_statspin_matches_oracle(L, W):
+    say("case statspin fp64", L, W)
+    seed = 77 + L
+    Ns = L * L
+    occ = occ_for(L, 0.25, 0.1)
+    spins = model.random_spins(L, W, seed)
+    nsteps, meas = 3 * Ns + 1, Ns
+    with binding.Batch(L, W, binding.FP64, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        ok = b.init(spins)
+        b.walk(nsteps, meas=meas, mask=binding.MEAS_PROJHEIS | binding.MEAS_STATSPIN)
+        acc, n = b.statspin()
+        gacc, gn = b.statspin_groups()
+        s_f = b.state()[0]
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    checked = 0
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo, oacc, on = oracle_statspin_walk(L, occ, spins[w], u[w], nsteps, meas)
+        assert np.array_equal(s_f[w], wo.spins())
+        assert n[w] == on == 4
+        # szz and the diagonals are occupation counts: exact
+        np.testing.assert_array_equal(acc[w, 0], oacc[0])
+        np.testing.assert_array_equal(np.diag(acc[w, 1]), np.diag(oacc[1]))
+        np.testing.assert_array_equal(np.diag(acc[w, 2]), np.diag(oacc[2]))
+        # swap ratios: FP64 table entries vs the oracle's inverse-row dot products
+        scale = max(1.0, np.abs(oacc[1:3]).max())
+        np.testing.assert_allclose(acc[w, 1:3], oacc[1:3], rtol=0, atol=1e-9 * scale)
+        assert not acc[w, 3:].any()
+        # sum rule at Sz = 0: sum_y szz[x][y] = 0 for every measurement
+        assert np.abs(acc[w, 0].sum(axis=1)).max() == 0.0
+        checked += 1
+    assert checked > 0
+    # group sums are the sums over walkers
+    np.testing.assert_allclose(gacc[0], acc.sum(axis=0), rtol=1e-12, atol=1e-12)
+    assert gn[0] == n.sum()
+
+
+@pytest.mark.parametrize("L,W", [(8, 4), (16, 2)])
+def test_mixed_statspin_tracks_oracle(L, W):
+    say("case statspin mixed", L, W)
+    seed = 91 + L
+    Ns = L * L
+    occ = occ_for(L, 0.1667, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        assert b.init(spins).all()
+        b.walk(1, meas=1, mask=binding.MEAS_STATSPIN)
+        acc, n = b.statspin()
+        s_f = b.state()[0]
+    u = model.philox_uniforms(seed, range(W), 0, 1)
+    for w in range(W):
+        wo, oacc, on = oracle_statspin_walk(L, occ, spins[w], u[w], 1, 1)
+        if not np.array_equal(s_f[w], wo.spins()):
+            continue  # decision flipped on a near-tie
+        assert n[w] == on == 1
+        np.testing.assert_array_equal(acc[w, 0], oacc[0])
+        # FP32 tables: ratios within FP32 rounding of the largest entry
+        scale = max(1.0, np.abs(oacc[1:3]).max())
+        np.testing.assert_allclose(acc[w, 1:3], oacc[1:3], rtol=0, atol=2e-5 * scale)
