@@ -97,6 +97,8 @@ struct gpvmc_ctx {
     size_t scratch_bytes = 0;
     // StatSpinStruct sums [W][ss_block(Ns)], allocated on first request
     double *d_ss = nullptr;
+    // working matrices of the streamed Gauss-Jordan variant (large N only)
+    double2 *d_gjwork = nullptr;
 };
 
 static void free_ctx(gpvmc_ctx *c)
@@ -104,7 +106,7 @@ static void free_ctx(gpvmc_ctx *c)
     if (!c) return;
     void *ptrs[] = {c->d_G, c->d_Ai64, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
                     c->d_status, c->d_edge, c->d_logabs, c->d_phase, c->d_acc, c->d_drift,
-                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss};
+                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -171,6 +173,10 @@ static int run_refresh(gpvmc_ctx *c, double *drift_dev)
     a.logabs = c->d_logabs;
     a.phase = c->d_phase;
     a.status = c->d_status;
+    if (const size_t wb = gv_gj_work_bytes(c->N)) {
+        if (!c->d_gjwork) HIPCHK(hipMalloc(&c->d_gjwork, wb * (size_t)c->W));
+        a.work = c->d_gjwork;
+    }
     gv::GBArgs b{};
     b.G = c->d_G;
     b.Ai = c->d_Ai64;

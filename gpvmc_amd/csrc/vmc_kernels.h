@@ -52,6 +52,7 @@ struct GJArgs {
     double *logabs;           // [W][2]
     double *phase;            // [W][2]
     int32_t *status;          // [W][2]
+    double2 *work;            // [W][2][N][N] working matrices of the streamed variant, or null
 };
 
 struct GBArgs {
@@ -70,6 +71,8 @@ int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
 hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
                            hipStream_t st);
 int gv_refresh_supported(int prec, int N);
+// bytes of GJArgs::work the refresh of N x N matrices needs per walker (0: none)
+size_t gv_gj_work_bytes(int N);
 hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
 // out[g][e] = sum over the walkers idx[off[g] .. off[g+1]) of ss[walker][e], e < per

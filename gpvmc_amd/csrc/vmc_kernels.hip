@@ -181,6 +181,9 @@ sweep_kernel(SweepArgs a)
     constexpr int RPW = 64 / LPR;                    // table rows per wavefront
     constexpr int KW = 16;                           // steps evaluated per window
     constexpr int RP = CPT + 16 / (int)sizeof(T);    // LDS row pitch: one 16-byte pad
+    // slot loops: fully unrolled over register tables (static register indices), partly
+    // over streamed ones (so the loads of a long row are not all hoisted at once)
+    constexpr int UNR = RES ? CPT : 4;
     static_assert(N % LPR == 0, "LPR must divide N");
     static_assert(64 % LPR == 0, "row groups must not straddle wavefronts");
     static_assert(NT <= 1024, "workgroup too large");
@@ -227,7 +230,7 @@ sweep_kernel(SweepArgs a)
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
-#pragma unroll
+#pragma unroll UNR
                 for (int k = 0; k < CPT; ++k) {
                     const C e = gat(fl, row, c + LPR * k);
                     gr[fl][k] = e.x;
@@ -266,7 +269,7 @@ sweep_kernel(SweepArgs a)
                 T *sT = reinterpret_cast<T *>(s_tr);
                 __syncthreads();
                 if (has_row && row >= LPR * k0 && row < LPR * (k0 + CH)) {
-#pragma unroll
+#pragma unroll UNR
                     for (int k = 0; k < CPT; ++k) {
                         const int i = c + LPR * k;
                         const size_t o = ((size_t)(row - LPR * k0) * N + i) * 2;
@@ -276,7 +279,7 @@ sweep_kernel(SweepArgs a)
                 }
                 __syncthreads();
                 if (has_row) {
-#pragma unroll
+#pragma unroll UNR
                     for (int k = k0; k < k0 + CH; ++k) {
                         const int j = c + LPR * k;
                         const size_t o = ((size_t)(j - LPR * k0) * N + row) * 2;
@@ -289,7 +292,7 @@ sweep_kernel(SweepArgs a)
                 }
             } else {
                 if (has_row) {
-#pragma unroll
+#pragma unroll UNR
                     for (int k = k0; k < k0 + CH; ++k) {
                         const int j = c + LPR * k;
                         const C ga = gat(0, row, j), gb = gat(1, j, row);
@@ -322,7 +325,7 @@ sweep_kernel(SweepArgs a)
                                 // between lanes): no run-time register index
                                 const int kj = j / LPR;
                                 C val = cmk<C>(0, 0);
-#pragma unroll
+#pragma unroll UNR
                                 for (int k = 0; k < CPT; ++k) {
                                     const T m = (k == kj) ? (T)1 : (T)0;
                                     val.x = fma(m, gr[fl][k], val.x);
@@ -533,7 +536,7 @@ sweep_kernel(SweepArgs a)
                     const int rsel = fl ? cj_j : ci_j;
                     if (row == rsel) {
                         T *dr = &s_row[fl][c][0][0], *di = &s_row[fl][c][1][0];
-#pragma unroll
+#pragma unroll UNR
                         for (int k = 0; k < CPT; ++k) {
                             if constexpr (RES) {
                                 dr[k] = gr[fl][k];
@@ -567,7 +570,7 @@ sweep_kernel(SweepArgs a)
                     const C nc = cmk<C>(-coef.x, -coef.y);
                     if (has_row) {
                         const T *sr = &s_row[fl][c][0][0], *si = &s_row[fl][c][1][0];
-#pragma unroll
+#pragma unroll UNR
                         for (int k = 0; k < CPT; ++k) {
                             const T vr = sr[k], vi = si[k];
                             if constexpr (RES) {
@@ -623,7 +626,7 @@ sweep_kernel(SweepArgs a)
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
-#pragma unroll
+#pragma unroll UNR
                 for (int k = 0; k < CPT; ++k) gat(fl, row, c + LPR * k) = cmk<C>(gr[fl][k], gi[fl][k]);
         }
     }
@@ -815,10 +818,154 @@ gj_kernel(GJArgs a)
 }
 
 // ---------------------------------------------------------------------------------
+// The same Gauss-Jordan inversion for matrices too large for a CU's registers
+// (N = 200: 640 KiB per matrix in FP64).  Thread (row, c) keeps its part of the working
+// row in global memory (GJArgs::work, read and written only by that thread); the pivot
+// row of each step is snapshotted in LDS as in gj_kernel, so the pivot search, the
+// arithmetic and the output permutation are those of gj_kernel.
+template <int N, int LPR>
+__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
+gj_stream_kernel(GJArgs a)
+{
+    using C = double2;
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    constexpr int NW = NT / 64;
+    constexpr int CPT = N / LPR;
+    constexpr int NS = 2 * N;
+    static_assert(N % LPR == 0, "LPR must divide N");
+
+    __shared__ int s_part[N];
+    __shared__ C s_prow[2][NW][N];
+    __shared__ unsigned long long s_ckey[2][NW];
+    __shared__ int s_perm[N];
+    __shared__ int s_invp[N];
+    __shared__ int s_flag;
+    __shared__ unsigned char s_seen[N];
+
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int row = tid / LPR, c = tid % LPR;
+    const bool has_row = row < N;
+
+    for (int i = tid; i < N; i += NT) {
+        s_part[i] = a.part[(size_t)w * 2 * N + (size_t)fl * N + i];
+        s_invp[i] = 0;
+    }
+    if (tid == 0) s_flag = 0;
+    __syncthreads();
+
+    const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
+    C *M = a.work + ((size_t)w * 2 + fl) * N * N;     // M[row][j], row-major
+    if (has_row) {
+        for (int t = 0; t < CPT; ++t) {
+            const int j = c + LPR * t;
+            M[(size_t)row * N + j] = occ[(size_t)s_part[j] * N + row];
+        }
+    }
+    bool pivoted = !has_row;
+    C dm = make_double2(1.0, 0.0);
+    int de = 0;
+    int par = 0;
+    for (int k = 0; k < N; ++k) {
+        const C bk = has_row ? M[(size_t)row * N + k] : make_double2(0.0, 0.0);
+        const float mag = (float)(fabs(bk.x) + fabs(bk.y));
+        unsigned long long key = 0ull;
+        if (!pivoted)
+            key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long ok2 = __shfl_xor(key, o);
+            key = ok2 > key ? ok2 : key;
+        }
+        const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
+        if (has_row && !pivoted && key != 0ull && row == wrow) {
+            for (int t = 0; t < CPT; ++t) {
+                const int j = c + LPR * t;
+                s_prow[par][wid][j] = M[(size_t)row * N + j];
+            }
+        }
+        if (lane == 0) s_ckey[par][wid] = key;
+        __syncthreads();
+        int bw = 0;
+        unsigned long long bkey = s_ckey[par][0];
+        for (int q = 1; q < NW; ++q) {
+            const unsigned long long v = s_ckey[par][q];
+            if (v > bkey) { bkey = v; bw = q; }
+        }
+        const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
+        const C piv = s_prow[par][bw][k];
+        const double np = piv.x * piv.x + piv.y * piv.y;
+        C ainv;
+        if (bkey != 0ull && np > 0.0) {
+            ainv = make_double2(piv.x / np, -piv.y / np);
+        } else {
+            ainv = make_double2(0.0, 0.0);
+            if (tid == 0) s_flag = 1;
+        }
+        if (tid == 0) {
+            s_perm[k] = (p >= 0 && p < N) ? p : 0;
+            if (p >= 0 && p < N) s_invp[p] = k;
+            const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
+            int e;
+            (void)frexp(fmax(fabs(re), fabs(im)), &e);
+            dm = make_double2(ldexp(re, -e), ldexp(im, -e));
+            de += e;
+        }
+        if (has_row) {
+            const bool me = (row == p);
+            const C bka = cmul(bk, ainv);
+            for (int t = 0; t < CPT; ++t) {
+                const int j = c + LPR * t;
+                const C rj = s_prow[par][bw][j];
+                C &m = M[(size_t)row * N + j];
+                if (me) {
+                    m = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
+                } else {
+                    m = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, m);
+                }
+            }
+            if (me) pivoted = true;
+        }
+        par ^= 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int sgn = 1;
+        for (int i = 0; i < N; ++i) s_seen[i] = 0;
+        for (int i = 0; i < N; ++i) {
+            if (s_seen[i]) continue;
+            int len = 0, j = i;
+            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
+            if ((len & 1) == 0) sgn = -sgn;
+        }
+        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
+        double ph = atan2(dm.y, dm.x);
+        if (sgn < 0) ph += M_PI;
+        const int sing = s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
+        a.logabs[2 * w + fl] = la;
+        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
+        a.status[2 * w + fl] = sing;
+    }
+    C *out = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    if (has_row) {
+        int orow = s_invp[row];
+        orow = orow < 0 ? 0 : (orow < N ? orow : N - 1);
+        for (int t = 0; t < CPT; ++t) {
+            const int j = c + LPR * t;
+            int oc = s_perm[j];
+            oc = oc < 0 ? 0 : (oc < N ? oc : N - 1);
+            out[(size_t)orow * N + oc] = M[(size_t)row * N + j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Ratio tables from the FP64 inverses, one block per (walker, flavour):
 // G_fl[i][j] = sum_f Ai_fl[i][f] Phi_fl(orb_f, site of particle j of the other flavour),
 // accumulated in FP64 over LDS-staged chunks of FB orbitals and written as OUT.  If
 // `drift` is given, max|G_old - G_new| / max|G_new| is stored first (drift refresh).
+// The thread's CPT output slots are formed CB at a time, CB sized to the VGPR budget of
+// the launch shape (one pass unless N is large).
 template <typename OUT, int N, int LPR>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 gbuild_kernel(GBArgs a)
@@ -831,6 +978,9 @@ gbuild_kernel(GBArgs a)
     constexpr int NS = 2 * N;
     constexpr int FB = (N % 8 == 0) ? 8 : ((N % 6 == 0) ? 6 : ((N % 2 == 0) ? 2 : 1));
     static_assert(N % FB == 0, "orbital chunk must divide N");
+    constexpr int WPS = (NW + 3) / 4;                             // waves per SIMD
+    constexpr int VB = (512 / WPS < 256 ? 512 / WPS : 256) / 2;   // VGPRs for accumulators
+    constexpr int CB = divisor_within(CPT, 4, VB);                // slots per pass
 
     __shared__ int s_other[N];        // sites of the other flavour's particles
     __shared__ C s_ai[N][FB + 1];     // +1 pads the row stride against bank conflicts
@@ -844,44 +994,46 @@ gbuild_kernel(GBArgs a)
     for (int i = tid; i < N; i += NT) s_other[i] = a.part[(size_t)w * 2 * N + (size_t)(1 - fl) * N + i];
     const C *ai = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
     const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
-    C acc[CPT];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) acc[k] = make_double2(0.0, 0.0);
-    __syncthreads();
-    for (int f0 = 0; f0 < N; f0 += FB) {
-        for (int i = tid; i < N * FB; i += NT) {
-            const int r = i / FB, ff = i % FB;
-            s_ai[r][ff] = ai[(size_t)r * N + f0 + ff];
-            const int j = i / FB;
-            s_b[ff][j] = occ[(size_t)s_other[j] * N + f0 + ff];
-        }
-        __syncthreads();
-        if (has_row) {
-#pragma unroll
-            for (int ff = 0; ff < FB; ++ff) {
-                const C x = s_ai[row][ff];
-#pragma unroll
-                for (int k = 0; k < CPT; ++k) acc[k] = cfma(x, s_b[ff][c + LPR * k], acc[k]);
-            }
-        }
-        __syncthreads();
-    }
     CO *out = reinterpret_cast<CO *>(a.G) + (size_t)w * 2 * N * N + (size_t)fl * N * N;
     double dmax = 0.0, nmax = 0.0;
-    if (has_row) {
+    __syncthreads();
+    for (int k0 = 0; k0 < CPT; k0 += CB) {
+        C acc[CB];
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const size_t o = (size_t)row * N + c + LPR * k;
-            if (a.drift) {
-                const CO old = out[o];
-                const double dx = (double)old.x - acc[k].x, dy = (double)old.y - acc[k].y;
-                dmax = fmax(dmax, sqrt(dx * dx + dy * dy));
-                nmax = fmax(nmax, sqrt(acc[k].x * acc[k].x + acc[k].y * acc[k].y));
+        for (int k = 0; k < CB; ++k) acc[k] = make_double2(0.0, 0.0);
+        for (int f0 = 0; f0 < N; f0 += FB) {
+            for (int i = tid; i < N * FB; i += NT) {
+                const int r = i / FB, ff = i % FB;
+                s_ai[r][ff] = ai[(size_t)r * N + f0 + ff];
+                const int j = i / FB;
+                s_b[ff][j] = occ[(size_t)s_other[j] * N + f0 + ff];
             }
-            CO v;
-            v.x = (OUT)acc[k].x;
-            v.y = (OUT)acc[k].y;
-            out[o] = v;
+            __syncthreads();
+            if (has_row) {
+#pragma unroll
+                for (int ff = 0; ff < FB; ++ff) {
+                    const C x = s_ai[row][ff];
+#pragma unroll
+                    for (int k = 0; k < CB; ++k) acc[k] = cfma(x, s_b[ff][c + LPR * (k0 + k)], acc[k]);
+                }
+            }
+            __syncthreads();
+        }
+        if (has_row) {
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const size_t o = (size_t)row * N + c + LPR * (k0 + k);
+                if (a.drift) {
+                    const CO old = out[o];
+                    const double dx = (double)old.x - acc[k].x, dy = (double)old.y - acc[k].y;
+                    dmax = fmax(dmax, sqrt(dx * dx + dy * dy));
+                    nmax = fmax(nmax, sqrt(acc[k].x * acc[k].x + acc[k].y * acc[k].y));
+                }
+                CO v;
+                v.x = (OUT)acc[k].x;
+                v.y = (OUT)acc[k].y;
+                out[o] = v;
+            }
         }
     }
     if (a.drift) {
@@ -937,6 +1089,15 @@ static hipError_t launch_gj_t(const GJArgs &a, int nwalkers, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int N, int LPR>
+static hipError_t launch_gj_stream_t(const GJArgs &a, int nwalkers, hipStream_t st)
+{
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    if (!a.work) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gj_stream_kernel<N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
 template <typename OUT, int N, int LPR>
 static hipError_t launch_gb_t(const GBArgs &a, int nwalkers, hipStream_t st)
 {
@@ -950,7 +1111,7 @@ struct SweepEntry {
     hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
 };
 struct GJEntry {
-    int N, lpr;
+    int N, lpr, streamed;
     hipError_t (*fn)(const GJArgs &, int, hipStream_t);
 };
 struct GBEntry {
@@ -966,30 +1127,43 @@ static const SweepEntry kSweep[] = {
     {0, 32, 4, 1, launch_sweep_t<double, 32, 4, true>},
     {0, 32, 2, 1, launch_sweep_t<double, 32, 2, true>},
     {0, 32, 8, 1, launch_sweep_t<double, 32, 8, true>},
+    {0, 50, 2, 0, launch_sweep_t<double, 50, 2, false>},
+    {0, 72, 4, 0, launch_sweep_t<double, 72, 4, false>},
     {0, 128, 2, 0, launch_sweep_t<double, 128, 2, false>},
+    {0, 200, 2, 0, launch_sweep_t<double, 200, 2, false>},
     {1, 8, 8, 1, launch_sweep_t<float, 8, 8, true>},
     {1, 18, 2, 1, launch_sweep_t<float, 18, 2, true>},
     {1, 32, 4, 1, launch_sweep_t<float, 32, 4, true>},
     {1, 32, 2, 1, launch_sweep_t<float, 32, 2, true>},
+    {1, 50, 2, 1, launch_sweep_t<float, 50, 2, true>},
+    {1, 72, 4, 1, launch_sweep_t<float, 72, 4, true>},
     {1, 128, 4, 1, launch_sweep_t<float, 128, 4, true>},
+    {1, 200, 2, 0, launch_sweep_t<float, 200, 2, false>},
 };
-
 static const GJEntry kGJ[] = {
-    {8, 8, launch_gj_t<8, 8>},
-    {18, 2, launch_gj_t<18, 2>},
-    {32, 4, launch_gj_t<32, 4>},
-    {128, 4, launch_gj_t<128, 4>},
+    {8, 8, 0, launch_gj_t<8, 8>},
+    {18, 2, 0, launch_gj_t<18, 2>},
+    {32, 4, 0, launch_gj_t<32, 4>},
+    {50, 2, 0, launch_gj_t<50, 2>},
+    {72, 4, 0, launch_gj_t<72, 4>},
+    {128, 4, 0, launch_gj_t<128, 4>},
+    {200, 2, 1, launch_gj_stream_t<200, 2>},
 };
-
 static const GBEntry kGB[] = {
     {0, 8, 8, launch_gb_t<double, 8, 8>},
     {0, 18, 2, launch_gb_t<double, 18, 2>},
     {0, 32, 4, launch_gb_t<double, 32, 4>},
+    {0, 50, 2, launch_gb_t<double, 50, 2>},
+    {0, 72, 4, launch_gb_t<double, 72, 4>},
     {0, 128, 8, launch_gb_t<double, 128, 8>},
+    {0, 200, 2, launch_gb_t<double, 200, 2>},
     {1, 8, 8, launch_gb_t<float, 8, 8>},
     {1, 18, 2, launch_gb_t<float, 18, 2>},
     {1, 32, 4, launch_gb_t<float, 32, 4>},
+    {1, 50, 2, launch_gb_t<float, 50, 2>},
+    {1, 72, 4, launch_gb_t<float, 72, 4>},
     {1, 128, 8, launch_gb_t<float, 128, 8>},
+    {1, 200, 2, launch_gb_t<float, 200, 2>},
 };
 
 }  // namespace gv
@@ -1029,6 +1203,13 @@ extern "C" int gv_refresh_supported(int prec, int N)
     if (!ok) return 0;
     for (const auto &e : gv::kGB)
         if (e.prec == prec && e.N == N) return 1;
+    return 0;
+}
+
+extern "C" size_t gv_gj_work_bytes(int N)
+{
+    for (const auto &e : gv::kGJ)
+        if (e.N == N) return e.streamed ? (size_t)2 * N * N * sizeof(double2) : 0;
     return 0;
 }
 

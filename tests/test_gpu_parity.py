@@ -230,6 +230,48 @@ def test_fp64_streaming_L16_matches_oracle():
         assert eloc_close(out["eloc"][w][:nm], o["eloc"])
 
 
+@pytest.mark.parametrize("L,W", [(10, 3), (12, 3), (20, 2)])
+def test_fp64_other_sizes_match_oracle(L, W):
+    """L = 10 and 12 (N = 50, 72) and L = 20 (N = 200: streamed tables and the streamed
+    Gauss-Jordan inverse) against the oracle: initial inverse, trajectory, tables, E_loc."""
+    Ns = L * L
+    nsteps, meas = 2 * Ns + 1, Ns
+    say("case fp64 size", L, W, nsteps)
+    seed = 1234 + L
+    occ = occ_for(L, 0.2333, 0.1)
+    spins = model.random_spins(L, W, seed)
+    with binding.Batch(L, W, binding.FP64, seed=seed) as b:
+        b.set_orbitals(0, occ)
+        ok = b.init(spins)
+        inv0 = [b.inverse(w) for w in range(W)]
+        la0 = b.state()[2]
+        out = b.walk(nsteps, meas=meas, trace=True)
+        s_f, _, la, _ = b.state()
+        tab = [b.ratio_table(w) for w in range(W)]
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    checked = 0
+    for w in range(W):
+        w0 = O.Walker(L, occ, spins[w])
+        assert ok[w] == w0.ok
+        if not ok[w]:
+            continue
+        assert abs(la0[w] - w0.logabs) < 1e-8
+        np.testing.assert_allclose(inv0[w], w0.Ai.transpose(0, 2, 1),
+                                   atol=1e-9 * max(1.0, np.abs(w0.Ai).max()))
+        wo, o = oracle_walk(L, occ, spins[w], u[w], meas=meas)
+        assert np.array_equal(o["accept"], out["accept"][w])
+        live = ~o["noop"].astype(bool)
+        ratios_match(out["ratio"][w], o["ratio"], live)
+        assert np.array_equal(s_f[w], wo.spins())
+        assert abs(la[w] - wo.logabs) < 1e-8
+        gt = oracle_table(wo)
+        np.testing.assert_allclose(tab[w], gt, atol=1e-8 * max(1.0, np.abs(gt).max()))
+        nm = len(o["eloc"])
+        assert eloc_close(out["eloc"][w][:nm], o["eloc"])
+        checked += 1
+    assert checked > 0
+
+
 def test_groups_and_walker_order():
     # walkers on different parameter groups each follow their own orbitals
     L, W = 8, 24
@@ -318,7 +360,7 @@ def test_mixed_ratios_track_oracle(L, W, nsweeps):
     assert agree_steps > 0.5 * W * nsteps
 
 
-@pytest.mark.parametrize("L,W", [(8, 8), (16, 4)])
+@pytest.mark.parametrize("L,W", [(8, 8), (16, 4), (10, 4), (12, 4), (20, 2)])
 def test_mixed_refresh_and_energy_on_identical_configurations(L, W):
     say("case mixed refresh", L, W)
     seed = 99 + L
