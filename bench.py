@@ -152,7 +152,8 @@ def cpu_eloc_rate(L, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed sweeps")
+    ap.add_argument("--steps", type=int, default=32,
+                    help="timed sweeps (default: two refresh periods at the usual tuned interval)")
     ap.add_argument("--warmup", type=int, default=3, help="untimed thermalisation sweeps")
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--walkers", type=int, default=16384)
