@@ -18,6 +18,12 @@
  *   or_stagmagnz            closed form on Neel and random configurations
  *   or_statspinstruct       exact <S^z S^z> and <S^+ S^-> at L=4 (exact enumeration);
  *                           sum rules
+ *   or_exc_amps             numpy determinants of the same matrices; ratios a_k / a_0
+ *                           equal to second-quantised site-basis amplitudes at L=4; the
+ *                           reference's slot arrangement times its HopSign (fixtures)
+ *   or_exc_walk             per-configuration h_k / a_k equal to the exact local
+ *                           (H psi_k) / psi_k at L=4 (trans and long); the chain follows
+ *                           the same proposal/accept code as or_walk
  */
 #include "gpvmc_oracle.h"
 #include <complex.h>
@@ -525,5 +531,195 @@ int or_walk(int L, const double *occin, const int32_t *nbr, int32_t *fock, int32
         }
     }
     free(edges);
+    return nm;
+}
+
+/* ---------------------------------------------------------------- channels
+ * Amplitude of channel state k at a configuration, restated from
+ * SlaterDeterminant::VirtUpdate (SlaterDeterminant.cpp:96-268) with the wavefunction
+ * sign GetSign()*HopSign() (:133-137): the product over flavours of
+ * det[Phi_fl(o_i, site of particle j)], rows o_i = the state's orbitals in increasing
+ * order, columns = the flavour's particles by label.  Computed from scratch by LU with
+ * partial pivoting, in log form (ln|a|, arg a), so it carries no update history.  The
+ * reference evaluates the crossing terms of its capacitance determinant with the matrix
+ * A of the last Init (linalg.cpp:196,320,457), which differs from these exact amplitudes
+ * once particles have moved; these are the exact ones. */
+static void logdet(int n, cplx *M, double *la, double *ph)
+{
+    double l = 0.0, p = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double best = -1.0;
+        for (int i = k; i < n; ++i) {
+            double m = fabs(creal(M[i * n + k])) + fabs(cimag(M[i * n + k]));
+            if (m > best) { best = m; piv = i; }
+        }
+        if (best == 0.0) { *la = -INFINITY; *ph = 0.0; return; }
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) {
+                cplx t = M[k * n + j]; M[k * n + j] = M[piv * n + j]; M[piv * n + j] = t;
+            }
+            p += M_PI;
+        }
+        cplx d = M[k * n + k];
+        l += log(cabs(d));
+        p += carg(d);
+        for (int i = k + 1; i < n; ++i) {
+            cplx f = M[i * n + k] / d;
+            if (f == 0) continue;
+            for (int j = k + 1; j < n; ++j) M[i * n + j] -= f * M[k * n + j];
+        }
+    }
+    *la = l;
+    *ph = remainder(p, 2 * M_PI);
+}
+
+/* tab[fl][orbital][site]; orbs[k][nup + ndn] (up orbitals then down, each increasing);
+ * part[nup + ndn] = sites of the up particles by label, then of the down particles. */
+void or_exc_amps(int Ns, int nup, int ndn, const double *tabin, int nk, const int32_t *orbs,
+                 const int32_t *part, double *la, double *ph)
+{
+    const cplx *tab = (const cplx *)tabin;
+    int nmax = nup > ndn ? nup : ndn;
+    cplx *M = malloc(sizeof(cplx) * (size_t)nmax * nmax);
+    for (int k = 0; k < nk; ++k) {
+        double lt = 0.0, pt = 0.0;
+        for (int fl = 0; fl < 2; ++fl) {
+            int n = fl ? ndn : nup;
+            const int32_t *o = orbs + (size_t)k * (nup + ndn) + (fl ? nup : 0);
+            const int32_t *s = part + (fl ? nup : 0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    M[i * n + j] = tab[((size_t)fl * Ns + o[i]) * Ns + s[j]];
+            double l, p;
+            logdet(n, M, &l, &p);
+            lt += l;
+            pt += p;
+        }
+        la[k] = lt;
+        ph[k] = remainder(pt, 2 * M_PI);
+    }
+    free(M);
+}
+
+static double logsumexp2(int nk, const double *la)
+{
+    double m = -INFINITY, s = 0.0;
+    for (int k = 0; k < nk; ++k) if (la[k] > m) m = la[k];
+    if (m == -INFINITY) return -INFINITY;
+    for (int k = 0; k < nk; ++k) s += exp(2.0 * (la[k] - m));
+    return 2.0 * m + log(s);
+}
+
+/* ProjHeis::measure on a channel (ProjHeis.cpp:40-140) at one configuration: amps a_k
+ * (scaled by a common real factor so that max |a_k| = 1) and
+ * h_k = sum over edges [antiparallel: -(a_k/4 + a_k(swapped)/2); parallel: +a_k/4]
+ * (J = 1, Lattice.h:29; the swapped configuration keeps particle labels).  Returns the
+ * scaled weight sum_k |a_k|^2; H and O are invariant under the common scale. */
+double or_exc_local(int L, int nup, int ndn, const double *tabin, int nk, const int32_t *orbs,
+                    const int32_t *site, const int32_t *part, double *ain, double *hin)
+{
+    int Ns = L * L, np = nup + ndn;
+    cplx *a = (cplx *)ain, *h = (cplx *)hin;
+    double *la = malloc(sizeof(double) * nk), *ph = malloc(sizeof(double) * nk);
+    int32_t *pt = malloc(sizeof(int32_t) * np);
+    int32_t *edges = malloc(sizeof(int32_t) * 4 * (size_t)Ns);
+    or_edges(L, edges);
+    or_exc_amps(Ns, nup, ndn, tabin, nk, orbs, part, la, ph);
+    double m = -INFINITY, wn = 0.0;
+    for (int k = 0; k < nk; ++k) if (la[k] > m) m = la[k];
+    for (int k = 0; k < nk; ++k) {
+        a[k] = exp(la[k] - m) * cexp(I * ph[k]);
+        wn += creal(a[k] * conj(a[k]));
+        h[k] = 0;
+    }
+    for (int e = 0; e < 2 * Ns; ++e) {
+        int vi = edges[2 * e], vj = edges[2 * e + 1];
+        int fi = site[vi] >> 16, fj = site[vj] >> 16;
+        if (fi == fj) {
+            for (int k = 0; k < nk; ++k) h[k] += 0.25 * a[k];
+            continue;
+        }
+        int pi = (site[vi] & 0xFFFF) + (fi ? nup : 0), pj = (site[vj] & 0xFFFF) + (fj ? nup : 0);
+        memcpy(pt, part, sizeof(int32_t) * np);
+        pt[pi] = vj;
+        pt[pj] = vi;
+        or_exc_amps(Ns, nup, ndn, tabin, nk, orbs, pt, la, ph);
+        for (int k = 0; k < nk; ++k)
+            h[k] -= 0.25 * a[k] + 0.5 * exp(la[k] - m) * cexp(I * ph[k]);
+    }
+    free(la); free(ph); free(pt); free(edges);
+    return wn;
+}
+
+/* One chain on a channel (MetroMC::Walk/Step, MetroMC.cpp:45-103; LatticeStepper::trystep
+ * and step, LatticeStepper.cpp:36-147): the weight is w = sum_k |a_k|^2 (trystep's
+ * BigDouble sum of norm(qs), :90-91), proposals and the accept rule are those of or_walk,
+ * 4 uniforms per step.  site[v] = (flavour << 16) | label, part as in or_exc_amps.
+ * After step s, when meas > 0 and (meas == 1 or s % meas == 0), ProjHeis::measure
+ * (ProjHeis.cpp:40-154) adds, with amps a_k at the configuration and
+ * h_k = sum_bonds [antiparallel: -(a_k/4 + a_k(swapped)/2); parallel: a_k/4],
+ *   HO[k1][k2] += conj(a_k1) h_k2 / w   and   HO[nk + k1][k2] += conj(a_k1) a_k2 / w,
+ * and logw[m] = ln w (OverlapTrack).  noop/accept/ratio get one entry per step.
+ * Returns the number of measurements. */
+int or_exc_walk(int L, int nup, int ndn, const double *tabin, int nk, const int32_t *orbs,
+                const int32_t *nbr, int32_t *site, int32_t *part, const double *u,
+                int64_t nsteps, int64_t meas, int8_t *noop, int8_t *accept, double *ratio,
+                double *HOin, double *logw)
+{
+    int Ns = L * L, np = nup + ndn;
+    cplx *HO = (cplx *)HOin;
+    double *la = malloc(sizeof(double) * nk), *ph = malloc(sizeof(double) * nk);
+    double *lb = malloc(sizeof(double) * nk), *pb = malloc(sizeof(double) * nk);
+    cplx *a = malloc(sizeof(cplx) * nk), *h = malloc(sizeof(cplx) * nk);
+    int32_t *pt = malloc(sizeof(int32_t) * np);
+    or_exc_amps(Ns, nup, ndn, tabin, nk, orbs, part, la, ph);
+    double lw = logsumexp2(nk, la);
+    int nm = 0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const double *us = u + 4 * s;
+        int v1 = (int)(Ns * us[0]);
+        if (v1 >= Ns) v1 = Ns - 1;
+        int ed = (int)(4 * us[1]);
+        if (ed > 3) ed = 3;
+        int v2 = nbr[4 * v1 + ed];
+        int f1 = site[v1] >> 16, f2 = site[v2] >> 16;
+        double r = 1.0;
+        int acc;
+        if (f1 == f2) {
+            acc = us[3] < 1.0;
+            if (noop) noop[s] = 1;
+        } else {
+            if (noop) noop[s] = 0;
+            int p1 = (site[v1] & 0xFFFF) + (f1 ? nup : 0), p2 = (site[v2] & 0xFFFF) + (f2 ? nup : 0);
+            memcpy(pt, part, sizeof(int32_t) * np);
+            pt[p1] = v2;
+            pt[p2] = v1;
+            or_exc_amps(Ns, nup, ndn, tabin, nk, orbs, pt, lb, pb);
+            double lw2 = logsumexp2(nk, lb);
+            r = exp(lw2 - lw);
+            acc = r > 1.0 ? (us[3] < 1.0) : (us[3] < r);
+            if (acc) {
+                memcpy(part, pt, sizeof(int32_t) * np);
+                int t = site[v1]; site[v1] = site[v2]; site[v2] = t;
+                memcpy(la, lb, sizeof(double) * nk);
+                memcpy(ph, pb, sizeof(double) * nk);
+                lw = lw2;
+            }
+        }
+        if (accept) accept[s] = (int8_t)acc;
+        if (ratio) ratio[s] = r;
+        if (meas > 0 && (meas == 1 || s % meas == 0)) {
+            double wn = or_exc_local(L, nup, ndn, tabin, nk, orbs, site, part, (double *)a, (double *)h);
+            for (int k1 = 0; k1 < nk; ++k1)
+                for (int k2 = 0; k2 < nk; ++k2) {
+                    HO[(size_t)k1 * nk + k2] += conj(a[k1]) * h[k2] / wn;
+                    HO[(size_t)(nk + k1) * nk + k2] += conj(a[k1]) * a[k2] / wn;
+                }
+            if (logw) logw[nm] = lw;
+            ++nm;
+        }
+    }
+    free(la); free(ph); free(lb); free(pb); free(a); free(h); free(pt);
     return nm;
 }

@@ -49,6 +49,16 @@ void or_stagmagnz(int L, const int32_t *fock, double *mz);
 void or_statspinstruct(int L, const double *occ, const int32_t *fock, const int32_t *part,
                        const double *Ai, double *acc);
 
+/* channels (exciton states): exact amplitudes by LU; a chain on w = sum_k |a_k|^2 */
+void or_exc_amps(int Ns, int nup, int ndn, const double *tab, int nk, const int32_t *orbs,
+                 const int32_t *part, double *la, double *ph);
+double or_exc_local(int L, int nup, int ndn, const double *tab, int nk, const int32_t *orbs,
+                    const int32_t *site, const int32_t *part, double *amps, double *heis);
+int  or_exc_walk(int L, int nup, int ndn, const double *tab, int nk, const int32_t *orbs,
+                 const int32_t *nbr, int32_t *site, int32_t *part, const double *u,
+                 int64_t nsteps, int64_t meas, int8_t *noop, int8_t *accept, double *ratio,
+                 double *HO, double *logw);
+
 #ifdef __cplusplus
 }
 #endif

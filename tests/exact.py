@@ -83,3 +83,28 @@ def flip_flop_expectations(confs, psi):
             beta2 = order[np.searchsorted(ms, mb2)]
             smp[a, b] = np.sum(np.conj(psi[sel2]) * psi[beta2]) / Z
     return szz, spm, smp
+
+
+def configurations_n(L, nup):
+    """All configurations of L*L sites with nup particles of flavour 0 (spins[site])."""
+    Ns = L * L
+    confs = []
+    for ups in itertools.combinations(range(Ns), nup):
+        s = np.ones(Ns, np.int32)
+        s[list(ups)] = 0
+        confs.append(s)
+    return np.array(confs)
+
+
+def spin_basis_psi_orbitals(confs, table, orbs_up, orbs_dn):
+    """Site-basis amplitude of the Slater state with the given occupied orbitals
+    (second-quantised, creation operators in increasing orbital order per flavour, up
+    first): psi(alpha) = det M, M[i, (fl, p)] = delta(s_i, fl) table[fl][orb_p][i]."""
+    nconf, Ns = confs.shape
+    nu = len(orbs_up)
+    M = np.zeros((nconf, Ns, Ns), np.complex128)
+    for i in range(Ns):
+        up = confs[:, i] == 0
+        M[up, i, :nu] = table[0][orbs_up, i][None, :]
+        M[~up, i, nu:] = table[1][orbs_dn, i][None, :]
+    return np.linalg.det(M)

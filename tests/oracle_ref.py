@@ -45,6 +45,14 @@ def lib():
         L.or_eloc.argtypes = [ctypes.c_int, _c128p, _i32p, _i32p, _i32p, _c128p, _f64p]
         L.or_stagmagnz.argtypes = [ctypes.c_int, _i32p, _f64p]
         L.or_statspinstruct.argtypes = [ctypes.c_int, _c128p, _i32p, _i32p, _c128p, _c128p]
+        L.or_exc_amps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c128p, ctypes.c_int,
+                                  _i32p, _i32p, _f64p, _f64p]
+        L.or_exc_local.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c128p, ctypes.c_int,
+                                   _i32p, _i32p, _i32p, _c128p, _c128p]
+        L.or_exc_local.restype = ctypes.c_double
+        L.or_exc_walk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c128p, ctypes.c_int,
+                                  _i32p, _i32p, _i32p, _i32p, _f64p, ctypes.c_int64, ctypes.c_int64,
+                                  dp, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -175,3 +183,62 @@ def glibc_uniforms(seed, n):
     libc.srand(ctypes.c_uint(seed & 0xFFFFFFFF))
     libc.rand.restype = ctypes.c_int
     return np.array([libc.rand() for _ in range(n)], np.float64) / 2147483647.0
+
+
+class ExcWalker:
+    """One chain on a channel in the oracle (exact amplitudes by LU at every step).
+
+    table: full orbital table [2][Ns][Ns]; ch: gpvmc_amd.channels.Channel; spins[site] in
+    {0, 1} with ch.npt particles of each flavour.  Labels in increasing site order."""
+
+    def __init__(self, L, table, ch, spins):
+        self.L, self.Ns = L, L * L
+        self.nup, self.ndn = ch.npt
+        self.nk = ch.nexc
+        self.tab = np.ascontiguousarray(table, np.complex128)
+        self.orbs = np.ascontiguousarray(
+            np.array([np.concatenate([u, d]) for u, d in ch.states], np.int32))
+        spins = np.asarray(spins)
+        self.site = np.zeros(self.Ns, np.int32)
+        self.part = np.zeros(self.nup + self.ndn, np.int32)
+        cnt = [0, 0]
+        for v in range(self.Ns):
+            fl = int(spins[v])
+            self.site[v] = (fl << 16) | cnt[fl]
+            self.part[cnt[fl] + (self.nup if fl else 0)] = v
+            cnt[fl] += 1
+        assert cnt == [self.nup, self.ndn]
+        self.nbr = neighbours(L)
+
+    def spins(self):
+        return (self.site >> 16).astype(np.int32)
+
+    def amps(self, part=None):
+        la = np.zeros(self.nk)
+        ph = np.zeros(self.nk)
+        p = self.part if part is None else np.ascontiguousarray(part, np.int32)
+        lib().or_exc_amps(self.Ns, self.nup, self.ndn, self.tab, self.nk, self.orbs, p, la, ph)
+        return la, ph
+
+    def local(self):
+        """(a_k, h_k, w) at the current configuration, scaled so that max |a_k| = 1."""
+        a = np.zeros(self.nk, np.complex128)
+        h = np.zeros(self.nk, np.complex128)
+        w = lib().or_exc_local(self.L, self.nup, self.ndn, self.tab, self.nk, self.orbs,
+                               self.site, self.part, a, h)
+        return a, h, w
+
+    def walk(self, u, meas=0, HO=None):
+        u = np.ascontiguousarray(u, np.float64)
+        n = len(u) // 4
+        noop = np.zeros(n, np.int8)
+        acc = np.zeros(n, np.int8)
+        rat = np.zeros(n, np.float64)
+        nm = 0 if meas <= 0 else (n + meas - 1) // meas if meas > 1 else n
+        if HO is None:
+            HO = np.zeros((2 * self.nk, self.nk), np.complex128)
+        logw = np.zeros(max(nm, 1), np.float64)
+        got = lib().or_exc_walk(self.L, self.nup, self.ndn, self.tab, self.nk, self.orbs,
+                                self.nbr, self.site, self.part, u, n, meas, noop.ctypes.data,
+                                acc.ctypes.data, rat.ctypes.data, HO.ctypes.data, logw.ctypes.data)
+        return dict(noop=noop, accept=acc, ratio=rat, HO=HO, logw=logw[:got], n_meas=got)
