@@ -37,6 +37,9 @@ SIGNATURES = {
     "gpvmc_read_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gpvmc_read_inverse": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_ratio_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_create_channel": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "gpvmc_set_orbital_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_read_channel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "gpvmc_reset_kernel_time": (ctypes.c_int, [_vp]),
 }
@@ -51,6 +54,12 @@ class _Config(ctypes.Structure):
                 ("n_groups", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("lanes_per_row", ctypes.c_int32),
                 ("seed", ctypes.c_uint64)]
+
+
+class _Channel(ctypes.Structure):
+    _fields_ = [("n_up", ctypes.c_int32), ("n_dn", ctypes.c_int32), ("nk", ctypes.c_int32),
+                ("state_orb", ctypes.c_void_p), ("hop_n", ctypes.c_void_p),
+                ("hop_orb", ctypes.c_void_p), ("hopsign", ctypes.c_void_p)]
 
 
 def library_path():
@@ -216,3 +225,73 @@ class Batch:
 
     def reset_kernel_time(self):
         self._check(self.lib.gpvmc_reset_kernel_time(self.h))
+
+
+class ChannelBatch(Batch):
+    """A batch of walkers on a channel (gpvmc_amd.channels.Channel): the weight is
+    sum_k |a_k|^2 over the channel's states, ProjHeis accumulates H and O matrices."""
+
+    def __init__(self, L, n_walkers, channel, precision=FP64, n_groups=1, seed=0, device=0):
+        self.lib = load_library()
+        self.L, self.Ns = L, L * L
+        self.ch = channel
+        self.n_up, self.n_dn = channel.npt
+        self.N = max(channel.npt)
+        self.nk = channel.nexc
+        self.W, self.G = n_walkers, n_groups
+        self.precision = precision
+        nk = self.nk
+        self._state_orb = np.ascontiguousarray(
+            np.array([np.concatenate([u, d]) for u, d in channel.states], np.int32))
+        self._hop_n = np.zeros((nk, nk, 2), np.int32)
+        self._hop_orb = np.zeros((nk, nk, 2, 2, 2), np.int32)
+        for c in range(nk):
+            for k in range(nk):
+                for fl in range(2):
+                    path = channel.hops[c][k][fl]
+                    self._hop_n[c, k, fl] = len(path)
+                    for h, (a, b) in enumerate(path):
+                        self._hop_orb[c, k, fl, h] = (a, b)
+        self._hopsign = np.ascontiguousarray(channel.hopsign.astype(np.int32))
+        chs = _Channel(self.n_up, self.n_dn, nk, _ptr(self._state_orb), _ptr(self._hop_n),
+                       _ptr(self._hop_orb), _ptr(self._hopsign))
+        cfg = _Config(L, n_walkers, n_groups, precision, device, 0, seed)
+        h = ctypes.c_void_p()
+        self._check(self.lib.gpvmc_create_channel(ctypes.byref(cfg), ctypes.byref(chs), ctypes.byref(h)))
+        self.h = h
+
+    def set_orbital_table(self, group, table):
+        t = np.ascontiguousarray(table, np.complex128)
+        assert t.shape == (2, self.Ns, self.Ns)
+        self._check(self.lib.gpvmc_set_orbital_table(self.h, group, _ptr(t)))
+
+    def walk(self, n_steps, meas=0, mask=MEAS_PROJHEIS, uniforms=None, trace=False):
+        u = None
+        if uniforms is not None:
+            u = np.ascontiguousarray(uniforms, np.float64).reshape(self.W, n_steps, 4)
+        out = {}
+        if trace:
+            nm = 0 if meas <= 0 else (n_steps if meas == 1 else (n_steps + meas - 1) // meas)
+            out = dict(noop=np.zeros((self.W, n_steps), np.int8),
+                       accept=np.zeros((self.W, n_steps), np.int8),
+                       ratio=np.zeros((self.W, n_steps)),
+                       logw=np.zeros((self.W, max(nm, 1))))
+        g = out.get
+        self._check(self.lib.gpvmc_walk(self.h, n_steps, meas, mask, _ptr(u), _ptr(g("noop")),
+                                        _ptr(g("accept")), _ptr(g("ratio")), None, None,
+                                        _ptr(g("logw"))))
+        return out
+
+    def measure(self, mask=MEAS_PROJHEIS):
+        lw = np.zeros(self.W)
+        self._check(self.lib.gpvmc_measure(self.h, mask, None, None, _ptr(lw)))
+        return dict(logw=lw)
+
+    def channel_sums(self, w0=0, nw=None):
+        """(ho[nw][2 nk][nk] complex, current state[nw], weight sum_k |a_k / a_c|^2 [nw])"""
+        nw = self.W - w0 if nw is None else nw
+        ho = np.zeros((nw, 2 * self.nk, self.nk), np.complex128)
+        cs = np.zeros(nw, np.int32)
+        wt = np.zeros(nw)
+        self._check(self.lib.gpvmc_read_channel(self.h, w0, nw, _ptr(ho), _ptr(cs), _ptr(wt)))
+        return ho, cs, wt

@@ -22,7 +22,7 @@ HIP_FLAGS = [
 
 
 def sources():
-    return [os.path.join(CSRC, f) for f in ("vmc_kernels.hip", "gpvmc_api.cpp")]
+    return [os.path.join(CSRC, f) for f in ("vmc_kernels.hip", "exc_kernels.hip", "gpvmc_api.cpp")]
 
 
 def build(verbose=False):

@@ -1,0 +1,779 @@
+// gfx950 kernels of the channel (exciton) walk: one workgroup per walker, runtime sizes.
+//
+// A channel is a set of N_k Slater states k; the walker's weight is w = sum_k |a_k|^2
+// (LatticeStepper::trystep, LatticeStepper.cpp:81-95).  Each walker keeps the inverse
+// Ai_fl of its "current" state c (the dominant state after the last accepted move, as
+// LatticeStepper::step hops the wavefunction to argmax_k |a_k|^2, :102-113) in the slot
+// arrangement the reference keeps (WaveFunction::Hop, State::Hop: an orbital replacement
+// keeps its slot).  Rows of Ai are lattice particles (labels), columns are slots.
+//
+// Ratios (SlaterDeterminant::VirtUpdate, SlaterDeterminant.cpp:96-268, and
+// linalg::DetUpdate, linalg.cpp:107-400).  A move of particle r to site s together with
+// the replacement of the orbitals in slots f_m by o'_m (the hop path from c to k) changes
+// one column and R rows of A; by the matrix determinant lemma the amplitude ratio is the
+// determinant of the (1+R) x (1+R) capacitance matrix B with, for c = Phi(orbitals, s),
+// y = Ai c, rho_m(r') = Phi(o'_m, site r'), and
+// D_m = Phi(o'_m, s) - Phi(o'_m, site r) - Phi(o_m, s) + Phi(o_m, site r):
+//   B00 = y[r],  B0n = Ai[r][f_n],
+//   Bm0 = rho_m . y + D_m y[r] - Phi(o'_m, s),  Bmn = (rho_m Ai)[f_n] + D_m Ai[r][f_n].
+// The crossing terms use the current A, so the ratios are exact (the reference uses the
+// A of its last Init there, linalg.cpp:196,320,457).  a_k(x') / a_c(x) is the product of
+// both flavours' determinants times HopSign(c -> k).  Without a lattice move B is the
+// R x R block (rho_m Ai)[f_n].
+//
+// Accepted move to state k*: Woodbury, Ai <- Ai - U B^-1 V with U = [y - e_r, Ai[:, f_m]],
+// V = [Ai[r, :]; (rho_m Ai) + D_m Ai[r, :] - e_{f_m}] (InvUpdate, linalg.cpp:402-573).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vmc_kernels.h"
+
+namespace gx {
+
+using gv::ExcArgs;
+
+template <typename T> struct cpx;
+template <> struct cpx<float> { using t = float2; };
+template <> struct cpx<double> { using t = double2; };
+
+struct zc {  // FP64 complex for the small determinants
+    double x, y;
+};
+__device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// conj(a) * b
+__device__ __forceinline__ zc zconjmul(zc a, zc b) { return zmk(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+__device__ __forceinline__ double znorm(zc a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ zc zdiv(zc a, zc b)
+{
+    const double n = znorm(b);
+    return zmk((a.x * b.x + a.y * b.y) / n, (a.y * b.x - a.x * b.y) / n);
+}
+template <typename C> __device__ __forceinline__ zc toz(C v) { return zmk((double)v.x, (double)v.y); }
+
+__device__ zc det_small(int n, const zc *B)  // n <= 3, row-major n x n
+{
+    if (n == 0) return zmk(1.0, 0.0);
+    if (n == 1) return B[0];
+    if (n == 2) return zsub(zmul(B[0], B[3]), zmul(B[1], B[2]));
+    zc t0 = zsub(zmul(B[4], B[8]), zmul(B[5], B[7]));
+    zc t1 = zsub(zmul(B[3], B[8]), zmul(B[5], B[6]));
+    zc t2 = zsub(zmul(B[3], B[7]), zmul(B[4], B[6]));
+    return zadd(zsub(zmul(B[0], t0), zmul(B[1], t1)), zmul(B[2], t2));
+}
+
+// inverse of an n x n (n <= 3) complex matrix by the adjugate
+__device__ void inv_small(int n, const zc *B, zc *Bi)
+{
+    const zc d = det_small(n, B);
+    if (n == 1) { Bi[0] = zdiv(zmk(1.0, 0.0), d); return; }
+    if (n == 2) {
+        Bi[0] = zdiv(B[3], d);
+        Bi[1] = zdiv(zmk(-B[1].x, -B[1].y), d);
+        Bi[2] = zdiv(zmk(-B[2].x, -B[2].y), d);
+        Bi[3] = zdiv(B[0], d);
+        return;
+    }
+    auto m = [&](int r, int c) { return B[r * 3 + c]; };
+    auto cof = [&](int r0, int r1, int c0, int c1) {
+        return zsub(zmul(m(r0, c0), m(r1, c1)), zmul(m(r0, c1), m(r1, c0)));
+    };
+    // Bi[i][j] = C[j][i] / d
+    zc C[9];
+    C[0] = cof(1, 2, 1, 2);
+    C[1] = zsub(zmk(0, 0), cof(1, 2, 0, 2));
+    C[2] = cof(1, 2, 0, 1);
+    C[3] = zsub(zmk(0, 0), cof(0, 2, 1, 2));
+    C[4] = cof(0, 2, 0, 2);
+    C[5] = zsub(zmk(0, 0), cof(0, 2, 0, 1));
+    C[6] = cof(0, 1, 1, 2);
+    C[7] = zsub(zmk(0, 0), cof(0, 1, 0, 2));
+    C[8] = cof(0, 1, 0, 1);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bi[i * 3 + j] = zdiv(C[j * 3 + i], d);
+}
+
+// LDS layout of one walker (dynamic shared memory; offsets computed by exc_lds_layout)
+struct Lds {
+    int *site;      // [Ns]   (flavour << 16) | label
+    int *part;      // [2][nmax] site of each label
+    int *cur;       // [2][nmax] orbital in each slot of the current state
+    int *slot;      // [2][Ns] slot of each orbital or -1
+    double *vc;     // [2][nmax] complex: c = Phi(cur orbitals, target site)
+    double *vy;     // [2][nmax] complex: y = Ai c
+    double *kg;     // [2][nk] complex: rho_m . y per new orbital of the current state
+    double *ra;     // [nk][2][2][2] complex: (rho_m Ai)[f_n] blocks of every state
+    double *alpha;  // [nk] complex: a_k(x') / a_c(x)
+    double *dets;   // [nk][2] complex: per-flavour determinant
+    double *a0;     // [nk] complex: amplitudes at the current configuration
+    double *hk;     // [nk] complex: ProjHeis h_k
+    double *U;      // [2][3][nmax] complex
+    double *V;      // [2][3][nmax] complex
+    double *red;    // [16]
+    int *ired;      // [16]
+};
+
+__host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk)
+{
+    size_t b = 0;
+    b += (size_t)Ns * 4 + 2 * (size_t)nmax * 4 * 2 + 2 * (size_t)Ns * 4;
+    b = (b + 15) & ~(size_t)15;
+    b += 16 * ((size_t)2 * nmax * 2 + (size_t)2 * nk + (size_t)nk * 8 + nk + (size_t)nk * 2 + nk + nk +
+               (size_t)2 * 3 * nmax * 2);
+    b += 16 * 8 + 16 * 4;
+    return b;
+}
+
+__device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk)
+{
+    Lds s;
+    int *ip = reinterpret_cast<int *>(base);
+    s.site = ip; ip += Ns;
+    s.part = ip; ip += 2 * nmax;
+    s.cur = ip; ip += 2 * nmax;
+    s.slot = ip; ip += 2 * Ns;
+    size_t off = (reinterpret_cast<char *>(ip) - base + 15) & ~(size_t)15;
+    double *dp = reinterpret_cast<double *>(base + off);
+    s.vc = dp; dp += 2 * (2 * nmax);
+    s.vy = dp; dp += 2 * (2 * nmax);
+    s.kg = dp; dp += 2 * (2 * nk);
+    s.ra = dp; dp += 2 * (nk * 8);
+    s.alpha = dp; dp += 2 * nk;
+    s.dets = dp; dp += 2 * (2 * nk);
+    s.a0 = dp; dp += 2 * nk;
+    s.hk = dp; dp += 2 * nk;
+    s.U = dp; dp += 2 * (2 * 3 * nmax);
+    s.V = dp; dp += 2 * (2 * 3 * nmax);
+    s.red = dp; dp += 16;
+    s.ired = reinterpret_cast<int *>(dp);
+    return s;
+}
+
+template <typename T> struct Walker {
+    using C = typename cpx<T>::t;
+    const ExcArgs &a;
+    Lds s;
+    int w, tid, nt, Ns, n[2], nmax, nk;
+    C *Ai[2];
+    const C *tab[2];
+
+    __device__ Walker(const ExcArgs &args, char *base, int wk)
+        : a(args), w(wk), tid(threadIdx.x), nt(blockDim.x)
+    {
+        Ns = a.L * a.L;
+        n[0] = a.n_up;
+        n[1] = a.n_dn;
+        nmax = n[0] > n[1] ? n[0] : n[1];
+        nk = a.nk;
+        s = lds_carve(base, Ns, nmax, nk);
+        for (int fl = 0; fl < 2; ++fl) {
+            Ai[fl] = reinterpret_cast<C *>(a.Ai) + ((size_t)w * 2 + fl) * nmax * nmax;
+            tab[fl] = reinterpret_cast<const C *>(a.tab) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
+        }
+    }
+    __device__ zc phi(int fl, int orb, int site) const { return toz(tab[fl][(size_t)orb * Ns + site]); }
+    __device__ zc ai(int fl, int r, int f) const { return toz(Ai[fl][(size_t)r * nmax + f]); }
+    __device__ zc lz(const double *p, int i) const { return zmk(p[2 * i], p[2 * i + 1]); }
+    __device__ void sz(double *p, int i, zc v) const { p[2 * i] = v.x; p[2 * i + 1] = v.y; }
+
+    __device__ int cstate() const { return a.cstate[w]; }
+    __device__ int hop_n(int c, int k, int fl) const { return a.hop_n[((size_t)c * nk + k) * 2 + fl]; }
+    __device__ int hop_from(int c, int k, int fl, int h) const
+    {
+        return a.hop_orb[((((size_t)c * nk + k) * 2 + fl) * 2 + h) * 2];
+    }
+    __device__ int hop_to(int c, int k, int fl, int h) const
+    {
+        return a.hop_orb[((((size_t)c * nk + k) * 2 + fl) * 2 + h) * 2 + 1];
+    }
+    __device__ int hop_new(int c, int k, int fl, int h) const  // index into newo[c][fl]
+    {
+        return a.hop_new[(((size_t)c * nk + k) * 2 + fl) * 2 + h];
+    }
+    __device__ int n_new(int c, int fl) const { return a.new_off[((size_t)c * 2 + fl) + 1] - a.new_off[(size_t)c * 2 + fl]; }
+    __device__ int new_orb(int c, int fl, int m) const { return a.new_orb[a.new_off[(size_t)c * 2 + fl] + m]; }
+
+    // (rho_o Ai)[f] = sum_r' Phi(o, site r') Ai[r'][f]
+    __device__ zc rho_ai(int fl, int o, int f) const
+    {
+        zc t = zmk(0, 0);
+        for (int r = 0; r < n[fl]; ++r) t = zadd(t, zmul(phi(fl, o, s.part[fl * nmax + r]), ai(fl, r, f)));
+        return t;
+    }
+
+    // RA blocks of every state relative to the current state (after any change of Ai or
+    // of the current state)
+    __device__ void build_ra()
+    {
+        const int c = cstate();
+        for (int e = tid; e < nk * 8; e += nt) {
+            const int k = e / 8, fl = (e / 4) & 1, m = (e / 2) & 1, nn = e & 1;
+            zc v = zmk(0, 0);
+            if (m < hop_n(c, k, fl) && nn < hop_n(c, k, fl))
+                v = rho_ai(fl, hop_to(c, k, fl, m), s.slot[fl * Ns + hop_from(c, k, fl, nn)]);
+            sz(s.ra, e, v);
+        }
+        __syncthreads();
+    }
+
+    // Ratios of every state for the swap moving up particle ci to site sb and down
+    // particle cj to site sa (mv = true), or at the current configuration (mv = false):
+    // fills alpha[k] and dets[k][fl]; leaves c, y, kg in LDS for apply().
+    __device__ void eval(bool mv, int ci, int sa, int sb, int cj)
+    {
+        const int c = cstate();
+        if (mv) {
+            for (int fl = 0; fl < 2; ++fl) {
+                const int tgt = fl ? sa : sb;
+                for (int f = tid; f < n[fl]; f += nt)
+                    sz(s.vc, fl * nmax + f, phi(fl, s.cur[fl * nmax + f], tgt));
+            }
+            __syncthreads();
+            for (int e = tid; e < n[0] + n[1]; e += nt) {
+                const int fl = e < n[0] ? 0 : 1, r = fl ? e - n[0] : e;
+                zc t = zmk(0, 0);
+                for (int f = 0; f < n[fl]; ++f) t = zadd(t, zmul(ai(fl, r, f), lz(s.vc, fl * nmax + f)));
+                sz(s.vy, fl * nmax + r, t);
+            }
+            __syncthreads();
+            const int nn0 = n_new(c, 0), nn1 = n_new(c, 1);
+            for (int e = tid; e < nn0 + nn1; e += nt) {
+                const int fl = e < nn0 ? 0 : 1, m = fl ? e - nn0 : e;
+                const int o = new_orb(c, fl, m);
+                zc t = zmk(0, 0);
+                for (int r = 0; r < n[fl]; ++r)
+                    t = zadd(t, zmul(phi(fl, o, s.part[fl * nmax + r]), lz(s.vy, fl * nmax + r)));
+                sz(s.kg, fl * nk + m, t);
+            }
+            __syncthreads();
+        }
+        for (int k = tid; k < nk; k += nt) {
+            zc tot = zmk((double)a.hopsign[(size_t)c * nk + k], 0.0);
+            for (int fl = 0; fl < 2; ++fl) {
+                const int R = hop_n(c, k, fl);
+                zc B[9];
+                zc d;
+                if (!mv) {
+                    for (int m = 0; m < R; ++m)
+                        for (int q = 0; q < R; ++q) B[m * R + q] = lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q);
+                    d = det_small(R, B);
+                } else {
+                    const int r = fl ? cj : ci;
+                    const int sold = fl ? sb : sa, snew = fl ? sa : sb;
+                    const int D = R + 1;
+                    B[0] = lz(s.vy, fl * nmax + r);
+                    for (int q = 0; q < R; ++q)
+                        B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, k, fl, q)]);
+                    for (int m = 0; m < R; ++m) {
+                        const int o2 = hop_to(c, k, fl, m), o1 = hop_from(c, k, fl, m);
+                        const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
+                        const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
+                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nk + hop_new(c, k, fl, m)),
+                                                   zmul(Dm, B[0])), p2s);
+                        for (int q = 0; q < R; ++q)
+                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q),
+                                                          zmul(Dm, B[1 + q]));
+                    }
+                    d = det_small(D, B);
+                }
+                sz(s.dets, k * 2 + fl, d);
+                tot = zmul(tot, d);
+            }
+            sz(s.alpha, k, tot);
+        }
+        __syncthreads();
+    }
+
+    // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element)
+    __device__ void reduce_alpha(double &sum, int &kmax)
+    {
+        double ls = 0.0, bv = -1.0;
+        int bk = nk;
+        for (int k = tid; k < nk; k += nt) {
+            const double v = znorm(lz(s.alpha, k));
+            ls += v;
+            if (v > bv) { bv = v; bk = k; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ls += __shfl_xor(ls, o);
+            const double ov = __shfl_xor(bv, o);
+            const int ok = __shfl_xor(bk, o);
+            if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+        }
+        const int lane = tid & 63, wid = tid >> 6;
+        __syncthreads();
+        if (lane == 0) { s.red[wid] = ls; s.red[8 + wid] = bv; s.ired[wid] = bk; }
+        __syncthreads();
+        sum = 0.0;
+        bv = -1.0;
+        bk = nk;
+        for (int q = 0; q < (nt + 63) / 64; ++q) {
+            sum += s.red[q];
+            if (s.red[8 + q] > bv || (s.red[8 + q] == bv && s.ired[q] < bk)) { bv = s.red[8 + q]; bk = s.ired[q]; }
+        }
+        kmax = bk;
+        __syncthreads();
+    }
+
+    // Moves the walker to state kn after eval(mv, ...): Woodbury update of both inverses,
+    // slot bookkeeping, lattice hop.
+    __device__ void apply(bool mv, int ci, int sa, int sb, int cj, int kn)
+    {
+        const int c = cstate();
+        for (int fl = 0; fl < 2; ++fl) {
+            const int R = hop_n(c, kn, fl);
+            const int r = fl ? cj : ci;
+            const int sold = fl ? sb : sa, snew = fl ? sa : sb;
+            const int m0 = mv ? 1 : 0, D = R + m0;
+            if (D == 0) continue;
+            double *U = s.U + 2 * (fl * 3 * nmax), *Vv = s.V + 2 * (fl * 3 * nmax);
+            // U columns and V rows, from the old inverse
+            for (int i = tid; i < n[fl]; i += nt) {
+                if (mv) {
+                    zc y = lz(s.vy, fl * nmax + i);
+                    if (i == r) y.x -= 1.0;
+                    sz(U, i, y);
+                    sz(Vv, i, ai(fl, r, i));
+                }
+                for (int m = 0; m < R; ++m) {
+                    const int fm = s.slot[fl * Ns + hop_from(c, kn, fl, m)];
+                    sz(U, (m0 + m) * nmax + i, ai(fl, i, fm));
+                    const int o2 = hop_to(c, kn, fl, m), o1 = hop_from(c, kn, fl, m);
+                    zc v = rho_ai(fl, o2, i);
+                    if (mv) {
+                        const zc Dm = zadd(zsub(zsub(phi(fl, o2, snew), phi(fl, o2, sold)), phi(fl, o1, snew)),
+                                           phi(fl, o1, sold));
+                        v = zadd(v, zmul(Dm, ai(fl, r, i)));
+                    }
+                    if (i == fm) v.x -= 1.0;
+                    sz(Vv, (m0 + m) * nmax + i, v);
+                }
+            }
+        }
+        // capacitance matrices of state kn, rebuilt per flavour from the same inputs
+        __shared__ zc s_binv[2][9];
+        if (tid < 2) {
+            const int fl = tid;
+            const int R = hop_n(c, kn, fl);
+            const int D = R + (mv ? 1 : 0);
+            zc B[9];
+            if (D > 0) {
+                if (!mv) {
+                    for (int m = 0; m < R; ++m)
+                        for (int q = 0; q < R; ++q) B[m * R + q] = lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q);
+                } else {
+                    const int r = fl ? cj : ci;
+                    const int sold = fl ? sb : sa, snew = fl ? sa : sb;
+                    B[0] = lz(s.vy, fl * nmax + r);
+                    for (int q = 0; q < R; ++q) B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, kn, fl, q)]);
+                    for (int m = 0; m < R; ++m) {
+                        const int o2 = hop_to(c, kn, fl, m), o1 = hop_from(c, kn, fl, m);
+                        const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
+                        const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
+                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nk + hop_new(c, kn, fl, m)), zmul(Dm, B[0])), p2s);
+                        for (int q = 0; q < R; ++q)
+                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
+                    }
+                }
+                inv_small(D, B, s_binv[fl]);
+            }
+        }
+        __syncthreads();
+        // W = B^-1 V (D x n) into V in place is not safe; form per element:
+        // Ai[i][f] -= sum_a U[a][i] sum_b Binv[a][b] V[b][f]
+        for (int fl = 0; fl < 2; ++fl) {
+            const int R = hop_n(c, kn, fl);
+            const int D = R + (mv ? 1 : 0);
+            if (D == 0) continue;
+            const double *U = s.U + 2 * (fl * 3 * nmax), *Vv = s.V + 2 * (fl * 3 * nmax);
+            for (int e = tid; e < n[fl] * n[fl]; e += nt) {
+                const int i = e / n[fl], f = e - i * n[fl];
+                zc acc = zmk(0, 0);
+                for (int aa = 0; aa < D; ++aa) {
+                    zc wv = zmk(0, 0);
+                    for (int bb = 0; bb < D; ++bb) wv = zadd(wv, zmul(s_binv[fl][aa * D + bb], lz(Vv, bb * nmax + f)));
+                    acc = zadd(acc, zmul(lz(U, aa * nmax + i), wv));
+                }
+                C &el = Ai[fl][(size_t)i * nmax + f];
+                el.x = (T)((double)el.x - acc.x);
+                el.y = (T)((double)el.y - acc.y);
+            }
+        }
+        __syncthreads();
+        // amplitude of the new current state (per flavour) and the slot bookkeeping
+        if (tid == 0) {
+            for (int fl = 0; fl < 2; ++fl) {
+                const zc d = lz(s.dets, kn * 2 + fl);
+                a.logabs[2 * w + fl] += 0.5 * log(znorm(d));
+                a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
+            }
+            for (int fl = 0; fl < 2; ++fl) {
+                const int R = hop_n(c, kn, fl);
+                for (int m = 0; m < R; ++m) {
+                    const int o1 = hop_from(c, kn, fl, m), o2 = hop_to(c, kn, fl, m);
+                    const int fm = s.slot[fl * Ns + o1];
+                    s.slot[fl * Ns + o1] = -1;
+                    s.slot[fl * Ns + o2] = fm;
+                    s.cur[fl * nmax + fm] = o2;
+                }
+            }
+            if (mv) {
+                s.site[sb] = ci;
+                s.site[sa] = (1 << 16) | cj;
+                s.part[ci] = sb;
+                s.part[nmax + cj] = sa;
+            }
+            a.cstate[w] = kn;
+        }
+        __syncthreads();
+    }
+};
+
+// mode 0: walk n_steps; 1: measurement only (traces, no accumulators); 2: initial state
+// choice (the state of largest |a_k|^2 at the configuration, vmc.cpp:243-254) and weight
+// bookkeeping after an init or a refresh of the inverse
+template <typename T>
+__global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+    Walker<T> wk(a, smem, w);
+    const int tid = wk.tid, nt = wk.nt, Ns = wk.Ns, nmax = wk.nmax, nk = wk.nk;
+    auto &s = wk.s;
+    for (int i = tid; i < Ns; i += nt) {
+        s.site[i] = a.site[(size_t)w * Ns + i];
+        s.slot[i] = -1;
+        s.slot[Ns + i] = -1;
+    }
+    for (int i = tid; i < 2 * nmax; i += nt) {
+        s.part[i] = a.part[(size_t)w * 2 * nmax + i];
+        s.cur[i] = a.cur[(size_t)w * 2 * nmax + i];
+    }
+    __syncthreads();
+    for (int i = tid; i < 2 * nmax; i += nt) {
+        const int fl = i / nmax, f = i % nmax;
+        if (f < wk.n[fl]) s.slot[fl * Ns + s.cur[i]] = f;
+    }
+    __syncthreads();
+    wk.build_ra();
+    double S;  // sum_k |a_k / a_c|^2 at the current configuration
+    int kmax;
+    wk.eval(false, 0, 0, 0, 0);
+    wk.reduce_alpha(S, kmax);
+    if (a.mode == 2) {
+        if (kmax != wk.cstate() && kmax < nk) {
+            wk.apply(false, 0, 0, 0, 0, kmax);
+            wk.build_ra();
+            wk.eval(false, 0, 0, 0, 0);
+            wk.reduce_alpha(S, kmax);
+        }
+    }
+    int64_t n_noop = 0, n_acc = 0, n_rej = 0;
+    double sacc[6] = {0, 0, 0, 0, 0, 0};  // [0] n_meas, [1] sum ln w
+    const uint64_t ctr0 = a.counter[w];
+    int64_t mslot = 0;
+    auto measure = [&](bool acc_on) {
+        // ProjHeis::measure on the channel (ProjHeis.cpp:40-154)
+        wk.eval(false, 0, 0, 0, 0);
+        double S0;
+        int kk;
+        wk.reduce_alpha(S0, kk);
+        for (int k = tid; k < nk; k += nt) {
+            wk.sz(s.a0, k, wk.lz(s.alpha, k));
+            wk.sz(s.hk, k, zmk(0, 0));
+        }
+        __syncthreads();
+        for (int e = 0; e < 2 * Ns; ++e) {
+            const int4 eg = a.edge[e];
+            const int vi = eg.x, vj = eg.z;
+            const int ii = s.site[vi], ij = s.site[vj];
+            if ((ii >> 16) == (ij >> 16)) {
+                for (int k = tid; k < nk; k += nt) {
+                    zc h = wk.lz(s.hk, k), a0 = wk.lz(s.a0, k);
+                    wk.sz(s.hk, k, zmk(h.x + 0.25 * a0.x, h.y + 0.25 * a0.y));
+                }
+                __syncthreads();
+                continue;
+            }
+            int ci, cj, sa, sb;
+            if ((ii >> 16) == 0) { ci = ii & 0xFFFF; sa = vi; cj = ij & 0xFFFF; sb = vj; }
+            else { ci = ij & 0xFFFF; sa = vj; cj = ii & 0xFFFF; sb = vi; }
+            wk.eval(true, ci, sa, sb, cj);
+            for (int k = tid; k < nk; k += nt) {
+                zc h = wk.lz(s.hk, k), a0 = wk.lz(s.a0, k), ab = wk.lz(s.alpha, k);
+                wk.sz(s.hk, k, zmk(h.x - 0.25 * a0.x - 0.5 * ab.x, h.y - 0.25 * a0.y - 0.5 * ab.y));
+            }
+            __syncthreads();
+        }
+        const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
+        if (acc_on) {
+            double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
+            for (int e = tid; e < nk * nk; e += nt) {
+                const int k1 = e / nk, k2 = e % nk;
+                const zc a1 = wk.lz(s.a0, k1);
+                const zc h2 = wk.lz(s.hk, k2), a2 = wk.lz(s.a0, k2);
+                const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
+                HO[e].x += hv.x / S0;
+                HO[e].y += hv.y / S0;
+                HO[(size_t)nk * nk + e].x += ov.x / S0;
+                HO[(size_t)nk * nk + e].y += ov.y / S0;
+            }
+            if (tid == 0) { sacc[0] += 1; sacc[1] += lw; }
+        }
+        if (tid == 0 && a.tr_logw) a.tr_logw[(size_t)w * a.n_meas_slots + mslot] = lw;
+        ++mslot;
+        __syncthreads();
+    };
+    if (a.mode == 1) {
+        measure(false);
+    } else if (a.mode == 0) {
+        for (int64_t st = 0; st < a.n_steps; ++st) {
+            // proposal: 4 uniforms per step, as the ground-state kernel
+            int v1, ed;
+            double u3;
+            if (a.uniforms) {
+                const double *up = a.uniforms + ((size_t)w * a.n_steps + st) * 4;
+                v1 = (int)(Ns * up[0]);
+                v1 = v1 < Ns ? v1 : Ns - 1;
+                ed = (int)(4 * up[1]);
+                ed = ed < 4 ? ed : 3;
+                u3 = up[3];
+            } else {
+                const uint64_t sc = ctr0 + (uint64_t)st;
+                uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
+                gv::philox4x32_10_dev(cc, a.key0, a.key1);
+                v1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
+                ed = (int)(cc[1] >> 30);
+                u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
+            }
+            const int v2 = a.nbr[v1 * 4 + ed];
+            const int i1 = s.site[v1], i2 = s.site[v2];
+            const bool noop = (i1 >> 16) == (i2 >> 16);
+            double ratio = 1.0;
+            bool acc;
+            if (noop) {
+                acc = u3 < 1.0;
+                ++n_noop;
+            } else {
+                int ci, cj, sa, sb;
+                if ((i1 >> 16) == 0) { ci = i1 & 0xFFFF; sa = v1; cj = i2 & 0xFFFF; sb = v2; }
+                else { ci = i2 & 0xFFFF; sa = v2; cj = i1 & 0xFFFF; sb = v1; }
+                wk.eval(true, ci, sa, sb, cj);
+                double S2;
+                int kn;
+                wk.reduce_alpha(S2, kn);
+                ratio = S2 / S;
+                acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
+                if (acc) {
+                    const zc ak = wk.lz(s.alpha, kn);
+                    wk.apply(true, ci, sa, sb, cj, kn);
+                    S = S2 / znorm(ak);
+                    wk.build_ra();
+                    ++n_acc;
+                } else {
+                    ++n_rej;
+                }
+            }
+            if (tid == 0) {
+                const size_t ti = (size_t)w * a.n_steps + st;
+                if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
+                if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
+                if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+            }
+            if (a.meas > 0 && (a.meas == 1 || st % a.meas == 0)) measure(true);
+        }
+    }
+    // write back
+    __syncthreads();
+    for (int i = tid; i < Ns; i += nt) a.site[(size_t)w * Ns + i] = s.site[i];
+    for (int i = tid; i < 2 * nmax; i += nt) {
+        a.part[(size_t)w * 2 * nmax + i] = s.part[i];
+        a.cur[(size_t)w * 2 * nmax + i] = s.cur[i];
+    }
+    if (tid == 0) {
+        a.weight[w] = S;
+        if (a.mode == 0) {
+            a.counter[w] = ctr0 + (uint64_t)a.n_steps;
+            double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
+            ac[3] += sacc[0];
+            ac[7] += (double)a.n_steps;
+            ac[8] += (double)n_acc;
+            ac[9] += (double)n_rej;
+            ac[10] += (double)n_noop;
+            ac[11] += sacc[1];
+        }
+    }
+}
+
+// The current state's Slater matrices of every walker, A[fl][f][r] = Phi(cur orbital of
+// slot f, site of label r), into the FP64 working buffer of the streamed Gauss-Jordan.
+__global__ void __launch_bounds__(256) exc_build_kernel(ExcArgs a, double2 *work)
+{
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int Ns = a.L * a.L;
+    const int n = fl ? a.n_dn : a.n_up;
+    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
+    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
+    const int *cur = a.cur + (size_t)w * 2 * nmax + (size_t)fl * nmax;
+    const int *part = a.part + (size_t)w * 2 * nmax + (size_t)fl * nmax;
+    double2 *M = work + ((size_t)w * 2 + fl) * nmax * nmax;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int f = e / n, r = e % n;
+        M[(size_t)f * nmax + r] = tab[(size_t)cur[f] * Ns + part[r]];
+    }
+}
+
+// Runtime-size FP64 Gauss-Jordan with implicit partial pivoting on a prebuilt matrix
+// (the algorithm of gj_kernel: pivot = largest |re|+|im| among unpivoted rows, lowest row
+// on ties; linalg.cpp:9-44).  One block per (walker, flavour), one row per thread.
+__global__ void __launch_bounds__(256) gj_dyn_kernel(gv::GJDynArgs a)
+{
+    using C = double2;
+    extern __shared__ __align__(16) char smem[];
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int n = fl ? a.n_dn : a.n_up;
+    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    constexpr int NW = 4;
+    C *s_prow = reinterpret_cast<C *>(smem);                      // [2][NW][nmax]
+    unsigned long long *s_ckey = reinterpret_cast<unsigned long long *>(s_prow + 2 * NW * nmax);  // [2][NW]
+    int *s_perm = reinterpret_cast<int *>(s_ckey + 2 * NW);      // [nmax]
+    int *s_invp = s_perm + nmax;                                  // [nmax]
+    int *s_flag = s_invp + nmax;                                  // [1]
+    unsigned char *s_seen = reinterpret_cast<unsigned char *>(s_flag + 1);
+    const int row = tid;
+    const bool has_row = row < n;
+    for (int i = tid; i < nmax; i += 256) s_invp[i] = 0;
+    if (tid == 0) *s_flag = 0;
+    __syncthreads();
+    C *M = a.work + ((size_t)w * 2 + fl) * nmax * nmax;
+    bool pivoted = !has_row;
+    C dm = make_double2(1.0, 0.0);
+    int de = 0;
+    int par = 0;
+    for (int k = 0; k < n; ++k) {
+        const C bk = has_row ? M[(size_t)row * nmax + k] : make_double2(0.0, 0.0);
+        const float mag = (float)(fabs(bk.x) + fabs(bk.y));
+        unsigned long long key = 0ull;
+        if (!pivoted) key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long ok2 = __shfl_xor(key, o);
+            key = ok2 > key ? ok2 : key;
+        }
+        const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
+        if (has_row && !pivoted && key != 0ull && row == wrow)
+            for (int j = 0; j < n; ++j) s_prow[(par * NW + wid) * nmax + j] = M[(size_t)row * nmax + j];
+        if (lane == 0) s_ckey[par * NW + wid] = key;
+        __syncthreads();
+        int bw = 0;
+        unsigned long long bkey = s_ckey[par * NW];
+        for (int q = 1; q < NW; ++q) {
+            const unsigned long long v = s_ckey[par * NW + q];
+            if (v > bkey) { bkey = v; bw = q; }
+        }
+        const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
+        const C *prow = s_prow + (size_t)(par * NW + bw) * nmax;
+        const C piv = prow[k];
+        const double np = piv.x * piv.x + piv.y * piv.y;
+        C ainv;
+        if (bkey != 0ull && np > 0.0) {
+            ainv = make_double2(piv.x / np, -piv.y / np);
+        } else {
+            ainv = make_double2(0.0, 0.0);
+            if (tid == 0) *s_flag = 1;
+        }
+        if (tid == 0) {
+            s_perm[k] = (p >= 0 && p < n) ? p : 0;
+            if (p >= 0 && p < n) s_invp[p] = k;
+            const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
+            int e;
+            (void)frexp(fmax(fabs(re), fabs(im)), &e);
+            dm = make_double2(ldexp(re, -e), ldexp(im, -e));
+            de += e;
+        }
+        if (has_row) {
+            const bool me = (row == p);
+            const double bax = bk.x * ainv.x - bk.y * ainv.y, bay = bk.x * ainv.y + bk.y * ainv.x;
+            for (int j = 0; j < n; ++j) {
+                const C rj = prow[j];
+                C &m = M[(size_t)row * nmax + j];
+                if (me) {
+                    m = (j == k) ? ainv : make_double2(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
+                } else if (j == k) {
+                    m = make_double2(bax, bay);
+                } else {
+                    m.x = fma(-bax, rj.x, m.x);
+                    m.x = fma(bay, rj.y, m.x);
+                    m.y = fma(-bax, rj.y, m.y);
+                    m.y = fma(-bay, rj.x, m.y);
+                }
+            }
+            if (me) pivoted = true;
+        }
+        par ^= 1;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int sgn = 1;
+        for (int i = 0; i < n; ++i) s_seen[i] = 0;
+        for (int i = 0; i < n; ++i) {
+            if (s_seen[i]) continue;
+            int len = 0, j = i;
+            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
+            if ((len & 1) == 0) sgn = -sgn;
+        }
+        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
+        double ph = atan2(dm.y, dm.x);
+        if (sgn < 0) ph += M_PI;
+        a.logabs[2 * w + fl] = la;
+        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
+        a.status[2 * w + fl] = *s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
+    }
+    C *out = a.Ai64 + ((size_t)w * 2 + fl) * nmax * nmax;
+    float2 *out32 = a.Ai32 ? a.Ai32 + ((size_t)w * 2 + fl) * nmax * nmax : nullptr;
+    if (has_row) {
+        int orow = s_invp[row];
+        orow = orow < 0 ? 0 : (orow < n ? orow : n - 1);
+        for (int j = 0; j < n; ++j) {
+            int oc = s_perm[j];
+            oc = oc < 0 ? 0 : (oc < n ? oc : n - 1);
+            const C v = M[(size_t)row * nmax + j];
+            out[(size_t)orow * nmax + oc] = v;
+            if (out32) out32[(size_t)orow * nmax + oc] = make_float2((float)v.x, (float)v.y);
+        }
+    }
+}
+
+}  // namespace gx
+
+extern "C" hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st)
+{
+    const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
+    if (nmax > 256) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * 4 * nmax * sizeof(double2) + 2 * 4 * 8 + (size_t)2 * nmax * 4 + 16 + nmax + 16;
+    hipLaunchKernelGGL(gx::gj_dyn_kernel, dim3(2 * nwalkers), dim3(256), lds, st, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_t st)
+{
+    const int Ns = a->L * a->L;
+    const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
+    const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk) + 256;
+    if (lds > 160 * 1024 - 4096) return hipErrorInvalidValue;
+    if (prec == 0)
+        hipLaunchKernelGGL(gx::exc_kernel<double>, dim3(nblocks), dim3(256), lds, st, *a);
+    else
+        hipLaunchKernelGGL(gx::exc_kernel<float>, dim3(nblocks), dim3(256), lds, st, *a);
+    return hipGetLastError();
+}
+
+extern "C" size_t gv_exc_lds_bytes(int Ns, int nmax, int nk) { return gx::exc_lds_bytes(Ns, nmax, nk) + 256; }
+
+extern "C" hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st)
+{
+    hipLaunchKernelGGL(gx::exc_build_kernel, dim3(2 * nwalkers), dim3(256), 0, st, *a, work);
+    return hipGetLastError();
+}

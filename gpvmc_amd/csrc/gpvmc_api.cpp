@@ -99,6 +99,18 @@ struct gpvmc_ctx {
     double *d_ss = nullptr;
     // working matrices of the streamed Gauss-Jordan variant (large N only)
     double2 *d_gjwork = nullptr;
+    // channel (exciton) contexts: N is the larger flavour size, part is [W][2][N]
+    bool channel = false;
+    int n_up = 0, n_dn = 0, nk = 0;
+    int32_t *d_cur = nullptr, *d_cstate = nullptr;
+    double *d_weight = nullptr, *d_ho = nullptr;
+    double2 *d_tab64 = nullptr;
+    float2 *d_tab32 = nullptr;
+    float2 *d_Ai32 = nullptr;
+    double2 *d_work = nullptr;
+    int32_t *d_hop_n = nullptr, *d_hop_orb = nullptr, *d_hop_new = nullptr, *d_new_off = nullptr,
+            *d_new_orb = nullptr, *d_hopsign = nullptr;
+    std::vector<int32_t> state_orb;
 };
 
 static void free_ctx(gpvmc_ctx *c)
@@ -106,7 +118,10 @@ static void free_ctx(gpvmc_ctx *c)
     if (!c) return;
     void *ptrs[] = {c->d_G, c->d_Ai64, c->d_site, c->d_part, c->d_group, c->d_order, c->d_nbr,
                     c->d_status, c->d_edge, c->d_logabs, c->d_phase, c->d_acc, c->d_drift,
-                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork};
+                    c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork,
+                    c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
+                    c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
+                    c->d_hopsign};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -161,6 +176,69 @@ static int clear_statspin(gpvmc_ctx *c)
     return GPVMC_OK;
 }
 
+static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
+{
+    gv::ExcArgs a{};
+    a.Ai = c->prec == GPVMC_FP64 ? (void *)c->d_Ai64 : (void *)c->d_Ai32;
+    a.site = c->d_site;
+    a.part = c->d_part;
+    a.cur = c->d_cur;
+    a.cstate = c->d_cstate;
+    a.weight = c->d_weight;
+    a.logabs = c->d_logabs;
+    a.phase = c->d_phase;
+    a.counter = c->d_counter;
+    a.acc = c->d_acc;
+    a.ho = c->d_ho;
+    a.tab = c->prec == GPVMC_FP64 ? (const void *)c->d_tab64 : (const void *)c->d_tab32;
+    a.tab64 = c->d_tab64;
+    a.group = c->d_group;
+    a.order = c->d_order;
+    a.nbr = c->d_nbr;
+    a.edge = c->d_edge;
+    a.hop_n = c->d_hop_n;
+    a.hop_orb = c->d_hop_orb;
+    a.hop_new = c->d_hop_new;
+    a.new_off = c->d_new_off;
+    a.new_orb = c->d_new_orb;
+    a.hopsign = c->d_hopsign;
+    a.key0 = (uint32_t)c->cfg.seed;
+    a.key1 = (uint32_t)(c->cfg.seed >> 32);
+    a.L = c->L;
+    a.n_up = c->n_up;
+    a.n_dn = c->n_dn;
+    a.nk = c->nk;
+    a.mode = mode;
+    return a;
+}
+
+// channel contexts: FP64 Gauss-Jordan of the current state's matrices, then the weight
+// bookkeeping (mode 2: also the choice of the dominant state, mode 3: weight only)
+static int run_refresh_channel(gpvmc_ctx *c, int mode)
+{
+    gv::ExcArgs a = exc_args(c, mode);
+    gv::GJDynArgs g{};
+    g.work = c->d_work;
+    g.Ai64 = c->d_Ai64;
+    g.Ai32 = c->prec == GPVMC_MIXED ? c->d_Ai32 : nullptr;
+    g.logabs = c->d_logabs;
+    g.phase = c->d_phase;
+    g.status = c->d_status;
+    g.n_up = c->n_up;
+    g.n_dn = c->n_dn;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(gv_launch_exc_build(&a, c->d_work, c->W, c->stream));
+    HIPCHK(gv_launch_gj_dyn(&g, c->W, c->stream));
+    HIPCHK(gv_launch_exc(c->prec, &a, c->W, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->refresh_ms += ms;
+    c->n_refresh += 1;
+    return GPVMC_OK;
+}
+
 // FP64 Gauss-Jordan inverse of every walker's Slater matrices, then the ratio tables
 // G = Ai Phi_other in FP64, stored in the working precision.
 static int run_refresh(gpvmc_ctx *c, double *drift_dev)
@@ -193,6 +271,111 @@ static int run_refresh(gpvmc_ctx *c, double *drift_dev)
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->refresh_ms += ms;
     c->n_refresh += 1;
+    return GPVMC_OK;
+}
+
+static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
+{
+    if (!spins) return fail(GPVMC_ERR_ARG, "null spins");
+    for (int w = 0; w < c->W; ++w)
+        if (!c->have_orb[c->groups[w]])
+            return fail(GPVMC_ERR_STATE, "orbital table of group " + std::to_string(c->groups[w]) + " not set");
+    const int Ns = c->Ns, nm = c->N, W = c->W;
+    const int npt[2] = {c->n_up, c->n_dn};
+    std::vector<int32_t> site((size_t)W * Ns), part((size_t)W * 2 * nm, 0), cur((size_t)W * 2 * nm, 0);
+    for (int w = 0; w < W; ++w) {
+        int cnt[2] = {0, 0};
+        for (int v = 0; v < Ns; ++v) {
+            const int fl = spins[(size_t)w * Ns + v];
+            if (fl != 0 && fl != 1) return fail(GPVMC_ERR_ARG, "spins must be 0 or 1");
+            if (cnt[fl] >= npt[fl])
+                return fail(GPVMC_ERR_ARG, "walker " + std::to_string(w) + " has the wrong number of particles");
+            site[(size_t)w * Ns + v] = (fl << 16) | cnt[fl];  // State::InitFock order
+            part[(size_t)w * 2 * nm + (size_t)fl * nm + cnt[fl]] = v;
+            cnt[fl]++;
+        }
+        // the current state starts as channel state 0, slots in increasing orbital order
+        for (int fl = 0; fl < 2; ++fl)
+            for (int f = 0; f < npt[fl]; ++f)
+                cur[(size_t)w * 2 * nm + (size_t)fl * nm + f] = c->state_orb[(fl ? c->n_up : 0) + f];
+    }
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipMemcpy(c->d_site, site.data(), site.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_cur, cur.data(), cur.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(c->d_cstate, 0, (size_t)W * sizeof(int32_t), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_ho, 0, (size_t)W * 4 * c->nk * c->nk * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
+    int rc = run_refresh_channel(c, 2);
+    if (rc) return rc;
+    std::vector<int32_t> st((size_t)W * 2);
+    HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (ok)
+        for (int w = 0; w < W; ++w) ok[w] = (st[2 * w] == 0 && st[2 * w + 1] == 0) ? 1 : 0;
+    c->initialised = true;
+    return GPVMC_OK;
+}
+
+static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask, const double *uniforms,
+                        int8_t *noop, int8_t *accept, double *ratio, double *eloc, double *mz, double *logw,
+                        int measure_only)
+{
+    (void)mask;
+    if (eloc || mz) return fail(GPVMC_ERR_UNSUPPORTED, "channel contexts accumulate ProjHeis H and O "
+                                                      "(gpvmc_read_channel); no per-measurement E_loc or m_z");
+    const size_t W = c->W;
+    const int64_t nmeas = measure_only ? 1 : meas > 0 ? (meas == 1 ? n_steps : (n_steps + meas - 1) / meas) : 0;
+    if (uniforms) {
+        const size_t n = W * (size_t)n_steps * 4;
+        for (size_t i = 0; i < n; ++i)
+            if (!(uniforms[i] >= 0.0 && uniforms[i] < 1.0))
+                return fail(GPVMC_ERR_ARG, "replay uniforms must lie in [0, 1)");
+    }
+    size_t off = 0, o_u = 0, o_no = 0, o_ac = 0, o_ra = 0, o_lw = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    if (uniforms) o_u = take(W * n_steps * 4 * sizeof(double));
+    if (noop) o_no = take(W * n_steps);
+    if (accept) o_ac = take(W * n_steps);
+    if (ratio) o_ra = take(W * n_steps * sizeof(double));
+    if (logw && nmeas) o_lw = take(W * nmeas * sizeof(double));
+    HIPCHK(hipSetDevice(c->cfg.device));
+    if (off) {
+        int rc = ensure_scratch(c, off);
+        if (rc) return rc;
+    }
+    char *sb = static_cast<char *>(c->d_scratch);
+    gv::ExcArgs a = exc_args(c, measure_only ? 1 : 0);
+    a.n_steps = measure_only ? 0 : n_steps;
+    a.meas = meas;
+    a.n_meas_slots = nmeas;
+    if (uniforms) {
+        a.uniforms = reinterpret_cast<const double *>(sb + o_u);
+        HIPCHK(hipMemcpyAsync(sb + o_u, uniforms, W * n_steps * 4 * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    if (noop) a.tr_noop = reinterpret_cast<int8_t *>(sb + o_no);
+    if (accept) a.tr_accept = reinterpret_cast<int8_t *>(sb + o_ac);
+    if (ratio) a.tr_ratio = reinterpret_cast<double *>(sb + o_ra);
+    if (logw && nmeas) a.tr_logw = reinterpret_cast<double *>(sb + o_lw);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(gv_launch_exc(c->prec, &a, c->W, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (!measure_only) {
+        c->sweep_ms += ms;
+        c->n_sweep += 1;
+    }
+    if (noop) HIPCHK(hipMemcpy(noop, a.tr_noop, W * n_steps, hipMemcpyDeviceToHost));
+    if (accept) HIPCHK(hipMemcpy(accept, a.tr_accept, W * n_steps, hipMemcpyDeviceToHost));
+    if (ratio) HIPCHK(hipMemcpy(ratio, a.tr_ratio, W * n_steps * sizeof(double), hipMemcpyDeviceToHost));
+    if (a.tr_logw) HIPCHK(hipMemcpy(logw, a.tr_logw, W * nmeas * sizeof(double), hipMemcpyDeviceToHost));
     return GPVMC_OK;
 }
 
@@ -292,6 +475,7 @@ int gpvmc_destroy(gpvmc_ctx *ctx)
 int gpvmc_set_orbitals(gpvmc_ctx *c, int32_t group, const double *occ)
 {
     if (!c || !occ) return fail(GPVMC_ERR_ARG, "null argument");
+    if (c->channel) return fail(GPVMC_ERR_STATE, "channel contexts take the full table (gpvmc_set_orbital_table)");
     if (group < 0 || group >= c->G) return fail(GPVMC_ERR_ARG, "group out of range");
     const size_t n = (size_t)2 * c->Ns * c->N;
     for (size_t i = 0; i < 2 * n; ++i)
@@ -320,6 +504,7 @@ int gpvmc_set_groups(gpvmc_ctx *c, const int32_t *g)
 
 int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
 {
+    if (c && c->channel) return init_channel(c, spins, ok);
     if (!c || !spins) return fail(GPVMC_ERR_ARG, "null argument");
     for (int w = 0; w < c->W; ++w)
         if (!c->have_orb[c->groups[w]])
@@ -360,6 +545,11 @@ int gpvmc_refresh(gpvmc_ctx *c, double *drift)
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
     HIPCHK(hipSetDevice(c->cfg.device));
+    if (c->channel) {
+        if (drift)
+            for (int w = 0; w < c->W; ++w) drift[w] = 0.0;
+        return run_refresh_channel(c, 3);
+    }
     int rc = run_refresh(c, drift ? c->d_drift : nullptr);
     if (rc) return rc;
     if (drift) {
@@ -381,6 +571,8 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
                            GPVMC_MEAS_STATSPIN))
         return fail(GPVMC_ERR_ARG, "unknown estimator bit in mask");
     if (n_steps == 0 && !measure_only) return GPVMC_OK;
+    if (c->channel) return walk_channel(c, n_steps, meas, mask, uniforms, noop, accept, ratio, eloc, mz, logw,
+                                        measure_only);
     const size_t W = c->W;
     const int64_t nmeas = measure_only ? 1
                           : meas > 0 ? (meas == 1 ? n_steps : (n_steps + meas - 1) / meas) : 0;
@@ -491,6 +683,8 @@ int gpvmc_reset_accumulators(gpvmc_ctx *c)
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)c->W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    if (c->d_ho)
+        HIPCHK(hipMemsetAsync(c->d_ho, 0, (size_t)c->W * 4 * c->nk * c->nk * sizeof(double), c->stream));
     int rc = clear_statspin(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -617,6 +811,7 @@ int gpvmc_read_inverse(gpvmc_ctx *c, int32_t walker, double *ai)
 int gpvmc_read_ratio_table(gpvmc_ctx *c, int32_t walker, double *g)
 {
     if (!c || !g) return fail(GPVMC_ERR_ARG, "null argument");
+    if (c->channel) return fail(GPVMC_ERR_UNSUPPORTED, "channel contexts keep inverses, not ratio tables");
     if (walker < 0 || walker >= c->W) return fail(GPVMC_ERR_ARG, "walker out of range");
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -651,6 +846,174 @@ int gpvmc_reset_kernel_time(gpvmc_ctx *c)
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
     c->sweep_ms = c->refresh_ms = 0;
     c->n_sweep = c->n_refresh = 0;
+    return GPVMC_OK;
+}
+
+int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc_ctx **out)
+{
+    if (!cfg || !ch || !out) return fail(GPVMC_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int L = cfg->L;
+    if (L < 4 || (L * L) % 2) return fail(GPVMC_ERR_ARG, "L must be >= 4 with L*L even");
+    if (cfg->n_walkers < 1) return fail(GPVMC_ERR_ARG, "n_walkers must be >= 1");
+    if (cfg->n_groups < 1) return fail(GPVMC_ERR_ARG, "n_groups must be >= 1");
+    if (cfg->precision != GPVMC_FP64 && cfg->precision != GPVMC_MIXED)
+        return fail(GPVMC_ERR_ARG, "precision must be GPVMC_FP64 or GPVMC_MIXED");
+    const int Ns = L * L, nk = ch->nk, nu = ch->n_up, nd = ch->n_dn;
+    if (nu < 1 || nd < 1 || nu + nd != Ns) return fail(GPVMC_ERR_ARG, "n_up + n_dn must equal L*L");
+    if (nk < 1) return fail(GPVMC_ERR_ARG, "nk must be >= 1");
+    if (!ch->state_orb || !ch->hop_n || !ch->hop_orb || !ch->hopsign) return fail(GPVMC_ERR_ARG, "null channel array");
+    const int nm = std::max(nu, nd);
+    if (nm > 256) return fail(GPVMC_ERR_UNSUPPORTED, "at most 256 particles per flavour");
+    if (gv_exc_lds_bytes(Ns, nm, nk) > 156 * 1024)
+        return fail(GPVMC_ERR_UNSUPPORTED, "channel too large for one workgroup's LDS");
+    for (int i = 0; i < nk * (nu + nd); ++i)
+        if (ch->state_orb[i] < 0 || ch->state_orb[i] >= Ns) return fail(GPVMC_ERR_ARG, "state orbital out of range");
+    // new-orbital lists per (current state, flavour) and the index of every hop's target
+    std::vector<int32_t> hop_new((size_t)nk * nk * 4, 0), new_off((size_t)nk * 2 + 1, 0), new_orb;
+    for (int cc = 0; cc < nk; ++cc)
+        for (int fl = 0; fl < 2; ++fl) {
+            std::vector<int32_t> lst;
+            for (int k = 0; k < nk; ++k) {
+                const int hn = ch->hop_n[((size_t)cc * nk + k) * 2 + fl];
+                if (hn < 0 || hn > 2) return fail(GPVMC_ERR_ARG, "at most two orbital hops per flavour");
+                for (int h = 0; h < hn; ++h) {
+                    const int32_t *ho = ch->hop_orb + ((((size_t)cc * nk + k) * 2 + fl) * 2 + h) * 2;
+                    if (ho[0] < 0 || ho[0] >= Ns || ho[1] < 0 || ho[1] >= Ns)
+                        return fail(GPVMC_ERR_ARG, "hop orbital out of range");
+                    if (std::find(lst.begin(), lst.end(), ho[1]) == lst.end()) lst.push_back(ho[1]);
+                }
+            }
+            std::sort(lst.begin(), lst.end());
+            for (int k = 0; k < nk; ++k) {
+                const int hn = ch->hop_n[((size_t)cc * nk + k) * 2 + fl];
+                for (int h = 0; h < hn; ++h) {
+                    const int32_t to = ch->hop_orb[((((size_t)cc * nk + k) * 2 + fl) * 2 + h) * 2 + 1];
+                    hop_new[(((size_t)cc * nk + k) * 2 + fl) * 2 + h] =
+                        (int32_t)(std::lower_bound(lst.begin(), lst.end(), to) - lst.begin());
+                }
+            }
+            new_off[(size_t)cc * 2 + fl] = (int32_t)new_orb.size();
+            new_orb.insert(new_orb.end(), lst.begin(), lst.end());
+        }
+    new_off[(size_t)nk * 2] = (int32_t)new_orb.size();
+    if (new_orb.empty()) new_orb.push_back(0);
+
+    gpvmc_ctx *c = new gpvmc_ctx();
+    c->cfg = *cfg;
+    c->channel = true;
+    c->L = L;
+    c->Ns = Ns;
+    c->N = nm;
+    c->n_up = nu;
+    c->n_dn = nd;
+    c->nk = nk;
+    c->W = cfg->n_walkers;
+    c->G = cfg->n_groups;
+    c->prec = cfg->precision;
+    c->have_orb.assign(c->G, 0);
+    c->groups.assign(c->W, 0);
+    c->state_orb.assign(ch->state_orb, ch->state_orb + (size_t)nk * (nu + nd));
+    auto bail = [&](int rc) {
+        free_ctx(c);
+        return rc;
+    };
+#define CK(expr)                                                                           \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return bail(fail(GPVMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    CK(hipSetDevice(cfg->device));
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipEventCreate(&c->ev0));
+    CK(hipEventCreate(&c->ev1));
+    const size_t W = c->W;
+    CK(hipMalloc(&c->d_Ai64, W * 2 * nm * nm * sizeof(double2)));
+    CK(hipMemset(c->d_Ai64, 0, W * 2 * nm * nm * sizeof(double2)));
+    if (c->prec == GPVMC_MIXED) {
+        CK(hipMalloc(&c->d_Ai32, W * 2 * nm * nm * sizeof(float2)));
+        CK(hipMemset(c->d_Ai32, 0, W * 2 * nm * nm * sizeof(float2)));
+    }
+    CK(hipMalloc(&c->d_work, W * 2 * nm * nm * sizeof(double2)));
+    CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_part, W * 2 * nm * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_cur, W * 2 * nm * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_cstate, W * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_weight, W * sizeof(double)));
+    CK(hipMalloc(&c->d_ho, W * 4 * nk * nk * sizeof(double)));
+    CK(hipMalloc(&c->d_group, W * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_order, W * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_status, W * 2 * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_logabs, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_phase, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_drift, W * 2 * sizeof(double)));
+    CK(hipMalloc(&c->d_acc, W * gv::ACC_STRIDE * sizeof(double)));
+    CK(hipMalloc(&c->d_counter, W * sizeof(uint64_t)));
+    CK(hipMalloc(&c->d_tab64, (size_t)c->G * 2 * Ns * Ns * sizeof(double2)));
+    if (c->prec == GPVMC_MIXED) CK(hipMalloc(&c->d_tab32, (size_t)c->G * 2 * Ns * Ns * sizeof(float2)));
+    CK(hipMemset(c->d_group, 0, W * sizeof(int32_t)));
+    CK(hipMemset(c->d_acc, 0, W * gv::ACC_STRIDE * sizeof(double)));
+    CK(hipMemset(c->d_ho, 0, W * 4 * nk * nk * sizeof(double)));
+    CK(hipMemset(c->d_counter, 0, W * sizeof(uint64_t)));
+    std::vector<int32_t> hop_orb((size_t)nk * nk * 8), hop_n((size_t)nk * nk * 2), hopsign((size_t)nk * nk);
+    std::copy(ch->hop_orb, ch->hop_orb + hop_orb.size(), hop_orb.begin());
+    std::copy(ch->hop_n, ch->hop_n + hop_n.size(), hop_n.begin());
+    std::copy(ch->hopsign, ch->hopsign + hopsign.size(), hopsign.begin());
+    auto up = [&](int32_t **d, const std::vector<int32_t> &h) -> hipError_t {
+        hipError_t e = hipMalloc(d, h.size() * sizeof(int32_t));
+        if (e != hipSuccess) return e;
+        return hipMemcpy(*d, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    };
+    CK(up(&c->d_hop_n, hop_n));
+    CK(up(&c->d_hop_orb, hop_orb));
+    CK(up(&c->d_hop_new, hop_new));
+    CK(up(&c->d_new_off, new_off));
+    CK(up(&c->d_new_orb, new_orb));
+    CK(up(&c->d_hopsign, hopsign));
+    std::vector<int32_t> nbr, edge4;
+    build_lattice(L, nbr, edge4);
+    CK(hipMalloc(&c->d_nbr, nbr.size() * sizeof(int32_t)));
+    CK(hipMalloc(&c->d_edge, edge4.size() * sizeof(int32_t)));
+    CK(hipMemcpy(c->d_nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->d_edge, edge4.data(), edge4.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+#undef CK
+    int rc = upload_order(c);
+    if (rc) return bail(rc);
+    *out = c;
+    return GPVMC_OK;
+}
+
+int gpvmc_set_orbital_table(gpvmc_ctx *c, int32_t group, const double *table)
+{
+    if (!c || !table) return fail(GPVMC_ERR_ARG, "null argument");
+    if (!c->channel) return fail(GPVMC_ERR_STATE, "ground-state contexts take occupied orbitals (gpvmc_set_orbitals)");
+    if (group < 0 || group >= c->G) return fail(GPVMC_ERR_ARG, "group out of range");
+    const size_t n = (size_t)2 * c->Ns * c->Ns;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(table[i])) return fail(GPVMC_ERR_ARG, "non-finite orbital value");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipMemcpy(c->d_tab64 + (size_t)group * n, table, n * sizeof(double2), hipMemcpyHostToDevice));
+    if (c->prec == GPVMC_MIXED) {
+        std::vector<float2> f(n);
+        for (size_t i = 0; i < n; ++i) f[i] = make_float2((float)table[2 * i], (float)table[2 * i + 1]);
+        HIPCHK(hipMemcpy(c->d_tab32 + (size_t)group * n, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    c->have_orb[group] = 1;
+    return GPVMC_OK;
+}
+
+int gpvmc_read_channel(gpvmc_ctx *c, int32_t w0, int32_t nw, double *ho, int32_t *cstate, double *weight)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (!c->channel) return fail(GPVMC_ERR_STATE, "not a channel context");
+    if (w0 < 0 || nw < 0 || (int64_t)w0 + nw > c->W) return fail(GPVMC_ERR_ARG, "walker range out of bounds");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t per = (size_t)4 * c->nk * c->nk;
+    if (ho) HIPCHK(hipMemcpy(ho, c->d_ho + (size_t)w0 * per, (size_t)nw * per * sizeof(double), hipMemcpyDeviceToHost));
+    if (cstate) HIPCHK(hipMemcpy(cstate, c->d_cstate + w0, (size_t)nw * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (weight) HIPCHK(hipMemcpy(weight, c->d_weight + w0, (size_t)nw * sizeof(double), hipMemcpyDeviceToHost));
     return GPVMC_OK;
 }
 

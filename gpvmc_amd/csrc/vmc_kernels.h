@@ -64,6 +64,74 @@ struct GBArgs {
     double *drift;            // [W][2] or null
 };
 
+// Channel (exciton) walk: runtime sizes n_up, n_dn, N_k; one workgroup per walker.
+struct ExcArgs {
+    void *Ai;                 // T2 [W][2][nmax][nmax] inverse of the current state, [label][slot]
+    int32_t *site;            // [W][Ns] (flavour << 16) | label
+    int32_t *part;            // [W][2][nmax] site of each label
+    int32_t *cur;             // [W][2][nmax] orbital in each slot of the current state
+    int32_t *cstate;          // [W] current channel state
+    double *weight;           // [W] sum_k |a_k / a_c|^2 at the current configuration
+    double *logabs;           // [W][2] ln|det| of the current state's matrices
+    double *phase;            // [W][2]
+    uint64_t *counter;        // [W]
+    double *acc;              // [W][ACC_STRIDE]
+    double *ho;               // [W][2 nk][nk] complex ProjHeis H and O sums
+    const void *tab;          // T2 [G][2][Ns][Ns] orbital table Phi[fl][orbital][site]
+    const void *tab64;        // double2, same layout
+    const int32_t *group;     // [W]
+    const int32_t *order;     // [blocks] block -> walker (may be null)
+    const int32_t *nbr;       // [Ns][4]
+    const int4 *edge;         // [2Ns]
+    const int32_t *hop_n;     // [nk][nk][2] orbital hops from state c to k per flavour (<= 2)
+    const int32_t *hop_orb;   // [nk][nk][2][2][2] (from, to) orbitals
+    const int32_t *hop_new;   // [nk][nk][2][2] index of the `to` orbital in new_orb of state c
+    const int32_t *new_off;   // [nk*2 + 1] offsets of new_orb per (c, fl)
+    const int32_t *new_orb;   // orbitals some hop from state c fills, per (c, fl)
+    const int32_t *hopsign;   // [nk][nk] State::HopSign of the path c -> k on state c
+    const double *uniforms;
+    int8_t *tr_noop;
+    int8_t *tr_accept;
+    double *tr_ratio;
+    double *tr_logw;
+    int64_t n_steps;
+    int64_t meas;
+    int64_t n_meas_slots;
+    uint32_t key0, key1;
+    int32_t L, n_up, n_dn, nk;
+    int32_t mode;             // 0 walk, 1 measurement only, 2 initial state choice, 3 weight only
+};
+
+// Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
+__device__ __forceinline__ void philox4x32_10_dev(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// runtime-size FP64 Gauss-Jordan on prebuilt matrices (channel contexts)
+struct GJDynArgs {
+    double2 *work;            // [W][2][nmax][nmax] A[slot][label] in, destroyed
+    double2 *Ai64;            // [W][2][nmax][nmax] out, Ai[label][slot]
+    float2 *Ai32;             // same in FP32, or null
+    double *logabs, *phase;   // [W][2]
+    int32_t *status;          // [W][2] 1 = singular
+    int32_t n_up, n_dn;
+};
+
 }  // namespace gv
 
 extern "C" {
@@ -76,6 +144,10 @@ size_t gv_gj_work_bytes(int N);
 hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
 // out[g][e] = sum over the walkers idx[off[g] .. off[g+1]) of ss[walker][e], e < per
+hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_t st);
+size_t gv_exc_lds_bytes(int Ns, int nmax, int nk);
+hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st);
+hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st);
 hipError_t gv_launch_ss_reduce(const double *ss, const int32_t *idx, const int32_t *off, int G,
                                size_t per, double *out, hipStream_t st);
 }

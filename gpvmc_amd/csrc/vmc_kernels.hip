@@ -60,26 +60,6 @@ template <int LPR, typename C> __device__ __forceinline__ C row_reduce(C v)
     return v;
 }
 
-// Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
-__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c[0];
-        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-        uint32_t lo1 = 0xCD9E8D57u * c[2];
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        uint32_t n0 = hi1 ^ c[1] ^ k0;
-        uint32_t n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = lo1;
-        c[2] = n2;
-        c[3] = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
 __device__ __forceinline__ double u32_to_open01(uint32_t x)
 {
     return ((double)x + 0.5) * 2.3283064365386963e-10;  // 2^-32
@@ -419,7 +399,7 @@ sweep_kernel(SweepArgs a)
                 } else {
                     const uint64_t st = ctr0 + (uint64_t)sl;
                     uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
-                    philox4x32_10(cc, a.key0, a.key1);
+                    philox4x32_10_dev(cc, a.key0, a.key1);
                     // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers
                     // (exact: NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused
                     v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);

@@ -120,6 +120,39 @@ int gpvmc_read_statspin(gpvmc_ctx *ctx, int32_t w0, int32_t nw, double *acc, dou
  * post-processing, which averages the per-rank files). */
 int gpvmc_read_statspin_groups(gpvmc_ctx *ctx, double *acc, double *n_meas);
 
+/* ---- channels (exciton states) -------------------------------------------------
+ * A channel context walks on w = sum_k |a_k|^2 over the N_k states of a channel
+ * (StagFluxTransExciton / StagFluxLongExciton; LatticeStepper::trystep,
+ * LatticeStepper.cpp:81-95), keeping the inverse of its dominant ("current") state and
+ * hopping the wavefunction to the largest |a_k|^2 after every accepted move
+ * (LatticeStepper::step, :102-113; SlaterDeterminant::Update, Woodbury).  The graph is
+ * WaveFunction::m_exc (WaveFunction.cpp:39-67) flattened: hop_n[c][k][fl] orbital hops
+ * (at most 2) from state c to state k, hop_orb[c][k][fl][h] = (from, to) orbitals,
+ * hopsign[c][k] = State::HopSign of that path on state c (State.cpp:86-115), and
+ * state_orb[k] = the n_up then n_dn occupied orbitals of state k, each increasing.
+ * ProjHeis is accumulated as the reference's Nexc > 1 matrices (ProjHeis.cpp:146-153):
+ * H[k1][k2] = sum conj(a_k1) h_k2 / w and O[k1][k2] = sum conj(a_k1) a_k2 / w. */
+typedef struct {
+    int32_t n_up, n_dn;         /* particles per flavour, n_up + n_dn = L*L */
+    int32_t nk;                 /* channel states */
+    const int32_t *state_orb;   /* [nk][n_up + n_dn] */
+    const int32_t *hop_n;       /* [nk][nk][2] */
+    const int32_t *hop_orb;     /* [nk][nk][2][2][2] */
+    const int32_t *hopsign;     /* [nk][nk] */
+} gpvmc_channel;
+
+/* StagFluxTransExciton / StagFluxLongExciton construction + SlaterDeterminant on them
+ * (vmc.cpp:140-156).  The context's part arrays are [w][2][max(n_up, n_dn)]. */
+int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc_ctx **out);
+/* Full orbital table of one group: table[fl][orbital][site] complex, 2*Ns*Ns values
+ * (WaveFunction::init_matrices / MatEl, WaveFunction.cpp:27-37). */
+int gpvmc_set_orbital_table(gpvmc_ctx *ctx, int32_t group, const double *table);
+/* ProjHeis sums ho[k][2 nk][nk] complex (rows 0..nk-1: H, nk..2nk-1: O) of walkers
+ * w0 .. w0+nw-1, their current state and weight sum_k |a_k / a_current|^2; any output may
+ * be NULL.  The measurement count is acc[3] of gpvmc_read_accumulators. */
+int gpvmc_read_channel(gpvmc_ctx *ctx, int32_t w0, int32_t nw, double *ho, int32_t *cstate,
+                       double *weight);
+
 /* Current state: spins[w][site], labels part[w][2][N] (site of each particle),
  * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */
 int gpvmc_read_state(gpvmc_ctx *ctx, int32_t *spins, int32_t *part, double *logabs,
