@@ -38,6 +38,9 @@ SIGNATURES = {
     "gpvmc_read_inverse": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_ratio_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_create_channel": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "gpvmc_create_channels": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.POINTER(_vp)]),
+    "gpvmc_read_amplitudes": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_channel_kernel": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_set_orbital_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_channel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -228,37 +231,65 @@ class Batch:
 
 
 class ChannelBatch(Batch):
-    """A batch of walkers on a channel (gpvmc_amd.channels.Channel): the weight is
-    sum_k |a_k|^2 over the channel's states, ProjHeis accumulates H and O matrices."""
+    """A batch of walkers on channels (gpvmc_amd.channels.Channel): the weight is
+    sum_k |a_k|^2 over the channel's states, ProjHeis accumulates H and O matrices.
+    `channel` is one Channel or a list of Channels of equal sizes (walker w walks on
+    channel w mod len(list))."""
 
-    def __init__(self, L, n_walkers, channel, precision=FP64, n_groups=1, seed=0, device=0):
+    def __init__(self, L, n_walkers, channel, precision=FP64, n_groups=1, seed=0, device=0,
+                 kernel="auto"):
+        """kernel: "auto" (register-resident where the walker fits) or "streamed"."""
         self.lib = load_library()
         self.L, self.Ns = L, L * L
-        self.ch = channel
-        self.n_up, self.n_dn = channel.npt
-        self.N = max(channel.npt)
-        self.nk = channel.nexc
+        if kernel not in ("auto", "streamed"):
+            raise ValueError("kernel must be 'auto' or 'streamed'")
+        chans = list(channel) if isinstance(channel, (list, tuple)) else [channel]
+        self.channels = chans
+        self.ch = chans[0]
+        self.n_up, self.n_dn = self.ch.npt
+        self.N = max(self.ch.npt)
+        self.nk = self.ch.nexc
         self.W, self.G = n_walkers, n_groups
         self.precision = precision
         nk = self.nk
-        self._state_orb = np.ascontiguousarray(
-            np.array([np.concatenate([u, d]) for u, d in channel.states], np.int32))
-        self._hop_n = np.zeros((nk, nk, 2), np.int32)
-        self._hop_orb = np.zeros((nk, nk, 2, 2, 2), np.int32)
-        for c in range(nk):
-            for k in range(nk):
-                for fl in range(2):
-                    path = channel.hops[c][k][fl]
-                    self._hop_n[c, k, fl] = len(path)
-                    for h, (a, b) in enumerate(path):
-                        self._hop_orb[c, k, fl, h] = (a, b)
-        self._hopsign = np.ascontiguousarray(channel.hopsign.astype(np.int32))
-        chs = _Channel(self.n_up, self.n_dn, nk, _ptr(self._state_orb), _ptr(self._hop_n),
-                       _ptr(self._hop_orb), _ptr(self._hopsign))
-        cfg = _Config(L, n_walkers, n_groups, precision, device, 0, seed)
+        self._keep = []
+        structs = (_Channel * len(chans))()
+        for q, ch in enumerate(chans):
+            if ch.npt != self.ch.npt or ch.nexc != nk:
+                raise ValueError("channels of one batch must have equal sizes")
+            state_orb = np.ascontiguousarray(
+                np.array([np.concatenate([u, d]) for u, d in ch.states], np.int32))
+            hop_n = np.zeros((nk, nk, 2), np.int32)
+            hop_orb = np.zeros((nk, nk, 2, 2, 2), np.int32)
+            for c in range(nk):
+                for k in range(nk):
+                    for fl in range(2):
+                        path = ch.hops[c][k][fl]
+                        hop_n[c, k, fl] = len(path)
+                        for h, (a, b) in enumerate(path):
+                            hop_orb[c, k, fl, h] = (a, b)
+            hopsign = np.ascontiguousarray(ch.hopsign.astype(np.int32))
+            self._keep += [state_orb, hop_n, hop_orb, hopsign]
+            structs[q] = _Channel(self.n_up, self.n_dn, nk, _ptr(state_orb), _ptr(hop_n),
+                                  _ptr(hop_orb), _ptr(hopsign))
+        cfg = _Config(L, n_walkers, n_groups, precision, device, 0 if kernel == "auto" else -1, seed)
         h = ctypes.c_void_p()
-        self._check(self.lib.gpvmc_create_channel(ctypes.byref(cfg), ctypes.byref(chs), ctypes.byref(h)))
+        self._check(self.lib.gpvmc_create_channels(ctypes.byref(cfg), len(chans),
+                                                   ctypes.cast(structs, ctypes.c_void_p),
+                                                   ctypes.byref(h)))
         self.h = h
+
+    @property
+    def register_resident(self):
+        v = ctypes.c_int32()
+        self._check(self.lib.gpvmc_channel_kernel(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def amplitudes(self):
+        """ln|a_k| + i arg a_k of every state at each walker's configuration, [W][nk] complex"""
+        out = np.zeros((self.W, self.nk, 2))
+        self._check(self.lib.gpvmc_read_amplitudes(self.h, _ptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
 
     def set_orbital_table(self, group, table):
         t = np.ascontiguousarray(table, np.complex128)

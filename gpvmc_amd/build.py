@@ -13,29 +13,46 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # no packed-FP32 instructions (v_pk_*_f32) anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"),
     "-I" + CSRC,
 ]
 
+SOURCES = ("vmc_kernels.hip", "exc_kernels.hip", "exc_reg_kernels.hip", "gpvmc_api.cpp")
+
 
 def sources():
-    return [os.path.join(CSRC, f) for f in ("vmc_kernels.hip", "exc_kernels.hip", "gpvmc_api.cpp")]
+    return [os.path.join(CSRC, f) for f in SOURCES]
 
 
 def build(verbose=False):
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    """Compiles every source to an object in parallel, then links the shared library."""
+    objdir = os.path.join(os.path.dirname(LIB), "obj")
+    os.makedirs(objdir, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    procs, objs = [], []
+    for src in sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        objs.append(obj)
+        cmd = [hipcc] + HIP_FLAGS + ["-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed building libgpvmc_hip.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + HIP_FLAGS + ["-o", tmp] + sources()
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libgpvmc_hip.so")
+        raise RuntimeError("hipcc failed linking libgpvmc_hip.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,103 @@
+// Small complex FP64 helpers shared by the channel (exciton) kernels: capacitance
+// determinants and inverses of at most 3 x 3.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gz {
+
+template <typename T> struct cpx;
+template <> struct cpx<float> { using t = float2; };
+template <> struct cpx<double> { using t = double2; };
+
+struct zc {  // FP64 complex for the small determinants
+    double x, y;
+};
+__device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// conj(a) * b
+__device__ __forceinline__ zc zconjmul(zc a, zc b) { return zmk(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+__device__ __forceinline__ double znorm(zc a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ zc zdiv(zc a, zc b)
+{
+    const double n = znorm(b);
+    return zmk((a.x * b.x + a.y * b.y) / n, (a.y * b.x - a.x * b.y) / n);
+}
+template <typename C> __device__ __forceinline__ zc toz(C v) { return zmk((double)v.x, (double)v.y); }
+
+__device__ inline zc det_small(int n, const zc *B)  // n <= 3, row-major n x n
+{
+    if (n == 0) return zmk(1.0, 0.0);
+    if (n == 1) return B[0];
+    if (n == 2) return zsub(zmul(B[0], B[3]), zmul(B[1], B[2]));
+    zc t0 = zsub(zmul(B[4], B[8]), zmul(B[5], B[7]));
+    zc t1 = zsub(zmul(B[3], B[8]), zmul(B[5], B[6]));
+    zc t2 = zsub(zmul(B[3], B[7]), zmul(B[4], B[6]));
+    return zadd(zsub(zmul(B[0], t0), zmul(B[1], t1)), zmul(B[2], t2));
+}
+
+// inverse of an n x n (n <= 3) complex matrix by the adjugate
+__device__ inline void inv_small(int n, const zc *B, zc *Bi)
+{
+    const zc d = det_small(n, B);
+    if (n == 1) { Bi[0] = zdiv(zmk(1.0, 0.0), d); return; }
+    if (n == 2) {
+        Bi[0] = zdiv(B[3], d);
+        Bi[1] = zdiv(zmk(-B[1].x, -B[1].y), d);
+        Bi[2] = zdiv(zmk(-B[2].x, -B[2].y), d);
+        Bi[3] = zdiv(B[0], d);
+        return;
+    }
+    auto m = [&](int r, int c) { return B[r * 3 + c]; };
+    auto cof = [&](int r0, int r1, int c0, int c1) {
+        return zsub(zmul(m(r0, c0), m(r1, c1)), zmul(m(r0, c1), m(r1, c0)));
+    };
+    // Bi[i][j] = C[j][i] / d
+    zc C[9];
+    C[0] = cof(1, 2, 1, 2);
+    C[1] = zsub(zmk(0, 0), cof(1, 2, 0, 2));
+    C[2] = cof(1, 2, 0, 1);
+    C[3] = zsub(zmk(0, 0), cof(0, 2, 1, 2));
+    C[4] = cof(0, 2, 0, 2);
+    C[5] = zsub(zmk(0, 0), cof(0, 2, 0, 1));
+    C[6] = cof(0, 1, 1, 2);
+    C[7] = zsub(zmk(0, 0), cof(0, 1, 0, 2));
+    C[8] = cof(0, 1, 0, 1);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bi[i * 3 + j] = zdiv(C[j * 3 + i], d);
+}
+
+// 3 x 3 determinant and inverse (a smaller capacitance is padded with the identity, which
+// leaves its determinant and its inverse block unchanged)
+__device__ __forceinline__ zc det3(const zc *B)
+{
+    const zc t0 = zsub(zmul(B[4], B[8]), zmul(B[5], B[7]));
+    const zc t1 = zsub(zmul(B[3], B[8]), zmul(B[5], B[6]));
+    const zc t2 = zsub(zmul(B[3], B[7]), zmul(B[4], B[6]));
+    return zadd(zsub(zmul(B[0], t0), zmul(B[1], t1)), zmul(B[2], t2));
+}
+__device__ __forceinline__ void inv3(const zc *B, zc *Bi)
+{
+    const zc d = det3(B);
+    auto cof = [&](int r0, int r1, int c0, int c1) {
+        return zsub(zmul(B[r0 * 3 + c0], B[r1 * 3 + c1]), zmul(B[r0 * 3 + c1], B[r1 * 3 + c0]));
+    };
+    // Bi[i][j] = cofactor C[j][i] / d
+    Bi[0] = zdiv(cof(1, 2, 1, 2), d);
+    Bi[3] = zdiv(zsub(zmk(0, 0), cof(1, 2, 0, 2)), d);
+    Bi[6] = zdiv(cof(1, 2, 0, 1), d);
+    Bi[1] = zdiv(zsub(zmk(0, 0), cof(0, 2, 1, 2)), d);
+    Bi[4] = zdiv(cof(0, 2, 0, 2), d);
+    Bi[7] = zdiv(zsub(zmk(0, 0), cof(0, 2, 0, 1)), d);
+    Bi[2] = zdiv(cof(0, 1, 1, 2), d);
+    Bi[5] = zdiv(zsub(zmk(0, 0), cof(0, 1, 0, 2)), d);
+    Bi[8] = zdiv(cof(0, 1, 0, 1), d);
+}
+__device__ __forceinline__ void eye3(zc *B)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) B[i] = zmk((i % 4) == 0 ? 1.0 : 0.0, 0.0);
+}
+
+}  // namespace gz

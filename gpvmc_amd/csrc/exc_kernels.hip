@@ -27,73 +27,13 @@
 #include <stdint.h>
 
 #include "vmc_kernels.h"
+#include "exc_common.h"
 
 namespace gx {
 
 using gv::ExcArgs;
 
-template <typename T> struct cpx;
-template <> struct cpx<float> { using t = float2; };
-template <> struct cpx<double> { using t = double2; };
-
-struct zc {  // FP64 complex for the small determinants
-    double x, y;
-};
-__device__ __forceinline__ zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
-__device__ __forceinline__ zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-// conj(a) * b
-__device__ __forceinline__ zc zconjmul(zc a, zc b) { return zmk(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
-__device__ __forceinline__ double znorm(zc a) { return a.x * a.x + a.y * a.y; }
-__device__ __forceinline__ zc zdiv(zc a, zc b)
-{
-    const double n = znorm(b);
-    return zmk((a.x * b.x + a.y * b.y) / n, (a.y * b.x - a.x * b.y) / n);
-}
-template <typename C> __device__ __forceinline__ zc toz(C v) { return zmk((double)v.x, (double)v.y); }
-
-__device__ zc det_small(int n, const zc *B)  // n <= 3, row-major n x n
-{
-    if (n == 0) return zmk(1.0, 0.0);
-    if (n == 1) return B[0];
-    if (n == 2) return zsub(zmul(B[0], B[3]), zmul(B[1], B[2]));
-    zc t0 = zsub(zmul(B[4], B[8]), zmul(B[5], B[7]));
-    zc t1 = zsub(zmul(B[3], B[8]), zmul(B[5], B[6]));
-    zc t2 = zsub(zmul(B[3], B[7]), zmul(B[4], B[6]));
-    return zadd(zsub(zmul(B[0], t0), zmul(B[1], t1)), zmul(B[2], t2));
-}
-
-// inverse of an n x n (n <= 3) complex matrix by the adjugate
-__device__ void inv_small(int n, const zc *B, zc *Bi)
-{
-    const zc d = det_small(n, B);
-    if (n == 1) { Bi[0] = zdiv(zmk(1.0, 0.0), d); return; }
-    if (n == 2) {
-        Bi[0] = zdiv(B[3], d);
-        Bi[1] = zdiv(zmk(-B[1].x, -B[1].y), d);
-        Bi[2] = zdiv(zmk(-B[2].x, -B[2].y), d);
-        Bi[3] = zdiv(B[0], d);
-        return;
-    }
-    auto m = [&](int r, int c) { return B[r * 3 + c]; };
-    auto cof = [&](int r0, int r1, int c0, int c1) {
-        return zsub(zmul(m(r0, c0), m(r1, c1)), zmul(m(r0, c1), m(r1, c0)));
-    };
-    // Bi[i][j] = C[j][i] / d
-    zc C[9];
-    C[0] = cof(1, 2, 1, 2);
-    C[1] = zsub(zmk(0, 0), cof(1, 2, 0, 2));
-    C[2] = cof(1, 2, 0, 1);
-    C[3] = zsub(zmk(0, 0), cof(0, 2, 1, 2));
-    C[4] = cof(0, 2, 0, 2);
-    C[5] = zsub(zmk(0, 0), cof(0, 2, 0, 1));
-    C[6] = cof(0, 1, 1, 2);
-    C[7] = zsub(zmk(0, 0), cof(0, 1, 0, 2));
-    C[8] = cof(0, 1, 0, 1);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Bi[i * 3 + j] = zdiv(C[j * 3 + i], d);
-}
+using namespace gz;
 
 // LDS layout of one walker (dynamic shared memory; offsets computed by exc_lds_layout)
 struct Lds {
@@ -156,6 +96,7 @@ template <typename T> struct Walker {
     const ExcArgs &a;
     Lds s;
     int w, tid, nt, Ns, n[2], nmax, nk;
+    size_t cb;  // channel of the walker times nk: base of its graph arrays
     C *Ai[2];
     const C *tab[2];
 
@@ -167,6 +108,7 @@ template <typename T> struct Walker {
         n[1] = a.n_dn;
         nmax = n[0] > n[1] ? n[0] : n[1];
         nk = a.nk;
+        cb = (size_t)(a.chan ? a.chan[w] : 0) * nk;
         s = lds_carve(base, Ns, nmax, nk);
         for (int fl = 0; fl < 2; ++fl) {
             Ai[fl] = reinterpret_cast<C *>(a.Ai) + ((size_t)w * 2 + fl) * nmax * nmax;
@@ -179,21 +121,19 @@ template <typename T> struct Walker {
     __device__ void sz(double *p, int i, zc v) const { p[2 * i] = v.x; p[2 * i + 1] = v.y; }
 
     __device__ int cstate() const { return a.cstate[w]; }
-    __device__ int hop_n(int c, int k, int fl) const { return a.hop_n[((size_t)c * nk + k) * 2 + fl]; }
-    __device__ int hop_from(int c, int k, int fl, int h) const
+    __device__ size_t ck(int c, int k) const { return (cb + c) * nk + k; }
+    __device__ int hop_n(int c, int k, int fl) const { return a.hop_n[ck(c, k) * 2 + fl]; }
+    __device__ int hop_from(int c, int k, int fl, int h) const { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2]; }
+    __device__ int hop_to(int c, int k, int fl, int h) const { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2 + 1]; }
+    __device__ int hop_new(int c, int k, int fl, int h) const  // index into the new orbitals of (c, fl)
     {
-        return a.hop_orb[((((size_t)c * nk + k) * 2 + fl) * 2 + h) * 2];
+        return a.hop_new[(ck(c, k) * 2 + fl) * 2 + h];
     }
-    __device__ int hop_to(int c, int k, int fl, int h) const
-    {
-        return a.hop_orb[((((size_t)c * nk + k) * 2 + fl) * 2 + h) * 2 + 1];
-    }
-    __device__ int hop_new(int c, int k, int fl, int h) const  // index into newo[c][fl]
-    {
-        return a.hop_new[(((size_t)c * nk + k) * 2 + fl) * 2 + h];
-    }
-    __device__ int n_new(int c, int fl) const { return a.new_off[((size_t)c * 2 + fl) + 1] - a.new_off[(size_t)c * 2 + fl]; }
-    __device__ int new_orb(int c, int fl, int m) const { return a.new_orb[a.new_off[(size_t)c * 2 + fl] + m]; }
+    __device__ int hopsign(int c, int k) const { return a.hopsign[ck(c, k)]; }
+    // new_off holds 2 nk + 1 offsets per channel
+    __device__ const int32_t *noff(int c, int fl) const { return a.new_off + (cb / nk) * (2 * nk + 1) + c * 2 + fl; }
+    __device__ int n_new(int c, int fl) const { return noff(c, fl)[1] - noff(c, fl)[0]; }
+    __device__ int new_orb(int c, int fl, int m) const { return a.new_orb[noff(c, fl)[0] + m]; }
 
     // (rho_o Ai)[f] = sum_r' Phi(o, site r') Ai[r'][f]
     __device__ zc rho_ai(int fl, int o, int f) const
@@ -250,7 +190,7 @@ template <typename T> struct Walker {
             __syncthreads();
         }
         for (int k = tid; k < nk; k += nt) {
-            zc tot = zmk((double)a.hopsign[(size_t)c * nk + k], 0.0);
+            zc tot = zmk((double)hopsign(c, k), 0.0);
             for (int fl = 0; fl < 2; ++fl) {
                 const int R = hop_n(c, k, fl);
                 zc B[9];
@@ -433,7 +373,8 @@ template <typename T> struct Walker {
 
 // mode 0: walk n_steps; 1: measurement only (traces, no accumulators); 2: initial state
 // choice (the state of largest |a_k|^2 at the configuration, vmc.cpp:243-254) and weight
-// bookkeeping after an init or a refresh of the inverse
+// bookkeeping after an init or a refresh of the inverse; 3: weight bookkeeping only;
+// 4: amplitudes of every state into amp_out
 template <typename T>
 __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
 {
@@ -526,7 +467,16 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         ++mslot;
         __syncthreads();
     };
-    if (a.mode == 1) {
+    if (a.mode == 4) {
+        // amplitudes of every state at the current configuration, as ln|a_k| and arg a_k
+        const double la = a.logabs[2 * w] + a.logabs[2 * w + 1], ph = a.phase[2 * w] + a.phase[2 * w + 1];
+        for (int k = tid; k < nk; k += nt) {
+            const zc al = wk.lz(s.alpha, k);
+            const double nrm = znorm(al);
+            a.amp_out[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
+            a.amp_out[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
+        }
+    } else if (a.mode == 1) {
         measure(false);
     } else if (a.mode == 0) {
         for (int64_t st = 0; st < a.n_steps; ++st) {

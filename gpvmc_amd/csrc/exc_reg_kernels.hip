@@ -1,0 +1,960 @@
+// gfx950 register-resident channel (exciton) walk: one 512-thread workgroup per walker.
+//
+// The chain, estimator and slot bookkeeping are those of the streamed channel kernel
+// (LatticeStepper::trystep/step on a channel, LatticeStepper.cpp:36-147;
+// SlaterDeterminant::VirtUpdate/Update, SlaterDeterminant.cpp:96-326; linalg::DetUpdate
+// and InvUpdate, linalg.cpp:107-573; ProjHeis::measure, ProjHeis.cpp:40-154).  Here each
+// walker's matrices stay in VGPRs for a whole launch:
+//
+//   M_fl = [Ai_fl ; Q_fl],   Ai_fl[r][f] = (A_fl^-1)(label r, slot f),
+//                            Q_fl[q][f]  = sum_r Phi_fl(o_q, site r) Ai_fl[r][f],
+//
+// o_q running over the orbitals that some hop from the current state fills (the rows
+// rho_m Ai of the capacitance).  Rows are cut in blocks of 16 and columns in blocks of 4,
+// held in the A-operand layout of v_mfma_*_16x16x4 (lane l: row l % 16, column l / 16 of
+// each 16 x 4 tile), so the ProjHeis contraction Z = M x Phi(current orbitals, sites of
+// the other flavour) runs on MFMA straight from the registers.
+//
+// Capacitance of a move of particle r to site s with the orbital hops o_m -> o'_m in
+// slots f_m (c = Phi(current orbitals, s), y = Ai c, KG = Q c):
+//   B = [[ y[r],                 Ai[r][f_n] ],
+//        [ KG_m - Phi(o'_m, s),  Q_m[f_n]   ]].
+// This is the streamed kernel's capacitance with the multiples D_m of row 0 taken out of
+// rows m: a row operation, which changes neither det B nor B^-1 V (V's rows m carry the
+// same multiple of V's row 0).
+//
+// Accepted move: Ai' = Ai - U W with W = B^-1 V, U = [y - e_r, Ai[:, f_m]],
+// V = [Ai[r, :]; Q_m - e_{f_m}].  The Q rows follow from Q'_q = Phi'(o_q) Ai' with
+// Phi'(o_q) = Phi(o_q) + delta_q e_r^T, delta_q = Phi(o_q, s) - Phi(o_q, site r):
+//   Q'_q = Q_q + delta_q V_0 - sum_a kappa_a W_a,
+//   kappa_0 = KG_q - Phi(o_q, s) + delta_q y[r],   kappa_m = Q_q[f_m] + delta_q Ai[r][f_m].
+// The row of a filled orbital o'_m is taken over by the emptied o_m, whose row before the
+// move is e_{f_m}: the same formula with Q_q = e_{f_m}, KG_q - Phi(o_q, s) = 0 and
+// delta = Phi(o_m, s) - Phi(o_m, site r).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "exc_common.h"
+#include "vmc_kernels.h"
+
+namespace gr {
+
+using gv::ExcArgs;
+using namespace gz;
+
+template <typename T> struct Mf;
+template <> struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // row of output element v of lane l; its column is l % 16
+    __device__ static int orow(int l, int v) { return 4 * (l >> 4) + v; }
+};
+template <> struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int orow(int l, int v) { return (l >> 4) + 4 * v; }
+};
+
+constexpr int NW = 8, NT = 64 * NW;
+
+struct Sizes {
+    int Ns, n0, n1, nmax, nq0, nq1, nqm, rb0, rb1, r16, nc, nk, zt, ngrp;
+};
+
+__host__ __device__ __forceinline__ Sizes make_sizes(int L, int n0, int n1, int nq0, int nq1, int rb0, int rb1, int cb, int nk)
+{
+    Sizes z;
+    z.Ns = L * L;
+    z.n0 = n0;
+    z.n1 = n1;
+    z.nmax = n0 > n1 ? n0 : n1;
+    z.nq0 = nq0;
+    z.nq1 = nq1;
+    z.nqm = nq0 > nq1 ? nq0 : nq1;
+    z.rb0 = rb0;
+    z.rb1 = rb1;
+    z.r16 = 16 * (rb0 > rb1 ? rb0 : rb1);
+    z.nc = 4 * cb;
+    z.nk = nk;
+    z.zt = 16 * ((z.nmax + 15) / 16);
+    const int g = NT / nk;
+    z.ngrp = g < 1 ? 1 : (g > 8 ? 8 : g);
+    return z;
+}
+
+// LDS of one walker (byte offsets); ts = bytes of one complex element of the working
+// precision
+struct Off {
+    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, un, total;
+};
+
+__host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
+{
+    const size_t o = off;
+    off += (bytes + 15) & ~(size_t)15;
+    return o;
+}
+
+__host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts)
+{
+    Off o;
+    size_t off = 0;
+    o.site = take(off, 4 * (size_t)z.Ns);
+    o.part = take(off, 8 * (size_t)z.nmax);
+    o.cur = take(off, 8 * (size_t)z.nmax);
+    o.slot = take(off, 8 * (size_t)z.Ns);
+    o.qrow = take(off, 8 * (size_t)z.Ns);
+    o.qorb = take(off, 8 * (size_t)z.nqm);
+    o.rep = take(off, 8 * (size_t)z.r16);
+    // s_c[2 nc] s_row[2 nc] s_z[2 r16] colb[4 r16] vq[4 nc] wr[6 nc] kap[6 r16] dl[2 r16]
+    o.c2 = take(off, ts * (14 * (size_t)z.nc + 14 * (size_t)z.r16));
+    o.ra = take(off, 16 * 8 * (size_t)z.nk);
+    o.alpha = take(off, 16 * (size_t)z.nk);
+    o.dets = take(off, 32 * (size_t)z.nk);
+    o.a0 = take(off, 16 * (size_t)z.nk);
+    o.hpart = take(off, 16 * (size_t)z.nk * z.ngrp);
+    o.binv = take(off, 16 * 18);
+    o.red = take(off, 16 * 16);
+    o.ired = take(off, 4 * 16);
+    const size_t un = (size_t)z.nqm * z.nc > (size_t)z.nc * 16 ? (size_t)z.nqm * z.nc : (size_t)z.nc * 16;
+    o.un = take(off, ts * un);
+    o.total = off;
+    return o;
+}
+
+template <typename C2> __device__ __forceinline__ C2 mk2(double x, double y)
+{
+    C2 v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
+// mode 0: walk n_steps; 1: measurement only; 2: initial state choice (largest |a_k|^2,
+// vmc.cpp:243-254) and weight; 3: weight only; 4: amplitudes into amp_out
+template <typename T, int CB, int RBW>
+__global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
+{
+    using C2 = typename cpx<T>::t;
+    using MF = Mf<T>;
+    constexpr int NC = 4 * CB;
+    extern __shared__ __align__(16) char smem[];
+    const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15, kq = lane >> 4;
+    const Sizes z = make_sizes(a.L, a.n_up, a.n_dn, a.nq0, a.nq1, a.rb0, a.rb1, CB, a.nk);
+    const int Ns = z.Ns, nk = z.nk, nmax = z.nmax, nqm = z.nqm, r16 = z.r16, RB0 = z.rb0;
+    const int RBT = z.rb0 + z.rb1, RS = 16 * RBT;
+    auto NF = [&](int fl) { return fl ? z.n1 : z.n0; };
+    auto NQ = [&](int fl) { return fl ? z.nq1 : z.nq0; };
+    const Off lo = carve(z, sizeof(C2));
+    int *const s_site = reinterpret_cast<int *>(smem + lo.site);
+    int *const s_part = reinterpret_cast<int *>(smem + lo.part);
+    int *const s_cur = reinterpret_cast<int *>(smem + lo.cur);
+    int *const s_slot = reinterpret_cast<int *>(smem + lo.slot);
+    int *const s_qrow = reinterpret_cast<int *>(smem + lo.qrow);
+    int *const s_qorb = reinterpret_cast<int *>(smem + lo.qorb);
+    int *const s_rep = reinterpret_cast<int *>(smem + lo.rep);
+    zc *const s_ra = reinterpret_cast<zc *>(smem + lo.ra);
+    zc *const s_alpha = reinterpret_cast<zc *>(smem + lo.alpha);
+    zc *const s_dets = reinterpret_cast<zc *>(smem + lo.dets);
+    zc *const s_a0 = reinterpret_cast<zc *>(smem + lo.a0);
+    zc *const s_hpart = reinterpret_cast<zc *>(smem + lo.hpart);
+    zc *const s_binv = reinterpret_cast<zc *>(smem + lo.binv);
+    zc *const s_red = reinterpret_cast<zc *>(smem + lo.red);
+    int *const s_ired = reinterpret_cast<int *>(smem + lo.ired);
+    char *const s_un = smem + lo.un;
+    C2 *s_c = reinterpret_cast<C2 *>(smem + lo.c2);
+    C2 *s_row = s_c + 2 * NC;
+    C2 *s_z = s_row + 2 * NC;
+    C2 *s_colb = s_z + 2 * r16;
+    C2 *s_vq = s_colb + 4 * r16;
+    C2 *s_wr = s_vq + 4 * NC;
+    C2 *s_kap = s_wr + 6 * NC;
+    C2 *s_dl = s_kap + 6 * r16;
+
+    const size_t chb = (size_t)(a.chan ? a.chan[w] : 0) * nk;
+    auto ck = [&](int c, int k) { return (chb + c) * nk + k; };
+    auto hop_n = [&](int c, int k, int fl) { return a.hop_n[ck(c, k) * 2 + fl]; };
+    auto hop_from = [&](int c, int k, int fl, int h) { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2]; };
+    auto hop_to = [&](int c, int k, int fl, int h) { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2 + 1]; };
+    auto hsign = [&](int c, int k) { return a.hopsign[ck(c, k)]; };
+    const C2 *tabT = reinterpret_cast<const C2 *>(a.tabT) + (size_t)a.group[w] * 2 * Ns * Ns;
+    auto phi = [&](int fl, int orb, int site) { return toz(tabT[((size_t)fl * Ns + site) * Ns + orb]); };
+    C2 *Mg = reinterpret_cast<C2 *>(a.M) + (size_t)w * RS * NC;
+
+    // ---- walker state into LDS and registers
+    for (int i = tid; i < Ns; i += NT) {
+        s_site[i] = a.site[(size_t)w * Ns + i];
+        s_slot[i] = -1;
+        s_slot[Ns + i] = -1;
+        s_qrow[i] = -1;
+        s_qrow[Ns + i] = -1;
+    }
+    for (int i = tid; i < 2 * nmax; i += NT) {
+        s_part[i] = a.part[(size_t)w * 2 * nmax + i];
+        s_cur[i] = a.cur[(size_t)w * 2 * nmax + i];
+    }
+    for (int i = tid; i < 2 * nqm; i += NT) s_qorb[i] = a.qorb[(size_t)w * 2 * nqm + i];
+    int cs = a.cstate[w];
+    __syncthreads();
+    for (int i = tid; i < 2 * nmax; i += NT) {
+        const int fl = i / nmax, f = i - fl * nmax;
+        if (f < NF(fl)) s_slot[fl * Ns + s_cur[i]] = f;
+    }
+    for (int i = tid; i < 2 * nqm; i += NT) {
+        const int fl = i / nqm, q = i - fl * nqm;
+        if (q < NQ(fl) && s_qorb[i] >= 0) s_qrow[fl * Ns + s_qorb[i]] = q;
+    }
+    T mr[RBW][CB], mi[RBW][CB];
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+        const int rbg = wv + j * NW;
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            C2 v = mk2<C2>(0.0, 0.0);
+            if (rbg < RBT) v = Mg[(size_t)(rbg * 16 + li) * NC + 4 * cb + kq];
+            mr[j][cb] = v.x;
+            mi[j][cb] = v.y;
+        }
+    }
+    __syncthreads();
+
+    // row block j of this wave: flavour and row within the flavour (false past the end)
+    auto rinfo = [&](int j, int &fl, int &row) __attribute__((always_inline)) {
+        const int rbg = wv + j * NW;
+        if (rbg >= RBT) return false;
+        fl = rbg >= RB0 ? 1 : 0;
+        row = (rbg - (fl ? RB0 : 0)) * 16 + li;
+        return true;
+    };
+    auto store_regs = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < RBW; ++j) {
+            const int rbg = wv + j * NW;
+            if (rbg < RBT) {
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+                    Mg[(size_t)(rbg * 16 + li) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+            }
+        }
+    };
+    // RA[k][fl][m][n] = Q_fl[row of hop_to(c, k, fl, m)][slot of hop_from(c, k, fl, n)]
+    auto build_ra = [&]() __attribute__((always_inline)) {
+        C2 *Qd = reinterpret_cast<C2 *>(s_un);
+        for (int fl = 0; fl < 2; ++fl) {
+            if (NQ(fl) > 0) {
+#pragma unroll
+                for (int j = 0; j < RBW; ++j) {
+                    int f_, row;
+                    if (rinfo(j, f_, row) && f_ == fl && row >= NF(fl) && row < NF(fl) + NQ(fl)) {
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb)
+                            Qd[(row - NF(fl)) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+                    }
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < nk * 4; e += NT) {
+                const int k = e >> 2, m = (e >> 1) & 1, q = e & 1;
+                const int R = hop_n(cs, k, fl);
+                zc v = zmk(0.0, 0.0);
+                if (m < R && q < R)
+                    v = toz(Qd[s_qrow[fl * Ns + hop_to(cs, k, fl, m)] * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+                s_ra[((k * 2 + fl) * 2 + m) * 2 + q] = v;
+            }
+            __syncthreads();
+        }
+    };
+    // amplitudes of every state at the current configuration, relative to the current state
+    auto eval0 = [&]() __attribute__((always_inline)) {
+        for (int k = tid; k < nk; k += NT) {
+            zc tot = zmk((double)hsign(cs, k), 0.0);
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                const int R = hop_n(cs, k, fl);
+                zc B[9];
+                eye3(B);
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (m < R && q < R) B[m * 3 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+                const zc d = det3(B);
+                s_dets[k * 2 + fl] = d;
+                tot = zmul(tot, d);
+            }
+            s_alpha[k] = tot;
+        }
+        __syncthreads();
+    };
+    // capacitance of state k for the move (ci -> sb, cj -> sa), flavour fl, from the
+    // step's LDS vectors
+    auto cap_move = [&](int k, int fl, int ci, int sa, int sb, int cj, zc *B) __attribute__((always_inline)) {
+        const int R = hop_n(cs, k, fl), r = fl ? cj : ci, snew = fl ? sa : sb;
+        eye3(B);
+        B[0] = toz(s_z[fl * r16 + r]);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (q < R) B[1 + q] = toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+            if (m < R) {
+                const int o2 = hop_to(cs, k, fl, m);
+                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_qrow[fl * Ns + o2]]), phi(fl, o2, snew));
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                    if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+            }
+    };
+    // ratios of every state for the swap moving up particle ci to sb and down particle
+    // cj to sa: c vectors, one GEMV per flavour, the moving rows, the capacitances
+    auto eval_mv = [&](int ci, int sa, int sb, int cj) __attribute__((always_inline)) {
+        for (int t = tid; t < 2 * NC; t += NT) {
+            const int fl = t >= NC ? 1 : 0, f = t - fl * NC;
+            C2 v = mk2<C2>(0.0, 0.0);
+            if (f < NF(fl)) v = tabT[((size_t)fl * Ns + (fl ? sa : sb)) * Ns + s_cur[fl * nmax + f]];
+            s_c[t] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RBW; ++j) {
+            int fl, row;
+            if (rinfo(j, fl, row)) {
+                const C2 *cv = s_c + fl * NC;
+                T ar = 0, ai = 0;
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) {
+                    const C2 c = cv[4 * cb + kq];
+                    ar += mr[j][cb] * c.x - mi[j][cb] * c.y;
+                    ai += mr[j][cb] * c.y + mi[j][cb] * c.x;
+                }
+                ar += __shfl_xor(ar, 16);
+                ai += __shfl_xor(ai, 16);
+                ar += __shfl_xor(ar, 32);
+                ai += __shfl_xor(ai, 32);
+                if (kq == 0) s_z[fl * r16 + row] = mk2<C2>(ar, ai);
+                if (row == (fl ? cj : ci)) {
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb) s_row[fl * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < nk; k += NT) {
+            zc tot = zmk((double)hsign(cs, k), 0.0);
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                zc B[9];
+                cap_move(k, fl, ci, sa, sb, cj, B);
+                const zc d = det3(B);
+                s_dets[k * 2 + fl] = d;
+                tot = zmul(tot, d);
+            }
+            s_alpha[k] = tot;
+        }
+        __syncthreads();
+    };
+    // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element)
+    auto reduce_alpha = [&](double &sum, int &kmax) __attribute__((always_inline)) {
+        double ls = 0.0, bv = -1.0;
+        int bk = nk;
+        for (int k = tid; k < nk; k += NT) {
+            const double v = znorm(s_alpha[k]);
+            ls += v;
+            if (v > bv) {
+                bv = v;
+                bk = k;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ls += __shfl_xor(ls, o);
+            const double ov = __shfl_xor(bv, o);
+            const int ok = __shfl_xor(bk, o);
+            if (ov > bv || (ov == bv && ok < bk)) {
+                bv = ov;
+                bk = ok;
+            }
+        }
+        if (lane == 0) {
+            s_red[wv] = zmk(ls, bv);
+            s_ired[wv] = bk;
+        }
+        __syncthreads();
+        sum = 0.0;
+        bv = -1.0;
+        bk = nk;
+        for (int q = 0; q < NW; ++q) {
+            sum += s_red[q].x;
+            if (s_red[q].y > bv || (s_red[q].y == bv && s_ired[q] < bk)) {
+                bv = s_red[q].y;
+                bk = s_ired[q];
+            }
+        }
+        kmax = bk;
+        __syncthreads();
+    };
+    // move to state kn after eval0 (mv false) or eval_mv (mv true)
+    auto apply = [&](bool mv, int ci, int sa, int sb, int cj, int kn) __attribute__((always_inline)) {
+        const int m0 = mv ? 1 : 0;
+        const int Rk0 = hop_n(cs, kn, 0), Rk1 = hop_n(cs, kn, 1);
+        auto RK = [&](int fl) { return fl ? Rk1 : Rk0; };
+        // columns f_m of M and the Q rows of the filled orbitals, before any change
+#pragma unroll
+        for (int j = 0; j < RBW; ++j) {
+            int fl, row;
+            if (rinfo(j, fl, row)) {
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    if (m >= RK(fl)) break;
+                    const int fm = s_slot[fl * Ns + hop_from(cs, kn, fl, m)];
+                    if (kq == (fm & 3)) {
+                        T vr = 0, vi = 0;
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb)
+                            if (cb == (fm >> 2)) {
+                                vr = mr[j][cb];
+                                vi = mi[j][cb];
+                            }
+                        s_colb[(fl * 2 + m) * r16 + row] = mk2<C2>(vr, vi);
+                    }
+                    if (row == NF(fl) + s_qrow[fl * Ns + hop_to(cs, kn, fl, m)]) {
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb)
+                            s_vq[(fl * 2 + m) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < 2) {
+            const int fl = tid, R = RK(fl), D = R + m0;
+            if (D > 0) {
+                zc B[9];
+                if (mv) {
+                    cap_move(kn, fl, ci, sa, sb, cj, B);
+                } else {
+                    eye3(B);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (m < R && q < R) B[m * 3 + q] = s_ra[((kn * 2 + fl) * 2 + m) * 2 + q];
+                }
+                inv3(B, s_binv + fl * 9);
+            }
+        }
+        // update coefficients of every row
+        for (int e = tid; e < 2 * r16; e += NT) {
+            const int fl = e >= r16 ? 1 : 0, i = e - fl * r16, R = RK(fl);
+            const int r = fl ? cj : ci, snew = fl ? sa : sb, sold = fl ? sb : sa;
+            zc kp[3] = {zmk(0, 0), zmk(0, 0), zmk(0, 0)}, dd = zmk(0, 0);
+            int rp = -1;
+            if (i < NF(fl)) {
+                if (mv) {
+                    kp[0] = zsub(toz(s_z[fl * r16 + i]), zmk(i == r ? 1.0 : 0.0, 0.0));
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        if (m < R) kp[1 + m] = toz(s_colb[(fl * 2 + m) * r16 + i]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        if (m < R) kp[m] = toz(s_colb[(fl * 2 + m) * r16 + i]);
+                }
+            } else if (i < NF(fl) + NQ(fl)) {
+                const int o = s_qorb[fl * nqm + (i - NF(fl))];
+                if (o >= 0) {
+                    int ms = -1;
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        if (m < R && hop_to(cs, kn, fl, m) == o) ms = m;
+                    if (ms >= 0) {
+                        const int o1 = hop_from(cs, kn, fl, ms);
+                        rp = s_slot[fl * Ns + o1];
+                        if (mv) {
+                            const zc eps = zsub(phi(fl, o1, snew), phi(fl, o1, sold));
+                            dd = eps;
+                            kp[0] = zmul(eps, toz(s_z[fl * r16 + r]));
+#pragma unroll
+                            for (int m = 0; m < 2; ++m)
+                                if (m < R)
+                                    kp[1 + m] = zadd(zmk(m == ms ? 1.0 : 0.0, 0.0),
+                                                     zmul(eps, toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, kn, fl, m)]])));
+                        } else {
+#pragma unroll
+                            for (int m = 0; m < 2; ++m)
+                                if (m < R) kp[m] = zmk(m == ms ? 1.0 : 0.0, 0.0);
+                        }
+                    } else if (mv) {
+                        const zc pn = phi(fl, o, snew), dq = zsub(pn, phi(fl, o, sold));
+                        dd = dq;
+                        kp[0] = zadd(zsub(toz(s_z[fl * r16 + i]), pn), zmul(dq, toz(s_z[fl * r16 + r])));
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+                            if (m < R)
+                                kp[1 + m] = zadd(toz(s_colb[(fl * 2 + m) * r16 + i]),
+                                                 zmul(dq, toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, kn, fl, m)]])));
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+                            if (m < R) kp[m] = toz(s_colb[(fl * 2 + m) * r16 + i]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) s_kap[(fl * r16 + i) * 3 + q] = mk2<C2>(kp[q].x, kp[q].y);
+            s_dl[fl * r16 + i] = mk2<C2>(dd.x, dd.y);
+            s_rep[fl * r16 + i] = rp;
+        }
+        __syncthreads();
+        // W = B^-1 V
+        for (int e = tid; e < 2 * NC; e += NT) {
+            const int fl = e >= NC ? 1 : 0, f = e - fl * NC, R = RK(fl), D = R + m0;
+            zc V[3] = {zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+            if (mv) {
+                V[0] = toz(s_row[fl * NC + f]);
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    if (m < R) {
+                        zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
+                        if (f == s_slot[fl * Ns + hop_from(cs, kn, fl, m)]) v.x -= 1.0;
+                        V[1 + m] = v;
+                    }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    if (m < R) {
+                        zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
+                        if (f == s_slot[fl * Ns + hop_from(cs, kn, fl, m)]) v.x -= 1.0;
+                        V[m] = v;
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q < D) {
+                    zc acc = zmk(0.0, 0.0);
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) acc = zadd(acc, zmul(s_binv[fl * 9 + q * 3 + b], V[b]));
+                    s_wr[(fl * 3 + q) * NC + f] = mk2<C2>(acc.x, acc.y);
+                }
+        }
+        __syncthreads();
+        // the update of every register row
+#pragma unroll
+        for (int j = 0; j < RBW; ++j) {
+            int fl, row;
+            if (rinfo(j, fl, row)) {
+                const int D = RK(fl) + m0;
+                if (D > 0) {
+                    const int ix = fl * r16 + row;
+                    const C2 k0 = s_kap[ix * 3], k1 = s_kap[ix * 3 + 1], k2 = s_kap[ix * 3 + 2], dd = s_dl[ix];
+                    const int rp = s_rep[ix];
+                    const C2 *w0 = s_wr + (fl * 3) * NC, *w1 = w0 + NC, *w2 = w1 + NC, *v0 = s_row + fl * NC;
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb) {
+                        const int f = 4 * cb + kq;
+                        T br = mr[j][cb], bi = mi[j][cb];
+                        if (rp >= 0) {
+                            br = f == rp ? (T)1 : (T)0;
+                            bi = 0;
+                        }
+                        C2 wq = w0[f];
+                        br -= k0.x * wq.x - k0.y * wq.y;
+                        bi -= k0.x * wq.y + k0.y * wq.x;
+                        if (D > 1) {
+                            wq = w1[f];
+                            br -= k1.x * wq.x - k1.y * wq.y;
+                            bi -= k1.x * wq.y + k1.y * wq.x;
+                        }
+                        if (D > 2) {
+                            wq = w2[f];
+                            br -= k2.x * wq.x - k2.y * wq.y;
+                            bi -= k2.x * wq.y + k2.y * wq.x;
+                        }
+                        if (mv) {
+                            const C2 vv = v0[f];
+                            br += dd.x * vv.x - dd.y * vv.y;
+                            bi += dd.x * vv.y + dd.y * vv.x;
+                        }
+                        mr[j][cb] = br;
+                        mi[j][cb] = bi;
+                    }
+                }
+            }
+        }
+        // amplitude and slot / Q-row bookkeeping
+        if (tid == 0) {
+            for (int fl = 0; fl < 2; ++fl) {
+                const zc d = s_dets[kn * 2 + fl];
+                a.logabs[2 * w + fl] += 0.5 * log(znorm(d));
+                a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
+            }
+            for (int fl = 0; fl < 2; ++fl)
+                for (int m = 0; m < RK(fl); ++m) {
+                    const int o1 = hop_from(cs, kn, fl, m), o2 = hop_to(cs, kn, fl, m);
+                    const int fm = s_slot[fl * Ns + o1];
+                    s_slot[fl * Ns + o1] = -1;
+                    s_slot[fl * Ns + o2] = fm;
+                    s_cur[fl * nmax + fm] = o2;
+                    const int q = s_qrow[fl * Ns + o2];
+                    s_qrow[fl * Ns + o2] = -1;
+                    s_qrow[fl * Ns + o1] = q;
+                    s_qorb[fl * nqm + q] = o1;
+                }
+            if (mv) {
+                s_site[sb] = ci;
+                s_site[sa] = (1 << 16) | cj;
+                s_part[ci] = sb;
+                s_part[nmax + cj] = sa;
+            }
+        }
+        cs = kn;
+        __syncthreads();
+        build_ra();
+    };
+
+    build_ra();
+    double S;
+    int kmax;
+    eval0();
+    reduce_alpha(S, kmax);
+    if (a.mode == 2 && kmax != cs && kmax < nk) {
+        apply(false, 0, 0, 0, 0, kmax);
+        eval0();
+        reduce_alpha(S, kmax);
+    }
+    int64_t n_noop = 0, n_acc = 0, n_rej = 0;
+    double sacc[2] = {0, 0};  // n_meas, sum ln w
+    const uint64_t ctr0 = a.counter[w];
+    int64_t mslot = 0;
+    C2 *Zg = a.zscr ? reinterpret_cast<C2 *>(a.zscr) + (size_t)w * z.zt * RS : nullptr;
+    // ProjHeis::measure on the channel (ProjHeis.cpp:40-154)
+    auto measure = [&](bool acc_on) __attribute__((always_inline)) {
+        eval0();
+        double S0;
+        int kk;
+        reduce_alpha(S0, kk);
+        for (int k = tid; k < nk; k += NT) s_a0[k] = s_alpha[k];
+        store_regs();
+        // Z_fl = M_fl x Phi_fl(current orbitals, sites of the other flavour's particles),
+        // 16 targets per tile, on MFMA: Zg[target][row]
+        C2 *Ct = reinterpret_cast<C2 *>(s_un);
+        for (int fl = 0; fl < 2; ++fl) {
+            const int no = NF(1 - fl);
+            for (int t0 = 0; t0 < no; t0 += 16) {
+                for (int e = tid; e < NC * 16; e += NT) {
+                    const int f = e >> 4, tg = t0 + (e & 15);
+                    C2 v = mk2<C2>(0.0, 0.0);
+                    if (f < NF(fl) && tg < no)
+                        v = tabT[((size_t)fl * Ns + s_part[(1 - fl) * nmax + tg]) * Ns + s_cur[fl * nmax + f]];
+                    Ct[e] = v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < RBW; ++j) {
+                    int f_, row;
+                    if (rinfo(j, f_, row) && f_ == fl) {
+                        typename MF::v4 zr = {0, 0, 0, 0}, zi = {0, 0, 0, 0};
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb) {
+                            const C2 b = Ct[(4 * cb + kq) * 16 + li];
+                            zr = MF::mma(mr[j][cb], b.x, zr);
+                            zr = MF::mma(mi[j][cb], -b.y, zr);
+                            zi = MF::mma(mr[j][cb], b.y, zi);
+                            zi = MF::mma(mi[j][cb], b.x, zi);
+                        }
+                        const int rbg = wv + j * NW;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            Zg[(size_t)(t0 + li) * RS + rbg * 16 + MF::orow(lane, v)] = mk2<C2>(zr[v], zi[v]);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // every antiparallel bond's ratio of every state
+        const int ngrp = z.ngrp;
+        for (int e2 = tid; e2 < nk * ngrp; e2 += NT) {
+            const int k = e2 % nk, gq = e2 / nk;
+            zc hs = zmk(0.0, 0.0);
+            for (int e = gq; e < 2 * Ns; e += ngrp) {
+                const int4 eg = a.edge[e];
+                const int ii = s_site[eg.x], ij = s_site[eg.z];
+                if ((ii >> 16) == (ij >> 16)) continue;
+                int ci, cj, sa, sb;
+                if ((ii >> 16) == 0) {
+                    ci = ii & 0xFFFF;
+                    sa = eg.x;
+                    cj = ij & 0xFFFF;
+                    sb = eg.z;
+                } else {
+                    ci = ij & 0xFFFF;
+                    sa = eg.z;
+                    cj = ii & 0xFFFF;
+                    sb = eg.x;
+                }
+                zc tot = zmk((double)hsign(cs, k), 0.0);
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int R = hop_n(cs, k, fl), r = fl ? cj : ci, tg = fl ? ci : cj;
+                    const int snew = fl ? sa : sb, base = fl ? RB0 * 16 : 0;
+                    const C2 *zcol = Zg + (size_t)tg * RS + base;
+                    zc B[9];
+                    eye3(B);
+                    B[0] = toz(zcol[r]);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (q < R) B[1 + q] = toz(Mg[(size_t)(base + r) * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        if (m < R) {
+                            const int o2 = hop_to(cs, k, fl, m);
+                            B[(1 + m) * 3] = zsub(toz(zcol[NF(fl) + s_qrow[fl * Ns + o2]]), phi(fl, o2, snew));
+#pragma unroll
+                            for (int q = 0; q < 2; ++q)
+                                if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+                        }
+                    tot = zmul(tot, det3(B));
+                }
+                hs = zadd(hs, tot);
+            }
+            s_hpart[gq * nk + k] = hs;
+        }
+        int ap = 0;
+        if (tid < 2 * Ns) {
+            const int4 eg = a.edge[tid];
+            ap = (s_site[eg.x] >> 16) != (s_site[eg.z] >> 16);
+        }
+        const int nap = __syncthreads_count(ap);
+        const double cpar = 0.25 * (double)(2 * Ns - nap) - 0.25 * (double)nap;
+        for (int k = tid; k < nk; k += NT) {
+            zc h = zmk(0.0, 0.0);
+            for (int q = 0; q < ngrp; ++q) h = zadd(h, s_hpart[q * nk + k]);
+            const zc a0 = s_a0[k];
+            s_alpha[k] = zmk(-0.5 * h.x + cpar * a0.x, -0.5 * h.y + cpar * a0.y);  // h_k
+        }
+        __syncthreads();
+        const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
+        if (acc_on) {
+            double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
+            for (int e = tid; e < nk * nk; e += NT) {
+                const int k1 = e / nk, k2 = e - k1 * nk;
+                const zc a1 = s_a0[k1], h2 = s_alpha[k2], a2 = s_a0[k2];
+                const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
+                HO[e].x += hv.x / S0;
+                HO[e].y += hv.y / S0;
+                HO[(size_t)nk * nk + e].x += ov.x / S0;
+                HO[(size_t)nk * nk + e].y += ov.y / S0;
+            }
+            if (tid == 0) {
+                sacc[0] += 1;
+                sacc[1] += lw;
+            }
+        }
+        if (tid == 0 && a.tr_logw) a.tr_logw[(size_t)w * a.n_meas_slots + mslot] = lw;
+        ++mslot;
+        __syncthreads();
+    };
+
+    if (a.mode == 4) {
+        const double la = a.logabs[2 * w] + a.logabs[2 * w + 1], ph = a.phase[2 * w] + a.phase[2 * w + 1];
+        for (int k = tid; k < nk; k += NT) {
+            const zc al = s_alpha[k];
+            const double nrm = znorm(al);
+            a.amp_out[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
+            a.amp_out[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
+        }
+    } else if (a.mode == 1) {
+        measure(false);
+    } else if (a.mode == 0) {
+        for (int64_t st = 0; st < a.n_steps; ++st) {
+            int v1, ed;
+            double u3;
+            if (a.uniforms) {
+                const double *up = a.uniforms + ((size_t)w * a.n_steps + st) * 4;
+                v1 = (int)(Ns * up[0]);
+                v1 = v1 < Ns ? v1 : Ns - 1;
+                ed = (int)(4 * up[1]);
+                ed = ed < 4 ? ed : 3;
+                u3 = up[3];
+            } else {
+                const uint64_t sc = ctr0 + (uint64_t)st;
+                uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
+                gv::philox4x32_10_dev(cc, a.key0, a.key1);
+                v1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
+                ed = (int)(cc[1] >> 30);
+                u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
+            }
+            const int v2 = a.nbr[v1 * 4 + ed];
+            const int i1 = s_site[v1], i2 = s_site[v2];
+            const bool noop = (i1 >> 16) == (i2 >> 16);
+            double ratio = 1.0;
+            bool acc;
+            if (noop) {
+                acc = u3 < 1.0;
+                ++n_noop;
+            } else {
+                int ci, cj, sa, sb;
+                if ((i1 >> 16) == 0) {
+                    ci = i1 & 0xFFFF;
+                    sa = v1;
+                    cj = i2 & 0xFFFF;
+                    sb = v2;
+                } else {
+                    ci = i2 & 0xFFFF;
+                    sa = v2;
+                    cj = i1 & 0xFFFF;
+                    sb = v1;
+                }
+                eval_mv(ci, sa, sb, cj);
+                double S2;
+                int kn;
+                reduce_alpha(S2, kn);
+                ratio = S2 / S;
+                acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
+                if (acc) {
+                    const zc ak = s_alpha[kn];
+                    apply(true, ci, sa, sb, cj, kn);
+                    S = S2 / znorm(ak);
+                    ++n_acc;
+                } else {
+                    ++n_rej;
+                }
+            }
+            if (tid == 0) {
+                const size_t ti = (size_t)w * a.n_steps + st;
+                if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
+                if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
+                if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+            }
+            if (a.meas > 0 && (a.meas == 1 || st % a.meas == 0)) measure(true);
+        }
+    }
+    // write back
+    __syncthreads();
+    store_regs();
+    for (int i = tid; i < Ns; i += NT) a.site[(size_t)w * Ns + i] = s_site[i];
+    for (int i = tid; i < 2 * nmax; i += NT) {
+        a.part[(size_t)w * 2 * nmax + i] = s_part[i];
+        a.cur[(size_t)w * 2 * nmax + i] = s_cur[i];
+    }
+    for (int i = tid; i < 2 * nqm; i += NT) a.qorb[(size_t)w * 2 * nqm + i] = s_qorb[i];
+    if (tid == 0) {
+        a.cstate[w] = cs;
+        a.weight[w] = S;
+        if (a.mode == 0) {
+            a.counter[w] = ctr0 + (uint64_t)a.n_steps;
+            double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
+            ac[3] += sacc[0];
+            ac[7] += (double)a.n_steps;
+            ac[8] += (double)n_acc;
+            ac[9] += (double)n_rej;
+            ac[10] += (double)n_noop;
+            ac[11] += sacc[1];
+        }
+    }
+}
+
+// M of every walker from the FP64 inverse of its current state (after the Gauss-Jordan
+// of a refresh): the Ai rows, then Q[q][f] = sum_r Phi(o_q, site r) Ai[r][f] in FP64 for
+// the new orbitals of the current state in increasing order.  One block per
+// (walker, flavour).
+template <typename T> __global__ void __launch_bounds__(256) excr_mbuild_kernel(ExcArgs a, const double2 *Ai64)
+{
+    using C2 = typename cpx<T>::t;
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int Ns = a.L * a.L, nk = a.nk, NC = 4 * a.cb;
+    const int n = fl ? a.n_dn : a.n_up, nq = fl ? a.nq1 : a.nq0;
+    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn, nqm = a.nq0 > a.nq1 ? a.nq0 : a.nq1;
+    const int RBT = a.rb0 + a.rb1, rows = 16 * (fl ? a.rb1 : a.rb0);
+    const size_t chb = (size_t)(a.chan ? a.chan[w] : 0);
+    const int c = a.cstate[w];
+    const int32_t *off = a.new_off + chb * (2 * nk + 1) + c * 2 + fl;
+    const int32_t *no = a.new_orb + off[0];
+    const int nnew = off[1] - off[0];
+    const double2 *Ai = Ai64 + ((size_t)w * 2 + fl) * nmax * nmax;
+    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
+    const int32_t *part = a.part + (size_t)w * 2 * nmax + (size_t)fl * nmax;
+    C2 *M = reinterpret_cast<C2 *>(a.M) + ((size_t)w * RBT * 16 + (fl ? a.rb0 * 16 : 0)) * NC;
+    for (int e = threadIdx.x; e < rows * NC; e += blockDim.x) {
+        const int row = e / NC, f = e - row * NC;
+        double x = 0.0, y = 0.0;
+        if (f < n) {
+            if (row < n) {
+                const double2 v = Ai[(size_t)row * nmax + f];
+                x = v.x;
+                y = v.y;
+            } else if (row < n + nq && row - n < nnew) {
+                const int o = no[row - n];
+                for (int r = 0; r < n; ++r) {
+                    const double2 p = tab[(size_t)o * Ns + part[r]], q = Ai[(size_t)r * nmax + f];
+                    x += p.x * q.x - p.y * q.y;
+                    y += p.x * q.y + p.y * q.x;
+                }
+            }
+        }
+        M[(size_t)row * NC + f] = mk2<C2>(x, y);
+    }
+    for (int q = threadIdx.x; q < nqm; q += blockDim.x)
+        a.qorb[((size_t)w * 2 + fl) * nqm + q] = (q < nq && q < nnew) ? no[q] : -1;
+}
+
+struct Shape {
+    int prec, cb, rbw;
+    void (*kern)(ExcArgs);
+};
+
+#define GR_SHAPE(P, TT, C, R) {P, C, R, excr_kernel<TT, C, R>}
+const Shape kShapes[] = {
+    GR_SHAPE(1, float, 3, 1),  GR_SHAPE(1, float, 5, 1),   GR_SHAPE(1, float, 9, 2),
+    GR_SHAPE(1, float, 13, 2), GR_SHAPE(1, float, 19, 2),  GR_SHAPE(1, float, 19, 3),
+    GR_SHAPE(0, double, 3, 1), GR_SHAPE(0, double, 5, 1),  GR_SHAPE(0, double, 9, 2),
+    GR_SHAPE(0, double, 13, 2), GR_SHAPE(0, double, 19, 2),
+};
+#undef GR_SHAPE
+
+const Shape *find_shape(int prec, int cb, int rbw)
+{
+    for (const Shape &sh : kShapes)
+        if (sh.prec == prec && sh.cb == cb && sh.rbw == rbw) return &sh;
+    return nullptr;
+}
+
+}  // namespace gr
+
+extern "C" int gv_excr_shape(int prec, int cb_min, int row_blocks, int *cb, int *rbw)
+{
+    for (const gr::Shape &sh : gr::kShapes)
+        if (sh.prec == prec && sh.cb >= cb_min && sh.rbw * gr::NW >= row_blocks) {
+            *cb = sh.cb;
+            *rbw = sh.rbw;
+            return 1;
+        }
+    return 0;
+}
+
+extern "C" size_t gv_excr_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0, int nq1, int rb0, int rb1, int cb,
+                                    int nk)
+{
+    const gr::Sizes z = gr::make_sizes(L, n_up, n_dn, nq0, nq1, rb0, rb1, cb, nk);
+    return gr::carve(z, prec == 0 ? 16 : 8).total + 64;
+}
+
+extern "C" hipError_t gv_launch_excr(int prec, int rbw, const gv::ExcArgs *a, int nblocks, hipStream_t st)
+{
+    const gr::Shape *sh = gr::find_shape(prec, a->cb, rbw);
+    if (!sh || (a->rb0 + a->rb1) > rbw * gr::NW || 2 * a->L * a->L > gr::NT) return hipErrorInvalidValue;
+    const size_t lds = gv_excr_lds_bytes(prec, a->L, a->n_up, a->n_dn, a->nq0, a->nq1, a->rb0, a->rb1, a->cb, a->nk);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sh->kern, dim3(nblocks), dim3(gr::NT), lds, st, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t gv_launch_excr_mbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers,
+                                            hipStream_t st)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL(gr::excr_mbuild_kernel<double>, dim3(2 * nwalkers), dim3(256), 0, st, *a, Ai64);
+    else
+        hipLaunchKernelGGL(gr::excr_mbuild_kernel<float>, dim3(2 * nwalkers), dim3(256), 0, st, *a, Ai64);
+    return hipGetLastError();
+}

@@ -110,7 +110,15 @@ struct gpvmc_ctx {
     double2 *d_work = nullptr;
     int32_t *d_hop_n = nullptr, *d_hop_orb = nullptr, *d_hop_new = nullptr, *d_new_off = nullptr,
             *d_new_orb = nullptr, *d_hopsign = nullptr;
-    std::vector<int32_t> state_orb;
+    std::vector<int32_t> state_orb;  // [n_chan][nk][n_up + n_dn]
+    int n_chan = 1;
+    int32_t *d_chan = nullptr;       // [W] channel of each walker
+    std::vector<int32_t> chans;
+    // register-resident channel kernel (exc_reg_kernels.hip) when the walker fits
+    bool regk = false;
+    int rk_cb = 0, rk_rbw = 0, nq0 = 0, nq1 = 0, rb0 = 0, rb1 = 0;
+    void *d_M = nullptr, *d_tabT = nullptr, *d_zscr = nullptr;
+    int32_t *d_qorb = nullptr;
 };
 
 static void free_ctx(gpvmc_ctx *c)
@@ -121,7 +129,7 @@ static void free_ctx(gpvmc_ctx *c)
                     c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork,
                     c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
                     c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
-                    c->d_hopsign};
+                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -202,6 +210,16 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     a.new_off = c->d_new_off;
     a.new_orb = c->d_new_orb;
     a.hopsign = c->d_hopsign;
+    a.chan = c->d_chan;
+    a.M = c->d_M;
+    a.qorb = c->d_qorb;
+    a.tabT = c->d_tabT;
+    a.zscr = c->d_zscr;
+    a.nq0 = c->nq0;
+    a.nq1 = c->nq1;
+    a.rb0 = c->rb0;
+    a.rb1 = c->rb1;
+    a.cb = c->rk_cb;
     a.key0 = (uint32_t)c->cfg.seed;
     a.key1 = (uint32_t)(c->cfg.seed >> 32);
     a.L = c->L;
@@ -212,6 +230,12 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     return a;
 }
 
+static hipError_t launch_exc(gpvmc_ctx *c, const gv::ExcArgs &a)
+{
+    if (c->regk) return gv_launch_excr(c->prec, c->rk_rbw, &a, c->W, c->stream);
+    return gv_launch_exc(c->prec, &a, c->W, c->stream);
+}
+
 // channel contexts: FP64 Gauss-Jordan of the current state's matrices, then the weight
 // bookkeeping (mode 2: also the choice of the dominant state, mode 3: weight only)
 static int run_refresh_channel(gpvmc_ctx *c, int mode)
@@ -220,7 +244,7 @@ static int run_refresh_channel(gpvmc_ctx *c, int mode)
     gv::GJDynArgs g{};
     g.work = c->d_work;
     g.Ai64 = c->d_Ai64;
-    g.Ai32 = c->prec == GPVMC_MIXED ? c->d_Ai32 : nullptr;
+    g.Ai32 = (c->prec == GPVMC_MIXED && !c->regk) ? c->d_Ai32 : nullptr;
     g.logabs = c->d_logabs;
     g.phase = c->d_phase;
     g.status = c->d_status;
@@ -229,7 +253,8 @@ static int run_refresh_channel(gpvmc_ctx *c, int mode)
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(gv_launch_exc_build(&a, c->d_work, c->W, c->stream));
     HIPCHK(gv_launch_gj_dyn(&g, c->W, c->stream));
-    HIPCHK(gv_launch_exc(c->prec, &a, c->W, c->stream));
+    if (c->regk) HIPCHK(gv_launch_excr_mbuild(c->prec, &a, c->d_Ai64, c->W, c->stream));
+    HIPCHK(launch_exc(c, a));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -294,10 +319,12 @@ static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
             part[(size_t)w * 2 * nm + (size_t)fl * nm + cnt[fl]] = v;
             cnt[fl]++;
         }
-        // the current state starts as channel state 0, slots in increasing orbital order
+        // the current state starts as state 0 of the walker's channel, slots in increasing
+        // orbital order
+        const int32_t *so = c->state_orb.data() + (size_t)c->chans[w] * c->nk * Ns;
         for (int fl = 0; fl < 2; ++fl)
             for (int f = 0; f < npt[fl]; ++f)
-                cur[(size_t)w * 2 * nm + (size_t)fl * nm + f] = c->state_orb[(fl ? c->n_up : 0) + f];
+                cur[(size_t)w * 2 * nm + (size_t)fl * nm + f] = so[(fl ? c->n_up : 0) + f];
     }
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipMemcpy(c->d_site, site.data(), site.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -363,7 +390,7 @@ static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t ma
     if (ratio) a.tr_ratio = reinterpret_cast<double *>(sb + o_ra);
     if (logw && nmeas) a.tr_logw = reinterpret_cast<double *>(sb + o_lw);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(gv_launch_exc(c->prec, &a, c->W, c->stream));
+    HIPCHK(launch_exc(c, a));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -851,8 +878,15 @@ int gpvmc_reset_kernel_time(gpvmc_ctx *c)
 
 int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc_ctx **out)
 {
-    if (!cfg || !ch || !out) return fail(GPVMC_ERR_ARG, "null argument");
+    return gpvmc_create_channels(cfg, 1, ch, out);
+}
+
+int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpvmc_channel *chs, gpvmc_ctx **out)
+{
+    if (!cfg || !chs || !out) return fail(GPVMC_ERR_ARG, "null argument");
     *out = nullptr;
+    if (n_channels < 1) return fail(GPVMC_ERR_ARG, "n_channels must be >= 1");
+    const gpvmc_channel *ch = chs;
     const int L = cfg->L;
     if (L < 4 || (L * L) % 2) return fail(GPVMC_ERR_ARG, "L must be >= 4 with L*L even");
     if (cfg->n_walkers < 1) return fail(GPVMC_ERR_ARG, "n_walkers must be >= 1");
@@ -862,16 +896,27 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
     const int Ns = L * L, nk = ch->nk, nu = ch->n_up, nd = ch->n_dn;
     if (nu < 1 || nd < 1 || nu + nd != Ns) return fail(GPVMC_ERR_ARG, "n_up + n_dn must equal L*L");
     if (nk < 1) return fail(GPVMC_ERR_ARG, "nk must be >= 1");
-    if (!ch->state_orb || !ch->hop_n || !ch->hop_orb || !ch->hopsign) return fail(GPVMC_ERR_ARG, "null channel array");
     const int nm = std::max(nu, nd);
     if (nm > 256) return fail(GPVMC_ERR_UNSUPPORTED, "at most 256 particles per flavour");
-    if (gv_exc_lds_bytes(Ns, nm, nk) > 156 * 1024)
-        return fail(GPVMC_ERR_UNSUPPORTED, "channel too large for one workgroup's LDS");
-    for (int i = 0; i < nk * (nu + nd); ++i)
-        if (ch->state_orb[i] < 0 || ch->state_orb[i] >= Ns) return fail(GPVMC_ERR_ARG, "state orbital out of range");
-    // new-orbital lists per (current state, flavour) and the index of every hop's target
-    std::vector<int32_t> hop_new((size_t)nk * nk * 4, 0), new_off((size_t)nk * 2 + 1, 0), new_orb;
-    for (int cc = 0; cc < nk; ++cc)
+    if (cfg->lanes_per_row != 0 && cfg->lanes_per_row != -1)
+        return fail(GPVMC_ERR_ARG, "channel contexts take lanes_per_row 0 (register-resident kernel where the "
+                                   "walker fits) or -1 (streamed kernel)");
+    const size_t nch = (size_t)n_channels, kk = (size_t)nk * nk;
+    std::vector<int32_t> hop_orb(nch * kk * 8), hop_n(nch * kk * 2), hopsign(nch * kk), state_orb;
+    // new-orbital lists per (channel, current state, flavour) and the index of every hop's target
+    std::vector<int32_t> hop_new(nch * kk * 4, 0), new_off(nch * (2 * nk + 1), 0), new_orb;
+    for (size_t q = 0; q < nch; ++q) {
+      ch = chs + q;
+      if (ch->nk != nk || ch->n_up != nu || ch->n_dn != nd)
+          return fail(GPVMC_ERR_ARG, "all channels of a context must have the same n_up, n_dn and nk");
+      if (!ch->state_orb || !ch->hop_n || !ch->hop_orb || !ch->hopsign) return fail(GPVMC_ERR_ARG, "null channel array");
+      for (int i = 0; i < nk * (nu + nd); ++i)
+          if (ch->state_orb[i] < 0 || ch->state_orb[i] >= Ns) return fail(GPVMC_ERR_ARG, "state orbital out of range");
+      state_orb.insert(state_orb.end(), ch->state_orb, ch->state_orb + (size_t)nk * (nu + nd));
+      std::copy(ch->hop_orb, ch->hop_orb + kk * 8, hop_orb.begin() + q * kk * 8);
+      std::copy(ch->hop_n, ch->hop_n + kk * 2, hop_n.begin() + q * kk * 2);
+      std::copy(ch->hopsign, ch->hopsign + kk, hopsign.begin() + q * kk);
+      for (int cc = 0; cc < nk; ++cc)
         for (int fl = 0; fl < 2; ++fl) {
             std::vector<int32_t> lst;
             for (int k = 0; k < nk; ++k) {
@@ -889,15 +934,31 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
                 const int hn = ch->hop_n[((size_t)cc * nk + k) * 2 + fl];
                 for (int h = 0; h < hn; ++h) {
                     const int32_t to = ch->hop_orb[((((size_t)cc * nk + k) * 2 + fl) * 2 + h) * 2 + 1];
-                    hop_new[(((size_t)cc * nk + k) * 2 + fl) * 2 + h] =
+                    hop_new[q * kk * 4 + (((size_t)cc * nk + k) * 2 + fl) * 2 + h] =
                         (int32_t)(std::lower_bound(lst.begin(), lst.end(), to) - lst.begin());
                 }
             }
-            new_off[(size_t)cc * 2 + fl] = (int32_t)new_orb.size();
+            new_off[q * (2 * nk + 1) + (size_t)cc * 2 + fl] = (int32_t)new_orb.size();
             new_orb.insert(new_orb.end(), lst.begin(), lst.end());
         }
-    new_off[(size_t)nk * 2] = (int32_t)new_orb.size();
+      new_off[q * (2 * nk + 1) + (size_t)nk * 2] = (int32_t)new_orb.size();
+    }
     if (new_orb.empty()) new_orb.push_back(0);
+    // rows of the register-resident kernel: the new orbitals of any current state
+    int nqf[2] = {0, 0};
+    for (size_t q = 0; q < nch; ++q)
+        for (int cc = 0; cc < nk; ++cc)
+            for (int fl = 0; fl < 2; ++fl) {
+                const size_t o = q * (2 * nk + 1) + (size_t)cc * 2 + fl;
+                nqf[fl] = std::max(nqf[fl], new_off[o + 1] - new_off[o]);
+            }
+    const int rb0 = (nu + nqf[0] + 15) / 16, rb1 = (nd + nqf[1] + 15) / 16;
+    int kcb = 0, krbw = 0;
+    const bool regk = cfg->lanes_per_row == 0 && 2 * Ns <= 512 &&
+                      gv_excr_shape(cfg->precision, (nm + 3) / 4, rb0 + rb1, &kcb, &krbw) &&
+                      gv_excr_lds_bytes(cfg->precision, L, nu, nd, nqf[0], nqf[1], rb0, rb1, kcb, nk) <= 160 * 1024;
+    if (!regk && gv_exc_lds_bytes(Ns, nm, nk) > 156 * 1024)
+        return fail(GPVMC_ERR_UNSUPPORTED, "channel too large for one workgroup's LDS");
 
     gpvmc_ctx *c = new gpvmc_ctx();
     c->cfg = *cfg;
@@ -913,7 +974,15 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
     c->prec = cfg->precision;
     c->have_orb.assign(c->G, 0);
     c->groups.assign(c->W, 0);
-    c->state_orb.assign(ch->state_orb, ch->state_orb + (size_t)nk * (nu + nd));
+    c->n_chan = n_channels;
+    c->state_orb = state_orb;
+    c->regk = regk;
+    c->rk_cb = kcb;
+    c->rk_rbw = krbw;
+    c->nq0 = nqf[0];
+    c->nq1 = nqf[1];
+    c->rb0 = rb0;
+    c->rb1 = rb1;
     auto bail = [&](int rc) {
         free_ctx(c);
         return rc;
@@ -931,7 +1000,17 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
     const size_t W = c->W;
     CK(hipMalloc(&c->d_Ai64, W * 2 * nm * nm * sizeof(double2)));
     CK(hipMemset(c->d_Ai64, 0, W * 2 * nm * nm * sizeof(double2)));
-    if (c->prec == GPVMC_MIXED) {
+    const size_t ts = c->prec == GPVMC_FP64 ? sizeof(double2) : sizeof(float2);
+    if (regk) {
+        const size_t rows = (size_t)16 * (rb0 + rb1), ncol = (size_t)4 * kcb, zt = (size_t)16 * ((nm + 15) / 16);
+        const size_t nqm = (size_t)std::max(1, std::max(nqf[0], nqf[1]));
+        CK(hipMalloc(&c->d_M, W * rows * ncol * ts));
+        CK(hipMemset(c->d_M, 0, W * rows * ncol * ts));
+        CK(hipMalloc(&c->d_zscr, W * zt * rows * ts));
+        CK(hipMalloc(&c->d_qorb, W * 2 * nqm * sizeof(int32_t)));
+        CK(hipMemset(c->d_qorb, 0xff, W * 2 * nqm * sizeof(int32_t)));
+        CK(hipMalloc(&c->d_tabT, (size_t)c->G * 2 * Ns * Ns * ts));
+    } else if (c->prec == GPVMC_MIXED) {
         CK(hipMalloc(&c->d_Ai32, W * 2 * nm * nm * sizeof(float2)));
         CK(hipMemset(c->d_Ai32, 0, W * 2 * nm * nm * sizeof(float2)));
     }
@@ -956,10 +1035,9 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
     CK(hipMemset(c->d_acc, 0, W * gv::ACC_STRIDE * sizeof(double)));
     CK(hipMemset(c->d_ho, 0, W * 4 * nk * nk * sizeof(double)));
     CK(hipMemset(c->d_counter, 0, W * sizeof(uint64_t)));
-    std::vector<int32_t> hop_orb((size_t)nk * nk * 8), hop_n((size_t)nk * nk * 2), hopsign((size_t)nk * nk);
-    std::copy(ch->hop_orb, ch->hop_orb + hop_orb.size(), hop_orb.begin());
-    std::copy(ch->hop_n, ch->hop_n + hop_n.size(), hop_n.begin());
-    std::copy(ch->hopsign, ch->hopsign + hopsign.size(), hopsign.begin());
+    std::vector<int32_t> chan(W);
+    for (size_t w = 0; w < W; ++w) chan[w] = (int32_t)(w % nch);
+    c->chans.assign(chan.begin(), chan.end());
     auto up = [&](int32_t **d, const std::vector<int32_t> &h) -> hipError_t {
         hipError_t e = hipMalloc(d, h.size() * sizeof(int32_t));
         if (e != hipSuccess) return e;
@@ -971,6 +1049,7 @@ int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc
     CK(up(&c->d_new_off, new_off));
     CK(up(&c->d_new_orb, new_orb));
     CK(up(&c->d_hopsign, hopsign));
+    CK(up(&c->d_chan, chan));
     std::vector<int32_t> nbr, edge4;
     build_lattice(L, nbr, edge4);
     CK(hipMalloc(&c->d_nbr, nbr.size() * sizeof(int32_t)));
@@ -999,7 +1078,57 @@ int gpvmc_set_orbital_table(gpvmc_ctx *c, int32_t group, const double *table)
         for (size_t i = 0; i < n; ++i) f[i] = make_float2((float)table[2 * i], (float)table[2 * i + 1]);
         HIPCHK(hipMemcpy(c->d_tab32 + (size_t)group * n, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
     }
+    if (c->regk) {
+        // [fl][site][orbital] copy in the working precision
+        const size_t Ns = c->Ns;
+        if (c->prec == GPVMC_FP64) {
+            std::vector<double2> t(n);
+            for (size_t fl = 0; fl < 2; ++fl)
+                for (size_t o = 0; o < Ns; ++o)
+                    for (size_t v = 0; v < Ns; ++v) {
+                        const size_t i = (fl * Ns + o) * Ns + v;
+                        t[(fl * Ns + v) * Ns + o] = make_double2(table[2 * i], table[2 * i + 1]);
+                    }
+            HIPCHK(hipMemcpy(static_cast<double2 *>(c->d_tabT) + (size_t)group * n, t.data(), n * sizeof(double2),
+                             hipMemcpyHostToDevice));
+        } else {
+            std::vector<float2> t(n);
+            for (size_t fl = 0; fl < 2; ++fl)
+                for (size_t o = 0; o < Ns; ++o)
+                    for (size_t v = 0; v < Ns; ++v) {
+                        const size_t i = (fl * Ns + o) * Ns + v;
+                        t[(fl * Ns + v) * Ns + o] = make_float2((float)table[2 * i], (float)table[2 * i + 1]);
+                    }
+            HIPCHK(hipMemcpy(static_cast<float2 *>(c->d_tabT) + (size_t)group * n, t.data(), n * sizeof(float2),
+                             hipMemcpyHostToDevice));
+        }
+    }
     c->have_orb[group] = 1;
+    return GPVMC_OK;
+}
+
+int gpvmc_channel_kernel(gpvmc_ctx *c, int32_t *register_resident)
+{
+    if (!c || !register_resident) return fail(GPVMC_ERR_ARG, "null argument");
+    if (!c->channel) return fail(GPVMC_ERR_STATE, "not a channel context");
+    *register_resident = c->regk ? 1 : 0;
+    return GPVMC_OK;
+}
+
+int gpvmc_read_amplitudes(gpvmc_ctx *c, double *amp)
+{
+    if (!c || !amp) return fail(GPVMC_ERR_ARG, "null argument");
+    if (!c->channel) return fail(GPVMC_ERR_STATE, "not a channel context");
+    if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    const size_t bytes = (size_t)c->W * c->nk * 2 * sizeof(double);
+    int rc = ensure_scratch(c, bytes);
+    if (rc) return rc;
+    gv::ExcArgs a = exc_args(c, 4);
+    a.amp_out = static_cast<double *>(c->d_scratch);
+    HIPCHK(launch_exc(c, a));
+    HIPCHK(hipMemcpyAsync(amp, a.amp_out, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return GPVMC_OK;
 }
 

@@ -83,23 +83,33 @@ struct ExcArgs {
     const int32_t *order;     // [blocks] block -> walker (may be null)
     const int32_t *nbr;       // [Ns][4]
     const int4 *edge;         // [2Ns]
-    const int32_t *hop_n;     // [nk][nk][2] orbital hops from state c to k per flavour (<= 2)
-    const int32_t *hop_orb;   // [nk][nk][2][2][2] (from, to) orbitals
-    const int32_t *hop_new;   // [nk][nk][2][2] index of the `to` orbital in new_orb of state c
-    const int32_t *new_off;   // [nk*2 + 1] offsets of new_orb per (c, fl)
-    const int32_t *new_orb;   // orbitals some hop from state c fills, per (c, fl)
-    const int32_t *hopsign;   // [nk][nk] State::HopSign of the path c -> k on state c
+    const int32_t *chan;      // [W] channel of each walker (may be null: channel 0)
+    const int32_t *hop_n;     // [nch][nk][nk][2] orbital hops from state c to k per flavour (<= 2)
+    const int32_t *hop_orb;   // [nch][nk][nk][2][2][2] (from, to) orbitals
+    const int32_t *hop_new;   // [nch][nk][nk][2][2] index of the `to` orbital in new_orb of state c
+    const int32_t *new_off;   // [nch][nk*2 + 1] offsets of new_orb per (c, fl)
+    const int32_t *new_orb;   // orbitals some hop from state c fills, per (channel, c, fl)
+    const int32_t *hopsign;   // [nch][nk][nk] State::HopSign of the path c -> k on state c
     const double *uniforms;
     int8_t *tr_noop;
     int8_t *tr_accept;
     double *tr_ratio;
     double *tr_logw;
+    double *amp_out;          // [W][nk][2] ln|a_k|, arg a_k (mode 4)
+    // register-resident variant: per walker the rows [Ai_fl; Q_fl] of both flavours,
+    // Q_fl[q][f] = sum_r Phi_fl(qorb_fl[q], site r) Ai_fl[r][f] (the new orbitals' rows)
+    void *M;                  // T2 [W][(rb0 + rb1) * 16][4 * cb] (flavour 0 row blocks first)
+    int32_t *qorb;            // [W][2][max(nq0, nq1)] orbital of each Q row
+    const void *tabT;         // T2 [G][2][Ns sites][Ns orbitals] transposed orbital table
+    void *zscr;               // T2 [W][16 * ceil(nmax / 16) targets][(rb0 + rb1) * 16] measurement tiles
+    int32_t nq0, nq1;         // Q rows per flavour
+    int32_t rb0, rb1, cb;     // row blocks of 16 per flavour, column blocks of 4
     int64_t n_steps;
     int64_t meas;
     int64_t n_meas_slots;
     uint32_t key0, key1;
     int32_t L, n_up, n_dn, nk;
-    int32_t mode;             // 0 walk, 1 measurement only, 2 initial state choice, 3 weight only
+    int32_t mode;             // 0 walk, 1 measurement only, 2 initial state choice, 3 weight only, 4 amplitudes
 };
 
 // Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
@@ -148,6 +158,12 @@ hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_
 size_t gv_exc_lds_bytes(int Ns, int nmax, int nk);
 hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st);
+// register-resident channel kernel: the smallest registered shape (column blocks >= cb_min,
+// row-block slots >= row_blocks) of a precision (1 = found)
+int gv_excr_shape(int prec, int cb_min, int row_blocks, int *cb, int *rbw);
+size_t gv_excr_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0, int nq1, int rb0, int rb1, int cb, int nk);
+hipError_t gv_launch_excr(int prec, int rbw, const gv::ExcArgs *a, int nblocks, hipStream_t st);
+hipError_t gv_launch_excr_mbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers, hipStream_t st);
 hipError_t gv_launch_ss_reduce(const double *ss, const int32_t *idx, const int32_t *off, int G,
                                size_t per, double *out, hipStream_t st);
 }

@@ -59,7 +59,9 @@ typedef struct {
     int32_t n_groups;      /* parameter groups (distinct orbital tables), >= 1 */
     int32_t precision;     /* GPVMC_FP64 or GPVMC_MIXED */
     int32_t device;        /* HIP device ordinal */
-    int32_t lanes_per_row; /* 0 = library default; else 1,2,4,8,16 (tuning) */
+    int32_t lanes_per_row; /* 0 = library default; else 1,2,4,8,16 (tuning).  Channel
+                              contexts: 0 = register-resident kernel where the walker
+                              fits, -1 = streamed kernel */
     uint64_t seed;         /* Philox4x32-10 key */
 } gpvmc_config;
 
@@ -144,6 +146,11 @@ typedef struct {
 /* StagFluxTransExciton / StagFluxLongExciton construction + SlaterDeterminant on them
  * (vmc.cpp:140-156).  The context's part arrays are [w][2][max(n_up, n_dn)]. */
 int gpvmc_create_channel(const gpvmc_config *cfg, const gpvmc_channel *ch, gpvmc_ctx **out);
+/* Several channels of equal n_up, n_dn and nk in one context (for instance one per
+ * momentum q, each of which the reference runs as its own vmc process with --qx --qy,
+ * vmc.cpp:140-156); walker w walks on channel w mod n_channels. */
+int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpvmc_channel *chs,
+                          gpvmc_ctx **out);
 /* Full orbital table of one group: table[fl][orbital][site] complex, 2*Ns*Ns values
  * (WaveFunction::init_matrices / MatEl, WaveFunction.cpp:27-37). */
 int gpvmc_set_orbital_table(gpvmc_ctx *ctx, int32_t group, const double *table);
@@ -152,6 +159,16 @@ int gpvmc_set_orbital_table(gpvmc_ctx *ctx, int32_t group, const double *table);
  * be NULL.  The measurement count is acc[3] of gpvmc_read_accumulators. */
 int gpvmc_read_channel(gpvmc_ctx *ctx, int32_t w0, int32_t nw, double *ho, int32_t *cstate,
                        double *weight);
+
+/* 1 if the context runs the register-resident channel kernel (each walker's inverse and
+ * new-orbital rows held in VGPRs, chosen when lanes_per_row is 0 and the walker fits),
+ * 0 for the streamed one (lanes_per_row -1, or a walker too large for registers). */
+int gpvmc_channel_kernel(gpvmc_ctx *ctx, int32_t *register_resident);
+/* Amplitude of every channel state at each walker's current configuration,
+ * amp[w][k] = (ln|a_k|, arg a_k) with a_k the product of both flavours' determinants times
+ * HopSign, as SlaterDeterminant::VirtUpdate returns it for k (SlaterDeterminant.cpp:96-268)
+ * times the current amplitude (SlaterDeterminant::Amp). */
+int gpvmc_read_amplitudes(gpvmc_ctx *ctx, double *amp);
 
 /* Current state: spins[w][site], labels part[w][2][N] (site of each particle),
  * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */

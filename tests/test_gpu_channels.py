@@ -28,17 +28,23 @@ def rand_spins(L, nup, W, seed):
     return out
 
 
+# "auto" runs the register-resident kernel at these sizes, "streamed" the streamed one
+KERNELS = ["auto", "streamed"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,L,q,W,sweeps", [("trans", 4, (1, 2), 6, 6), ("long", 4, (0, 0), 6, 6),
                                                 ("long", 4, (1, 0), 4, 6), ("trans", 6, (2, 3), 3, 3)])
-def test_fp64_channel_walk_matches_oracle(name, L, q, W, sweeps):
-    say("case channel fp64", name, L, q, W)
+def test_fp64_channel_walk_matches_oracle(name, L, q, W, sweeps, kernel):
+    say("case channel fp64", name, L, q, W, kernel)
     ch = channels.Channel(name, L, q)
     tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
     seed = 500 + L + q[0]
     spins = rand_spins(L, ch.npt[0], W, seed)
     Ns = L * L
     nsteps, meas = sweeps * Ns + 1, Ns
-    with binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed) as b:
+    with binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed, kernel=kernel) as b:
+        assert b.register_resident == (kernel == "auto")
         b.set_orbital_table(0, tab)
         ok = b.init(spins)
         out = b.walk(nsteps, meas=meas, trace=True)
@@ -69,15 +75,17 @@ def test_fp64_channel_walk_matches_oracle(name, L, q, W, sweeps):
     assert checked > 0
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,L,q,W", [("trans", 6, (1, 2), 4), ("long", 8, (2, 2), 2)])
-def test_mixed_channel_tracks_oracle(name, L, q, W):
-    say("case channel mixed", name, L, q, W)
+def test_mixed_channel_tracks_oracle(name, L, q, W, kernel):
+    say("case channel mixed", name, L, q, W, kernel)
     ch = channels.Channel(name, L, q)
     tab = model.orbital_table(L, 0.1667 * np.pi, 0.1)
     seed = 700 + L
     spins = rand_spins(L, ch.npt[0], W, seed)
     Ns = L * L
-    with binding.ChannelBatch(L, W, ch, binding.MIXED, seed=seed) as b:
+    with binding.ChannelBatch(L, W, ch, binding.MIXED, seed=seed, kernel=kernel) as b:
+        assert b.register_resident == (kernel == "auto")
         b.set_orbital_table(0, tab)
         ok = b.init(spins)
         out = b.walk(2 * Ns, trace=True)
@@ -107,3 +115,98 @@ def test_mixed_channel_tracks_oracle(name, L, q, W):
                 np.testing.assert_allclose(ho[w], o2["HO"], rtol=0, atol=1e-4 * scale)
     rel = np.concatenate(rels)
     assert np.median(rel) < 1e-3
+
+
+def _same_up_to_sign(la_g, ph_g, la_o, ph_o, live, tol):
+    """a_g = +-a_o on the live states (one sign per walker: the slot order of the current
+    state relative to increasing orbital order)"""
+    assert np.all(np.abs(la_g[live] - la_o[live]) < tol)
+    d = np.angle(np.exp(1j * (ph_g[live] - ph_o[live])))
+    d0 = d[0]
+    assert abs(abs(d0) - np.pi) < 1e-6 or abs(d0) < 1e-6
+    assert np.all(np.abs(np.angle(np.exp(1j * (d - d0)))) < tol)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("prec,tol", [(binding.FP64, 1e-9), (binding.MIXED, 2e-3)])
+@pytest.mark.parametrize("name,L,qs,W", [("trans", 4, [(0, 0), (1, 2), (2, 1)], 6),
+                                         ("long", 6, [(0, 0), (3, 3)], 4),
+                                         ("trans", 8, [(1, 3), (4, 0)], 4)])
+def test_channel_amplitudes_and_several_channels(name, L, qs, W, kernel, prec, tol):
+    """Walkers of several channels (momenta) in one context, walker w on channel w mod n:
+    every state's amplitude at the configuration reached equals the oracle's determinant
+    (up to one sign per walker), and the hop graph used is the walker's own channel's."""
+    say("case channel amplitudes", name, L, qs, W, kernel, prec)
+    chs = [channels.Channel(name, L, q) for q in qs]
+    tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+    seed = 900 + L
+    spins = rand_spins(L, chs[0].npt[0], W, seed)
+    Ns = L * L
+    with binding.ChannelBatch(L, W, chs, prec, seed=seed, kernel=kernel) as b:
+        assert b.register_resident == (kernel == "auto")
+        b.set_orbital_table(0, tab)
+        ok = b.init(spins)
+        b.walk(3 * Ns)
+        if prec == binding.MIXED:
+            b.refresh()
+        amp = b.amplitudes()
+        s_f = b.state()[0]
+        _, cs, _ = b.channel_sums()
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo = O.ExcWalker(L, tab, chs[w % len(qs)], s_f[w])
+        la_o, ph_o = wo.amps()
+        live = la_o > la_o.max() - np.log(1e4)
+        _same_up_to_sign(amp[w].real, amp[w].imag, la_o, ph_o, live, tol)
+        # the current state is the dominant one (first of the largest |a_k|)
+        assert la_o[cs[w]] >= la_o.max() - 1e-6
+
+
+@pytest.mark.parametrize("W,nsteps", [(2, 40)])
+def test_l12_trans_matches_oracle(W, nsteps):
+    """The C4 channel size (L = 12, N_up = 73, N_dn = 71, 72 states): FP64 through both
+    kernels (the FP64 walker needs more LDS than the register-resident kernel has, so
+    both run streamed) with the oracle's accept sequence and ratios; mixed FP32 through
+    the register-resident kernel with ratios within 1e-3 at the median until the first
+    decision that differs."""
+    L, q = 12, (3, 5)
+    ch = channels.Channel("trans", L, q)
+    tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+    seed = 1212
+    spins = rand_spins(L, ch.npt[0], W, seed)
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    outs = {}
+    for kernel in KERNELS:
+        say("case channel L12 fp64", kernel)
+        with binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed, kernel=kernel) as b:
+            b.set_orbital_table(0, tab)
+            ok = b.init(spins)
+            outs[kernel] = (b.register_resident, ok, b.walk(nsteps, trace=True))
+    for w in range(W):
+        wo = O.ExcWalker(L, tab, ch, spins[w])
+        o = wo.walk(u[w].reshape(-1))
+        for kernel, (rk, ok, out) in outs.items():
+            if not ok[w]:
+                continue
+            assert np.array_equal(o["accept"], out["accept"][w]), kernel
+            live = ~o["noop"].astype(bool)
+            r_g, r_o = out["ratio"][w][live], o["ratio"][live]
+            assert (np.abs(r_g - r_o) <= 1e-8 * r_o + 1e-12).all(), kernel
+    say("case channel L12 mixed auto")
+    with binding.ChannelBatch(L, W, ch, binding.MIXED, seed=seed) as b:
+        assert b.register_resident
+        b.set_orbital_table(0, tab)
+        ok = b.init(spins)
+        out = b.walk(nsteps, trace=True)
+    rels = []
+    for w in range(W):
+        if not ok[w]:
+            continue
+        o = O.ExcWalker(L, tab, ch, spins[w]).walk(u[w].reshape(-1))
+        d = np.nonzero(o["accept"] != out["accept"][w])[0]
+        upto = d[0] if len(d) else nsteps
+        live = ~o["noop"][:upto].astype(bool)
+        rels.append(np.abs(out["ratio"][w][:upto][live] - o["ratio"][:upto][live]) / o["ratio"][:upto][live])
+    rel = np.concatenate(rels)
+    assert len(rel) > 10 and np.median(rel) < 1e-3
