@@ -55,6 +55,7 @@ template <> struct Mf<double> {
     __device__ static int orow(int l, int v) { return (l >> 4) + 4 * v; }
 };
 
+// eight waves per walker, two per SIMD
 constexpr int NW = 8, NT = 64 * NW;
 
 struct Sizes {
@@ -130,9 +131,11 @@ template <typename C2> __device__ __forceinline__ C2 mk2(double x, double y)
     return v;
 }
 
-// mode 0: walk n_steps; 1: measurement only; 2: initial state choice (largest |a_k|^2,
-// vmc.cpp:243-254) and weight; 3: weight only; 4: amplitudes into amp_out
-template <typename T, int CB, int RBW>
+// MK = 0, mode 0: walk n_steps (no measurement: the host launches the measurements
+// between walk segments); 2: initial state choice (largest |a_k|^2, vmc.cpp:243-254) and
+// weight; 3: weight only; 4: amplitudes into amp_out.
+// MK = 1: one ProjHeis measurement, into the sums (mode 5) or only the traces (mode 1).
+template <typename T, int CB, int RBW, int MK>
 __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
 {
     using C2 = typename cpx<T>::t;
@@ -140,7 +143,11 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     constexpr int NC = 4 * CB;
     extern __shared__ __align__(16) char smem[];
     const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, li = lane & 15, kq = lane >> 4;
+    // lane ids are re-declared opaque at every step (see the walk loop) so that the
+    // compiler recomputes lane-dependent addresses instead of holding them in VGPRs
+    int tid = threadIdx.x, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+    // wave-uniform values go through readfirstlane so that branches on them are scalar
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const Sizes z = make_sizes(a.L, a.n_up, a.n_dn, a.nq0, a.nq1, a.rb0, a.rb1, CB, a.nk);
     const int Ns = z.Ns, nk = z.nk, nmax = z.nmax, nqm = z.nqm, r16 = z.r16, RB0 = z.rb0;
     const int RBT = z.rb0 + z.rb1, RS = 16 * RBT;
@@ -195,7 +202,11 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         s_cur[i] = a.cur[(size_t)w * 2 * nmax + i];
     }
     for (int i = tid; i < 2 * nqm; i += NT) s_qorb[i] = a.qorb[(size_t)w * 2 * nqm + i];
-    int cs = a.cstate[w];
+    // s_c, s_row (adjacent) and the W rows start at zero: the branch-free update multiplies
+    // them by zero coefficients before the first move writes them
+    for (int i = tid; i < 4 * NC; i += NT) s_c[i] = mk2<C2>(0.0, 0.0);
+    for (int i = tid; i < 6 * NC; i += NT) s_wr[i] = mk2<C2>(0.0, 0.0);
+    int cs = __builtin_amdgcn_readfirstlane(a.cstate[w]);
     __syncthreads();
     for (int i = tid; i < 2 * nmax; i += NT) {
         const int fl = i / nmax, f = i - fl * nmax;
@@ -316,28 +327,47 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             s_c[t] = v;
         }
         __syncthreads();
+        {
+            // column blocks outer, row blocks inner: each c element is used at once
+            T ar[RBW], ai[RBW];
+            int flj[RBW], rowj[RBW];
+            bool okj[RBW];
 #pragma unroll
-        for (int j = 0; j < RBW; ++j) {
-            int fl, row;
-            if (rinfo(j, fl, row)) {
-                const C2 *cv = s_c + fl * NC;
-                T ar = 0, ai = 0;
-#pragma unroll
-                for (int cb = 0; cb < CB; ++cb) {
-                    const C2 c = cv[4 * cb + kq];
-                    ar += mr[j][cb] * c.x - mi[j][cb] * c.y;
-                    ai += mr[j][cb] * c.y + mi[j][cb] * c.x;
-                }
-                ar += __shfl_xor(ar, 16);
-                ai += __shfl_xor(ai, 16);
-                ar += __shfl_xor(ar, 32);
-                ai += __shfl_xor(ai, 32);
-                if (kq == 0) s_z[fl * r16 + row] = mk2<C2>(ar, ai);
-                if (row == (fl ? cj : ci)) {
-#pragma unroll
-                    for (int cb = 0; cb < CB; ++cb) s_row[fl * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
-                }
+            for (int j = 0; j < RBW; ++j) {
+                flj[j] = 0;
+                rowj[j] = 0;
+                okj[j] = rinfo(j, flj[j], rowj[j]);
+                ar[j] = 0;
+                ai[j] = 0;
             }
+            // no branch in the column loop (rows past the end are zero in registers), so
+            // the LDS loads of successive column blocks overlap
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+#pragma unroll
+                for (int j = 0; j < RBW; ++j) {
+                    const C2 c = s_c[flj[j] * NC + 4 * cb + kq];
+                    ar[j] += mr[j][cb] * c.x - mi[j][cb] * c.y;
+                    ai[j] += mr[j][cb] * c.y + mi[j][cb] * c.x;
+                }
+                // bounds how far the scheduler hoists loads (register pressure)
+                if ((cb & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int j = 0; j < RBW; ++j)
+                if (okj[j]) {
+                    const int fl = flj[j], row = rowj[j];
+                    T xr = ar[j], xi = ai[j];
+                    xr += __shfl_xor(xr, 16);
+                    xi += __shfl_xor(xi, 16);
+                    xr += __shfl_xor(xr, 32);
+                    xi += __shfl_xor(xi, 32);
+                    if (kq == 0) s_z[fl * r16 + row] = mk2<C2>(xr, xi);
+                    if (row == (fl ? cj : ci)) {
+#pragma unroll
+                        for (int cb = 0; cb < CB; ++cb) s_row[fl * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+                    }
+                }
         }
         __syncthreads();
         for (int k = tid; k < nk; k += NT) {
@@ -390,13 +420,13 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 bk = s_ired[q];
             }
         }
-        kmax = bk;
+        kmax = __builtin_amdgcn_readfirstlane(bk);
         __syncthreads();
     };
     // move to state kn after eval0 (mv false) or eval_mv (mv true)
     auto apply = [&](bool mv, int ci, int sa, int sb, int cj, int kn) __attribute__((always_inline)) {
         const int m0 = mv ? 1 : 0;
-        const int Rk0 = hop_n(cs, kn, 0), Rk1 = hop_n(cs, kn, 1);
+        const int Rk0 = __builtin_amdgcn_readfirstlane(hop_n(cs, kn, 0)), Rk1 = __builtin_amdgcn_readfirstlane(hop_n(cs, kn, 1));
         auto RK = [&](int fl) { return fl ? Rk1 : Rk0; };
         // columns f_m of M and the Q rows of the filled orbitals, before any change
 #pragma unroll
@@ -529,56 +559,66 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     }
             }
 #pragma unroll
-            for (int q = 0; q < 3; ++q)
+            for (int q = 0; q < 3; ++q) {
+                zc acc = zmk(0.0, 0.0);
                 if (q < D) {
-                    zc acc = zmk(0.0, 0.0);
 #pragma unroll
                     for (int b = 0; b < 3; ++b) acc = zadd(acc, zmul(s_binv[fl * 9 + q * 3 + b], V[b]));
-                    s_wr[(fl * 3 + q) * NC + f] = mk2<C2>(acc.x, acc.y);
                 }
+                s_wr[(fl * 3 + q) * NC + f] = mk2<C2>(acc.x, acc.y);  // zero past D
+            }
         }
         __syncthreads();
-        // the update of every register row
+        // the update of every register row: column blocks outer, no branch inside (the
+        // coefficients of rows that do not change, and the W rows past D, are zero)
+        {
+            C2 kj0[RBW], kj1[RBW], kj2[RBW], dj[RBW];
+            int rpj[RBW], flj[RBW];
 #pragma unroll
-        for (int j = 0; j < RBW; ++j) {
-            int fl, row;
-            if (rinfo(j, fl, row)) {
-                const int D = RK(fl) + m0;
-                if (D > 0) {
+            for (int j = 0; j < RBW; ++j) {
+                int fl = 0, row = 0;
+                kj0[j] = kj1[j] = kj2[j] = dj[j] = mk2<C2>(0.0, 0.0);
+                rpj[j] = -1;
+                flj[j] = 0;
+                if (rinfo(j, fl, row) && RK(fl) + m0 > 0) {
                     const int ix = fl * r16 + row;
-                    const C2 k0 = s_kap[ix * 3], k1 = s_kap[ix * 3 + 1], k2 = s_kap[ix * 3 + 2], dd = s_dl[ix];
-                    const int rp = s_rep[ix];
-                    const C2 *w0 = s_wr + (fl * 3) * NC, *w1 = w0 + NC, *w2 = w1 + NC, *v0 = s_row + fl * NC;
+                    flj[j] = fl;
+                    kj0[j] = s_kap[ix * 3];
+                    kj1[j] = s_kap[ix * 3 + 1];
+                    kj2[j] = s_kap[ix * 3 + 2];
+                    dj[j] = s_dl[ix];
+                    rpj[j] = s_rep[ix];
+                }
+            }
+            // four rank-1 passes (W_0, W_1, W_2, then + delta V_0), each loading one complex
+            // value per element: the register pressure of one pass, not of four
+            const int dmax = __builtin_amdgcn_readfirstlane(Rk0 > Rk1 ? Rk0 + m0 : Rk1 + m0);
 #pragma unroll
-                    for (int cb = 0; cb < CB; ++cb) {
-                        const int f = 4 * cb + kq;
+            for (int pass = 0; pass < 4; ++pass) {
+                // a scalar branch per pass: passes stay separate blocks, and unneeded ones
+                // (W_a past every D, V_0 without a move) are skipped
+                if (pass < 3 ? pass >= dmax : !mv) continue;
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) {
+                    const int f = 4 * cb + kq;
+#pragma unroll
+                    for (int j = 0; j < RBW; ++j) {
+                        const C2 kc = pass == 0 ? kj0[j] : pass == 1 ? kj1[j] : pass == 2 ? kj2[j] : dj[j];
+                        const C2 wv2 = pass < 3 ? s_wr[(flj[j] * 3 + pass) * NC + f] : s_row[flj[j] * NC + f];
                         T br = mr[j][cb], bi = mi[j][cb];
-                        if (rp >= 0) {
-                            br = f == rp ? (T)1 : (T)0;
+                        if (pass == 0 && rpj[j] >= 0) {
+                            br = f == rpj[j] ? (T)1 : (T)0;
                             bi = 0;
                         }
-                        C2 wq = w0[f];
-                        br -= k0.x * wq.x - k0.y * wq.y;
-                        bi -= k0.x * wq.y + k0.y * wq.x;
-                        if (D > 1) {
-                            wq = w1[f];
-                            br -= k1.x * wq.x - k1.y * wq.y;
-                            bi -= k1.x * wq.y + k1.y * wq.x;
-                        }
-                        if (D > 2) {
-                            wq = w2[f];
-                            br -= k2.x * wq.x - k2.y * wq.y;
-                            bi -= k2.x * wq.y + k2.y * wq.x;
-                        }
-                        if (mv) {
-                            const C2 vv = v0[f];
-                            br += dd.x * vv.x - dd.y * vv.y;
-                            bi += dd.x * vv.y + dd.y * vv.x;
-                        }
+                        const T sg = pass < 3 ? (T)-1 : (T)1;
+                        br += sg * (kc.x * wv2.x - kc.y * wv2.y);
+                        bi += sg * (kc.x * wv2.y + kc.y * wv2.x);
                         mr[j][cb] = br;
                         mi[j][cb] = bi;
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                asm volatile("" ::: "memory");
             }
         }
         // amplitude and slot / Q-row bookkeeping
@@ -617,7 +657,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     int kmax;
     eval0();
     reduce_alpha(S, kmax);
-    if (a.mode == 2 && kmax != cs && kmax < nk) {
+    if (MK == 0 && a.mode == 2 && kmax != cs && kmax < nk) {
         apply(false, 0, 0, 0, 0, kmax);
         eval0();
         reduce_alpha(S, kmax);
@@ -629,6 +669,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     C2 *Zg = a.zscr ? reinterpret_cast<C2 *>(a.zscr) + (size_t)w * z.zt * RS : nullptr;
     // ProjHeis::measure on the channel (ProjHeis.cpp:40-154)
     auto measure = [&](bool acc_on) __attribute__((always_inline)) {
+        if constexpr (MK == 0) return;
         eval0();
         double S0;
         int kk;
@@ -719,12 +760,18 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             }
             s_hpart[gq * nk + k] = hs;
         }
-        int ap = 0;
-        if (tid < 2 * Ns) {
-            const int4 eg = a.edge[tid];
-            ap = (s_site[eg.x] >> 16) != (s_site[eg.z] >> 16);
+        if (tid == 0) s_ired[15] = 0;
+        __syncthreads();
+        {
+            int ap = 0;
+            for (int e = tid; e < 2 * Ns; e += NT) {
+                const int4 eg = a.edge[e];
+                ap += (s_site[eg.x] >> 16) != (s_site[eg.z] >> 16);
+            }
+            atomicAdd(&s_ired[15], ap);
         }
-        const int nap = __syncthreads_count(ap);
+        __syncthreads();
+        const int nap = s_ired[15];
         const double cpar = 0.25 * (double)(2 * Ns - nap) - 0.25 * (double)nap;
         for (int k = tid; k < nk; k += NT) {
             zc h = zmk(0.0, 0.0);
@@ -750,12 +797,14 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 sacc[1] += lw;
             }
         }
-        if (tid == 0 && a.tr_logw) a.tr_logw[(size_t)w * a.n_meas_slots + mslot] = lw;
+        if (tid == 0 && a.tr_logw) a.tr_logw[(size_t)w * a.n_meas_slots + a.meas_slot + mslot] = lw;
         ++mslot;
         __syncthreads();
     };
 
-    if (a.mode == 4) {
+    if constexpr (MK == 1) {
+        measure(a.mode == 5);
+    } else if (a.mode == 4) {
         const double la = a.logabs[2 * w] + a.logabs[2 * w + 1], ph = a.phase[2 * w] + a.phase[2 * w + 1];
         for (int k = tid; k < nk; k += NT) {
             const zc al = s_alpha[k];
@@ -763,14 +812,13 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             a.amp_out[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
             a.amp_out[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
         }
-    } else if (a.mode == 1) {
-        measure(false);
     } else if (a.mode == 0) {
         for (int64_t st = 0; st < a.n_steps; ++st) {
+            asm volatile("" : "+v"(tid), "+v"(lane), "+v"(li), "+v"(kq));
             int v1, ed;
             double u3;
             if (a.uniforms) {
-                const double *up = a.uniforms + ((size_t)w * a.n_steps + st) * 4;
+                const double *up = a.uniforms + ((size_t)w * a.n_total + a.st0 + st) * 4;
                 v1 = (int)(Ns * up[0]);
                 v1 = v1 < Ns ? v1 : Ns - 1;
                 ed = (int)(4 * up[1]);
@@ -784,8 +832,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 ed = (int)(cc[1] >> 30);
                 u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
             }
-            const int v2 = a.nbr[v1 * 4 + ed];
-            const int i1 = s_site[v1], i2 = s_site[v2];
+            v1 = __builtin_amdgcn_readfirstlane(v1);
+            ed = __builtin_amdgcn_readfirstlane(ed);
+            const int v2 = __builtin_amdgcn_readfirstlane(a.nbr[v1 * 4 + ed]);
+            const int i1 = __builtin_amdgcn_readfirstlane(s_site[v1]), i2 = __builtin_amdgcn_readfirstlane(s_site[v2]);
             const bool noop = (i1 >> 16) == (i2 >> 16);
             double ratio = 1.0;
             bool acc;
@@ -811,6 +861,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 reduce_alpha(S2, kn);
                 ratio = S2 / S;
                 acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
+                acc = __builtin_amdgcn_readfirstlane(acc ? 1 : 0) != 0;
                 if (acc) {
                     const zc ak = s_alpha[kn];
                     apply(true, ci, sa, sb, cj, kn);
@@ -821,12 +872,11 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 }
             }
             if (tid == 0) {
-                const size_t ti = (size_t)w * a.n_steps + st;
+                const size_t ti = (size_t)w * a.n_total + a.st0 + st;
                 if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
                 if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
                 if (a.tr_ratio) a.tr_ratio[ti] = ratio;
             }
-            if (a.meas > 0 && (a.meas == 1 || st % a.meas == 0)) measure(true);
         }
     }
     // write back
@@ -841,15 +891,18 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     if (tid == 0) {
         a.cstate[w] = cs;
         a.weight[w] = S;
-        if (a.mode == 0) {
-            a.counter[w] = ctr0 + (uint64_t)a.n_steps;
+        if (MK == 1 && a.mode == 5) {
             double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
             ac[3] += sacc[0];
+            ac[11] += sacc[1];
+        }
+        if (MK == 0 && a.mode == 0) {
+            a.counter[w] = ctr0 + (uint64_t)a.n_steps;
+            double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
             ac[7] += (double)a.n_steps;
             ac[8] += (double)n_acc;
             ac[9] += (double)n_rej;
             ac[10] += (double)n_noop;
-            ac[11] += sacc[1];
         }
     }
 }
@@ -901,14 +954,15 @@ template <typename T> __global__ void __launch_bounds__(256) excr_mbuild_kernel(
 struct Shape {
     int prec, cb, rbw;
     void (*kern)(ExcArgs);
+    void (*meas)(ExcArgs);
 };
 
-#define GR_SHAPE(P, TT, C, R) {P, C, R, excr_kernel<TT, C, R>}
+#define GR_SHAPE(P, TT, C, R) {P, C, R, excr_kernel<TT, C, R, 0>, excr_kernel<TT, C, R, 1>}
 const Shape kShapes[] = {
     GR_SHAPE(1, float, 3, 1),  GR_SHAPE(1, float, 5, 1),   GR_SHAPE(1, float, 9, 2),
     GR_SHAPE(1, float, 13, 2), GR_SHAPE(1, float, 19, 2),  GR_SHAPE(1, float, 19, 3),
     GR_SHAPE(0, double, 3, 1), GR_SHAPE(0, double, 5, 1),  GR_SHAPE(0, double, 9, 2),
-    GR_SHAPE(0, double, 13, 2), GR_SHAPE(0, double, 19, 2),
+    GR_SHAPE(0, double, 13, 2),
 };
 #undef GR_SHAPE
 
@@ -942,10 +996,10 @@ extern "C" size_t gv_excr_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0
 extern "C" hipError_t gv_launch_excr(int prec, int rbw, const gv::ExcArgs *a, int nblocks, hipStream_t st)
 {
     const gr::Shape *sh = gr::find_shape(prec, a->cb, rbw);
-    if (!sh || (a->rb0 + a->rb1) > rbw * gr::NW || 2 * a->L * a->L > gr::NT) return hipErrorInvalidValue;
+    if (!sh || (a->rb0 + a->rb1) > rbw * gr::NW) return hipErrorInvalidValue;
     const size_t lds = gv_excr_lds_bytes(prec, a->L, a->n_up, a->n_dn, a->nq0, a->nq1, a->rb0, a->rb1, a->cb, a->nk);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(sh->kern, dim3(nblocks), dim3(gr::NT), lds, st, *a);
+    hipLaunchKernelGGL((a->mode == 1 || a->mode == 5) ? sh->meas : sh->kern, dim3(nblocks), dim3(gr::NT), lds, st, *a);
     return hipGetLastError();
 }
 

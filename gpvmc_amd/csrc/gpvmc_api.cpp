@@ -378,6 +378,7 @@ static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t ma
     char *sb = static_cast<char *>(c->d_scratch);
     gv::ExcArgs a = exc_args(c, measure_only ? 1 : 0);
     a.n_steps = measure_only ? 0 : n_steps;
+    a.n_total = a.n_steps;
     a.meas = meas;
     a.n_meas_slots = nmeas;
     if (uniforms) {
@@ -390,7 +391,36 @@ static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t ma
     if (ratio) a.tr_ratio = reinterpret_cast<double *>(sb + o_ra);
     if (logw && nmeas) a.tr_logw = reinterpret_cast<double *>(sb + o_lw);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(launch_exc(c, a));
+    if (!c->regk || measure_only) {
+        HIPCHK(launch_exc(c, a));
+    } else {
+        // the register-resident kernel walks the segments between measured steps (steps
+        // 0, meas, 2 meas, ... are followed by a measurement) and measures in a launch
+        // of its own
+        a.n_total = n_steps;
+        int64_t st = 0, slot = 0;
+        while (st < n_steps) {
+            int64_t last = n_steps - 1;
+            bool m_after = false;
+            if (meas > 0) {
+                const int64_t m = meas == 1 ? st : ((st + meas - 1) / meas) * meas;
+                if (m < n_steps) {
+                    last = m;
+                    m_after = true;
+                }
+            }
+            a.mode = 0;
+            a.st0 = st;
+            a.n_steps = last - st + 1;
+            HIPCHK(launch_exc(c, a));
+            if (m_after) {
+                a.mode = 5;
+                a.meas_slot = slot++;
+                HIPCHK(launch_exc(c, a));
+            }
+            st = last + 1;
+        }
+    }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -954,7 +984,7 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
             }
     const int rb0 = (nu + nqf[0] + 15) / 16, rb1 = (nd + nqf[1] + 15) / 16;
     int kcb = 0, krbw = 0;
-    const bool regk = cfg->lanes_per_row == 0 && 2 * Ns <= 512 &&
+    const bool regk = cfg->lanes_per_row == 0 &&
                       gv_excr_shape(cfg->precision, (nm + 3) / 4, rb0 + rb1, &kcb, &krbw) &&
                       gv_excr_lds_bytes(cfg->precision, L, nu, nd, nqf[0], nqf[1], rb0, rb1, kcb, nk) <= 160 * 1024;
     if (!regk && gv_exc_lds_bytes(Ns, nm, nk) > 156 * 1024)

@@ -32,6 +32,9 @@ struct SweepArgs {
     int64_t n_steps;
     int64_t meas;
     int64_t n_meas_slots;
+    // register-resident kernel: this launch's first step and trace slot within the call
+    // (walk segments between measurements), and the call's steps per walker
+    int64_t st0, n_total, meas_slot;
     uint32_t mask;
     uint32_t key0, key1;
     int32_t L;
@@ -107,9 +110,13 @@ struct ExcArgs {
     int64_t n_steps;
     int64_t meas;
     int64_t n_meas_slots;
+    // register-resident kernel: this launch's first step and trace slot within the call
+    // (walk segments between measurements), and the call's steps per walker
+    int64_t st0, n_total, meas_slot;
     uint32_t key0, key1;
     int32_t L, n_up, n_dn, nk;
-    int32_t mode;             // 0 walk, 1 measurement only, 2 initial state choice, 3 weight only, 4 amplitudes
+    int32_t mode;             // 0 walk, 1 measurement only, 2 initial state choice, 3 weight only, 4 amplitudes,
+                              // 5 measurement into the sums (register-resident kernel)
 };
 
 // Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
