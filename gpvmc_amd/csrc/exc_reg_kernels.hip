@@ -86,7 +86,7 @@ __host__ __device__ __forceinline__ Sizes make_sizes(int L, int n0, int n1, int 
 // LDS of one walker (byte offsets); ts = bytes of one complex element of the working
 // precision
 struct Off {
-    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, un, total;
+    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, total;
 };
 
 __host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
@@ -117,6 +117,7 @@ __host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts)
     o.binv = take(off, 16 * 18);
     o.red = take(off, 16 * 16);
     o.ired = take(off, 4 * 16);
+    o.hop = take(off, 4 * 19 * (size_t)z.nk);
     const size_t un = (size_t)z.nqm * z.nc > (size_t)z.nc * 16 ? (size_t)z.nqm * z.nc : (size_t)z.nc * 16;
     o.un = take(off, ts * un);
     o.total = off;
@@ -170,6 +171,15 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     zc *const s_red = reinterpret_cast<zc *>(smem + lo.red);
     int *const s_ired = reinterpret_cast<int *>(smem + lo.ired);
     char *const s_un = smem + lo.un;
+    // the current state's row of the hop graph, resolved to Q rows and slots:
+    // hn[k][fl] hops, hto/hfr[k][fl][m] orbitals, hq[k][fl][m] Q row of hto, hs[k][fl][m]
+    // slot of hfr, hsg[k] HopSign
+    int *const s_hn = reinterpret_cast<int *>(smem + lo.hop);
+    int *const s_hto = s_hn + 2 * nk;
+    int *const s_hfr = s_hto + 4 * nk;
+    int *const s_hq = s_hfr + 4 * nk;
+    int *const s_hs = s_hq + 4 * nk;
+    int *const s_hsg = s_hs + 4 * nk;
     C2 *s_c = reinterpret_cast<C2 *>(smem + lo.c2);
     C2 *s_row = s_c + 2 * NC;
     C2 *s_z = s_row + 2 * NC;
@@ -249,6 +259,28 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             }
         }
     };
+    auto cache_hops = [&]() __attribute__((always_inline)) {
+        for (int e = tid; e < nk * 2; e += NT) {
+            const int k = e >> 1, fl = e & 1;
+            const int n = hop_n(cs, k, fl);
+            s_hn[e] = n;
+            if (fl == 0) s_hsg[k] = hsign(cs, k);
+            for (int m = 0; m < 2; ++m) {
+                int o1 = -1, o2 = -1, q = 0, sl = 0;
+                if (m < n) {
+                    o1 = hop_from(cs, k, fl, m);
+                    o2 = hop_to(cs, k, fl, m);
+                    q = s_qrow[fl * Ns + o2];
+                    sl = s_slot[fl * Ns + o1];
+                }
+                s_hfr[e * 2 + m] = o1;
+                s_hto[e * 2 + m] = o2;
+                s_hq[e * 2 + m] = q;
+                s_hs[e * 2 + m] = sl;
+            }
+        }
+        __syncthreads();
+    };
     // RA[k][fl][m][n] = Q_fl[row of hop_to(c, k, fl, m)][slot of hop_from(c, k, fl, n)]
     auto build_ra = [&]() __attribute__((always_inline)) {
         C2 *Qd = reinterpret_cast<C2 *>(s_un);
@@ -267,10 +299,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             __syncthreads();
             for (int e = tid; e < nk * 4; e += NT) {
                 const int k = e >> 2, m = (e >> 1) & 1, q = e & 1;
-                const int R = hop_n(cs, k, fl);
+                const int R = s_hn[k * 2 + fl];
                 zc v = zmk(0.0, 0.0);
                 if (m < R && q < R)
-                    v = toz(Qd[s_qrow[fl * Ns + hop_to(cs, k, fl, m)] * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+                    v = toz(Qd[s_hq[(k * 2 + fl) * 2 + m] * NC + s_hs[(k * 2 + fl) * 2 + q]]);
                 s_ra[((k * 2 + fl) * 2 + m) * 2 + q] = v;
             }
             __syncthreads();
@@ -279,10 +311,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // amplitudes of every state at the current configuration, relative to the current state
     auto eval0 = [&]() __attribute__((always_inline)) {
         for (int k = tid; k < nk; k += NT) {
-            zc tot = zmk((double)hsign(cs, k), 0.0);
+            zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl) {
-                const int R = hop_n(cs, k, fl);
+                const int R = s_hn[k * 2 + fl];
                 zc B[9];
                 eye3(B);
 #pragma unroll
@@ -301,17 +333,17 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // capacitance of state k for the move (ci -> sb, cj -> sa), flavour fl, from the
     // step's LDS vectors
     auto cap_move = [&](int k, int fl, int ci, int sa, int sb, int cj, zc *B) __attribute__((always_inline)) {
-        const int R = hop_n(cs, k, fl), r = fl ? cj : ci, snew = fl ? sa : sb;
+        const int R = s_hn[k * 2 + fl], r = fl ? cj : ci, snew = fl ? sa : sb;
         eye3(B);
         B[0] = toz(s_z[fl * r16 + r]);
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-            if (q < R) B[1 + q] = toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+            if (q < R) B[1 + q] = toz(s_row[fl * NC + s_hs[(k * 2 + fl) * 2 + q]]);
 #pragma unroll
         for (int m = 0; m < 2; ++m)
             if (m < R) {
-                const int o2 = hop_to(cs, k, fl, m);
-                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_qrow[fl * Ns + o2]]), phi(fl, o2, snew));
+                const int o2 = s_hto[(k * 2 + fl) * 2 + m];
+                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_hq[(k * 2 + fl) * 2 + m]]), phi(fl, o2, snew));
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
                     if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
@@ -371,7 +403,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         }
         __syncthreads();
         for (int k = tid; k < nk; k += NT) {
-            zc tot = zmk((double)hsign(cs, k), 0.0);
+            zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl) {
                 zc B[9];
@@ -426,7 +458,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // move to state kn after eval0 (mv false) or eval_mv (mv true)
     auto apply = [&](bool mv, int ci, int sa, int sb, int cj, int kn) __attribute__((always_inline)) {
         const int m0 = mv ? 1 : 0;
-        const int Rk0 = __builtin_amdgcn_readfirstlane(hop_n(cs, kn, 0)), Rk1 = __builtin_amdgcn_readfirstlane(hop_n(cs, kn, 1));
+        const int Rk0 = __builtin_amdgcn_readfirstlane(s_hn[kn * 2]), Rk1 = __builtin_amdgcn_readfirstlane(s_hn[kn * 2 + 1]);
         auto RK = [&](int fl) { return fl ? Rk1 : Rk0; };
         // columns f_m of M and the Q rows of the filled orbitals, before any change
 #pragma unroll
@@ -436,7 +468,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
                     if (m >= RK(fl)) break;
-                    const int fm = s_slot[fl * Ns + hop_from(cs, kn, fl, m)];
+                    const int fm = s_hs[(kn * 2 + fl) * 2 + m];
                     if (kq == (fm & 3)) {
                         T vr = 0, vi = 0;
 #pragma unroll
@@ -447,7 +479,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             }
                         s_colb[(fl * 2 + m) * r16 + row] = mk2<C2>(vr, vi);
                     }
-                    if (row == NF(fl) + s_qrow[fl * Ns + hop_to(cs, kn, fl, m)]) {
+                    if (row == NF(fl) + s_hq[(kn * 2 + fl) * 2 + m]) {
 #pragma unroll
                         for (int cb = 0; cb < CB; ++cb)
                             s_vq[(fl * 2 + m) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
@@ -496,9 +528,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     int ms = -1;
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
-                        if (m < R && hop_to(cs, kn, fl, m) == o) ms = m;
+                        if (m < R && s_hto[(kn * 2 + fl) * 2 + m] == o) ms = m;
                     if (ms >= 0) {
-                        const int o1 = hop_from(cs, kn, fl, ms);
+                        const int o1 = s_hfr[(kn * 2 + fl) * 2 + ms];
                         rp = s_slot[fl * Ns + o1];
                         if (mv) {
                             const zc eps = zsub(phi(fl, o1, snew), phi(fl, o1, sold));
@@ -508,7 +540,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             for (int m = 0; m < 2; ++m)
                                 if (m < R)
                                     kp[1 + m] = zadd(zmk(m == ms ? 1.0 : 0.0, 0.0),
-                                                     zmul(eps, toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, kn, fl, m)]])));
+                                                     zmul(eps, toz(s_row[fl * NC + s_hs[(kn * 2 + fl) * 2 + m]])));
                         } else {
 #pragma unroll
                             for (int m = 0; m < 2; ++m)
@@ -522,7 +554,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                         for (int m = 0; m < 2; ++m)
                             if (m < R)
                                 kp[1 + m] = zadd(toz(s_colb[(fl * 2 + m) * r16 + i]),
-                                                 zmul(dq, toz(s_row[fl * NC + s_slot[fl * Ns + hop_from(cs, kn, fl, m)]])));
+                                                 zmul(dq, toz(s_row[fl * NC + s_hs[(kn * 2 + fl) * 2 + m]])));
                     } else {
 #pragma unroll
                         for (int m = 0; m < 2; ++m)
@@ -546,7 +578,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 for (int m = 0; m < 2; ++m)
                     if (m < R) {
                         zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
-                        if (f == s_slot[fl * Ns + hop_from(cs, kn, fl, m)]) v.x -= 1.0;
+                        if (f == s_hs[(kn * 2 + fl) * 2 + m]) v.x -= 1.0;
                         V[1 + m] = v;
                     }
             } else {
@@ -554,7 +586,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 for (int m = 0; m < 2; ++m)
                     if (m < R) {
                         zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
-                        if (f == s_slot[fl * Ns + hop_from(cs, kn, fl, m)]) v.x -= 1.0;
+                        if (f == s_hs[(kn * 2 + fl) * 2 + m]) v.x -= 1.0;
                         V[m] = v;
                     }
             }
@@ -630,7 +662,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             }
             for (int fl = 0; fl < 2; ++fl)
                 for (int m = 0; m < RK(fl); ++m) {
-                    const int o1 = hop_from(cs, kn, fl, m), o2 = hop_to(cs, kn, fl, m);
+                    const int o1 = s_hfr[(kn * 2 + fl) * 2 + m], o2 = s_hto[(kn * 2 + fl) * 2 + m];
                     const int fm = s_slot[fl * Ns + o1];
                     s_slot[fl * Ns + o1] = -1;
                     s_slot[fl * Ns + o2] = fm;
@@ -649,19 +681,19 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         }
         cs = kn;
         __syncthreads();
+        cache_hops();
         build_ra();
     };
 
+    cache_hops();
     build_ra();
     double S;
     int kmax;
     eval0();
     reduce_alpha(S, kmax);
-    if (MK == 0 && a.mode == 2 && kmax != cs && kmax < nk) {
-        apply(false, 0, 0, 0, 0, kmax);
-        eval0();
-        reduce_alpha(S, kmax);
-    }
+    // mode 2: the hop to the dominant state runs as step -1 of the walk loop (no lattice
+    // move), so that the kernel holds a single copy of apply
+    const bool hop0 = MK == 0 && a.mode == 2 && kmax != cs && kmax < nk;
     int64_t n_noop = 0, n_acc = 0, n_rej = 0;
     double sacc[2] = {0, 0};  // n_meas, sum ln w
     const uint64_t ctr0 = a.counter[w];
@@ -733,10 +765,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     cj = ii & 0xFFFF;
                     sb = eg.x;
                 }
-                zc tot = zmk((double)hsign(cs, k), 0.0);
+                zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = hop_n(cs, k, fl), r = fl ? cj : ci, tg = fl ? ci : cj;
+                    const int R = s_hn[k * 2 + fl], r = fl ? cj : ci, tg = fl ? ci : cj;
                     const int snew = fl ? sa : sb, base = fl ? RB0 * 16 : 0;
                     const C2 *zcol = Zg + (size_t)tg * RS + base;
                     zc B[9];
@@ -744,12 +776,12 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     B[0] = toz(zcol[r]);
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = toz(Mg[(size_t)(base + r) * NC + s_slot[fl * Ns + hop_from(cs, k, fl, q)]]);
+                        if (q < R) B[1 + q] = toz(Mg[(size_t)(base + r) * NC + s_hs[(k * 2 + fl) * 2 + q]]);
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
                         if (m < R) {
-                            const int o2 = hop_to(cs, k, fl, m);
-                            B[(1 + m) * 3] = zsub(toz(zcol[NF(fl) + s_qrow[fl * Ns + o2]]), phi(fl, o2, snew));
+                            const int o2 = s_hto[(k * 2 + fl) * 2 + m];
+                            B[(1 + m) * 3] = zsub(toz(zcol[NF(fl) + s_hq[(k * 2 + fl) * 2 + m]]), phi(fl, o2, snew));
 #pragma unroll
                             for (int q = 0; q < 2; ++q)
                                 if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
@@ -812,71 +844,78 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             a.amp_out[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
             a.amp_out[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
         }
-    } else if (a.mode == 0) {
-        for (int64_t st = 0; st < a.n_steps; ++st) {
+    } else if (MK == 0 && (a.mode == 0 || hop0)) {
+        const int64_t n_walk = a.mode == 0 ? a.n_steps : 0;
+        for (int64_t st = hop0 ? -1 : 0; st < n_walk; ++st) {
             asm volatile("" : "+v"(tid), "+v"(lane), "+v"(li), "+v"(kq));
-            int v1, ed;
-            double u3;
-            if (a.uniforms) {
-                const double *up = a.uniforms + ((size_t)w * a.n_total + a.st0 + st) * 4;
-                v1 = (int)(Ns * up[0]);
-                v1 = v1 < Ns ? v1 : Ns - 1;
-                ed = (int)(4 * up[1]);
-                ed = ed < 4 ? ed : 3;
-                u3 = up[3];
-            } else {
-                const uint64_t sc = ctr0 + (uint64_t)st;
-                uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
-                gv::philox4x32_10_dev(cc, a.key0, a.key1);
-                v1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
-                ed = (int)(cc[1] >> 30);
-                u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
-            }
-            v1 = __builtin_amdgcn_readfirstlane(v1);
-            ed = __builtin_amdgcn_readfirstlane(ed);
-            const int v2 = __builtin_amdgcn_readfirstlane(a.nbr[v1 * 4 + ed]);
-            const int i1 = __builtin_amdgcn_readfirstlane(s_site[v1]), i2 = __builtin_amdgcn_readfirstlane(s_site[v2]);
-            const bool noop = (i1 >> 16) == (i2 >> 16);
-            double ratio = 1.0;
-            bool acc;
-            if (noop) {
-                acc = u3 < 1.0;
-                ++n_noop;
-            } else {
-                int ci, cj, sa, sb;
-                if ((i1 >> 16) == 0) {
-                    ci = i1 & 0xFFFF;
-                    sa = v1;
-                    cj = i2 & 0xFFFF;
-                    sb = v2;
+            bool mv = st >= 0, acc = true, noop = false;
+            int ci = 0, cj = 0, sa = 0, sb = 0, kn = kmax;
+            double ratio = 1.0, S2 = 0.0;
+            if (mv) {
+                int v1, ed;
+                double u3;
+                if (a.uniforms) {
+                    const double *up = a.uniforms + ((size_t)w * a.n_total + a.st0 + st) * 4;
+                    v1 = (int)(Ns * up[0]);
+                    v1 = v1 < Ns ? v1 : Ns - 1;
+                    ed = (int)(4 * up[1]);
+                    ed = ed < 4 ? ed : 3;
+                    u3 = up[3];
                 } else {
-                    ci = i2 & 0xFFFF;
-                    sa = v2;
-                    cj = i1 & 0xFFFF;
-                    sb = v1;
+                    const uint64_t sc = ctr0 + (uint64_t)st;
+                    uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
+                    gv::philox4x32_10_dev(cc, a.key0, a.key1);
+                    v1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
+                    ed = (int)(cc[1] >> 30);
+                    u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
                 }
-                eval_mv(ci, sa, sb, cj);
-                double S2;
-                int kn;
-                reduce_alpha(S2, kn);
-                ratio = S2 / S;
-                acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
-                acc = __builtin_amdgcn_readfirstlane(acc ? 1 : 0) != 0;
-                if (acc) {
-                    const zc ak = s_alpha[kn];
-                    apply(true, ci, sa, sb, cj, kn);
+                v1 = __builtin_amdgcn_readfirstlane(v1);
+                ed = __builtin_amdgcn_readfirstlane(ed);
+                const int v2 = __builtin_amdgcn_readfirstlane(a.nbr[v1 * 4 + ed]);
+                const int i1 = __builtin_amdgcn_readfirstlane(s_site[v1]), i2 = __builtin_amdgcn_readfirstlane(s_site[v2]);
+                noop = (i1 >> 16) == (i2 >> 16);
+                if (noop) {
+                    acc = u3 < 1.0;
+                    ++n_noop;
+                } else {
+                    if ((i1 >> 16) == 0) {
+                        ci = i1 & 0xFFFF;
+                        sa = v1;
+                        cj = i2 & 0xFFFF;
+                        sb = v2;
+                    } else {
+                        ci = i2 & 0xFFFF;
+                        sa = v2;
+                        cj = i1 & 0xFFFF;
+                        sb = v1;
+                    }
+                    eval_mv(ci, sa, sb, cj);
+                    reduce_alpha(S2, kn);
+                    ratio = S2 / S;
+                    acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
+                    acc = __builtin_amdgcn_readfirstlane(acc ? 1 : 0) != 0;
+                }
+            }
+            if (!noop && acc) {
+                const zc ak = s_alpha[kn];
+                apply(mv, ci, sa, sb, cj, kn);
+                if (mv) {
                     S = S2 / znorm(ak);
                     ++n_acc;
-                } else {
-                    ++n_rej;
                 }
+            } else if (!noop) {
+                ++n_rej;
             }
-            if (tid == 0) {
+            if (mv && tid == 0) {
                 const size_t ti = (size_t)w * a.n_total + a.st0 + st;
                 if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
                 if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
                 if (a.tr_ratio) a.tr_ratio[ti] = ratio;
             }
+        }
+        if (hop0) {
+            eval0();
+            reduce_alpha(S, kmax);
         }
     }
     // write back
