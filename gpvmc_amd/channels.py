@@ -95,27 +95,48 @@ class Channel:
         self.nexc = len(self.states)
         self.npt = (len(self.states[0][0]), len(self.states[0][1]))
         K = self.nexc
-        occ = [[set(s[fl].tolist()) for fl in range(2)] for s in self.states]
+        # every state is E = {0, 2, ..., 2N-2} (the band-0 orbitals) with a few orbitals
+        # removed and added per flavour, so each pair's path and sign cost O(1)
+        E = set(even)
+        rem = [[sorted(E - set(st[fl].tolist())) for fl in range(2)] for st in self.states]
+        add = [[sorted(set(st[fl].tolist()) - E) for fl in range(2)] for st in self.states]
+
+        def n_even_between(lo, hi):  # members of E strictly between lo and hi
+            a, b = lo + 1, min(hi - 1, 2 * N - 2)
+            if b < a:
+                return 0
+            return b // 2 - (a + 1) // 2 + 1
+
         self.hops = [[[None, None] for _ in range(K)] for _ in range(K)]
         self.hopsign = np.ones((K, K), np.int8)
         for c in range(K):
             for k in range(K):
                 odd = False
                 for fl in range(2):
-                    src, dst = occ[c][fl], occ[k][fl]
-                    holes = sorted(src - dst)
-                    parts = sorted(dst - src)
+                    rc, ac, rk, ak = rem[c][fl], add[c][fl], rem[k][fl], add[k][fl]
+                    # occupied in c but not in k: c's additions k lacks, and E-orbitals k
+                    # removed that c kept; the reverse for the targets
+                    holes = sorted([o for o in ac if o not in ak] + [o for o in rk if o not in rc])
+                    parts = sorted([o for o in ak if o not in ac] + [o for o in rc if o not in rk])
                     path = list(zip(holes, parts))
                     self.hops[c][k][fl] = path
-                    # State::HopSign on state c's occupation (paths here have at most two
-                    # hops per flavour, where the reference's bookkeeping is exact)
-                    sf = set(src)
+                    # State::HopSign on state c's occupation, hop after hop (paths here
+                    # have at most two hops per flavour, where the reference's bookkeeping
+                    # is exact)
+                    gone, came = list(rc), list(ac)
                     for a, b in path:
                         lo, hi = min(a, b), max(a, b)
-                        n_between = sum(1 for o in sf if lo < o < hi)
+                        n_between = (n_even_between(lo, hi) - sum(1 for o in gone if lo < o < hi)
+                                     + sum(1 for o in came if lo < o < hi))
                         odd ^= bool(n_between & 1)
-                        sf.discard(a)
-                        sf.add(b)
+                        if a in came:
+                            came.remove(a)
+                        else:
+                            gone.append(a)
+                        if b in gone:
+                            gone.remove(b)
+                        else:
+                            came.append(b)
                 self.hopsign[c, k] = -1 if odd else 1
         self.max_hops = max(len(self.hops[c][k][fl]) for c in range(K) for k in range(K) for fl in range(2))
 
