@@ -311,9 +311,13 @@ def main():
     ach_gbs = bytes_launch / (avg_ms * 1e-3) / 1e9
     ach_tf = flops_launch / (avg_ms * 1e-3) / 1e12
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "sweep_L16_mixed_traffic.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
+    # plain copy's readings); recorded in the profile summary at the default workload
+    tf = os.path.join(ROOT, "profiles", "sweep_L16_mixed_traffic.md")
+    if os.path.exists(tf) and L == 16 and W == 16384:
+        for ln in open(tf):
+            if ln.startswith("hbm_bytes_per_launch:"):
+                traffic = float(ln.split(":")[1])
     # The ratio tables are register-resident for the whole launch, so HBM does not bound
     # the sweep: its roofline is FP32 arithmetic (157.3 TFLOP/s, the same peak for VALU
     # and MFMA FP32 on MI355X).  The byte model is kept beside it, with the measured HBM

@@ -41,6 +41,10 @@ SIGNATURES = {
     "gpvmc_create_channels": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.POINTER(_vp)]),
     "gpvmc_read_amplitudes": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_channel_kernel": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_batch_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "gpvmc_batch_add": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32]),
+    "gpvmc_batch_sweep": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "gpvmc_batch_destroy": (ctypes.c_int, [_vp]),
     "gpvmc_set_orbital_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_channel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -326,3 +330,42 @@ class ChannelBatch(Batch):
         wt = np.zeros(nw)
         self._check(self.lib.gpvmc_read_channel(self.h, w0, nw, _ptr(ho), _ptr(cs), _ptr(wt)))
         return ho, cs, wt
+
+
+class Sweeper:
+    """Contexts (Batch or ChannelBatch, any sizes) swept together through one captured
+    hipGraph per sweep (gpvmc_batch_*): add(batch, n_steps, meas, mask), then sweep(n)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.members = []
+        h = ctypes.c_void_p()
+        rc = self.lib.gpvmc_batch_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise GpvmcError(self.lib.gpvmc_last_error().decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GpvmcError(self.lib.gpvmc_last_error().decode())
+
+    def add(self, batch, n_steps, meas=0, mask=MEAS_PROJHEIS):
+        self._check(self.lib.gpvmc_batch_add(self.h, batch.h, n_steps, meas, mask))
+        self.members.append(batch)
+
+    def sweep(self, n=1):
+        """runs n sweeps of every member; returns their device time in ms"""
+        ms = ctypes.c_double()
+        self._check(self.lib.gpvmc_batch_sweep(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h is not None:
+            self.lib.gpvmc_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -230,10 +230,43 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     return a;
 }
 
-static hipError_t launch_exc(gpvmc_ctx *c, const gv::ExcArgs &a)
+static hipError_t launch_exc(gpvmc_ctx *c, const gv::ExcArgs &a, hipStream_t st = nullptr)
 {
-    if (c->regk) return gv_launch_excr(c->prec, c->rk_rbw, &a, c->W, c->stream);
-    return gv_launch_exc(c->prec, &a, c->W, c->stream);
+    if (!st) st = c->stream;
+    if (c->regk) return gv_launch_excr(c->prec, c->rk_rbw, &a, c->W, st);
+    return gv_launch_exc(c->prec, &a, c->W, st);
+}
+
+// The register-resident channel kernel walks the segments between measured steps (steps
+// 0, meas, 2 meas, ... are followed by a measurement) and measures in a launch of its own.
+static hipError_t launch_exc_walk(gpvmc_ctx *c, gv::ExcArgs a, int64_t n_steps, int64_t meas, hipStream_t st)
+{
+    a.n_total = n_steps;
+    int64_t s0 = 0, slot = 0;
+    while (s0 < n_steps) {
+        int64_t last = n_steps - 1;
+        bool m_after = false;
+        if (meas > 0) {
+            const int64_t m = meas == 1 ? s0 : ((s0 + meas - 1) / meas) * meas;
+            if (m < n_steps) {
+                last = m;
+                m_after = true;
+            }
+        }
+        a.mode = 0;
+        a.st0 = s0;
+        a.n_steps = last - s0 + 1;
+        hipError_t e = launch_exc(c, a, st);
+        if (e != hipSuccess) return e;
+        if (m_after) {
+            a.mode = 5;
+            a.meas_slot = slot++;
+            e = launch_exc(c, a, st);
+            if (e != hipSuccess) return e;
+        }
+        s0 = last + 1;
+    }
+    return hipSuccess;
 }
 
 // channel contexts: FP64 Gauss-Jordan of the current state's matrices, then the weight
@@ -394,32 +427,7 @@ static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t ma
     if (!c->regk || measure_only) {
         HIPCHK(launch_exc(c, a));
     } else {
-        // the register-resident kernel walks the segments between measured steps (steps
-        // 0, meas, 2 meas, ... are followed by a measurement) and measures in a launch
-        // of its own
-        a.n_total = n_steps;
-        int64_t st = 0, slot = 0;
-        while (st < n_steps) {
-            int64_t last = n_steps - 1;
-            bool m_after = false;
-            if (meas > 0) {
-                const int64_t m = meas == 1 ? st : ((st + meas - 1) / meas) * meas;
-                if (m < n_steps) {
-                    last = m;
-                    m_after = true;
-                }
-            }
-            a.mode = 0;
-            a.st0 = st;
-            a.n_steps = last - st + 1;
-            HIPCHK(launch_exc(c, a));
-            if (m_after) {
-                a.mode = 5;
-                a.meas_slot = slot++;
-                HIPCHK(launch_exc(c, a));
-            }
-            st = last + 1;
-        }
+        HIPCHK(launch_exc_walk(c, a, n_steps, meas, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
@@ -617,6 +625,34 @@ int gpvmc_refresh(gpvmc_ctx *c, double *drift)
     return GPVMC_OK;
 }
 
+static gv::SweepArgs sweep_args(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask, int64_t nmeas,
+                                int measure_only)
+{
+    gv::SweepArgs a{};
+    a.G = c->d_G;
+    a.site = c->d_site;
+    a.part = c->d_part;
+    a.logabs = c->d_logabs;
+    a.phase = c->d_phase;
+    a.counter = c->d_counter;
+    a.acc = c->d_acc;
+    a.occ = c->prec == GPVMC_FP64 ? (const void *)c->d_occ64 : (const void *)c->d_occ32;
+    a.group = c->d_group;
+    a.order = c->d_order;
+    a.nbr = c->d_nbr;
+    a.edge = c->d_edge;
+    a.n_steps = n_steps;
+    a.meas = meas;
+    a.n_meas_slots = nmeas;
+    a.mask = mask;
+    a.key0 = (uint32_t)c->cfg.seed;
+    a.key1 = (uint32_t)(c->cfg.seed >> 32);
+    a.L = c->L;
+    a.measure_only = measure_only;
+    a.ss = c->d_ss;
+    return a;
+}
+
 static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
                      const double *uniforms, int8_t *noop, int8_t *accept, double *ratio,
                      double *eloc, double *mz, double *logw, int measure_only)
@@ -663,28 +699,7 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
         if (rc) return rc;
     }
     char *sb = static_cast<char *>(c->d_scratch);
-    gv::SweepArgs a{};
-    a.G = c->d_G;
-    a.site = c->d_site;
-    a.part = c->d_part;
-    a.logabs = c->d_logabs;
-    a.phase = c->d_phase;
-    a.counter = c->d_counter;
-    a.acc = c->d_acc;
-    a.occ = c->prec == GPVMC_FP64 ? (const void *)c->d_occ64 : (const void *)c->d_occ32;
-    a.group = c->d_group;
-    a.order = c->d_order;
-    a.nbr = c->d_nbr;
-    a.edge = c->d_edge;
-    a.n_steps = n_steps;
-    a.meas = meas;
-    a.n_meas_slots = nmeas;
-    a.mask = mask;
-    a.key0 = (uint32_t)c->cfg.seed;
-    a.key1 = (uint32_t)(c->cfg.seed >> 32);
-    a.L = c->L;
-    a.measure_only = measure_only;
-    a.ss = c->d_ss;
+    gv::SweepArgs a = sweep_args(c, n_steps, meas, mask, nmeas, measure_only);
     if (uniforms) {
         a.uniforms = reinterpret_cast<const double *>(sb + o_u);
         HIPCHK(hipMemcpyAsync(sb + o_u, uniforms, W * n_steps * 4 * sizeof(double),
@@ -1134,6 +1149,147 @@ int gpvmc_set_orbital_table(gpvmc_ctx *c, int32_t group, const double *table)
         }
     }
     c->have_orb[group] = 1;
+    return GPVMC_OK;
+}
+
+// ---- batches of contexts swept together ------------------------------------------------
+
+struct gpvmc_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, fork = nullptr;
+    struct Member {
+        gpvmc_ctx *c;
+        int64_t n_steps, meas;
+        uint32_t mask;
+        hipStream_t s;
+        hipEvent_t join;
+    };
+    std::vector<Member> m;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void drop_graph(gpvmc_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// one sweep of one member, kernels only (no host synchronisation, no traces), on stream st
+static hipError_t enqueue_member(const gpvmc_batch::Member &mb, hipStream_t st)
+{
+    gpvmc_ctx *c = mb.c;
+    const int64_t nmeas = mb.meas > 0 ? (mb.meas == 1 ? mb.n_steps : (mb.n_steps + mb.meas - 1) / mb.meas) : 0;
+    if (c->channel) {
+        gv::ExcArgs a = exc_args(c, 0);
+        a.n_steps = mb.n_steps;
+        a.n_total = mb.n_steps;
+        a.meas = mb.meas;
+        a.n_meas_slots = nmeas;
+        if (c->regk) return launch_exc_walk(c, a, mb.n_steps, mb.meas, st);
+        return launch_exc(c, a, st);
+    }
+    gv::SweepArgs a = sweep_args(c, mb.n_steps, mb.meas, mb.mask, nmeas, 0);
+    return gv_launch_sweep(c->prec, c->N, c->lpr, &a, c->W, st);
+}
+
+int gpvmc_batch_create(int32_t device, gpvmc_batch **out)
+{
+    if (!out) return fail(GPVMC_ERR_ARG, "null argument");
+    *out = nullptr;
+    gpvmc_batch *b = new gpvmc_batch();
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        gpvmc_batch_destroy(b);
+        return fail(GPVMC_ERR_HIP, std::string("batch creation: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return GPVMC_OK;
+}
+
+int gpvmc_batch_add(gpvmc_batch *b, gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask)
+{
+    if (!b || !c) return fail(GPVMC_ERR_ARG, "null argument");
+    if (c->cfg.device != b->device) return fail(GPVMC_ERR_ARG, "context on another device");
+    if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
+    if (n_steps < 1 || meas < 0) return fail(GPVMC_ERR_ARG, "n_steps must be >= 1 and meas >= 0");
+    if (mask & ~(uint32_t)(GPVMC_MEAS_PROJHEIS | GPVMC_MEAS_STAGMAGNZ | GPVMC_MEAS_OVERLAP | GPVMC_MEAS_STATSPIN))
+        return fail(GPVMC_ERR_ARG, "unknown estimator bit in mask");
+    for (const auto &mb : b->m)
+        if (mb.c == c) return fail(GPVMC_ERR_ARG, "context already in the batch");
+    HIPCHK(hipSetDevice(b->device));
+    if (!c->channel && (mask & GPVMC_MEAS_STATSPIN) && meas > 0) {
+        int rc = ensure_statspin(c);
+        if (rc) return rc;
+    }
+    gpvmc_batch::Member mb{c, n_steps, meas, mask, nullptr, nullptr};
+    HIPCHK(hipStreamCreateWithFlags(&mb.s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&mb.join, hipEventDisableTiming));
+    b->m.push_back(mb);
+    drop_graph(b);
+    return GPVMC_OK;
+}
+
+int gpvmc_batch_sweep(gpvmc_batch *b, int32_t n_sweeps, double *device_ms)
+{
+    if (!b) return fail(GPVMC_ERR_ARG, "null batch");
+    if (n_sweeps < 0) return fail(GPVMC_ERR_ARG, "negative n_sweeps");
+    if (b->m.empty()) return fail(GPVMC_ERR_STATE, "empty batch");
+    HIPCHK(hipSetDevice(b->device));
+    // members' own streams may hold earlier work (init, refresh, walks)
+    for (const auto &mb : b->m) HIPCHK(hipStreamSynchronize(mb.c->stream));
+    if (!b->exec) {
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipEventRecord(b->fork, b->stream);
+        for (size_t i = 0; e == hipSuccess && i < b->m.size(); ++i) {
+            const auto &mb = b->m[i];
+            e = hipStreamWaitEvent(mb.s, b->fork, 0);
+            if (e == hipSuccess) e = enqueue_member(mb, mb.s);
+            if (e == hipSuccess) e = hipEventRecord(mb.join, mb.s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, mb.join, 0);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(GPVMC_ERR_HIP, std::string("sweep capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int i = 0; i < n_sweeps; ++i) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (device_ms) *device_ms = ms;
+    return GPVMC_OK;
+}
+
+int gpvmc_batch_destroy(gpvmc_batch *b)
+{
+    if (!b) return GPVMC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto &mb : b->m) {
+        if (mb.join) (void)hipEventDestroy(mb.join);
+        if (mb.s) (void)hipStreamDestroy(mb.s);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
     return GPVMC_OK;
 }
 

@@ -170,6 +170,22 @@ int gpvmc_channel_kernel(gpvmc_ctx *ctx, int32_t *register_resident);
  * times the current amplitude (SlaterDeterminant::Amp). */
 int gpvmc_read_amplitudes(gpvmc_ctx *ctx, double *amp);
 
+/* ---- batches: contexts of any sizes and channels swept together --------------------
+ * vmc.cpp runs one chain per process and has no counterpart; a batch is how walkers of
+ * uneven lattice sizes and channels share one GPU.  Each member contributes one sweep
+ * (n_steps steps with a measurement every `meas` steps, as gpvmc_walk without traces).
+ * The first gpvmc_batch_sweep captures one sweep of every member into a hipGraph, each
+ * member's kernels on a stream of its own forked from the batch's stream, so the members
+ * run concurrently; later calls replay it.  Adding a member drops the graph.  Members
+ * must not be walked through other calls while a sweep runs; refresh (gpvmc_refresh)
+ * between sweeps as usual.  gpvmc_batch_sweep returns after the sweeps complete and
+ * reports their device time. */
+typedef struct gpvmc_batch gpvmc_batch;
+int gpvmc_batch_create(int32_t device, gpvmc_batch **out);
+int gpvmc_batch_add(gpvmc_batch *b, gpvmc_ctx *ctx, int64_t n_steps, int64_t meas, uint32_t mask);
+int gpvmc_batch_sweep(gpvmc_batch *b, int32_t n_sweeps, double *device_ms);
+int gpvmc_batch_destroy(gpvmc_batch *b);
+
 /* Current state: spins[w][site], labels part[w][2][N] (site of each particle),
  * logabs[w] = ln|amp|, phase[w] = arg(amp); any pointer may be NULL. */
 int gpvmc_read_state(gpvmc_ctx *ctx, int32_t *spins, int32_t *part, double *logabs,
