@@ -233,6 +233,66 @@ def exciton_leg(seed, warmup, steps, W=8192, L=12, refresh_every=16, n_cmp=1024)
     }
 
 
+def mixed_batch_leg(seed, sweeps, total=20000, Ls=(8, 10, 12, 16, 20), warmup=1):
+    """Walkers of uneven lattice sizes, ground state and longitudinal-exciton channels
+    interleaved: lattice side and kind drawn per walker from a seeded categorical
+    (uniform over Ls x {groundstate, long}); one context per (L, kind) bucket, all swept
+    together through one captured hipGraph per sweep (gpvmc_batch_*).  A sweep is L*L steps
+    and one measurement per walker (ProjHeis + StagMagnZ for the ground state, ProjHeis
+    H/O for the channels), mixed FP32; an FP64 refresh of every context after warm-up."""
+    from gpvmc_amd import binding, channels, model
+    rng = np.random.default_rng(seed)
+    Lw = rng.choice(np.array(Ls), size=total)
+    kind = rng.integers(0, 2, size=total)
+    sw = binding.Sweeper()
+    buckets = []
+    members = []
+    for L in Ls:
+        for kd in (0, 1):
+            n = int(np.sum((Lw == L) & (kind == kd)))
+            if n == 0:
+                continue
+            tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+            if kd == 0:
+                b = binding.Batch(L, n, binding.MIXED, n_groups=1, seed=seed + L)
+                b.set_orbitals(0, model.ground_occ(tab))
+                ok = b.init(model.random_spins(L, n, seed + L))
+                mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ
+                name = "groundstate"
+            else:
+                # four momenta per size, walker w on q = w mod 4
+                qs = [(0, 0), (L // 2, L // 2), (1, 0), (L // 4, L // 2)]
+                chs = [channels.Channel("long", L, q) for q in qs]
+                nk = max(c.nexc for c in chs)
+                chs = [c for c in chs if c.nexc == nk]
+                b = binding.ChannelBatch(L, n, chs, binding.MIXED, seed=seed + L)
+                b.set_orbital_table(0, tab)
+                ok = b.init(model.random_spins(L, n, seed + 100 + L))
+                mask = binding.MEAS_PROJHEIS
+                name = "long (%d states, %s kernel)" % (nk, "register" if b.register_resident else "streamed")
+            sw.add(b, L * L, L * L, mask)
+            members.append(b)
+            buckets.append({"L": L, "kind": name, "walkers": n, "init_ok": float(np.mean(ok))})
+    sw.sweep(warmup)
+    for b in members:
+        b.refresh()
+    for b in members:
+        b.reset_accumulators()
+    ms = sw.sweep(sweeps)
+    steps = sum(bk["walkers"] * bk["L"] ** 2 for bk in buckets) * sweeps
+    for bk, b in zip(buckets, members):
+        acc = b.accumulators()
+        bk["p_accept_move"] = round(float(acc[:, 8].sum() / max(acc[:, 7].sum(), 1)), 4)
+    sw.close()
+    for b in members:
+        b.close()
+    return {"workload": "%d walkers, L in %s x {groundstate, long} from a seeded uniform categorical, "
+                        "one captured hipGraph per sweep (L*L steps + 1 measurement per walker), mixed FP32"
+                        % (total, list(Ls)),
+            "mc_steps_per_s": round(steps / (ms * 1e-3), 1), "ms_per_sweep": round(ms / sweeps, 3),
+            "sweeps": sweeps, "buckets": buckets}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
