@@ -312,6 +312,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exciton-steps", type=int, default=6,
                     help="timed sweeps of the L=12 transverse-exciton leg (0: skip)")
+    ap.add_argument("--mixed-sweeps", type=int, default=2,
+                    help="timed sweeps of the uneven mixed batch (L in 8..20, ground state and "
+                         "long exciton, one hipGraph per sweep; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; walkers are independent chains and the "
@@ -396,6 +399,10 @@ def main():
     if args.exciton_steps > 0:
         print("bench: transverse-exciton channel, L=12, all q", file=sys.stderr, flush=True)
         exc = exciton_leg(args.seed, 1, args.exciton_steps)
+    mixed = None
+    if args.mixed_sweeps > 0:
+        print("bench: uneven mixed batch, 20000 walkers", file=sys.stderr, flush=True)
+        mixed = mixed_batch_leg(args.seed, args.mixed_sweeps)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
@@ -422,6 +429,7 @@ def main():
         "p_antiparallel": round(float(p_ap), 4), "p_accept_move": round(float(a_ap), 4),
         "roofline": roofline, "cpu_baseline": cpu,
         "exciton_c4": exc,
+        "mixed_batch_c5": mixed,
     }
     print(json.dumps(line), flush=True)
 
