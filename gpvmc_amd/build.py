@@ -54,7 +54,37 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed linking libgpvmc_hip.so")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_LIB = os.path.join(PKG, "lib", "libgpvmc_host.so")
+HOST_EXE = os.path.join(PKG, "lib", "vmc_host")
+
+
+def build_host(verbose=False):
+    """The C++ host classes (include/gpvmc_host.hpp) as libgpvmc_host.so over
+    libgpvmc_hip.so, and the vmc_host driver; plain C++ (no device code)."""
+    cxx = os.environ.get("CXX", "g++")
+    libdir = os.path.dirname(LIB)
+    host = os.path.join(PKG, "host")
+    common = ["-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include")]
+    cmds = [
+        [cxx] + common + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp", os.path.join(host, "gpvmc_host.cpp"),
+                          "-L" + libdir, "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
+        [cxx] + common + ["-o", HOST_EXE + ".tmp", os.path.join(host, "vmc_host.cpp"), "-L" + libdir,
+                          "-lgpvmc_host", "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
+    ]
+    for cmd, out in zip(cmds, (HOST_LIB, HOST_EXE)):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        if out == HOST_EXE:
+            os.replace(HOST_LIB + ".tmp", HOST_LIB)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("building %s failed" % os.path.basename(out))
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,476 @@
+// C++ host classes over the C ABI (include/gpvmc_host.hpp): the reference's object graph
+// for a batch of chains on one GPU.  Host-side work only: the orbital table, the channel
+// states and their hop graph, initial configurations, and the calls into libgpvmc_hip.so.
+#include "gpvmc_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <utility>
+
+namespace gpvmc {
+
+namespace {
+
+void check(int rc)
+{
+    if (rc != GPVMC_OK) throw Error(gpvmc_last_error());
+}
+
+struct QN {
+    int kx, ky, band;
+};
+
+// StagFluxWaveFunction::inmbz (StagFluxWaveFunction.cpp:89-101)
+bool inmbz(int L, double bcx, double bcy, int kx, int ky)
+{
+    const double kkx = (kx + bcx / 2.0) / L, kky = (ky + bcy / 2.0) / L, eps = 1e-9;
+    return (0 <= kkx && kkx < 1 && 0 <= kky && kky < 1) &&
+           (kkx + kky - 0.5 < 0 || kkx + kky - 1.5 >= -eps || kky - kkx - 0.5 > eps || kky - kkx + 0.5 <= eps);
+}
+
+// orbital quantum numbers, kx outer and ky inner over the magnetic zone, two bands each
+// (StagFluxWaveFunction.cpp:24-41)
+std::vector<QN> magnetic_zone(int L, double bcx, double bcy)
+{
+    std::vector<QN> qn;
+    for (int kx = 0; kx < L; ++kx)
+        for (int ky = 0; ky < L; ++ky)
+            if (inmbz(L, bcx, bcy, kx, ky)) {
+                qn.push_back({kx, ky, 0});
+                qn.push_back({kx, ky, 1});
+            }
+    return qn;
+}
+
+// StagFluxWaveFunction::mbzmod (StagFluxWaveFunction.cpp:103-120)
+std::pair<int, int> mbzmod(int L, double bcx, double bcy, int kx, int ky)
+{
+    if (inmbz(L, bcx, bcy, kx, ky)) return {kx, ky};
+    const int h = L / 2;
+    if (kx < h && ky < h) return {kx + h, ky + h};
+    if (kx < h && ky >= h) return {kx + h, ky - h};
+    if (kx >= h && ky < h) return {kx - h, ky + h};
+    return {kx - h, ky - h};
+}
+
+// Philox4x32-10 (Salmon et al., SC'11) with its Weyl key schedule
+void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+}  // namespace
+
+StagFluxWaveFunction::StagFluxWaveFunction(int L, double phi, double neel, double neel_exp, double bcx, double bcy,
+                                           Channel channel, int qx, int qy)
+    : L_(L), Ns_(L * L), N_(L * L / 2), nk_(0), ch_(channel)
+{
+    if (L < 2 || (L * L) % 2) throw Error("L must be >= 2 with L*L even");
+    const std::vector<QN> qn = magnetic_zone(L, bcx, bcy);
+    if ((int)qn.size() != Ns_) throw Error("magnetic zone enumeration does not give L*L orbitals");
+    // the orbital table (StagFluxWaveFunction.cpp:51-79, 138-163)
+    const double pi = std::acos(-1.0), ph = phi * pi;
+    table_.assign((size_t)2 * Ns_ * Ns_, 0.0);
+    for (int f = 0; f < Ns_; ++f) {
+        const double k0 = (qn[f].kx + bcx / 2.0) * 2 * pi / L, k1 = (qn[f].ky + bcy / 2.0) * 2 * pi / L;
+        const std::complex<double> d =
+            0.5 * (std::cos(k0) * std::polar(1.0, ph) + std::cos(k1) * std::polar(1.0, -ph));
+        const double nk = neel_exp == 0.0 ? neel
+                                          : neel * std::pow(0.5 * (std::cos(k0) * std::cos(k0) +
+                                                                   std::cos(k1) * std::cos(k1)), neel_exp);
+        const double ad = std::abs(d), om = std::sqrt(nk * nk + ad * ad);
+        if (!(ad > 0.0)) throw Error("degenerate delta(k): choose bc or neel so that |delta| > 0");
+        for (int fl = 0; fl < 2; ++fl) {
+            const double sig = fl ? -1.0 : 1.0;
+            const double up = std::sqrt(0.5 * (1 + sig * nk / om)), dn = std::sqrt(0.5 * (1 - sig * nk / om));
+            const std::complex<double> U = qn[f].band == 0 ? std::complex<double>(up) : -std::conj(d) / ad * dn;
+            const std::complex<double> V = qn[f].band == 0 ? d / ad * dn : std::complex<double>(up);
+            for (int r = 0; r < Ns_; ++r) {
+                const int ry = r / L, rx = r - ry * L;
+                table_[((size_t)fl * Ns_ + f) * Ns_ + r] =
+                    std::sqrt(2.0) * std::polar(1.0, k0 * rx + k1 * ry) * (((rx + ry) % 2 == 0) ? U : V);
+            }
+        }
+    }
+    // the channel's states, each flavour's occupied orbitals in increasing order
+    std::vector<int> qn2fock((size_t)2 * Ns_, Ns_);
+    for (int i = 0; i < Ns_; ++i) qn2fock[((size_t)qn[i].kx * L + qn[i].ky) * 2 + qn[i].band] = i;
+    qx = ((qx % L) + L) % L;
+    qy = ((qy % L) + L) % L;
+    auto hole_of = [&](int fe) {
+        const QN &o = qn[2 * fe + 1];
+        const auto k = mbzmod(L, bcx, bcy, ((o.kx - qx) % L + L) % L, ((o.ky - qy) % L + L) % L);
+        return qn2fock[((size_t)k.first * L + k.second) * 2];
+    };
+    std::vector<int> even(N_);
+    for (int f = 0; f < N_; ++f) even[f] = 2 * f;
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> st;
+    auto without = [&](int h) {
+        std::vector<int> v;
+        for (int o : even)
+            if (o != h) v.push_back(o);
+        return v;
+    };
+    if (channel == GroundState) {
+        st.push_back({even, even});
+    } else if (channel == TransExciton) {
+        for (int fe = 0; fe < N_; ++fe) {
+            std::vector<int> u = even;
+            u.push_back(2 * fe + 1);
+            st.push_back({u, without(hole_of(fe))});
+        }
+    } else {
+        for (int fe = 0; fe < N_; ++fe) {
+            std::vector<int> p = without(hole_of(fe));
+            p.push_back(2 * fe + 1);
+            st.push_back({p, even});
+            st.push_back({even, p});
+        }
+        if ((qx == 0 && qy == 0) || (qx == L / 2 && qy == L / 2)) st.push_back({even, even});
+    }
+    nk_ = (int)st.size();
+    npt_[0] = (int)st[0].first.size();
+    npt_[1] = (int)st[0].second.size();
+    for (auto &s : st) {
+        std::sort(s.first.begin(), s.first.end());
+        std::sort(s.second.begin(), s.second.end());
+        std::vector<int32_t> v(s.first.begin(), s.first.end());
+        v.insert(v.end(), s.second.begin(), s.second.end());
+        states_.push_back(v);
+        state_orb_.insert(state_orb_.end(), v.begin(), v.end());
+    }
+    // hop graph: every state is E = {0, 2, ..., 2N-2} with a few orbitals removed and added
+    // per flavour; the path c -> k pairs the orbitals occupied in c but not in k with those
+    // occupied in k but not in c, each in increasing order (State::get_hop_path,
+    // State.cpp:213-231), and HopSign counts the occupied orbitals between the two ends of
+    // every hop, hop after hop (State.cpp:86-115)
+    std::vector<std::vector<int>> rem((size_t)nk_ * 2), add((size_t)nk_ * 2);
+    for (int k = 0; k < nk_; ++k)
+        for (int fl = 0; fl < 2; ++fl) {
+            const std::vector<int> &o = fl ? st[k].second : st[k].first;
+            for (int e : even)
+                if (!std::binary_search(o.begin(), o.end(), e)) rem[k * 2 + fl].push_back(e);
+            for (int x : o)
+                if (x % 2 == 1 || x >= 2 * N_) add[k * 2 + fl].push_back(x);
+        }
+    auto has = [](const std::vector<int> &v, int x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    auto n_even_between = [&](int lo, int hi) {
+        const int a = lo + 1, b = std::min(hi - 1, 2 * N_ - 2);
+        if (b < a) return 0;
+        return b / 2 - (a + 1) / 2 + 1;
+    };
+    hop_n_.assign((size_t)nk_ * nk_ * 2, 0);
+    hop_orb_.assign((size_t)nk_ * nk_ * 8, 0);
+    hopsign_.assign((size_t)nk_ * nk_, 1);
+    for (int c = 0; c < nk_; ++c)
+        for (int k = 0; k < nk_; ++k) {
+            bool odd = false;
+            for (int fl = 0; fl < 2; ++fl) {
+                const auto &rc = rem[c * 2 + fl], &ac = add[c * 2 + fl], &rk = rem[k * 2 + fl], &ak = add[k * 2 + fl];
+                std::vector<int> holes, parts;
+                for (int o : ac)
+                    if (!has(ak, o)) holes.push_back(o);
+                for (int o : rk)
+                    if (!has(rc, o)) holes.push_back(o);
+                for (int o : ak)
+                    if (!has(ac, o)) parts.push_back(o);
+                for (int o : rc)
+                    if (!has(rk, o)) parts.push_back(o);
+                std::sort(holes.begin(), holes.end());
+                std::sort(parts.begin(), parts.end());
+                if (holes.size() != parts.size() || holes.size() > 2)
+                    throw Error("channel paths must have at most two hops per flavour");
+                const size_t base = ((size_t)c * nk_ + k) * 2 + fl;
+                hop_n_[base] = (int32_t)holes.size();
+                std::vector<int> gone = rc, came = ac;
+                for (size_t h = 0; h < holes.size(); ++h) {
+                    const int a = holes[h], b = parts[h];
+                    hop_orb_[(base * 2 + h) * 2] = a;
+                    hop_orb_[(base * 2 + h) * 2 + 1] = b;
+                    const int lo = std::min(a, b), hi = std::max(a, b);
+                    int nb = n_even_between(lo, hi);
+                    for (int o : gone)
+                        if (lo < o && o < hi) --nb;
+                    for (int o : came)
+                        if (lo < o && o < hi) ++nb;
+                    odd ^= (nb & 1) != 0;
+                    auto ia = std::find(came.begin(), came.end(), a);
+                    if (ia != came.end())
+                        came.erase(ia);
+                    else
+                        gone.push_back(a);
+                    auto ib = std::find(gone.begin(), gone.end(), b);
+                    if (ib != gone.end())
+                        gone.erase(ib);
+                    else
+                        came.push_back(b);
+                }
+            }
+            hopsign_[(size_t)c * nk_ + k] = odd ? -1 : 1;
+        }
+}
+
+std::vector<std::pair<int32_t, int32_t>> StagFluxWaveFunction::GetHop(int c, int k, int fl) const
+{
+    std::vector<std::pair<int32_t, int32_t>> p;
+    const size_t base = ((size_t)c * nk_ + k) * 2 + fl;
+    for (int h = 0; h < hop_n_[base]; ++h) p.push_back({hop_orb_[(base * 2 + h) * 2], hop_orb_[(base * 2 + h) * 2 + 1]});
+    return p;
+}
+
+gpvmc_channel StagFluxWaveFunction::GraphABI() const
+{
+    gpvmc_channel ch;
+    ch.n_up = npt_[0];
+    ch.n_dn = npt_[1];
+    ch.nk = nk_;
+    ch.state_orb = state_orb_.data();
+    ch.hop_n = hop_n_.data();
+    ch.hop_orb = hop_orb_.data();
+    ch.hopsign = hopsign_.data();
+    return ch;
+}
+
+std::vector<int32_t> RanInit(int L, int n_walkers, int n_up, uint64_t seed)
+{
+    const int Ns = L * L;
+    std::vector<int32_t> out((size_t)n_walkers * Ns);
+    for (int w = 0; w < n_walkers; ++w) {
+        int32_t *s = out.data() + (size_t)w * Ns;
+        for (int i = 0; i < Ns; ++i) s[i] = i < n_up ? 0 : 1;
+        for (int i = Ns - 1; i > 0; --i) {
+            uint32_t c[4] = {(uint32_t)i, 0u, (uint32_t)w, 1u};
+            philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+            const int j = (int)((double)c[0] * 0x1p-32 * (i + 1));
+            std::swap(s[i], s[j]);
+        }
+    }
+    return out;
+}
+
+SlaterDeterminant::SlaterDeterminant(const StagFluxWaveFunction &wav, int n_walkers, int precision, uint64_t seed,
+                                     int device)
+    : wav_(wav), W_(n_walkers)
+{
+    const int L = wav.GetL(), Ns = L * L;
+    gpvmc_config cfg = {L, n_walkers, 1, precision, device, 0, seed};
+    const std::vector<std::complex<double>> &t = wav.Table();
+    if (wav.GetChannel() == StagFluxWaveFunction::GroundState) {
+        check(gpvmc_create(&cfg, &ctx_));
+        // occ[fl][site][p] = Phi_fl(orbital of particle p = 2p, site) (StagFluxGroundState.cpp:11-14)
+        const int N = Ns / 2;
+        std::vector<double> occ((size_t)2 * Ns * N * 2);
+        for (int fl = 0; fl < 2; ++fl)
+            for (int r = 0; r < Ns; ++r)
+                for (int p = 0; p < N; ++p) {
+                    const std::complex<double> v = t[((size_t)fl * Ns + 2 * p) * Ns + r];
+                    occ[(((size_t)fl * Ns + r) * N + p) * 2] = v.real();
+                    occ[(((size_t)fl * Ns + r) * N + p) * 2 + 1] = v.imag();
+                }
+        try {
+            check(gpvmc_set_orbitals(ctx_, 0, occ.data()));
+        } catch (...) {
+            gpvmc_destroy(ctx_);
+            throw;
+        }
+    } else {
+        const gpvmc_channel ch = wav.GraphABI();
+        check(gpvmc_create_channel(&cfg, &ch, &ctx_));
+        try {
+            check(gpvmc_set_orbital_table(ctx_, 0, reinterpret_cast<const double *>(t.data())));
+        } catch (...) {
+            gpvmc_destroy(ctx_);
+            throw;
+        }
+    }
+}
+
+SlaterDeterminant::~SlaterDeterminant()
+{
+    if (ctx_) gpvmc_destroy(ctx_);
+}
+
+std::vector<int32_t> SlaterDeterminant::Init(const std::vector<int32_t> &spins)
+{
+    const size_t Ns = (size_t)wav_.GetL() * wav_.GetL();
+    if (spins.size() != (size_t)W_ * Ns) throw Error("spins must hold n_walkers * L * L values");
+    std::vector<int32_t> ok(W_);
+    check(gpvmc_init_walkers(ctx_, spins.data(), ok.data()));
+    return ok;
+}
+
+void SlaterDeterminant::Refresh() { check(gpvmc_refresh(ctx_, nullptr)); }
+
+std::vector<LogAmp> SlaterDeterminant::Amp() const
+{
+    std::vector<double> la(W_), ph(W_);
+    check(gpvmc_read_state(ctx_, nullptr, nullptr, la.data(), ph.data()));
+    std::vector<LogAmp> out(W_);
+    for (int w = 0; w < W_; ++w) out[w] = {la[w], ph[w]};
+    return out;
+}
+
+std::vector<int32_t> SlaterDeterminant::Spins() const
+{
+    std::vector<int32_t> s((size_t)W_ * wav_.GetL() * wav_.GetL());
+    check(gpvmc_read_state(ctx_, s.data(), nullptr, nullptr, nullptr));
+    return s;
+}
+
+void Quantity::init(SlaterDeterminant &amp) { check(gpvmc_reset_accumulators(amp.Context())); }
+
+namespace {
+std::vector<double> accumulators(SlaterDeterminant &amp)
+{
+    std::vector<double> acc((size_t)amp.NWalkers() * GPVMC_ACC_STRIDE);
+    check(gpvmc_read_accumulators(amp.Context(), acc.data()));
+    return acc;
+}
+}  // namespace
+
+void ProjHeis::save(SlaterDeterminant &amp)
+{
+    const int W = amp.NWalkers();
+    const std::vector<double> acc = accumulators(amp);
+    if (amp.GetWaveFunction().GetChannel() == StagFluxWaveFunction::GroundState) {
+        energy.assign(W, 0.0);
+        for (int w = 0; w < W; ++w) {
+            const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
+            if (a[3] > 0) energy[w] = std::complex<double>(a[0], a[1]) / a[3];
+        }
+        return;
+    }
+    nk = amp.GetWaveFunction().GetNExc();
+    std::vector<double> ho((size_t)W * 4 * nk * nk);
+    check(gpvmc_read_channel(amp.Context(), 0, W, ho.data(), nullptr, nullptr));
+    H.assign((size_t)W * nk * nk, 0.0);
+    O.assign((size_t)W * nk * nk, 0.0);
+    for (int w = 0; w < W; ++w) {
+        const double n = acc[(size_t)w * GPVMC_ACC_STRIDE + 3];
+        if (n <= 0) continue;
+        const double *b = ho.data() + (size_t)w * 4 * nk * nk;
+        for (int e = 0; e < nk * nk; ++e) {
+            H[(size_t)w * nk * nk + e] = std::complex<double>(b[2 * e], b[2 * e + 1]) / n;
+            O[(size_t)w * nk * nk + e] = std::complex<double>(b[2 * (nk * nk + e)], b[2 * (nk * nk + e) + 1]) / n;
+        }
+    }
+}
+
+void StagMagnZ::save(SlaterDeterminant &amp)
+{
+    const int W = amp.NWalkers();
+    const std::vector<double> acc = accumulators(amp);
+    mz.assign(W, 0.0);
+    mz2.assign(W, 0.0);
+    for (int w = 0; w < W; ++w) {
+        const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
+        if (a[3] <= 0) continue;
+        mz[w] = std::complex<double>(a[4], a[5]) / a[3];
+        mz2[w] = a[6] / a[3];
+    }
+}
+
+void StatSpinStruct::init(SlaterDeterminant &amp) { Quantity::init(amp); }
+
+void StatSpinStruct::save(SlaterDeterminant &amp)
+{
+    const size_t Ns = (size_t)amp.GetWaveFunction().GetL() * amp.GetWaveFunction().GetL();
+    std::vector<double> raw(5 * Ns * Ns * 2);
+    double n = 0;
+    check(gpvmc_read_statspin_groups(amp.Context(), raw.data(), &n));
+    corr.assign(5 * Ns * Ns, 0.0);
+    if (n > 0)
+        for (size_t e = 0; e < corr.size(); ++e) corr[e] = std::complex<double>(raw[2 * e], raw[2 * e + 1]) / n;
+}
+
+void OverlapTrack::save(SlaterDeterminant &amp)
+{
+    const int W = amp.NWalkers();
+    const std::vector<double> acc = accumulators(amp);
+    mean_logw.assign(W, 0.0);
+    for (int w = 0; w < W; ++w) {
+        const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
+        if (a[3] > 0) mean_logw[w] = a[11] / a[3];
+    }
+}
+
+void MetroMC::Walk(int64_t len, int64_t meas)
+{
+    uint32_t mask = 0;
+    for (Quantity *q : quantities_) mask |= q->Mask();
+    if (meas > 0 && mask == 0) meas = 0;
+    check(gpvmc_walk(st_.GetAmp().Context(), len, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr));
+}
+
+double MetroMC::Rejection() const
+{
+    SlaterDeterminant &amp = st_.GetAmp();
+    const std::vector<double> acc = accumulators(amp);
+    double rej = 0, steps = 0;
+    for (int w = 0; w < amp.NWalkers(); ++w) {
+        rej += acc[(size_t)w * GPVMC_ACC_STRIDE + 9];
+        steps += acc[(size_t)w * GPVMC_ACC_STRIDE + 7];
+    }
+    return steps > 0 ? rej / steps : 0.0;
+}
+
+}  // namespace gpvmc
+
+// Host-side tables for the CPU tests (no device needed): the orbital table and the
+// channel graph exactly as the C++ classes build them.
+extern "C" int gpvmc_host_orbital_table(int32_t L, double phi, double neel, double neel_exp, double bcx, double bcy,
+                                        double *out)
+{
+    try {
+        gpvmc::StagFluxWaveFunction wf(L, phi, neel, neel_exp, bcx, bcy);
+        const auto &t = wf.Table();
+        for (size_t i = 0; i < t.size(); ++i) {
+            out[2 * i] = t[i].real();
+            out[2 * i + 1] = t[i].imag();
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// channel 0 ground state, 1 trans, 2 long; sizes first (out == NULL), then the arrays
+extern "C" int gpvmc_host_channel(int32_t L, int32_t channel, int32_t qx, int32_t qy, int32_t *nk, int32_t *n_up,
+                                  int32_t *n_dn, int32_t *state_orb, int32_t *hop_n, int32_t *hop_orb,
+                                  int32_t *hopsign)
+{
+    try {
+        gpvmc::StagFluxWaveFunction wf(L, 0.25, 0.1, 0.0, 1.0, 1.0, (gpvmc::StagFluxWaveFunction::Channel)channel, qx,
+                                       qy);
+        const gpvmc_channel ch = wf.GraphABI();
+        *nk = ch.nk;
+        *n_up = ch.n_up;
+        *n_dn = ch.n_dn;
+        const size_t K = ch.nk;
+        if (state_orb) std::copy(ch.state_orb, ch.state_orb + K * (ch.n_up + ch.n_dn), state_orb);
+        if (hop_n) std::copy(ch.hop_n, ch.hop_n + K * K * 2, hop_n);
+        if (hop_orb) std::copy(ch.hop_orb, ch.hop_orb + K * K * 8, hop_orb);
+        if (hopsign) std::copy(ch.hopsign, ch.hopsign + K * K, hopsign);
+        return 0;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+// RanInit's configurations
+extern "C" int gpvmc_host_raninit(int32_t L, int32_t n_walkers, int32_t n_up, uint64_t seed, int32_t *spins)
+{
+    const std::vector<int32_t> s = gpvmc::RanInit(L, n_walkers, n_up, seed);
+    std::copy(s.begin(), s.end(), spins);
+    return 0;
+}

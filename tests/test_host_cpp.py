@@ -1,0 +1,168 @@
+"""The C++ host classes (include/gpvmc_host.hpp, libgpvmc_host.so, the vmc_host driver).
+
+CPU: the C++ orbital table, channel graphs and initial configurations equal the Python
+host's (which the oracle tests pin against the reference's dumps).  GPU: a walk through the
+C++ classes (StagFluxWaveFunction -> SlaterDeterminant -> LatticeStepper -> MetroMC with
+ProjHeis / StagMagnZ / OverlapTrack) equals the same walk through the Python binding.
+"""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpvmc_amd import binding, build, channels, model
+
+
+def host_lib():
+    if not os.path.exists(build.HOST_LIB):
+        pytest.skip("libgpvmc_host.so not built")
+    lib = ctypes.CDLL(build.HOST_LIB)
+    lib.gpvmc_host_orbital_table.argtypes = [ctypes.c_int32] + [ctypes.c_double] * 5 + [ctypes.c_void_p]
+    lib.gpvmc_host_channel.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 7
+    lib.gpvmc_host_raninit.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                       ctypes.c_void_p]
+    return lib
+
+
+def p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.mark.parametrize("L,phi,neel,nexp,bcx,bcy", [(4, 0.25, 0.1, 0.0, 1.0, 1.0), (6, 0.1667, 0.0, 0.0, 1.0, 1.0),
+                                                     (8, 0.3, 0.05, 1.5, 1.0, 0.0), (12, 0.25, 0.1, 0.0, 1.0, 1.0)])
+def test_orbital_table_equals_python(L, phi, neel, nexp, bcx, bcy):
+    lib = host_lib()
+    out = np.zeros((2, L * L, L * L), np.complex128)
+    assert lib.gpvmc_host_orbital_table(L, phi, neel, nexp, bcx, bcy, p(out)) == 0
+    ref = model.orbital_table(L, phi * np.pi, neel, nexp, bcx, bcy)
+    assert np.abs(out - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,L,q", [("groundstate", 4, (0, 0)), ("trans", 4, (1, 2)), ("long", 4, (0, 0)),
+                                      ("long", 6, (1, 0)), ("trans", 8, (3, 5)), ("long", 8, (4, 4)),
+                                      ("trans", 12, (5, 7))])
+def test_channel_graph_equals_python(name, L, q):
+    lib = host_lib()
+    kind = {"groundstate": 0, "trans": 1, "long": 2}[name]
+    nk, nu, nd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert lib.gpvmc_host_channel(L, kind, q[0], q[1], ctypes.byref(nk), ctypes.byref(nu), ctypes.byref(nd),
+                                  None, None, None, None) == 0
+    K = nk.value
+    so = np.zeros((K, nu.value + nd.value), np.int32)
+    hn = np.zeros((K, K, 2), np.int32)
+    ho = np.zeros((K, K, 2, 2, 2), np.int32)
+    hs = np.zeros((K, K), np.int32)
+    assert lib.gpvmc_host_channel(L, kind, q[0], q[1], ctypes.byref(nk), ctypes.byref(nu), ctypes.byref(nd),
+                                  p(so), p(hn), p(ho), p(hs)) == 0
+    ch = channels.Channel(name, L, q)
+    assert K == ch.nexc and (nu.value, nd.value) == ch.npt
+    for k, (u, d) in enumerate(ch.states):
+        assert np.array_equal(so[k], np.concatenate([u, d]))
+    assert np.array_equal(hs, ch.hopsign.astype(np.int32))
+    for c in range(K):
+        for k in range(K):
+            for fl in range(2):
+                path = ch.hops[c][k][fl]
+                assert hn[c, k, fl] == len(path)
+                for h, (a, b) in enumerate(path):
+                    assert (ho[c, k, fl, h, 0], ho[c, k, fl, h, 1]) == (a, b)
+
+
+def test_raninit_equals_python():
+    lib = host_lib()
+    L, W, seed = 8, 5, 12345
+    out = np.zeros((W, L * L), np.int32)
+    lib.gpvmc_host_raninit(L, W, L * L // 2, seed, p(out))
+    assert np.array_equal(out, model.random_spins(L, W, seed))
+
+
+def init_redrawing(b, draw, seed):
+    """init with singular starts redrawn from seed + 1000 * try, as vmc_host does"""
+    spins = draw(seed)
+    ok = b.init(spins)
+    for tries in range(1, 50):
+        if ok.all():
+            break
+        fresh = draw(seed + 1000 * tries)
+        spins[ok == 0] = fresh[ok == 0]
+        ok = b.init(spins)
+    return ok
+
+
+def run_host(*args):
+    if not os.path.exists(build.HOST_EXE):
+        pytest.skip("vmc_host not built")
+    r = subprocess.run([build.HOST_EXE] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,W,prec", [(4, 16, 0), (8, 8, 1)])
+def test_host_ground_state_walk_equals_binding(L, W, prec):
+    phi, neel, therm, samples, meas, seed = 0.25, 0.1, 2, 20, L * L, 77
+    print("case host ground", L, W, prec, flush=True)
+    out = run_host(L, W, phi, neel, 0, 0, 0, therm, samples, meas, prec, seed)
+    b = binding.Batch(L, W, prec, n_groups=1, seed=seed)
+    b.set_orbitals(0, model.ground_occ(model.orbital_table(L, phi * np.pi, neel)))
+    ok = init_redrawing(b, lambda s: model.random_spins(L, W, s), seed)
+    b.walk(therm * L * L)
+    b.reset_accumulators()
+    b.walk(samples * meas, meas=meas,
+           mask=binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ | binding.MEAS_OVERLAP)
+    acc = b.accumulators()
+    sp = b.state()[0]
+    b.close()
+    cs = int(np.sum(sp.astype(np.int64) * (np.arange(L * L) + 1)[None, :] * (np.arange(W) + 1)[:, None]))
+    assert ok.all() and out["init_ok"] == W
+    assert out["spins_checksum"] == cs
+    e = np.mean(acc[:, 0] / acc[:, 3])
+    m2 = np.mean(acc[:, 6] / acc[:, 3])
+    lw = np.mean(acc[:, 11] / acc[:, 3])
+    # the C++ and numpy orbital tables agree to ~1e-16 (different libm roundings), which can
+    # move an FP32 rounding: FP32-level bars for the mixed path, the configurations (and
+    # hence the accept counts) identical in both
+    tol = 1e-12 if prec == binding.FP64 else 1e-6
+    assert abs(out["energy_per_site"] - e) < tol
+    assert abs(out["stag_magn_sq"] - m2) < tol
+    assert abs(out["mean_logw"] - lw) < max(tol, 1e-9) * max(1.0, abs(lw))
+    assert abs(out["rejection"] - acc[:, 9].sum() / acc[:, 7].sum()) < 1e-12
+
+
+@pytest.mark.gpu
+def test_host_trans_channel_walk_equals_binding():
+    L, W, q, phi, neel, therm, samples, seed = 4, 6, (1, 2), 0.25, 0.1, 2, 10, 91
+    meas = L * L
+    print("case host trans", flush=True)
+    out = run_host(L, W, phi, neel, 1, q[0], q[1], therm, samples, meas, 0, seed)
+    lib = host_lib()
+    ch = channels.Channel("trans", L, q)
+
+    def draw(s):
+        spins = np.zeros((W, L * L), np.int32)
+        lib.gpvmc_host_raninit(L, W, ch.npt[0], s, p(spins))
+        return spins
+
+    b = binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed)
+    b.set_orbital_table(0, model.orbital_table(L, phi * np.pi, neel))
+    ok = init_redrawing(b, draw, seed)
+    assert ok.all()
+    b.walk(therm * L * L)
+    b.reset_accumulators()
+    b.walk(samples * meas, meas=meas)
+    ho, _, _ = b.channel_sums()
+    acc = b.accumulators()
+    sp = b.state()[0]
+    b.close()
+    cs = int(np.sum(sp.astype(np.int64) * (np.arange(L * L) + 1)[None, :] * (np.arange(W) + 1)[:, None]))
+    assert out["init_ok"] == W and out["spins_checksum"] == cs
+    nk = ch.nexc
+    n = acc[:, 3][:, None]
+    th = np.mean(np.real(np.trace(ho[:, :nk, :], axis1=1, axis2=2)) / n[:, 0])
+    to = np.mean(np.real(np.trace(ho[:, nk:, :], axis1=1, axis2=2)) / n[:, 0])
+    assert out["nk"] == nk
+    assert abs(out["trace_H"] - th) < 1e-10 * max(1.0, abs(th))
+    assert abs(out["trace_O"] - to) < 1e-10 * max(1.0, abs(to))
