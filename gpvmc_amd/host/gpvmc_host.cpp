@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <utility>
 
 namespace gpvmc {
@@ -346,21 +347,39 @@ void ProjHeis::save(SlaterDeterminant &amp)
         for (int w = 0; w < W; ++w) {
             const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
             if (a[3] > 0) energy[w] = std::complex<double>(a[0], a[1]) / a[3];
+            if (fm_) {
+                // MatrixQuantity(2, 1): Val(0,0) = E_loc per site, Val(0,1) (= m_vals[1]) counts
+                // the measurements, so its mean is 1 (ProjHeis.cpp:140-145)
+                const double v[4] = {energy[w].real(), energy[w].imag(), a[3] > 0 ? 1.0 : 0.0, 0.0};
+                fm_->FileStream(name_, w).Write(2, 2, v);
+            }
         }
         return;
     }
     nk = amp.GetWaveFunction().GetNExc();
-    std::vector<double> ho((size_t)W * 4 * nk * nk);
+    const size_t kk = (size_t)nk * nk;
+    std::vector<double> ho((size_t)W * 4 * kk);
     check(gpvmc_read_channel(amp.Context(), 0, W, ho.data(), nullptr, nullptr));
-    H.assign((size_t)W * nk * nk, 0.0);
-    O.assign((size_t)W * nk * nk, 0.0);
+    H.assign((size_t)W * kk, 0.0);
+    O.assign((size_t)W * kk, 0.0);
+    std::vector<double> v(4 * kk);
     for (int w = 0; w < W; ++w) {
         const double n = acc[(size_t)w * GPVMC_ACC_STRIDE + 3];
-        if (n <= 0) continue;
-        const double *b = ho.data() + (size_t)w * 4 * nk * nk;
-        for (int e = 0; e < nk * nk; ++e) {
-            H[(size_t)w * nk * nk + e] = std::complex<double>(b[2 * e], b[2 * e + 1]) / n;
-            O[(size_t)w * nk * nk + e] = std::complex<double>(b[2 * (nk * nk + e)], b[2 * (nk * nk + e) + 1]) / n;
+        const double *b = ho.data() + (size_t)w * 4 * kk;
+        if (n > 0)
+            for (size_t e = 0; e < kk; ++e) {
+                H[(size_t)w * kk + e] = std::complex<double>(b[2 * e], b[2 * e + 1]) / n;
+                O[(size_t)w * kk + e] = std::complex<double>(b[2 * (kk + e)], b[2 * (kk + e) + 1]) / n;
+            }
+        if (fm_) {
+            // rows k1 < nk: H[k1][k2]; rows nk + k1: O[k1][k2]; complex interleaved along rows
+            for (size_t e = 0; e < kk; ++e) {
+                v[2 * e] = H[(size_t)w * kk + e].real();
+                v[2 * e + 1] = H[(size_t)w * kk + e].imag();
+                v[2 * (kk + e)] = O[(size_t)w * kk + e].real();
+                v[2 * (kk + e) + 1] = O[(size_t)w * kk + e].imag();
+            }
+            fm_->FileStream(name_, w).Write(2 * nk, 2 * nk, v.data());
         }
     }
 }
@@ -373,9 +392,14 @@ void StagMagnZ::save(SlaterDeterminant &amp)
     mz2.assign(W, 0.0);
     for (int w = 0; w < W; ++w) {
         const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
-        if (a[3] <= 0) continue;
-        mz[w] = std::complex<double>(a[4], a[5]) / a[3];
-        mz2[w] = a[6] / a[3];
+        if (a[3] > 0) {
+            mz[w] = std::complex<double>(a[4], a[5]) / a[3];
+            mz2[w] = a[6] / a[3];
+        }
+        if (fm_) {
+            const double v[2] = {mz[w].real(), mz[w].imag()};
+            fm_->FileStream(name_, w).Write(1, 2, v);
+        }
     }
 }
 
@@ -384,12 +408,54 @@ void StatSpinStruct::init(SlaterDeterminant &amp) { Quantity::init(amp); }
 void StatSpinStruct::save(SlaterDeterminant &amp)
 {
     const size_t Ns = (size_t)amp.GetWaveFunction().GetL() * amp.GetWaveFunction().GetL();
-    std::vector<double> raw(5 * Ns * Ns * 2);
+    const size_t blk = 5 * Ns * Ns * 2;
+    std::vector<double> raw(blk);
     double n = 0;
     check(gpvmc_read_statspin_groups(amp.Context(), raw.data(), &n));
     corr.assign(5 * Ns * Ns, 0.0);
     if (n > 0)
         for (size_t e = 0; e < corr.size(); ++e) corr[e] = std::complex<double>(raw[2 * e], raw[2 * e + 1]) / n;
+    if (!fm_) return;
+    // per walker: MatrixQuantity(5, Ns^2) divided by its own count (StatSpinStruct.cpp:12-14),
+    // read in chunks of walkers
+    const int W = amp.NWalkers(), chunk = (int)std::max<size_t>(1, std::min<size_t>(256, (64u << 20) / (blk * 8)));
+    std::vector<double> part((size_t)chunk * blk), cnt(chunk);
+    for (int w0 = 0; w0 < W; w0 += chunk) {
+        const int nw = std::min(chunk, W - w0);
+        check(gpvmc_read_statspin(amp.Context(), w0, nw, part.data(), cnt.data()));
+        for (int k = 0; k < nw; ++k) {
+            double *b = part.data() + (size_t)k * blk;
+            if (cnt[k] > 0)
+                for (size_t e = 0; e < blk; ++e) b[e] /= cnt[k];
+            fm_->FileStream(name_, w0 + k).Write(5, (int)(2 * Ns * Ns), b);
+        }
+    }
+}
+
+std::vector<double> RelativeWeights(const std::vector<double> &logw)
+{
+    // ln mean(w) = max + ln(sum_m exp(ln w_m - max) / n): no overflow where the reference
+    // needs BigDouble (OverlapTrack.cpp:15-25)
+    std::vector<double> rel(logw.size());
+    if (logw.empty()) return rel;
+    const double mx = *std::max_element(logw.begin(), logw.end());
+    double sum = 0;
+    for (double x : logw) sum += std::exp(x - mx);
+    const double lmean = mx + std::log(sum / (double)logw.size());
+    for (size_t m = 0; m < logw.size(); ++m) rel[m] = std::exp(logw[m] - lmean);
+    return rel;
+}
+
+void OverlapTrack::init(SlaterDeterminant &amp)
+{
+    Quantity::init(amp);
+    logw_.assign(amp.NWalkers(), {});
+}
+
+void OverlapTrack::TakeTraces(int W, int64_t n_meas, const double *logw, const double *)
+{
+    if ((int)logw_.size() != W) logw_.assign(W, {});
+    for (int w = 0; w < W; ++w) logw_[w].insert(logw_[w].end(), logw + (size_t)w * n_meas, logw + (size_t)(w + 1) * n_meas);
 }
 
 void OverlapTrack::save(SlaterDeterminant &amp)
@@ -397,31 +463,227 @@ void OverlapTrack::save(SlaterDeterminant &amp)
     const int W = amp.NWalkers();
     const std::vector<double> acc = accumulators(amp);
     mean_logw.assign(W, 0.0);
+    rel.assign(W, {});
     for (int w = 0; w < W; ++w) {
         const double *a = acc.data() + (size_t)w * GPVMC_ACC_STRIDE;
         if (a[3] > 0) mean_logw[w] = a[11] / a[3];
+        if (w < (int)logw_.size()) rel[w] = RelativeWeights(logw_[w]);
+        if (fm_) fm_->FileStream(name_, w).Write((int)rel[w].size(), 1, rel[w].data());
+    }
+}
+
+void StagMagnTrack::init(SlaterDeterminant &amp)
+{
+    Quantity::init(amp);
+    mz.assign(amp.NWalkers(), {});
+}
+
+void StagMagnTrack::TakeTraces(int W, int64_t n_meas, const double *, const double *m)
+{
+    if ((int)mz.size() != W) mz.assign(W, {});
+    for (int w = 0; w < W; ++w)
+        for (int64_t k = 0; k < n_meas; ++k)
+            mz[w].emplace_back(m[2 * ((size_t)w * n_meas + k)], m[2 * ((size_t)w * n_meas + k) + 1]);
+}
+
+void StagMagnTrack::save(SlaterDeterminant &amp)
+{
+    if (!fm_) return;
+    for (int w = 0; w < amp.NWalkers(); ++w) {
+        const std::vector<std::complex<double>> &v = w < (int)mz.size() ? mz[w] : std::vector<std::complex<double>>();
+        fm_->FileStream(name_, w).Write((int)(2 * v.size()), 1, reinterpret_cast<const double *>(v.data()));
     }
 }
 
 void MetroMC::Walk(int64_t len, int64_t meas)
 {
-    uint32_t mask = 0;
-    for (Quantity *q : quantities_) mask |= q->Mask();
+    SlaterDeterminant &amp = st_.GetAmp();
+    uint32_t mask = 0, traces = 0;
+    for (Quantity *q : quantities_) {
+        mask |= q->Mask();
+        traces |= q->Traces();
+    }
     if (meas > 0 && mask == 0) meas = 0;
-    check(gpvmc_walk(st_.GetAmp().Context(), len, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     nullptr));
+    if (meas > 0)
+        for (Quantity *q : quantities_) q->init(amp);
+    const int W = amp.NWalkers();
+    const int64_t n_meas = meas > 0 ? (meas == 1 ? len : (len + meas - 1) / meas) : 0;
+    std::vector<double> logw, mz;
+    if (n_meas && (traces & TraceLogW)) logw.resize((size_t)W * n_meas);
+    if (n_meas && (traces & TraceStagMagn)) mz.resize((size_t)W * n_meas * 2);
+    check(gpvmc_walk(amp.Context(), len, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     mz.empty() ? nullptr : mz.data(), logw.empty() ? nullptr : logw.data()));
+    if (n_meas && traces)
+        for (Quantity *q : quantities_)
+            if (q->Traces()) q->TakeTraces(W, n_meas, logw.empty() ? nullptr : logw.data(), mz.empty() ? nullptr : mz.data());
+    if (meas > 0) {
+        // the accumulators were cleared by init: they hold this walk's counts
+        const std::vector<double> acc = accumulators(amp);
+        steps_.resize(W, 0.0);
+        rejected_.resize(W, 0.0);
+        for (int w = 0; w < W; ++w) {
+            steps_[w] += acc[(size_t)w * GPVMC_ACC_STRIDE + 7];
+            rejected_[w] += acc[(size_t)w * GPVMC_ACC_STRIDE + 9];
+        }
+    }
 }
 
 double MetroMC::Rejection() const
 {
-    SlaterDeterminant &amp = st_.GetAmp();
-    const std::vector<double> acc = accumulators(amp);
     double rej = 0, steps = 0;
-    for (int w = 0; w < amp.NWalkers(); ++w) {
-        rej += acc[(size_t)w * GPVMC_ACC_STRIDE + 9];
-        steps += acc[(size_t)w * GPVMC_ACC_STRIDE + 7];
+    for (size_t w = 0; w < steps_.size(); ++w) {
+        rej += rejected_[w];
+        steps += steps_[w];
     }
     return steps > 0 ? rej / steps : 0.0;
+}
+
+std::vector<double> MetroMC::RejectionPerChain() const
+{
+    std::vector<double> r(steps_.size(), 0.0);
+    for (size_t w = 0; w < r.size(); ++w)
+        if (steps_[w] > 0) r[w] = rejected_[w] / steps_[w];
+    return r;
+}
+
+void MatStream::Write(int m, int n, const double *mat)
+{
+    if (m < 0 || n < 0) throw Error("MatStream::Write: negative size");
+    const size_t sz = (size_t)m * n;
+    if (stat_ == 0) {
+        nrow_ = m;
+        ncol_ = n;
+        mat_.assign(mat, mat + sz);
+        stat_ = 1;
+        return;
+    }
+    if (m != nrow_ || n != ncol_) throw Error("MatStream::Write: sizes must match.");
+    for (size_t a = 0; a < sz; ++a) mat_[a] = (stat_ * mat_[a] + mat[a]) / (stat_ + 1);
+    ++stat_;
+}
+
+FileManager::FileManager(int n_chains) : n_chains_(n_chains)
+{
+    if (n_chains < 1) throw Error("FileManager: n_chains must be >= 1");
+}
+
+MatStream &FileManager::FileStream(const std::string &name, int chain)
+{
+    if (chain < 0 || chain >= n_chains_) throw Error("FileManager::FileStream: chain out of range");
+    std::vector<MatStream> &v = streams_[name];
+    if (v.empty()) v.resize(n_chains_);
+    return v[chain];
+}
+
+void FileManager::DataAttribute(const std::string &key, double v) { dattr_[key].assign(n_chains_, v); }
+
+void FileManager::DataAttribute(const std::string &key, const std::vector<double> &per_chain)
+{
+    if ((int)per_chain.size() != n_chains_) throw Error("FileManager::DataAttribute: one value per chain");
+    dattr_[key] = per_chain;
+}
+
+void FileManager::Write()
+{
+    for (auto &it : streams_) {
+        std::vector<std::vector<Dataset>> &rec = data_[it.first];
+        if (rec.empty()) rec.resize(n_chains_);
+        for (int c = 0; c < n_chains_; ++c) {
+            // a stream with nothing saved since its dataset closed (Stat() == 0) rewrites that
+            // dataset, attributes included, as the reference does
+            MatStream &ms = it.second[c];
+            std::vector<Dataset> &ds = rec[c];
+            if (ms.Stat() == 1 || ds.empty()) ds.emplace_back();
+            Dataset &d = ds.back();
+            d.rows = ms.Rows();
+            d.cols = ms.Cols();
+            d.mat = ms.Mat();
+            for (const auto &a : dattr_) d.attr[a.first] = a.second[c];
+            if (ms.Stat() == stat_per_sample_) ms.Stat() = 0;
+        }
+    }
+}
+
+std::vector<std::string> FileManager::Names() const
+{
+    std::vector<std::string> n;
+    for (const auto &it : data_) n.push_back(it.first);
+    return n;
+}
+
+const std::vector<FileManager::Dataset> &FileManager::Datasets(const std::string &name, int chain) const
+{
+    auto it = data_.find(name);
+    if (it == data_.end() || chain < 0 || chain >= n_chains_) throw Error("FileManager::Datasets: no such record");
+    return it->second[chain];
+}
+
+namespace {
+std::string json_str(const std::string &s)
+{
+    std::string o = "\"";
+    for (char ch : s) {
+        if (ch == '"' || ch == '\\') o += '\\';
+        if ((unsigned char)ch < 0x20) {
+            char b[8];
+            std::snprintf(b, sizeof b, "\\u%04x", ch);
+            o += b;
+        } else {
+            o += ch;
+        }
+    }
+    return o + "\"";
+}
+
+void json_num(FILE *f, double v)
+{
+    if (std::isfinite(v))
+        std::fprintf(f, "%.17g", v);
+    else
+        std::fprintf(f, "null");
+}
+}  // namespace
+
+void FileManager::WriteJSON(const std::string &dir, const std::string &prefix) const
+{
+    for (const auto &q : data_) {
+        const std::string path = dir + "/" + prefix + "-" + q.first + ".json";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) throw Error("FileManager::WriteJSON: cannot open " + path);
+        // root attributes, then "type" = the quantity (FileManager.cpp:474-490)
+        std::fprintf(f, "{\"attributes\": {");
+        for (const auto &a : fattr_num_) {
+            std::fprintf(f, "%s: ", json_str(a.first).c_str());
+            json_num(f, a.second);
+            std::fprintf(f, ", ");
+        }
+        for (const auto &a : fattr_str_)
+            std::fprintf(f, "%s: %s, ", json_str(a.first).c_str(), json_str(a.second).c_str());
+        std::fprintf(f, "\"type\": %s}", json_str(q.first).c_str());
+        for (int c = 0; c < n_chains_; ++c) {
+            std::fprintf(f, ",\n\"rank-%d\": {", c);
+            const std::vector<Dataset> &ds = q.second[c];
+            for (size_t k = 0; k < ds.size(); ++k) {
+                std::fprintf(f, "%s\"data-%zu\": {\"shape\": [%d, %d], \"attr\": {", k ? ", " : "", k, ds[k].rows,
+                             ds[k].cols);
+                bool fa = true;
+                for (const auto &a : ds[k].attr) {
+                    std::fprintf(f, "%s%s: ", fa ? "" : ", ", json_str(a.first).c_str());
+                    json_num(f, a.second);
+                    fa = false;
+                }
+                std::fprintf(f, "}, \"data\": [");
+                for (size_t e = 0; e < ds[k].mat.size(); ++e) {
+                    if (e) std::fputc(',', f);
+                    json_num(f, ds[k].mat[e]);
+                }
+                std::fprintf(f, "]}");
+            }
+            std::fprintf(f, "}");
+        }
+        std::fprintf(f, "}\n");
+        if (std::fclose(f)) throw Error("FileManager::WriteJSON: write failed for " + path);
+    }
 }
 
 }  // namespace gpvmc
@@ -473,4 +735,51 @@ extern "C" int gpvmc_host_raninit(int32_t L, int32_t n_walkers, int32_t n_up, ui
     const std::vector<int32_t> s = gpvmc::RanInit(L, n_walkers, n_up, seed);
     std::copy(s.begin(), s.end(), spins);
     return 0;
+}
+
+// OverlapTrack's w_m / mean(w) from ln w_m
+extern "C" int gpvmc_host_relative_weights(int32_t n, const double *logw, double *rel)
+{
+    const std::vector<double> r = gpvmc::RelativeWeights(std::vector<double>(logw, logw + n));
+    std::copy(r.begin(), r.end(), rel);
+    return 0;
+}
+
+// A FileManager driven through a scripted sequence of saves: save s writes mats[s][c]
+// (m x n) into chain c's stream "Q", with data attribute "statistics" = s; Write() after
+// every save.  Returns the number of datasets of each chain and their matrices
+// out[c][k][m * n] (k < max_ds) and statistics attributes stats[c][k]; -1 on an error
+// (a size mismatch when shapes[s] differs, message in gpvmc_host_last_error).
+static std::string g_host_error;
+extern "C" const char *gpvmc_host_last_error(void) { return g_host_error.c_str(); }
+extern "C" int gpvmc_host_filemanager_script(int32_t n_chains, int32_t stat_per_sample, int32_t n_saves,
+                                             const int32_t *shapes, const double *mats, int32_t max_ds,
+                                             int32_t *n_ds, double *out, double *stats)
+{
+    try {
+        gpvmc::FileManager fm(n_chains);
+        fm.StatPerSample() = stat_per_sample;
+        size_t off = 0, mn0 = (size_t)shapes[0] * shapes[1];
+        for (int s = 0; s < n_saves; ++s) {
+            const int m = shapes[2 * s], n = shapes[2 * s + 1];
+            for (int c = 0; c < n_chains; ++c) {
+                fm.FileStream("Q", c).Write(m, n, mats + off);
+                off += (size_t)m * n;
+            }
+            fm.DataAttribute("statistics", (double)s);
+            fm.Write();
+        }
+        for (int c = 0; c < n_chains; ++c) {
+            const auto &ds = fm.Datasets("Q", c);
+            n_ds[c] = (int32_t)ds.size();
+            for (size_t k = 0; k < ds.size() && (int)k < max_ds; ++k) {
+                std::copy(ds[k].mat.begin(), ds[k].mat.end(), out + ((size_t)c * max_ds + k) * mn0);
+                stats[(size_t)c * max_ds + k] = ds[k].attr.at("statistics");
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
 }

@@ -1,19 +1,253 @@
 // vmc_host: a batch of chains through the C++ host classes, as vmc.cpp wires one chain
-// (vmc.cpp:140-317): wavefunction, SlaterDeterminant, LatticeStepper, MetroMC with
-// ProjHeis / StagMagnZ / OverlapTrack, thermalisation then measured walks.  Prints one JSON
-// line of per-walker means.
+// (vmc.cpp:140-317): wavefunction, SlaterDeterminant, LatticeStepper, MetroMC with its
+// quantities, thermalisation then measured walks.
 //
-// usage: vmc_host L walkers phi neel channel(0 ground, 1 trans, 2 long) qx qy therm_sweeps
-//        samples meas_interval precision(0 FP64, 1 mixed) seed
+// Two forms:
+//   vmc_host --L=8 --samples=10 --meas_projheis ...   the reference's vmc flag set and defaults
+//       (vmc.cpp:69-94, ArgParse's --key=value / --flag syntax) and its sample / save loop
+//       (vmc.cpp:295-317); every walker is one chain (the reference's rank).  Extra flags:
+//       --walkers (default 1) and --precision=fp64|mixed (default fp64).  Each quantity's
+//       records go to <dir>/<prefix>-<Name>.json (FileManager::WriteJSON), the prefix from
+//       --prefix or, when it is negative, from <dir>/.runfile (FileManager.cpp:20-64).
+//   vmc_host L walkers phi neel channel(0 ground, 1 trans, 2 long) qx qy therm_sweeps
+//       samples meas_interval precision(0 FP64, 1 mixed) seed
+//       one measured walk; prints one JSON line of per-walker means.
+#include <chrono>
+#include <ctime>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <algorithm>
 #include <exception>
+#include <fstream>
+#include <map>
+#include <string>
 
 #include "gpvmc_host.hpp"
 
+namespace {
+
+struct Flags {
+    std::map<std::string, bool> bo;
+    std::map<std::string, double> num;  // size_t, int and double flags
+    std::map<std::string, std::string> str;
+};
+
+Flags default_flags()
+{
+    Flags f;
+    f.num = {{"L", 8},
+             {"samples", 10},
+             {"samples_saves", 1},
+             {"samples_saves_stat", 100},
+             {"qx", 0},
+             {"qy", 0},
+             {"meas_interv", 0},
+             {"therm", 100},
+             {"prefix", -1},
+             {"verbose", 1},
+             {"seed", (double)std::time(nullptr)},
+             {"phi", 0.085},
+             {"neel", 0.055},
+             {"neel_exp", 0.0},
+             {"phase_shift_x", 1.0},
+             {"phase_shift_y", 1.0},
+             {"jr", 0.0},
+             {"walkers", 1}};
+    f.bo = {{"track_stagmagn", false}, {"track_overlap", false},       {"meas_projheis", false},
+            {"meas_stagmagn", false},  {"meas_statspinstruct", false}, {"Neel_init", false}};
+    f.str = {{"dir", "."}, {"spinstate", ""}, {"channel", "groundstate"}, {"precision", "fp64"}};
+    return f;
+}
+
+void print_help(const Flags &f)
+{
+    std::printf("Available options:\n");
+    for (const auto &a : f.num) std::printf("--%s=<number> (default %g)\n", a.first.c_str(), a.second);
+    for (const auto &a : f.bo) std::printf("--%s\n", a.first.c_str());
+    for (const auto &a : f.str) std::printf("--%s=<string> (default \"%s\")\n", a.first.c_str(), a.second.c_str());
+}
+
+// --key=value, --key value, or --flag (booleans); returns 1 for --help
+int parse(int argc, char **argv, Flags &f)
+{
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--help" || a == "-h") return 1;
+        if (a.rfind("--", 0) != 0) throw gpvmc::Error("unexpected argument " + a);
+        a = a.substr(2);
+        std::string key = a, val;
+        const size_t eq = a.find('=');
+        bool has_val = eq != std::string::npos;
+        if (has_val) {
+            key = a.substr(0, eq);
+            val = a.substr(eq + 1);
+        }
+        if (f.bo.count(key)) {
+            if (has_val) throw gpvmc::Error("flag --" + key + " takes no value");
+            f.bo[key] = true;
+            continue;
+        }
+        if (!f.num.count(key) && !f.str.count(key)) throw gpvmc::Error("unknown option --" + key);
+        if (!has_val) {
+            if (i + 1 >= argc) throw gpvmc::Error("option --" + key + " needs a value");
+            val = argv[++i];
+        }
+        if (f.num.count(key)) {
+            char *end = nullptr;
+            const double v = std::strtod(val.c_str(), &end);
+            if (end == val.c_str() || *end) throw gpvmc::Error("option --" + key + ": not a number: " + val);
+            f.num[key] = v;
+        } else {
+            f.str[key] = val;
+        }
+    }
+    return 0;
+}
+
+// FileManager's run number: --prefix when >= 0, else one more than <dir>/.runfile (0 when
+// absent), written back (FileManager.cpp:20-64)
+int run_number(const std::string &dir, int prefix)
+{
+    if (prefix >= 0) return prefix;
+    int num = 0;
+    std::ifstream in(dir + "/.runfile");
+    if (in >> num)
+        ++num;
+    else
+        num = 0;
+    std::ofstream out(dir + "/.runfile");
+    if (!(out << num << "\n")) throw gpvmc::Error("error writing runfile in " + dir);
+    return num;
+}
+
+int run_flags(int argc, char **argv)
+{
+    Flags f = default_flags();
+    if (parse(argc, argv, f)) {
+        print_help(f);
+        return 0;
+    }
+    const int L = (int)f.num["L"], W = (int)f.num["walkers"];
+    const std::string channel = f.str["channel"];
+    gpvmc::StagFluxWaveFunction::Channel ch;
+    if (channel == "groundstate")
+        ch = gpvmc::StagFluxWaveFunction::GroundState;
+    else if (channel == "trans")
+        ch = gpvmc::StagFluxWaveFunction::TransExciton;
+    else if (channel == "long")
+        ch = gpvmc::StagFluxWaveFunction::LongExciton;
+    else
+        throw gpvmc::Error("unknown channel " + channel);
+    int prec;
+    if (f.str["precision"] == "fp64")
+        prec = GPVMC_FP64;
+    else if (f.str["precision"] == "mixed")
+        prec = GPVMC_MIXED;
+    else
+        throw gpvmc::Error("precision must be fp64 or mixed");
+    if (f.num["jr"] != 0.0) throw gpvmc::Error("--jr != 0 is not supported (the reference's vmc does not use it)");
+    if (!f.str["spinstate"].empty())
+        throw gpvmc::Error("--spinstate reads a FockState HDF5 file, which this build does not read");
+    if (f.bo["Neel_init"] && ch == gpvmc::StagFluxWaveFunction::TransExciton)
+        throw gpvmc::Error("--Neel_init fills L*L/2 sites per flavour; the trans channel has L*L/2 +- 1");
+    if (f.bo["track_stagmagn"] && ch != gpvmc::StagFluxWaveFunction::GroundState)
+        throw gpvmc::Error("--track_stagmagn is measured in ground-state contexts only");
+    const size_t samples = (size_t)f.num["samples"], saves = (size_t)f.num["samples_saves"],
+                 stat = (size_t)f.num["samples_saves_stat"], therm = (size_t)f.num["therm"];
+    size_t meas = (size_t)f.num["meas_interv"];
+    if (!meas) meas = (size_t)L * L;
+    const uint64_t seed = (uint64_t)f.num["seed"];
+    std::printf("seed=%llu\n", (unsigned long long)seed);
+
+    gpvmc::FileManager fm(W);
+    fm.StatPerSample() = (int)saves;
+    for (const auto &a : f.bo) fm.FileAttribute(a.first, a.second ? 1.0 : 0.0);
+    for (const auto &a : f.num) fm.FileAttribute(a.first, a.second);
+    for (const auto &a : f.str) fm.FileAttribute(a.first, a.second);
+    fm.FileAttribute("stagflux_wav", 1.0);
+    fm.FileAttribute("sfpnxphz_wav", 0.0);
+    fm.FileAttribute("sfpnzphx_wav", 0.0);
+
+    gpvmc::StagFluxWaveFunction wav(L, f.num["phi"], f.num["neel"], f.num["neel_exp"], f.num["phase_shift_x"],
+                                    f.num["phase_shift_y"], ch, (int)f.num["qx"], (int)f.num["qy"]);
+    gpvmc::SlaterDeterminant amp(wav, W, prec, seed);
+    std::vector<int32_t> spins, ok;
+    if (f.bo["Neel_init"]) {
+        // vmc.cpp:163-177: flavour 0 on the sites with x + y odd
+        spins.resize((size_t)W * L * L);
+        for (int w = 0; w < W; ++w)
+            for (int y = 0; y < L; ++y)
+                for (int x = 0; x < L; ++x) spins[(size_t)w * L * L + y * L + x] = (x + y) % 2 ? 0 : 1;
+        ok = amp.Init(spins);
+        for (int x : ok)
+            if (!x) throw gpvmc::Error("the Neel configuration has a zero amplitude");
+    } else {
+        // vmc.cpp:222-241: up to 10 RanInit draws per chain (draw t from seed + 1000 t)
+        spins = gpvmc::RanInit(L, W, wav.GetNpt(0), seed);
+        ok = amp.Init(spins);
+        for (int tries = 1;; ++tries) {
+            bool all = true;
+            for (int x : ok) all = all && x;
+            if (all) break;
+            if (tries == 10) throw gpvmc::Error("could not find a non-singular starting configuration.");
+            const std::vector<int32_t> fresh = gpvmc::RanInit(L, W, wav.GetNpt(0), seed + 1000ull * tries);
+            for (int w = 0; w < W; ++w)
+                if (!ok[w]) std::copy(fresh.begin() + (size_t)w * L * L, fresh.begin() + (size_t)(w + 1) * L * L,
+                                      spins.begin() + (size_t)w * L * L);
+            ok = amp.Init(spins);
+        }
+    }
+    gpvmc::LatticeStepper stepper(amp);
+    gpvmc::MetroMC mc(stepper);
+    gpvmc::ProjHeis heis(&fm);
+    gpvmc::StagMagnZ magn(&fm);
+    gpvmc::StatSpinStruct sss(&fm);
+    gpvmc::StagMagnTrack mtrack(&fm);
+    gpvmc::OverlapTrack otrack(&fm);
+    const bool ground = ch == gpvmc::StagFluxWaveFunction::GroundState;
+    if (f.bo["meas_projheis"]) mc.AddQuantity(&heis);
+    if (ground && f.bo["meas_stagmagn"]) mc.AddQuantity(&magn);
+    if (ground && f.bo["meas_statspinstruct"]) mc.AddQuantity(&sss);
+    if (f.bo["track_stagmagn"]) mc.AddQuantity(&mtrack);
+    if (f.bo["track_overlap"]) mc.AddQuantity(&otrack);
+    if (therm) mc.Walk((int64_t)(therm * L * L), 0);
+    const int num = run_number(f.str["dir"], (int)f.num["prefix"]);
+    double walk_s = 0;
+    for (size_t sample = 0; sample < samples; ++sample)
+        for (size_t m = 0; m < saves; ++m) {
+            const auto t0 = std::chrono::steady_clock::now();
+            mc.Walk((int64_t)(meas * stat), (int64_t)meas);
+            walk_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            for (gpvmc::Quantity *q : mc.GetQuantities()) q->save(amp);
+            fm.DataAttribute("rej", mc.RejectionPerChain());
+            fm.DataAttribute("time", walk_s);
+            fm.DataAttribute("statistics", (double)((m + 1) * stat));
+            fm.Write();
+            if (f.num["verbose"] > 0)
+                std::printf("sample %zu save %zu: rejection %.6f\n", sample, m, mc.Rejection());
+        }
+    fm.WriteJSON(f.str["dir"], std::to_string(num));
+    std::printf("{\"prefix\": %d, \"chains\": %d, \"rejection\": %.15g, \"quantities\": [", num, W, mc.Rejection());
+    const std::vector<std::string> names = fm.Names();
+    for (size_t i = 0; i < names.size(); ++i) std::printf("%s\"%s\"", i ? ", " : "", names[i].c_str());
+    std::printf("]}\n");
+    return 0;
+}
+
+}  // namespace
+
 int main(int argc, char **argv)
 {
+    if (argc > 1 && std::strncmp(argv[1], "-", 1) == 0) {
+        try {
+            return run_flags(argc, argv);
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "vmc_host: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc != 13) {
         std::fprintf(stderr, "usage: %s L walkers phi neel channel qx qy therm samples meas prec seed\n", argv[0]);
         return 2;

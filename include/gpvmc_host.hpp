@@ -16,16 +16,20 @@
  *   SlaterDeterminant      SlaterDeterminant::Init / Amp (SlaterDeterminant.cpp:60-92)
  *   LatticeStepper         LatticeStepper (LatticeStepper.cpp:36-147), proposal and step on
  *                          the device
- *   Quantity, ProjHeis, StagMagnZ, StatSpinStruct, OverlapTrack
+ *   Quantity, ProjHeis, StagMagnZ, StatSpinStruct, OverlapTrack, StagMagnTrack
  *                          Quantity::init / save (Quantity.cpp:19-27, ScalarQuantity.cpp,
- *                          MatrixQuantity.cpp)
+ *                          VectorQuantity.cpp, MatrixQuantity.cpp, OverlapTrack.cpp,
+ *                          StagMagnTrack.cpp)
  *   MetroMC                MetroMC::Walk / AddQuantity / Rejection (MetroMC.cpp:27-103)
+ *   MatStream, FileManager the running mean over saves and the data-k bookkeeping
+ *                          (FileManager.h:78-108, FileManager.cpp:465-551)
  */
 #ifndef GPVMC_HOST_HPP
 #define GPVMC_HOST_HPP
 
 #include <complex>
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -120,26 +124,93 @@ class LatticeStepper {
     SlaterDeterminant &amp_;
 };
 
-/* Quantity: init() clears the sums, save() reads them (per walker, averaged over its
- * measurements) */
+/* MatStream::Write (FileManager.h:78-108).  The first Write after a restart (Stat() == 0)
+ * stores the m x n doubles; every later Write must have the same shape (else Error) and
+ * replaces the stored matrix with the running mean (stat * old + new) / (stat + 1). */
+class MatStream {
+  public:
+    void Write(int m, int n, const double *mat);
+    int &Stat() { return stat_; }
+    int Rows() const { return nrow_; }
+    int Cols() const { return ncol_; }
+    const std::vector<double> &Mat() const { return mat_; }
+
+  private:
+    int nrow_ = 0, ncol_ = 0, stat_ = 0;
+    std::vector<double> mat_;
+};
+
+/* FileManager's stream and dataset bookkeeping (FileManager.cpp:465-551) for a batch of
+ * chains: the reference keeps one FileManager per rank (one chain), here chain c is walker c.
+ * FileStream(name, c) is the MatStream a quantity's save writes into.  Write() stores every
+ * stream's matrix as a dataset of its chain: a new one (data-k, k = the chain's count so far)
+ * when the stream's Stat() is 1, else over the last one; a stream whose Stat() reaches
+ * StatPerSample() restarts at its next Write (one dataset per sample of StatPerSample()
+ * saves).  The data attributes set before a Write (rej, time, statistics) go onto every
+ * dataset it writes.  The reference writes each quantity's records to <dir>/<prefix>-<name>.h5
+ * as /rank-<r>/data-<k> with the root attributes and "type" = name; WriteJSON writes the same
+ * tree to <dir>/<prefix>-<name>.json ("attributes", "rank-<r>": {"data-<k>": {"shape",
+ * "attr", "data" row-major}}). */
+class FileManager {
+  public:
+    struct Dataset {
+        int rows = 0, cols = 0;
+        std::vector<double> mat;
+        std::map<std::string, double> attr;
+    };
+    explicit FileManager(int n_chains);
+    int NChains() const { return n_chains_; }
+    MatStream &FileStream(const std::string &name, int chain);
+    int &StatPerSample() { return stat_per_sample_; }
+    void FileAttribute(const std::string &key, double v) { fattr_num_[key] = v; }
+    void FileAttribute(const std::string &key, const std::string &v) { fattr_str_[key] = v; }
+    /* one value for every chain, or one per chain */
+    void DataAttribute(const std::string &key, double v);
+    void DataAttribute(const std::string &key, const std::vector<double> &per_chain);
+    void Write();
+    std::vector<std::string> Names() const;
+    const std::vector<Dataset> &Datasets(const std::string &name, int chain) const;
+    void WriteJSON(const std::string &dir, const std::string &prefix) const;
+
+  private:
+    int n_chains_, stat_per_sample_ = 1;
+    std::map<std::string, std::vector<MatStream>> streams_;
+    std::map<std::string, std::vector<std::vector<Dataset>>> data_;
+    std::map<std::string, std::vector<double>> dattr_;
+    std::map<std::string, double> fattr_num_;
+    std::map<std::string, std::string> fattr_str_;
+};
+
+/* per-measurement traces a quantity asks MetroMC::Walk to record */
+enum Trace : uint32_t { TraceLogW = 1u, TraceStagMagn = 2u };
+
+/* Quantity: init() clears the sums (MetroMC::Walk calls it before every measuring walk, as
+ * MetroMC.cpp:49-51 does); save() averages them per walker over its measurements and, when
+ * the quantity has a FileManager, writes each walker's value into FileStream(name, walker)
+ * with the reference's shape. */
 class Quantity {
   public:
     virtual ~Quantity() = default;
     virtual uint32_t Mask() const = 0;
+    virtual uint32_t Traces() const { return 0; }
+    /* the traces of one walk: logw[w][m] = ln w, mz[w][m] complex, m < n_meas */
+    virtual void TakeTraces(int, int64_t, const double *, const double *) {}
     virtual void init(SlaterDeterminant &amp);
     virtual void save(SlaterDeterminant &amp) = 0;
     const std::string &GetName() const { return name_; }
 
   protected:
-    explicit Quantity(std::string name) : name_(std::move(name)) {}
+    Quantity(std::string name, FileManager *fm) : name_(std::move(name)), fm_(fm) {}
     std::string name_;
+    FileManager *fm_;
 };
 
 /* ProjHeis: ground state, E_loc per site averaged per walker (Val(0,0), ProjHeis.cpp:140-145);
- * channels, the H and O matrices averaged per walker (ProjHeis.cpp:146-153) */
+ * channels, the H and O matrices averaged per walker (ProjHeis.cpp:146-153).  Stream shape:
+ * 2 nk x 2 nk doubles, rows H then O (ground state: [E_loc; 1]). */
 class ProjHeis : public Quantity {
   public:
-    ProjHeis() : Quantity("ProjHeis") {}
+    explicit ProjHeis(FileManager *fm = nullptr) : Quantity("ProjHeis", fm) {}
     uint32_t Mask() const override { return GPVMC_MEAS_PROJHEIS; }
     void save(SlaterDeterminant &amp) override;
     std::vector<std::complex<double>> energy;  // [W] (ground state)
@@ -147,48 +218,78 @@ class ProjHeis : public Quantity {
     int nk = 0;
 };
 
-/* StagMagnZ: staggered magnetisation m_z averaged per walker (StagMagnZ.cpp:12-43) */
+/* StagMagnZ: staggered magnetisation m_z averaged per walker (StagMagnZ.cpp:12-43); stream
+ * 1 x 2 */
 class StagMagnZ : public Quantity {
   public:
-    StagMagnZ() : Quantity("StagMagnZ") {}
+    explicit StagMagnZ(FileManager *fm = nullptr) : Quantity("StagMagnZ", fm) {}
     uint32_t Mask() const override { return GPVMC_MEAS_STAGMAGNZ; }
     void save(SlaterDeterminant &amp) override;
     std::vector<std::complex<double>> mz;  // [W]
     std::vector<double> mz2;               // [W] <|m_z|^2>
 };
 
-/* StatSpinStruct: the 5 x Ns x Ns correlators summed over the walkers (StatSpinStruct.cpp:16-135) */
+/* StatSpinStruct: the 5 x Ns x Ns correlators (StatSpinStruct.cpp:16-135), `corr` summed over
+ * the walkers; stream 5 x 2 Ns^2 per walker */
 class StatSpinStruct : public Quantity {
   public:
-    StatSpinStruct() : Quantity("StatSpinStruct") {}
+    explicit StatSpinStruct(FileManager *fm = nullptr) : Quantity("StatSpinStruct", fm) {}
     uint32_t Mask() const override { return GPVMC_MEAS_STATSPIN; }
     void init(SlaterDeterminant &amp) override;
     void save(SlaterDeterminant &amp) override;
     std::vector<std::complex<double>> corr;  // [5][Ns][Ns], averaged over all measurements
 };
 
-/* OverlapTrack: the mean of ln w per walker (OverlapTrack.cpp:6-29) */
+/* w_m / mean(w) from ln w_m, without overflow */
+std::vector<double> RelativeWeights(const std::vector<double> &logw);
+
+/* OverlapTrack (OverlapTrack.cpp:6-29): the weights of one walk's measurements relative to
+ * their mean, w_m / mean(w), per walker (stream n_meas x 1); also the mean of ln w */
 class OverlapTrack : public Quantity {
   public:
-    OverlapTrack() : Quantity("OverlapTrack") {}
+    explicit OverlapTrack(FileManager *fm = nullptr) : Quantity("OverlapTrack", fm) {}
     uint32_t Mask() const override { return GPVMC_MEAS_OVERLAP; }
+    uint32_t Traces() const override { return TraceLogW; }
+    void TakeTraces(int W, int64_t n_meas, const double *logw, const double *mz) override;
+    void init(SlaterDeterminant &amp) override;
     void save(SlaterDeterminant &amp) override;
-    std::vector<double> mean_logw;  // [W]
+    std::vector<double> mean_logw;         // [W]
+    std::vector<std::vector<double>> rel;  // [W][n_meas] w_m / mean(w)
+
+  private:
+    std::vector<std::vector<double>> logw_;
+};
+
+/* StagMagnTrack (StagMagnTrack.cpp:17-50): m_z of every measurement of one walk, per walker
+ * (stream 2 n_meas x 1); ground-state contexts */
+class StagMagnTrack : public Quantity {
+  public:
+    explicit StagMagnTrack(FileManager *fm = nullptr) : Quantity("StagMagnTrack", fm) {}
+    uint32_t Mask() const override { return GPVMC_MEAS_STAGMAGNZ; }
+    uint32_t Traces() const override { return TraceStagMagn; }
+    void TakeTraces(int W, int64_t n_meas, const double *logw, const double *mz) override;
+    void init(SlaterDeterminant &amp) override;
+    void save(SlaterDeterminant &amp) override;
+    std::vector<std::vector<std::complex<double>>> mz;  // [W][n_meas]
 };
 
 class MetroMC {
   public:
     explicit MetroMC(LatticeStepper &stepper) : st_(stepper) {}
     void AddQuantity(Quantity *q) { quantities_.push_back(q); }
-    /* MetroMC::Walk(len, meas): len steps of every walker, measuring every meas steps
-     * (meas = 0: no measurement) */
+    const std::vector<Quantity *> &GetQuantities() const { return quantities_; }
+    /* MetroMC::Walk(len, meas): len steps of every walker; with meas > 0 the quantities are
+     * init()ed first and measured after step s when meas == 1 or s % meas == 0 */
     void Walk(int64_t len, int64_t meas);
-    /* rejected / proposed over every walker since the quantities' init */
+    /* rejected / proposed steps of the measuring walks so far (MetroMC.cpp:97-109: steps and
+     * rejections are counted only while measuring), over every walker, or per walker */
     double Rejection() const;
+    std::vector<double> RejectionPerChain() const;
 
   private:
     LatticeStepper &st_;
     std::vector<Quantity *> quantities_;
+    std::vector<double> steps_, rejected_;
 };
 
 }  // namespace gpvmc

@@ -166,3 +166,125 @@ def test_host_trans_channel_walk_equals_binding():
     assert out["nk"] == nk
     assert abs(out["trace_H"] - th) < 1e-10 * max(1.0, abs(th))
     assert abs(out["trace_O"] - to) < 1e-10 * max(1.0, abs(to))
+
+
+def fm_lib():
+    lib = host_lib()
+    lib.gpvmc_host_filemanager_script.argtypes = [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 2 + \
+        [ctypes.c_int32] + [ctypes.c_void_p] * 3
+    lib.gpvmc_host_last_error.restype = ctypes.c_char_p
+    lib.gpvmc_host_relative_weights.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+    return lib
+
+
+@pytest.mark.parametrize("chains,per_sample,saves", [(2, 3, 7), (3, 1, 4), (1, 4, 8)])
+def test_filemanager_dataset_per_sample_is_mean_of_its_saves(chains, per_sample, saves):
+    """FileManager.h:78-108 + FileManager.cpp:514-520: the running mean over the saves of a
+    sample equals their arithmetic mean, one dataset per sample of per_sample saves"""
+    lib = fm_lib()
+    m, n, max_ds = 2, 3, 16
+    rng = np.random.default_rng(chains * 100 + saves)
+    mats = rng.normal(size=(saves, chains, m * n))
+    shapes = np.tile(np.array([m, n], np.int32), saves)
+    n_ds = np.zeros(chains, np.int32)
+    out = np.zeros((chains, max_ds, m * n))
+    stats = np.zeros((chains, max_ds))
+    assert lib.gpvmc_host_filemanager_script(chains, per_sample, saves, p(shapes), p(mats), max_ds, p(n_ds),
+                                             p(out), p(stats)) == 0
+    groups = [list(range(s, min(s + per_sample, saves))) for s in range(0, saves, per_sample)]
+    assert list(n_ds) == [len(groups)] * chains
+    for c in range(chains):
+        for k, g in enumerate(groups):
+            np.testing.assert_allclose(out[c, k], mats[g, c].mean(axis=0), rtol=1e-13, atol=1e-14)
+            assert stats[c, k] == g[-1]
+
+
+def test_filemanager_shape_mismatch_is_an_error():
+    lib = fm_lib()
+    shapes = np.array([2, 2, 2, 3], np.int32)
+    mats = np.zeros(4 + 6)
+    n_ds = np.zeros(1, np.int32)
+    out = np.zeros((4, 4))
+    stats = np.zeros(4)
+    assert lib.gpvmc_host_filemanager_script(1, 5, 2, p(shapes), p(mats), 4, p(n_ds), p(out), p(stats)) == -1
+    assert b"sizes must match" in lib.gpvmc_host_last_error()
+
+
+def test_relative_weights():
+    """OverlapTrack.cpp:15-25: w_m / mean(w); also at ln w far beyond double range, where it
+    depends only on differences of ln w"""
+    lib = fm_lib()
+    lw = np.random.default_rng(3).uniform(-5, 5, 40)
+    rel = np.zeros_like(lw)
+    lib.gpvmc_host_relative_weights(len(lw), p(lw), p(rel))
+    w = np.exp(lw)
+    np.testing.assert_allclose(rel, w / w.mean(), rtol=1e-13)
+    big = lw + 5000.0
+    rel2 = np.zeros_like(lw)
+    lib.gpvmc_host_relative_weights(len(lw), p(big), p(rel2))
+    np.testing.assert_allclose(rel2, rel, rtol=1e-12)
+    # ln w near 5000 carries an ulp of ~1e-12, so the differences do too
+    assert abs(rel2.mean() - 1.0) < 1e-11
+
+
+@pytest.mark.gpu
+def test_host_flag_driver_sample_loop_equals_binding(tmp_path):
+    """vmc_host's flag form (vmc.cpp's sample / save loop) against the same walks through the
+    binding: per chain, each dataset is the mean over a sample's saves of the per-save averages,
+    rej is the chain's rejected / steps over the measuring walks so far"""
+    L, W, phi, neel, therm, samples, saves, stat, seed = 4, 3, 0.25, 0.1, 2, 2, 2, 5, 77
+    meas = L * L
+    if not os.path.exists(build.HOST_EXE):
+        pytest.skip("vmc_host not built")
+    print("case host flags", flush=True)
+    args = [build.HOST_EXE, f"--L={L}", f"--walkers={W}", f"--phi={phi}", f"--neel={neel}", f"--therm={therm}",
+            f"--samples={samples}", f"--samples_saves={saves}", f"--samples_saves_stat={stat}", "--meas_projheis",
+            "--meas_stagmagn", "--track_overlap", "--track_stagmagn", f"--seed={seed}", f"--dir={tmp_path}",
+            "--prefix=3", "--verbose=0"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    summary = json.loads(r.stdout.strip().splitlines()[-1])
+    assert summary["prefix"] == 3
+    files = {q: json.load(open(tmp_path / f"3-{q}.json")) for q in summary["quantities"]}
+    assert set(files) == {"ProjHeis", "StagMagnZ", "OverlapTrack", "StagMagnTrack"}
+    assert files["ProjHeis"]["attributes"]["type"] == "ProjHeis"
+    assert files["ProjHeis"]["attributes"]["L"] == L and files["ProjHeis"]["attributes"]["channel"] == "groundstate"
+
+    b = binding.Batch(L, W, binding.FP64, n_groups=1, seed=seed)
+    b.set_orbitals(0, model.ground_occ(model.orbital_table(L, phi * np.pi, neel)))
+    ok = init_redrawing(b, lambda s: model.random_spins(L, W, s), seed)
+    assert ok.all()
+    b.walk(therm * L * L)
+    mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ | binding.MEAS_OVERLAP
+    steps = np.zeros(W)
+    rejected = np.zeros(W)
+    for sample in range(samples):
+        per_save = []
+        for m in range(saves):
+            b.reset_accumulators()
+            tr = b.walk(meas * stat, meas=meas, mask=mask, trace=True)
+            acc = b.accumulators()
+            steps += acc[:, 7]
+            rejected += acc[:, 9]
+            e = (acc[:, 0] + 1j * acc[:, 1]) / acc[:, 3]
+            mz = (acc[:, 4] + 1j * acc[:, 5]) / acc[:, 3]
+            w = np.exp(tr["logw"] - tr["logw"].max(axis=1, keepdims=True))
+            per_save.append(dict(
+                ProjHeis=np.stack([e.real, e.imag, np.ones(W), np.zeros(W)], 1),
+                StagMagnZ=np.stack([mz.real, mz.imag], 1),
+                OverlapTrack=w / w.mean(axis=1, keepdims=True),
+                StagMagnTrack=np.stack([tr["mz"].real, tr["mz"].imag], 2).reshape(W, -1)))
+        # the sample's last save leaves its dataset at the mean of the sample's saves, with
+        # that save's attributes
+        shapes = dict(ProjHeis=[2, 2], StagMagnZ=[1, 2], OverlapTrack=[stat, 1], StagMagnTrack=[2 * stat, 1])
+        for q, f in files.items():
+            want = np.mean([ps[q] for ps in per_save], axis=0)
+            for c in range(W):
+                d = f[f"rank-{c}"][f"data-{sample}"]
+                assert d["shape"] == shapes[q]
+                assert d["attr"]["statistics"] == saves * stat
+                assert abs(d["attr"]["rej"] - rejected[c] / steps[c]) < 1e-15
+                np.testing.assert_allclose(np.array(d["data"]), want[c], rtol=1e-11, atol=1e-12)
+    for f in files.values():
+        assert all(len(f[f"rank-{c}"]) == samples for c in range(W))
+    b.close()
