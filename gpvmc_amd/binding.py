@@ -10,6 +10,9 @@ MEAS_PROJHEIS = 1
 MEAS_STAGMAGNZ = 2
 MEAS_OVERLAP = 4
 MEAS_STATSPIN = 8
+SWEEP_DEFAULT = 0
+SWEEP_ONE_MOVE = 1
+SWEEP_WINDOWS = 2
 ACC_STRIDE = 12
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -27,6 +30,8 @@ SIGNATURES = {
     "gpvmc_set_groups": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_init_walkers": (ctypes.c_int, [_vp, _vp, _vp]),
     "gpvmc_refresh": (ctypes.c_int, [_vp, _vp]),
+    "gpvmc_set_sweep_variant": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "gpvmc_sweep_variant": (ctypes.c_int, [_vp]),
     "gpvmc_walk": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32,
                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gpvmc_measure": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
@@ -149,6 +154,15 @@ class Batch:
         d = np.zeros(self.W) if drift else None
         self._check(self.lib.gpvmc_refresh(self.h, _ptr(d)))
         return d
+
+    @property
+    def sweep_variant(self):
+        """the kernel that runs: SWEEP_ONE_MOVE or SWEEP_WINDOWS (delayed-update windows)"""
+        return self.lib.gpvmc_sweep_variant(self.h)
+
+    @sweep_variant.setter
+    def sweep_variant(self, v):
+        self._check(self.lib.gpvmc_set_sweep_variant(self.h, int(v)))
 
     def walk(self, n_steps, meas=0, mask=MEAS_PROJHEIS | MEAS_STAGMAGNZ, uniforms=None,
              trace=False):

@@ -75,6 +75,7 @@ int default_lpr(int prec, int N)
 struct gpvmc_ctx {
     gpvmc_config cfg;
     int L = 0, Ns = 0, N = 0, W = 0, G = 0, prec = 0, lpr = 0, res = 0, nthreads = 0;
+    int variant = GPVMC_SWEEP_DEFAULT;  // gpvmc_set_sweep_variant
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t esize = 0;  // bytes per complex element of the ratio tables
@@ -377,6 +378,30 @@ static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     return GPVMC_OK;
 }
 
+// the sweep kernel of the context's variant: delayed-update windows where this shape has
+// them, else one accepted move per window
+// the default variant per shape: delayed-update windows where they are measured faster
+static bool windows_by_default(int prec, int N, int lpr)
+{
+    (void)prec;
+    (void)N;
+    (void)lpr;
+    return false;
+}
+
+static bool uses_windows(const gpvmc_ctx *c)
+{
+    if (c->channel || !gv_sweep_wdu_supported(c->prec, c->N, c->lpr)) return false;
+    return c->variant == GPVMC_SWEEP_WINDOWS ||
+           (c->variant == GPVMC_SWEEP_DEFAULT && windows_by_default(c->prec, c->N, c->lpr));
+}
+
+static hipError_t launch_sweep(const gpvmc_ctx *c, const gv::SweepArgs &a, hipStream_t st)
+{
+    if (uses_windows(c)) return gv_launch_sweep_wdu(c->prec, c->N, c->lpr, &a, c->W, st);
+    return gv_launch_sweep(c->prec, c->N, c->lpr, &a, c->W, st);
+}
+
 static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask, const double *uniforms,
                         int8_t *noop, int8_t *accept, double *ratio, double *eloc, double *mz, double *logw,
                         int measure_only)
@@ -605,6 +630,24 @@ int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     return GPVMC_OK;
 }
 
+int gpvmc_set_sweep_variant(gpvmc_ctx *c, int32_t variant)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (c->channel) return fail(GPVMC_ERR_UNSUPPORTED, "channel contexts have one sweep kernel per shape");
+    if (variant != GPVMC_SWEEP_DEFAULT && variant != GPVMC_SWEEP_ONE_MOVE && variant != GPVMC_SWEEP_WINDOWS)
+        return fail(GPVMC_ERR_ARG, "variant must be GPVMC_SWEEP_DEFAULT, _ONE_MOVE or _WINDOWS");
+    if (variant == GPVMC_SWEEP_WINDOWS && !gv_sweep_wdu_supported(c->prec, c->N, c->lpr))
+        return fail(GPVMC_ERR_UNSUPPORTED, "no delayed-update window kernel for this precision and N");
+    c->variant = variant;
+    return GPVMC_OK;
+}
+
+int gpvmc_sweep_variant(const gpvmc_ctx *c)
+{
+    if (!c) return -1;
+    return uses_windows(c) ? GPVMC_SWEEP_WINDOWS : GPVMC_SWEEP_ONE_MOVE;
+}
+
 int gpvmc_refresh(gpvmc_ctx *c, double *drift)
 {
     if (!c) return fail(GPVMC_ERR_ARG, "null context");
@@ -712,7 +755,7 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
     if (mz && nmeas) a.tr_mz = reinterpret_cast<double *>(sb + o_mz);
     if (logw && nmeas) a.tr_logw = reinterpret_cast<double *>(sb + o_lw);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(gv_launch_sweep(c->prec, c->N, c->lpr, &a, c->W, c->stream));
+    HIPCHK(launch_sweep(c, a, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -1193,7 +1236,7 @@ static hipError_t enqueue_member(const gpvmc_batch::Member &mb, hipStream_t st)
         return launch_exc(c, a, st);
     }
     gv::SweepArgs a = sweep_args(c, mb.n_steps, mb.meas, mb.mask, nmeas, 0);
-    return gv_launch_sweep(c->prec, c->N, c->lpr, &a, c->W, st);
+    return launch_sweep(c, a, st);
 }
 
 int gpvmc_batch_create(int32_t device, gpvmc_batch **out)

@@ -153,6 +153,9 @@ struct GJDynArgs {
 
 extern "C" {
 int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
+/* delayed-update windows (several accepted moves per table update) for this shape */
+int gv_sweep_wdu_supported(int prec, int N, int lpr);
+hipError_t gv_launch_sweep_wdu(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks, hipStream_t st);
 hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
                            hipStream_t st);
 int gv_refresh_supported(int prec, int N);

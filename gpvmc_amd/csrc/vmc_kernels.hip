@@ -121,6 +121,54 @@ __device__ __forceinline__ double readlane_real(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// number of set bits of the label mask m below bit x (x < 64 * NMW)
+template <int NMW> __device__ __forceinline__ int mask_rank(const uint64_t (&m)[NMW], int x)
+{
+    if constexpr (NMW == 1) {
+        return __popcll(m[0] & ((1ull << x) - 1ull));
+    } else {
+        return x < 64 ? __popcll(m[0] & ((1ull << x) - 1ull))
+                      : __popcll(m[0]) + __popcll(m[1] & ((1ull << (x - 64)) - 1ull));
+    }
+}
+
+// orders this wavefront's LDS accesses (one wavefront's LDS operations complete in issue
+// order; this keeps the compiler from moving them across)
+__device__ __forceinline__ void wave_lds_order()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Delayed-update windows (WDU sweep kernel), sizes of the dynamic LDS carve-up:
+//   two row dumps [WD_ROWS][LPR][2][RP] of T (double-buffered by window parity),
+//   then the window scratch of wavefront 0 (WduScratch).
+constexpr int WD_K = 16;     // steps per window
+constexpr int WD_M = 4;      // accepted moves per window at most
+constexpr int WD_ROWS = 2 * WD_K;
+template <typename T> struct WduScratch {
+    using C = typename cpx<T>::t;
+    C xu[WD_K * WD_K];                // up table rows U x columns D of the window
+    C xd[WD_K * WD_K];                // down table rows D x columns U
+    C buf[2][WD_M][WD_ROWS];          // per move and table: row (+e) then column (-e)
+    int lab[2][WD_ROWS];              // labels of the window, by position, per flavour
+    double2 coef[2][2][WD_M][WD_M];   // [table][gamma, beta][n][b]
+    double2 qv[2][WD_M];              // ratio of each move, per table
+    double2 st[2][2][WD_M][WD_M];     // [table][S, T][n][d]
+    C mm[2][WD_M][WD_M];              // M = S^T diag(1/q) T per table
+    int mv[WD_M][4];                  // ci, cj, sa, sb of each move
+    int pos[WD_M][2];                 // position of ci in U, of cj in D
+    int res[4];                       // moves, steps processed
+};
+template <typename T, int N, int LPR> constexpr size_t wdu_dump_elems()
+{
+    return (size_t)WD_ROWS * LPR * 2 * (N / LPR + 16 / sizeof(T));
+}
+template <typename T, int N, int LPR> constexpr size_t wdu_lds_bytes()
+{
+    return 2 * wdu_dump_elems<T, N, LPR>() * sizeof(T) + sizeof(WduScratch<T>);
+}
+
 // ---------------------------------------------------------------------------------
 // Sweep kernel: n_steps of MetroMC::Step (MetroMC.cpp:90-103) for one walker per block.
 //
@@ -150,7 +198,7 @@ __device__ __forceinline__ double readlane_real(double v, int l)
 //    applies the rank-1 update (its rows go through LDS, one more barrier), and the next
 //    window starts after it.  Without an accepted move the window simply advances.
 // Every decision is the one MetroMC::Step takes on the same uniforms, in the same order.
-template <typename T, int N, int LPR, bool RES, bool MONLY>
+template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
 {
@@ -183,7 +231,11 @@ sweep_kernel(SweepArgs a)
     // StatSpinStruct: a chunk of the transposed G_dn (RES kernels)
     constexpr int TRK = LPR * N * 2 * (int)sizeof(T);         // bytes per slot k of G_dn^T
     constexpr int CH = divisor_within(CPT, TRK, 32768);         // slots per chunk
-    __shared__ double s_tr[RES ? (CH * TRK + 7) / 8 : 1];
+    __shared__ double s_tr[(RES && !WDU) ? (CH * TRK + 7) / 8 : 1];
+    // WDU: row dumps and window scratch (dynamic), which the StatSpinStruct chunks reuse
+    extern __shared__ __align__(16) char smem[];
+    static_assert(!WDU || (RES && N <= 128 && !MONLY), "delayed-update windows need register tables, N <= 128");
+    static_assert(!WDU || (size_t)CH * TRK <= 2 * wdu_dump_elems<T, N, LPR>() * sizeof(T), "statspin chunk");
 
     const int w = __builtin_amdgcn_readfirstlane(a.order ? a.order[blockIdx.x] : (int)blockIdx.x);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -246,7 +298,7 @@ sweep_kernel(SweepArgs a)
         for (int k0 = 0; k0 < CPT; k0 += CH) {
             // rows j = c' + LPR k of G_dn, k in [k0, k0 + CH), i.e. j in [LPR k0, LPR (k0 + CH))
             if constexpr (RES) {
-                T *sT = reinterpret_cast<T *>(s_tr);
+                T *sT = WDU ? reinterpret_cast<T *>(smem) : reinterpret_cast<T *>(s_tr);
                 __syncthreads();
                 if (has_row && row >= LPR * k0 && row < LPR * (k0 + CH)) {
 #pragma unroll UNR
@@ -374,41 +426,45 @@ sweep_kernel(SweepArgs a)
         __syncthreads();
     };
 
+    // uniforms of the batch of 64 steps starting at s0 (s0 % 64 == 0): lane l decodes step
+    // s0 + l into bpv = v1 | (v2 << 16) and its acceptance uniform bu3
+    auto draw_batch = [&](int64_t s0, int &bpv, double &bu3) {
+        const int64_t sl = s0 + lane;
+        int v1 = 0, ed = 0;
+        double u3 = 1.0;
+        if (a.uniforms) {
+            if (sl < a.n_steps) {
+                const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
+                v1 = (int)(NS * up[0]);
+                v1 = v1 < NS ? v1 : NS - 1;
+                ed = (int)(4 * up[1]);
+                ed = ed < 4 ? ed : 3;
+                u3 = up[3];
+            }
+        } else {
+            const uint64_t st = ctr0 + (uint64_t)sl;
+            uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
+            philox4x32_10_dev(cc, a.key0, a.key1);
+            // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers
+            // (exact: NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused
+            v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
+            ed = (int)(cc[1] >> 30);
+            u3 = u32_to_open01(cc[3]);
+        }
+        bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
+        bu3 = u3;
+    };
+
     if constexpr (MONLY) {
         measure(-1);
-    } else {
+    } else if constexpr (!WDU) {
         int bpv = 0;          // batch: v1 | (v2 << 16) of step (s0 & ~63) + lane
         double bu3 = 1.0;     // batch: acceptance uniform of that step
         int par = 0;
         int64_t s0 = 0;
         while (s0 < a.n_steps) {
             const int b0 = (int)(s0 & 63);
-            if (b0 == 0) {
-                const int64_t sl = s0 + lane;
-                int v1 = 0, ed = 0;
-                double u3 = 1.0;
-                if (a.uniforms) {
-                    if (sl < a.n_steps) {
-                        const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
-                        v1 = (int)(NS * up[0]);
-                        v1 = v1 < NS ? v1 : NS - 1;
-                        ed = (int)(4 * up[1]);
-                        ed = ed < 4 ? ed : 3;
-                        u3 = up[3];
-                    }
-                } else {
-                    const uint64_t st = ctr0 + (uint64_t)sl;
-                    uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
-                    philox4x32_10_dev(cc, a.key0, a.key1);
-                    // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers
-                    // (exact: NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused
-                    v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
-                    ed = (int)(cc[1] >> 30);
-                    u3 = u32_to_open01(cc[3]);
-                }
-                bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
-                bu3 = u3;
-            }
+            if (b0 == 0) draw_batch(s0, bpv, bu3);
             // window: up to KW steps, within the batch, ending at the next measured step
             int64_t lim = a.n_steps - s0;
             if (lim > 64 - b0) lim = 64 - b0;
@@ -598,6 +654,367 @@ sweep_kernel(SweepArgs a)
             par ^= 1;
             const int64_t last = s0 - 1;
             if (a.meas > 0 && (a.meas == 1 || last % a.meas == 0)) measure(last);
+        }
+    } else {
+        // Delayed-update windows.  A window is up to WD_K steps (inside the batch, ending
+        // at the next measured step) with at most WD_M accepted moves.  The particles the
+        // window can touch are those at its sites: the up labels U and down labels D found
+        // there (a swap only exchanges particles between the window's sites), so every
+        // ratio the window needs is an entry of the sub-blocks G_up[U][D] and G_dn[D][U].
+        //  1. the owners of the rows U of G_up and D of G_dn copy them to an LDS dump
+        //     (double-buffered by window parity); barrier;
+        //  2. wavefront 0 copies the sub-blocks and runs MetroMC::Step on them for each step
+        //     in order, updating them after every accepted move with the table's own
+        //     rank-1 formula; it records for move n the sub-block row y_n (+e) and column
+        //     x_n (-e) and q_n, and from them the m x m matrix M with
+        //        G_after = G - C M R,   C[:,b] = G[:,col_b] - e_row_b,  R[d,:] = G[row_d,:] + e_col_d
+        //     (x_n = sum_b S_nb C[:,b], y_n = sum_d T_nd R[d,:], S and T unit lower
+        //     triangular from gamma_nb = y_b[col_n] / q_b and beta_nb = x_b[row_n] / q_b,
+        //     M = S^T diag(1/q) T); it writes the moves, counts and traces; barrier;
+        //  3. every thread applies the rank-m update to its rows, R from the dump.
+        // The decisions are MetroMC::Step's on the same uniforms in the same order; the
+        // tables after the window equal the m rank-1 updates up to rounding.
+        constexpr int NMW = (N + 63) / 64;
+        constexpr size_t DE = wdu_dump_elems<T, N, LPR>();
+        const int wu = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform: scalar branches
+        WduScratch<T> &ws = *reinterpret_cast<WduScratch<T> *>(smem + 2 * DE * sizeof(T));
+        int bpv = 0;
+        double bu3 = 1.0;
+        int par = 0;
+        int64_t s0 = 0;
+        int64_t next_meas = 0;  // the next measured step (meas > 0)
+        while (s0 < a.n_steps) {
+            const int b0 = (int)(s0 & 63);
+            if (b0 == 0) draw_batch(s0, bpv, bu3);
+            int64_t lim = a.n_steps - s0;
+            if (lim > 64 - b0) lim = 64 - b0;
+            if (a.meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
+            const int len = (int)(lim < WD_K ? lim : WD_K);
+            const bool inw = lane >= b0 && lane < b0 + len;
+            const int v1 = bpv & 0xFFFF, v2 = bpv >> 16;
+            const double u3 = bu3;
+            const int i1 = s_site[v1], i2 = s_site[v2];
+            // label masks of the window's sites (identical in every wavefront)
+            uint64_t um[NMW], dm[NMW];
+#pragma unroll
+            for (int q = 0; q < NMW; ++q) um[q] = dm[q] = 0ull;
+            for (int i = b0; i < b0 + len; ++i) {
+                const int x1 = __builtin_amdgcn_readlane(i1, i), x2 = __builtin_amdgcn_readlane(i2, i);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int x = h ? x2 : x1, l = x & 0xFFFF;
+#pragma unroll
+                    for (int q = 0; q < NMW; ++q) {
+                        const uint64_t bit = ((l >> 6) == q) ? (1ull << (l & 63)) : 0ull;
+                        if (x >> 16) dm[q] |= bit;
+                        else um[q] |= bit;
+                    }
+                }
+            }
+            int nU = 0, nD = 0;
+#pragma unroll
+            for (int q = 0; q < NMW; ++q) {
+                nU += __popcll(um[q]);
+                nD += __popcll(dm[q]);
+            }
+            T *dump = reinterpret_cast<T *>(smem) + (size_t)par * DE;
+            if (has_row) {
+                const int rw = row >> 6;
+                uint64_t ur = um[0], dr = dm[0];
+                if constexpr (NMW > 1) {
+                    ur = rw ? um[1] : um[0];
+                    dr = rw ? dm[1] : dm[0];
+                }
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    if (((fl ? dr : ur) >> (row & 63)) & 1ull) {
+                        const int p = fl ? nU + mask_rank<NMW>(dm, row) : mask_rank<NMW>(um, row);
+                        T *d = dump + ((size_t)(p * LPR + c) * 2) * RP;
+#pragma unroll
+                        for (int k = 0; k < CPT; ++k) {
+                            d[k] = gr[fl][k];
+                            d[RP + k] = gi[fl][k];
+                        }
+                    }
+                }
+            }
+            if (wu == 0 && inw) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int x = h ? i2 : i1, l = x & 0xFFFF;
+                    if (x >> 16) ws.lab[1][mask_rank<NMW>(dm, l)] = l;
+                    else ws.lab[0][mask_rank<NMW>(um, l)] = l;
+                }
+            }
+            __syncthreads();
+            if (wu == 0) {
+                for (int e = lane; e < nU * nD; e += 64) {
+                    const int ua = e / nD, db = e - ua * nD;
+                    const int lu = ws.lab[0][ua], ld = ws.lab[1][db];
+                    const T *ru = dump + ((size_t)(ua * LPR + ld % LPR) * 2) * RP + ld / LPR;
+                    ws.xu[e] = cmk<C>(ru[0], ru[RP]);
+                    const T *rd = dump + ((size_t)((nU + db) * LPR + lu % LPR) * 2) * RP + lu / LPR;
+                    ws.xd[db * nU + ua] = cmk<C>(rd[0], rd[RP]);
+                }
+                wave_lds_order();
+                // the particles at the step's two sites as the window's moves leave them,
+                // (flavour << 16) | label
+                int x1 = i1, x2 = i2;
+                int m = 0, sst = b0, jend = b0 + len - 1;
+                uint64_t noop_m = 0ull;  // finished steps that were trivial
+                for (;;) {
+                    const bool act = inw && lane >= sst;
+                    const bool noop = ((x1 ^ x2) >> 16) == 0;
+                    const bool up1 = (x1 >> 16) == 0;
+                    const int ci = (up1 ? x1 : x2) & 0xFFFF, cj = (up1 ? x2 : x1) & 0xFFFF;
+                    int pa = 0, pb = 0;
+                    C qu = cmk<C>(0, 0), qd = cmk<C>(0, 0);
+                    if (act && !noop) {
+                        pa = mask_rank<NMW>(um, ci);
+                        pb = mask_rank<NMW>(dm, cj);
+                        qu = ws.xu[pa * nD + pb];
+                        qd = ws.xd[pb * nU + pa];
+                    }
+                    double ratio = 1.0;
+                    bool accl;
+                    if (noop) {
+                        accl = u3 < 1.0;  // LatticeStepper.cpp:64-68: ratio 1
+                    } else {
+                        const double n0 = (double)qu.x * qu.x + (double)qu.y * qu.y;
+                        const double n1 = (double)qd.x * qd.x + (double)qd.y * qd.y;
+                        ratio = n0 * n1;
+                        accl = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
+                    }
+                    const uint64_t moved = __ballot(act && !noop && accl);
+                    const int j = moved ? (int)__builtin_ctzll(moved) : 64;
+                    // steps sst .. j are finished with these decisions
+                    const bool fin = act && lane <= j;
+                    noop_m |= __ballot(fin && noop);
+                    if (fin) {
+                        const size_t ti = (size_t)w * a.n_steps + s0 + (lane - b0);
+                        if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
+                        if (a.tr_accept) a.tr_accept[ti] = accl ? 1 : 0;
+                        if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+                    }
+                    if (!moved) break;
+                    const int ci_j = __builtin_amdgcn_readlane(ci, j), cj_j = __builtin_amdgcn_readlane(cj, j);
+                    const int pa_j = __builtin_amdgcn_readlane(pa, j), pb_j = __builtin_amdgcn_readlane(pb, j);
+                    const int u1_j = __builtin_amdgcn_readlane(up1 ? 1 : 0, j);
+                    const int v1_j = __builtin_amdgcn_readlane(v1, j), v2_j = __builtin_amdgcn_readlane(v2, j);
+                    const int sa_j = u1_j ? v1_j : v2_j, sb_j = u1_j ? v2_j : v1_j;
+                    const C qU = cmk<C>(readlane_real(qu.x, j), readlane_real(qu.y, j));
+                    const C qD = cmk<C>(readlane_real(qd.x, j), readlane_real(qd.y, j));
+                    if (lane == 0) {
+                        ws.mv[m][0] = ci_j;
+                        ws.mv[m][1] = cj_j;
+                        ws.mv[m][2] = sa_j;
+                        ws.mv[m][3] = sb_j;
+                        ws.pos[m][0] = pa_j;
+                        ws.pos[m][1] = pb_j;
+                        ws.qv[0][m] = make_double2((double)qU.x, (double)qU.y);
+                        ws.qv[1][m] = make_double2((double)qD.x, (double)qD.y);
+                    }
+                    // y (row + e) and x (column - e) of both sub-blocks before the move:
+                    // up table row ci (over D), column cj (over U); down table row cj
+                    // (over U), column ci (over D)
+                    C *bu = ws.buf[0][m], *bd = ws.buf[1][m];
+                    if (lane < nD) {
+                        C y = ws.xu[pa_j * nD + lane];
+                        if (lane == pb_j) y.x += (T)1;
+                        bu[lane] = y;
+                        C x = ws.xd[lane * nU + pa_j];
+                        if (lane == pb_j) x.x -= (T)1;
+                        bd[nU + lane] = x;
+                    }
+                    if (lane < nU) {
+                        C x = ws.xu[lane * nD + pb_j];
+                        if (lane == pa_j) x.x -= (T)1;
+                        bu[nD + lane] = x;
+                        C y = ws.xd[pb_j * nU + lane];
+                        if (lane == pa_j) y.x += (T)1;
+                        bd[lane] = y;
+                    }
+                    wave_lds_order();
+                    // gamma_mb = y_b[col_m] / q_b and beta_mb = x_b[row_m] / q_b, b < m
+                    if (lane < 4 * m) {
+                        const int b = lane >> 2, wh = lane & 3, tb = wh >> 1, gb = wh & 1;
+                        const int off = tb == 0 ? (gb ? nD + pa_j : pb_j) : (gb ? nU + pb_j : pa_j);
+                        const C v = ws.buf[tb][b][off];
+                        const double2 q = ws.qv[tb][b];
+                        const double nq = q.x * q.x + q.y * q.y;
+                        ws.coef[tb][gb][m][b] = make_double2(((double)v.x * q.x + (double)v.y * q.y) / nq,
+                                                             ((double)v.y * q.x - (double)v.x * q.y) / nq);
+                    }
+                    // the sub-blocks after the move, with the table's rank-1 formula
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb) {
+                        const C q = tb ? qD : qU;
+                        const double nq = (double)q.x * q.x + (double)q.y * q.y;
+                        const C qi = cmk<C>(q.x / nq, -q.y / nq);
+                        const int nr = tb ? nD : nU, nc = tb ? nU : nD;
+                        const C *bb = tb ? bd : bu;
+                        C *xx = tb ? ws.xd : ws.xu;
+                        for (int e = lane; e < nr * nc; e += 64) {
+                            const int r = e / nc, cc = e - r * nc;
+                            const C coef = cmul(bb[nc + r], qi);
+                            const C y = bb[cc];
+                            const C g = xx[e];
+                            T er = g.x, ei = g.y;
+                            er = fma(-coef.x, y.x, er);
+                            er = fma(coef.y, y.y, er);
+                            ei = fma(-coef.x, y.y, ei);
+                            ei = fma(-coef.y, y.x, ei);
+                            xx[e] = cmk<C>(er, ei);
+                        }
+                    }
+                    wave_lds_order();
+                    // State::Hop on the window's sites
+                    const int xa = (1 << 16) | cj_j, xb = ci_j;
+                    x1 = v1 == sa_j ? xa : (v1 == sb_j ? xb : x1);
+                    x2 = v2 == sa_j ? xa : (v2 == sb_j ? xb : x2);
+                    ++m;
+                    sst = j + 1;
+                    if (m == WD_M || sst >= b0 + len) {
+                        jend = j;
+                        break;
+                    }
+                }
+                wave_lds_order();
+                if (m > 0) {
+                    // columns of S (tb, 0) and T (tb, 1): v_n = delta_nd - sum_b coef_nb v_b
+                    if (lane < 4 * WD_M) {
+                        const int tb = lane / (2 * WD_M), gb = (lane / WD_M) & 1, d = lane % WD_M;
+                        double2 v[WD_M];
+#pragma unroll
+                        for (int n = 0; n < WD_M; ++n) {
+                            double2 t = make_double2(n == d ? 1.0 : 0.0, 0.0);
+                            if (n < m) {
+#pragma unroll
+                                for (int b = 0; b < n; ++b) {
+                                    const double2 cf = ws.coef[tb][gb][n][b];
+                                    t.x -= cf.x * v[b].x - cf.y * v[b].y;
+                                    t.y -= cf.x * v[b].y + cf.y * v[b].x;
+                                }
+                                ws.st[tb][gb][n][d] = t;
+                            }
+                            v[n] = t;
+                        }
+                    }
+                    wave_lds_order();
+                    if (lane < 2 * WD_M * WD_M) {
+                        const int tb = lane / (WD_M * WD_M), b = (lane / WD_M) % WD_M, d = lane % WD_M;
+                        if (b < m && d < m) {
+                            double2 acc = make_double2(0.0, 0.0);
+                            for (int n = 0; n < m; ++n) {
+                                const double2 sv = ws.st[tb][0][n][b], tv = ws.st[tb][1][n][d], q = ws.qv[tb][n];
+                                const double pr = sv.x * tv.x - sv.y * tv.y, pi = sv.x * tv.y + sv.y * tv.x;
+                                const double nq = q.x * q.x + q.y * q.y;
+                                acc.x += (pr * q.x + pi * q.y) / nq;
+                                acc.y += (pi * q.x - pr * q.y) / nq;
+                            }
+                            ws.mm[tb][b][d] = cmk<C>(acc.x, acc.y);
+                        }
+                    }
+                }
+                const int nproc = jend - b0 + 1;
+                const int nnoop = __popcll(noop_m);
+                n_noop += nnoop;
+                n_acc += m;
+                n_rej += nproc - nnoop - m;
+                if (lane == 0) {
+                    ws.res[0] = m;
+                    ws.res[1] = nproc;
+                    // State::Hop (State.cpp:73-84) of each move, in order
+                    for (int n = 0; n < m; ++n) {
+                        const int ci_n = ws.mv[n][0], cj_n = ws.mv[n][1], sa_n = ws.mv[n][2], sb_n = ws.mv[n][3];
+                        s_site[sb_n] = ci_n;
+                        s_site[sa_n] = (1 << 16) | cj_n;
+                        s_part[0][ci_n] = sb_n;
+                        s_part[1][cj_n] = sa_n;
+                    }
+                    // amplitude *= q per flavour and move (SlaterDeterminant.cpp:321)
+                    for (int n = 0; n < m; ++n) {
+#pragma unroll
+                        for (int fl = 0; fl < 2; ++fl) {
+                            const double2 q = ws.qv[fl][n];
+                            const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
+                            const double re = mr * q.x - mi * q.y;
+                            const double im = mr * q.y + mi * q.x;
+                            int e;
+                            (void)frexp(fmax(fabs(re), fabs(im)), &e);
+                            s_amp[2 * fl] = ldexp(re, -e);
+                            s_amp[2 * fl + 1] = ldexp(im, -e);
+                            s_amp[4 + fl] += e;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int m = __builtin_amdgcn_readfirstlane(ws.res[0]);
+            const int nproc = __builtin_amdgcn_readfirstlane(ws.res[1]);
+            if (m > 0 && has_row) {
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    int colv[WD_M], rowv[WD_M], dpos[WD_M];
+                    C wv[WD_M];
+#pragma unroll
+                    for (int d = 0; d < WD_M; ++d) wv[d] = cmk<C>(0, 0);
+#pragma unroll
+                    for (int b = 0; b < WD_M; ++b) {
+                        colv[b] = rowv[b] = dpos[b] = 0;
+                        if (b < m) {
+                            const int ci_b = __builtin_amdgcn_readfirstlane(ws.mv[b][0]);
+                            const int cj_b = __builtin_amdgcn_readfirstlane(ws.mv[b][1]);
+                            colv[b] = fl ? ci_b : cj_b;
+                            rowv[b] = fl ? cj_b : ci_b;
+                            dpos[b] = fl ? nU + __builtin_amdgcn_readfirstlane(ws.pos[b][1])
+                                         : __builtin_amdgcn_readfirstlane(ws.pos[b][0]);
+                            // C[row][b] = G[row][col_b] - delta(row, row_b), from lane col_b % LPR
+                            const int k = colv[b] / LPR, ls = colv[b] % LPR;
+                            T xr = gr[fl][k], xi = gi[fl][k];
+                            xr = __shfl(xr, gbase + ls);
+                            xi = __shfl(xi, gbase + ls);
+                            if (row == rowv[b]) xr -= (T)1;
+                            // w_d += C[row][b] M[b][d]
+                            const C cbv = cmk<C>(xr, xi);
+#pragma unroll
+                            for (int d = 0; d < WD_M; ++d)
+                                if (d < m) wv[d] = cfma(cbv, ws.mm[fl][b][d], wv[d]);
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < WD_M; ++d) {
+                        if (d < m) {
+                            const C wd = wv[d];
+                            // G[row][:] -= w (G[row_d][:] + e_col_d)
+                            const T *sr = dump + ((size_t)(dpos[d] * LPR + c) * 2) * RP, *si = sr + RP;
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
+                                const T vr = sr[k], vi = si[k];
+                                T er = gr[fl][k], ei = gi[fl][k];
+                                er = fma(-wd.x, vr, er);
+                                er = fma(wd.y, vi, er);
+                                ei = fma(-wd.x, vi, ei);
+                                ei = fma(-wd.y, vr, ei);
+                                gr[fl][k] = er;
+                                gi[fl][k] = ei;
+                            }
+                            if (c == colv[d] % LPR) {
+                                const int k = colv[d] / LPR;
+                                gr[fl][k] -= wd.x;
+                                gi[fl][k] -= wd.y;
+                            }
+                        }
+                    }
+                }
+            }
+            s0 += nproc;
+            par ^= 1;
+            const int64_t last = s0 - 1;
+            if (a.meas > 0 && last == next_meas) {
+                measure(last);
+                next_meas = last + a.meas;
+            }
         }
     }
 
@@ -1061,6 +1478,17 @@ static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st
     return hipGetLastError();
 }
 
+// delayed-update windows; the measurement-only pass is the one-move kernel's
+template <typename T, int N, int LPR>
+static hipError_t launch_sweep_wdu_t(const SweepArgs &a, int nblocks, hipStream_t st)
+{
+    constexpr int NT = ((N * LPR + 63) / 64) * 64;
+    constexpr size_t lds = wdu_lds_bytes<T, N, LPR>();
+    if (a.measure_only) return launch_sweep_t<T, N, LPR, true>(a, nblocks, st);
+    hipLaunchKernelGGL((sweep_kernel<T, N, LPR, true, false, true>), dim3(nblocks), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
 template <int N, int LPR>
 static hipError_t launch_gj_t(const GJArgs &a, int nwalkers, hipStream_t st)
 {
@@ -1088,6 +1516,10 @@ static hipError_t launch_gb_t(const GBArgs &a, int nwalkers, hipStream_t st)
 
 struct SweepEntry {
     int prec, N, lpr, res;
+    hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
+};
+struct SweepWduEntry {
+    int prec, N, lpr;
     hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
 };
 struct GJEntry {
@@ -1119,6 +1551,9 @@ static const SweepEntry kSweep[] = {
     {1, 72, 4, 1, launch_sweep_t<float, 72, 4, true>},
     {1, 128, 4, 1, launch_sweep_t<float, 128, 4, true>},
     {1, 200, 2, 0, launch_sweep_t<float, 200, 2, false>},
+};
+static const SweepWduEntry kSweepWdu[] = {
+    {1, 128, 4, launch_sweep_wdu_t<float, 128, 4>},
 };
 static const GJEntry kGJ[] = {
     {8, 8, 0, launch_gj_t<8, 8>},
@@ -1171,6 +1606,21 @@ extern "C" hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepA
                                       hipStream_t st)
 {
     for (const auto &e : gv::kSweep)
+        if (e.prec == prec && e.N == N && e.lpr == lpr) return e.fn(*a, nblocks, st);
+    return hipErrorInvalidValue;
+}
+
+extern "C" int gv_sweep_wdu_supported(int prec, int N, int lpr)
+{
+    for (const auto &e : gv::kSweepWdu)
+        if (e.prec == prec && e.N == N && e.lpr == lpr) return 1;
+    return 0;
+}
+
+extern "C" hipError_t gv_launch_sweep_wdu(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
+                                          hipStream_t st)
+{
+    for (const auto &e : gv::kSweepWdu)
         if (e.prec == prec && e.N == N && e.lpr == lpr) return e.fn(*a, nblocks, st);
     return hipErrorInvalidValue;
 }

@@ -84,6 +84,23 @@ int gpvmc_set_groups(gpvmc_ctx *ctx, const int32_t *group_of_walker);
  * the Philox counters. */
 int gpvmc_init_walkers(gpvmc_ctx *ctx, const int32_t *spins, int32_t *ok);
 
+/* Sweep kernel variant of a ground-state context.  GPVMC_SWEEP_ONE_MOVE updates the ratio
+ * tables after each accepted move.  GPVMC_SWEEP_WINDOWS runs delayed-update windows where the
+ * kernel has them (mixed precision, N = 128, i.e. L = 16): up to 16 steps and 4 accepted
+ * moves are decided on the window's sub-blocks of the tables, and the tables take one rank-m
+ * update per window.  GPVMC_SWEEP_DEFAULT (0) is the library's choice for the shape (the
+ * faster of the two as measured).  All take MetroMC::Step's decisions (MetroMC.cpp:90-103) on
+ * the same uniforms; their tables differ by rounding.  Setting GPVMC_SWEEP_WINDOWS on a shape
+ * without windows fails with GPVMC_ERR_UNSUPPORTED.  gpvmc_sweep_variant returns the variant
+ * that runs (ONE_MOVE or WINDOWS), -1 for a null context.  No counterpart in the reference,
+ * whose chain updates the inverse after every accepted move (SlaterDeterminant::Update,
+ * SlaterDeterminant.cpp:270-326). */
+#define GPVMC_SWEEP_DEFAULT 0
+#define GPVMC_SWEEP_ONE_MOVE 1
+#define GPVMC_SWEEP_WINDOWS 2
+int gpvmc_set_sweep_variant(gpvmc_ctx *ctx, int32_t variant);
+int gpvmc_sweep_variant(const gpvmc_ctx *ctx);
+
 /* Recompute every walker's inverse and log-amplitude from its configuration in FP64
  * (drift refresh).  drift[w] (may be NULL) receives max|Ai_old - Ai_new| / max|Ai_new|. */
 int gpvmc_refresh(gpvmc_ctx *ctx, double *drift);

@@ -471,3 +471,104 @@ def test_mixed_statspin_tracks_oracle(L, W):
         # FP32 tables: ratios within FP32 rounding of the largest entry
         scale = max(1.0, np.abs(oacc[1:3]).max())
         np.testing.assert_allclose(acc[w, 1:3], oacc[1:3], rtol=0, atol=2e-5 * scale)
+
+
+@pytest.mark.parametrize("meas,replay", [(256, False), (7, False), (1, False), (256, True)])
+def test_mixed_delayed_windows_match_one_move_and_oracle(meas, replay):
+    """L = 16 mixed: the delayed-update windows (up to 16 steps, 4 accepted moves per table
+    update) against the one-move kernel on the same uniforms, and both against the FP64
+    oracle.  Decisions agree except on near-ties; the ratios and the final tables carry FP32
+    rounding only; the counters add up."""
+    L, W, nsteps = 16, 8, 3 * 256
+    say("case wdu", meas, replay)
+    seed = 5150 + meas
+    occ = occ_for(L, 0.1667, 0.1)
+    spins = model.random_spins(L, W, seed)
+    u = model.philox_uniforms(seed + 1, range(W), 0, nsteps) if replay else None
+    runs = {}
+    for variant in (binding.SWEEP_WINDOWS, binding.SWEEP_ONE_MOVE):
+        with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
+            b.set_orbitals(0, occ)
+            ok = b.init(spins)
+            b.sweep_variant = variant
+            assert b.sweep_variant == variant
+            out = b.walk(nsteps, meas=meas, uniforms=u, trace=True)
+            acc = b.accumulators()
+            s_f, part, _, _ = b.state()
+            tabs = [b.ratio_table(w) for w in range(W)]
+            runs[variant] = (out, acc, s_f, part, tabs)
+    uu = u if replay else model.philox_uniforms(seed, range(W), 0, nsteps)
+    (od, ad, sd, pd, td), (o1, a1, s1, p1, t1) = runs[binding.SWEEP_WINDOWS], runs[binding.SWEEP_ONE_MOVE]
+    # counters: every step is a move, a rejection or a no-op; measurements at the cadence
+    nm = nsteps if meas == 1 else (nsteps + meas - 1) // meas
+    for acc in (ad, a1):
+        assert (acc[:, 7] == nsteps).all()
+        assert (acc[:, 8] + acc[:, 9] + acc[:, 10] == nsteps).all()
+        assert (acc[:, 3] == nm).all()
+    agree = 0
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo, o = oracle_walk(L, occ, spins[w], uu[w])
+        for out in (od, o1):
+            diff = np.nonzero(o["accept"] != out["accept"][w])[0]
+            upto = diff[0] if len(diff) else nsteps
+            if len(diff):
+                s = diff[0]
+                assert abs(min(o["ratio"][s], 1.0) - uu[w, s, 3]) < 1e-3
+            live = ~o["noop"][:upto].astype(bool)
+            sg, so = np.sqrt(out["ratio"][w][:upto][live]), np.sqrt(o["ratio"][:upto][live])
+            rel = np.abs(sg - so) / np.maximum(so, 1e-300)
+            assert np.median(rel) <= 1e-5
+            assert np.quantile(rel, 0.99) <= 1e-3
+            assert (out["noop"][w][:upto] == o["noop"][:upto]).all()
+        if np.array_equal(sd[w], wo.spins()):
+            agree += 1
+            # the window kernel's tables against the FP64 tables of the final configuration
+            # (a fresh oracle walker labels particles in site order, hence the relabelling)
+            # FP32 tables three sweeps after the last FP64 refresh: entries next to a node
+            # (large |G|) carry the largest absolute drift
+            gt = oracle_table(O.Walker(L, occ, sd[w]))
+            np.testing.assert_allclose(relabel(td[w], pd[w]), gt, atol=2e-3 * max(1.0, np.abs(gt).max()))
+            if np.array_equal(s1[w], sd[w]):
+                # one-move kernel, same bar
+                np.testing.assert_allclose(relabel(t1[w], p1[w]), gt, atol=2e-3 * max(1.0, np.abs(gt).max()))
+        if meas <= 256:
+            np.testing.assert_allclose(od["eloc"][w][:1], o1["eloc"][w][:1], rtol=1e-4, atol=1e-5)
+    assert agree >= W // 2
+
+
+def test_mixed_delayed_windows_drift_like_one_move():
+    """The FP32 drift of the tables (max |Ai - Ai_FP64| / max |Ai_FP64| at the refresh) after
+    four sweeps is of the same size with delayed-update windows as with one move per update."""
+    L, W = 16, 64
+    say("case wdu drift")
+    seed = 8080
+    occ = occ_for(L, 0.1667, 0.1)
+    spins = model.random_spins(L, W, seed)
+    drift = {}
+    for variant in (binding.SWEEP_WINDOWS, binding.SWEEP_ONE_MOVE):
+        with binding.Batch(L, W, binding.MIXED, seed=seed) as b:
+            b.set_orbitals(0, occ)
+            ok = b.init(spins)
+            b.sweep_variant = variant
+            b.walk(4 * L * L)
+            drift[variant] = b.refresh(drift=True)[ok]
+    dd, d1 = drift[binding.SWEEP_WINDOWS], drift[binding.SWEEP_ONE_MOVE]
+    print("drift median %.3g / %.3g, max %.3g / %.3g" % (np.median(dd), np.median(d1), dd.max(), d1.max()))
+    assert np.median(dd) <= 2.0 * np.median(d1) + 1e-7
+    assert np.quantile(dd, 0.9) <= 3.0 * np.quantile(d1, 0.9) + 1e-7
+
+
+def test_sweep_variant_errors():
+    say("case variant errors")
+    with binding.Batch(8, 2, binding.MIXED, seed=1) as b:
+        assert b.sweep_variant == binding.SWEEP_ONE_MOVE
+        with pytest.raises(binding.GpvmcError):
+            b.sweep_variant = binding.SWEEP_WINDOWS
+        with pytest.raises(binding.GpvmcError):
+            b.sweep_variant = 7
+        b.sweep_variant = binding.SWEEP_ONE_MOVE
+    with binding.Batch(16, 2, binding.MIXED, seed=1) as b:
+        b.sweep_variant = binding.SWEEP_WINDOWS
+        assert b.sweep_variant == binding.SWEEP_WINDOWS
