@@ -148,17 +148,21 @@ constexpr int WD_M = 4;      // accepted moves per window at most
 constexpr int WD_ROWS = 2 * WD_K;
 template <typename T> struct WduScratch {
     using C = typename cpx<T>::t;
-    C xu[WD_K * WD_K];                // up table rows U x columns D of the window
-    C xd[WD_K * WD_K];                // down table rows D x columns U
-    C buf[2][WD_M][WD_ROWS];          // per move and table: row (+e) then column (-e)
-    int lab[2][WD_ROWS];              // labels of the window, by position, per flavour
-    double2 coef[2][2][WD_M][WD_M];   // [table][gamma, beta][n][b]
-    double2 qv[2][WD_M];              // ratio of each move, per table
+    // vectors of move n by sub-block position: [0] x_up conj(q)/|q|^2 over U, [1] y_up over
+    // D, [2] x_dn conj(q)/|q|^2 over D, [3] y_dn over U (the rank-1 factors of the move on
+    // each table, the column one times 1/q as the table update forms it)
+    C vec[WD_M][4][WD_ROWS];
+    C xr[WD_M][2][WD_ROWS];           // the column factors x_up, x_dn themselves
+    double2 iq[2][WD_M];              // 1 / q in FP64, for the coefficients
+    double2 qd[2][WD_M];              // q in FP64, for the amplitude
+    double2 coef[2][2][WD_M][WD_M];   // [table][gamma, beta][n][m]
     double2 st[2][2][WD_M][WD_M];     // [table][S, T][n][d]
     C mm[2][WD_M][WD_M];              // M = S^T diag(1/q) T per table
+    int lab[2][WD_ROWS];              // labels of the window by position, per flavour
     int mv[WD_M][4];                  // ci, cj, sa, sb of each move
     int pos[WD_M][2];                 // position of ci in U, of cj in D
     int res[4];                       // moves, steps processed
+    uint32_t msk[16][2][4];           // per wavefront: label masks of the window's sites
 };
 template <typename T, int N, int LPR> constexpr size_t wdu_dump_elems()
 {
@@ -462,16 +466,16 @@ sweep_kernel(SweepArgs a)
         double bu3 = 1.0;     // batch: acceptance uniform of that step
         int par = 0;
         int64_t s0 = 0;
+        // the next measured step, meas > 0: measurements follow step s when meas == 1 or
+        // s % meas == 0, and no window runs past one
+        int64_t next_meas = 0;
         while (s0 < a.n_steps) {
             const int b0 = (int)(s0 & 63);
             if (b0 == 0) draw_batch(s0, bpv, bu3);
             // window: up to KW steps, within the batch, ending at the next measured step
             int64_t lim = a.n_steps - s0;
             if (lim > 64 - b0) lim = 64 - b0;
-            if (a.meas > 0) {
-                const int64_t nm = (a.meas == 1) ? s0 : ((s0 + a.meas - 1) / a.meas) * a.meas;
-                if (nm - s0 + 1 < lim) lim = nm - s0 + 1;
-            }
+            if (a.meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
             const int len = (int)(lim < KW ? lim : KW);
             // lane b0 + i holds step s0 + i of the window, as it holds it in the batch
             const bool inw = lane >= b0 && lane < b0 + len;
@@ -653,7 +657,10 @@ sweep_kernel(SweepArgs a)
             s0 += nproc;
             par ^= 1;
             const int64_t last = s0 - 1;
-            if (a.meas > 0 && (a.meas == 1 || last % a.meas == 0)) measure(last);
+            if (a.meas > 0 && last == next_meas) {
+                measure(last);
+                next_meas = last + a.meas;
+            }
         }
     } else {
         // Delayed-update windows.  A window is up to WD_K steps (inside the batch, ending
@@ -663,21 +670,22 @@ sweep_kernel(SweepArgs a)
         // ratio the window needs is an entry of the sub-blocks G_up[U][D] and G_dn[D][U].
         //  1. the owners of the rows U of G_up and D of G_dn copy them to an LDS dump
         //     (double-buffered by window parity); barrier;
-        //  2. wavefront 0 copies the sub-blocks and runs MetroMC::Step on them for each step
-        //     in order, updating them after every accepted move with the table's own
-        //     rank-1 formula; it records for move n the sub-block row y_n (+e) and column
-        //     x_n (-e) and q_n, and from them the m x m matrix M with
+        //  2. wavefront 0 runs MetroMC::Step for each step in order.  An entry after n moves
+        //     is the dumped entry with the n rank-1 terms of the moves applied in order, the
+        //     table's own formula; move n stores its factors x_n (column - e) and y_n
+        //     (row + e) by sub-block position.  From them follows the m x m matrix M with
         //        G_after = G - C M R,   C[:,b] = G[:,col_b] - e_row_b,  R[d,:] = G[row_d,:] + e_col_d
         //     (x_n = sum_b S_nb C[:,b], y_n = sum_d T_nd R[d,:], S and T unit lower
-        //     triangular from gamma_nb = y_b[col_n] / q_b and beta_nb = x_b[row_n] / q_b,
+        //     triangular from gamma_nm = y_m[col_n] / q_m and beta_nm = x_m[row_n] / q_m,
         //     M = S^T diag(1/q) T); it writes the moves, counts and traces; barrier;
         //  3. every thread applies the rank-m update to its rows, R from the dump.
         // The decisions are MetroMC::Step's on the same uniforms in the same order; the
         // tables after the window equal the m rank-1 updates up to rounding.
         constexpr int NMW = (N + 63) / 64;
         constexpr size_t DE = wdu_dump_elems<T, N, LPR>();
-        const int wu = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform: scalar branches
+        static_assert(NT / 64 <= 16, "mask slots");
         WduScratch<T> &ws = *reinterpret_cast<WduScratch<T> *>(smem + 2 * DE * sizeof(T));
+        const int wu = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform: scalar branches
         int bpv = 0;
         double bu3 = 1.0;
         int par = 0;
@@ -694,22 +702,23 @@ sweep_kernel(SweepArgs a)
             const int v1 = bpv & 0xFFFF, v2 = bpv >> 16;
             const double u3 = bu3;
             const int i1 = s_site[v1], i2 = s_site[v2];
-            // label masks of the window's sites (identical in every wavefront)
+            // label masks of the window's sites: LDS atomic ORs into this wavefront's slot
+            // (every wavefront derives the same masks)
+            uint32_t *mk = &ws.msk[wu][0][0];
+            if (lane < 8) mk[lane] = 0u;
+            wave_lds_order();
+            if (inw) {
+                atomicOr(&ws.msk[wu][i1 >> 16][(i1 & 0xFFFF) >> 5], 1u << (i1 & 31));
+                atomicOr(&ws.msk[wu][i2 >> 16][(i2 & 0xFFFF) >> 5], 1u << (i2 & 31));
+            }
+            wave_lds_order();
             uint64_t um[NMW], dm[NMW];
 #pragma unroll
-            for (int q = 0; q < NMW; ++q) um[q] = dm[q] = 0ull;
-            for (int i = b0; i < b0 + len; ++i) {
-                const int x1 = __builtin_amdgcn_readlane(i1, i), x2 = __builtin_amdgcn_readlane(i2, i);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int x = h ? x2 : x1, l = x & 0xFFFF;
-#pragma unroll
-                    for (int q = 0; q < NMW; ++q) {
-                        const uint64_t bit = ((l >> 6) == q) ? (1ull << (l & 63)) : 0ull;
-                        if (x >> 16) dm[q] |= bit;
-                        else um[q] |= bit;
-                    }
-                }
+            for (int q = 0; q < NMW; ++q) {
+                um[q] = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)mk[2 * q]) |
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)mk[2 * q + 1]) << 32);
+                dm[q] = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)mk[4 + 2 * q]) |
+                        ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)mk[4 + 2 * q + 1]) << 32);
             }
             int nU = 0, nD = 0;
 #pragma unroll
@@ -718,6 +727,11 @@ sweep_kernel(SweepArgs a)
                 nD += __popcll(dm[q]);
             }
             T *dump = reinterpret_cast<T *>(smem) + (size_t)par * DE;
+            // element (dump row p, column label l)
+            auto dat = [&](int p, int l) -> C {
+                const T *e = dump + ((size_t)(p * LPR + l % LPR) * 2) * RP + l / LPR;
+                return cmk<C>(e[0], e[RP]);
+            };
             if (has_row) {
                 const int rw = row >> 6;
                 uint64_t ur = um[0], dr = dm[0];
@@ -748,15 +762,25 @@ sweep_kernel(SweepArgs a)
             }
             __syncthreads();
             if (wu == 0) {
-                for (int e = lane; e < nU * nD; e += 64) {
-                    const int ua = e / nD, db = e - ua * nD;
-                    const int lu = ws.lab[0][ua], ld = ws.lab[1][db];
-                    const T *ru = dump + ((size_t)(ua * LPR + ld % LPR) * 2) * RP + ld / LPR;
-                    ws.xu[e] = cmk<C>(ru[0], ru[RP]);
-                    const T *rd = dump + ((size_t)((nU + db) * LPR + lu % LPR) * 2) * RP + lu / LPR;
-                    ws.xd[db * nU + ua] = cmk<C>(rd[0], rd[RP]);
-                }
-                wave_lds_order();
+                // entry (row position r, column position cp) of table tb after the first n
+                // moves, from the dumped entry v: the moves' rank-1 terms in order
+                auto moved_entry = [&](C v, int n, int tb, int r, int cp) -> C {
+#pragma unroll
+                    for (int mm = 0; mm < WD_M; ++mm) {
+                        if (mm < n) {
+                            const C cf = ws.vec[mm][2 * tb][r], y = ws.vec[mm][2 * tb + 1][cp];
+                            T er = v.x, ei = v.y;
+                            er = fma(-cf.x, y.x, er);
+                            er = fma(cf.y, y.y, er);
+                            ei = fma(-cf.x, y.y, ei);
+                            ei = fma(-cf.y, y.x, ei);
+                            v = cmk<C>(er, ei);
+                        }
+                    }
+                    return v;
+                };
+                const int lu = lane < nU ? ws.lab[0][lane] : 0;  // label of U position `lane`
+                const int ld = lane < nD ? ws.lab[1][lane] : 0;  // label of D position `lane`
                 // the particles at the step's two sites as the window's moves leave them,
                 // (flavour << 16) | label
                 int x1 = i1, x2 = i2;
@@ -772,8 +796,8 @@ sweep_kernel(SweepArgs a)
                     if (act && !noop) {
                         pa = mask_rank<NMW>(um, ci);
                         pb = mask_rank<NMW>(dm, cj);
-                        qu = ws.xu[pa * nD + pb];
-                        qd = ws.xd[pb * nU + pa];
+                        qu = moved_entry(dat(pa, cj), m, 0, pa, pb);
+                        qd = moved_entry(dat(nU + pb, ci), m, 1, pb, pa);
                     }
                     double ratio = 1.0;
                     bool accl;
@@ -804,6 +828,57 @@ sweep_kernel(SweepArgs a)
                     const int sa_j = u1_j ? v1_j : v2_j, sb_j = u1_j ? v2_j : v1_j;
                     const C qU = cmk<C>(readlane_real(qu.x, j), readlane_real(qu.y, j));
                     const C qD = cmk<C>(readlane_real(qd.x, j), readlane_real(qd.y, j));
+                    // the move's factors: up table column cj (over U) and row ci (over D),
+                    // down table column ci (over D) and row cj (over U), after the earlier moves
+                    C xu_ = cmk<C>(0, 0), yd_ = cmk<C>(0, 0), yu_ = cmk<C>(0, 0), xd_ = cmk<C>(0, 0);
+                    if (lane < nU) {
+                        xu_ = moved_entry(dat(lane, cj_j), m, 0, lane, pb_j);
+                        yd_ = moved_entry(dat(nU + pb_j, lu), m, 1, pb_j, lane);
+                    }
+                    if (lane < nD) {
+                        yu_ = moved_entry(dat(pa_j, ld), m, 0, pa_j, lane);
+                        xd_ = moved_entry(dat(nU + lane, ci_j), m, 1, lane, pa_j);
+                    }
+                    // gamma and beta of this move against the earlier ones (FP64)
+                    if (lane < 4 * m) {
+                        const int mb = lane >> 2, wh = lane & 3, tb = wh >> 1, gb = wh & 1;
+                        // gamma: y_m at this move's column; beta: x_m at this move's row
+                        const int off = tb == 0 ? (gb ? pa_j : pb_j) : (gb ? pb_j : pa_j);
+                        const C v = gb ? ws.xr[mb][tb][off] : ws.vec[mb][2 * tb + 1][off];
+                        const double2 iq = ws.iq[tb][mb];
+                        ws.coef[tb][gb][m][mb] = make_double2((double)v.x * iq.x - (double)v.y * iq.y,
+                                                              (double)v.x * iq.y + (double)v.y * iq.x);
+                    }
+                    {
+                        // 1/q as the table update forms it (FP64 norm, FP32 factor)
+                        const double nu = (double)qU.x * qU.x + (double)qU.y * qU.y;
+                        const double nd = (double)qD.x * qD.x + (double)qD.y * qD.y;
+                        const C qiu = cmk<C>(qU.x / nu, -qU.y / nu), qid = cmk<C>(qD.x / nd, -qD.y / nd);
+                        if (lane < nU) {
+                            if (lane == pa_j) {
+                                xu_.x -= (T)1;
+                                yd_.x += (T)1;
+                            }
+                            ws.xr[m][0][lane] = xu_;
+                            ws.vec[m][0][lane] = cmul(xu_, qiu);
+                            ws.vec[m][3][lane] = yd_;
+                        }
+                        if (lane < nD) {
+                            if (lane == pb_j) {
+                                yu_.x += (T)1;
+                                xd_.x -= (T)1;
+                            }
+                            ws.vec[m][1][lane] = yu_;
+                            ws.xr[m][1][lane] = xd_;
+                            ws.vec[m][2][lane] = cmul(xd_, qid);
+                        }
+                        if (lane < 2) {
+                            const C q = lane ? qD : qU;
+                            const double nq = lane ? nd : nu;
+                            ws.iq[lane][m] = make_double2((double)q.x / nq, -(double)q.y / nq);
+                            ws.qd[lane][m] = make_double2((double)q.x, (double)q.y);
+                        }
+                    }
                     if (lane == 0) {
                         ws.mv[m][0] = ci_j;
                         ws.mv[m][1] = cj_j;
@@ -811,61 +886,6 @@ sweep_kernel(SweepArgs a)
                         ws.mv[m][3] = sb_j;
                         ws.pos[m][0] = pa_j;
                         ws.pos[m][1] = pb_j;
-                        ws.qv[0][m] = make_double2((double)qU.x, (double)qU.y);
-                        ws.qv[1][m] = make_double2((double)qD.x, (double)qD.y);
-                    }
-                    // y (row + e) and x (column - e) of both sub-blocks before the move:
-                    // up table row ci (over D), column cj (over U); down table row cj
-                    // (over U), column ci (over D)
-                    C *bu = ws.buf[0][m], *bd = ws.buf[1][m];
-                    if (lane < nD) {
-                        C y = ws.xu[pa_j * nD + lane];
-                        if (lane == pb_j) y.x += (T)1;
-                        bu[lane] = y;
-                        C x = ws.xd[lane * nU + pa_j];
-                        if (lane == pb_j) x.x -= (T)1;
-                        bd[nU + lane] = x;
-                    }
-                    if (lane < nU) {
-                        C x = ws.xu[lane * nD + pb_j];
-                        if (lane == pa_j) x.x -= (T)1;
-                        bu[nD + lane] = x;
-                        C y = ws.xd[pb_j * nU + lane];
-                        if (lane == pa_j) y.x += (T)1;
-                        bd[lane] = y;
-                    }
-                    wave_lds_order();
-                    // gamma_mb = y_b[col_m] / q_b and beta_mb = x_b[row_m] / q_b, b < m
-                    if (lane < 4 * m) {
-                        const int b = lane >> 2, wh = lane & 3, tb = wh >> 1, gb = wh & 1;
-                        const int off = tb == 0 ? (gb ? nD + pa_j : pb_j) : (gb ? nU + pb_j : pa_j);
-                        const C v = ws.buf[tb][b][off];
-                        const double2 q = ws.qv[tb][b];
-                        const double nq = q.x * q.x + q.y * q.y;
-                        ws.coef[tb][gb][m][b] = make_double2(((double)v.x * q.x + (double)v.y * q.y) / nq,
-                                                             ((double)v.y * q.x - (double)v.x * q.y) / nq);
-                    }
-                    // the sub-blocks after the move, with the table's rank-1 formula
-#pragma unroll
-                    for (int tb = 0; tb < 2; ++tb) {
-                        const C q = tb ? qD : qU;
-                        const double nq = (double)q.x * q.x + (double)q.y * q.y;
-                        const C qi = cmk<C>(q.x / nq, -q.y / nq);
-                        const int nr = tb ? nD : nU, nc = tb ? nU : nD;
-                        const C *bb = tb ? bd : bu;
-                        C *xx = tb ? ws.xd : ws.xu;
-                        for (int e = lane; e < nr * nc; e += 64) {
-                            const int r = e / nc, cc = e - r * nc;
-                            const C coef = cmul(bb[nc + r], qi);
-                            const C y = bb[cc];
-                            const C g = xx[e];
-                            T er = g.x, ei = g.y;
-                            er = fma(-coef.x, y.x, er);
-                            er = fma(coef.y, y.y, er);
-                            ei = fma(-coef.x, y.y, ei);
-                            ei = fma(-coef.y, y.x, ei);
-                            xx[e] = cmk<C>(er, ei);
-                        }
                     }
                     wave_lds_order();
                     // State::Hop on the window's sites
@@ -879,7 +899,6 @@ sweep_kernel(SweepArgs a)
                         break;
                     }
                 }
-                wave_lds_order();
                 if (m > 0) {
                     // columns of S (tb, 0) and T (tb, 1): v_n = delta_nd - sum_b coef_nb v_b
                     if (lane < 4 * WD_M) {
@@ -906,11 +925,10 @@ sweep_kernel(SweepArgs a)
                         if (b < m && d < m) {
                             double2 acc = make_double2(0.0, 0.0);
                             for (int n = 0; n < m; ++n) {
-                                const double2 sv = ws.st[tb][0][n][b], tv = ws.st[tb][1][n][d], q = ws.qv[tb][n];
+                                const double2 sv = ws.st[tb][0][n][b], tv = ws.st[tb][1][n][d], iq = ws.iq[tb][n];
                                 const double pr = sv.x * tv.x - sv.y * tv.y, pi = sv.x * tv.y + sv.y * tv.x;
-                                const double nq = q.x * q.x + q.y * q.y;
-                                acc.x += (pr * q.x + pi * q.y) / nq;
-                                acc.y += (pi * q.x - pr * q.y) / nq;
+                                acc.x += pr * iq.x - pi * iq.y;
+                                acc.y += pr * iq.y + pi * iq.x;
                             }
                             ws.mm[tb][b][d] = cmk<C>(acc.x, acc.y);
                         }
@@ -932,21 +950,23 @@ sweep_kernel(SweepArgs a)
                         s_part[0][ci_n] = sb_n;
                         s_part[1][cj_n] = sa_n;
                     }
-                    // amplitude *= q per flavour and move (SlaterDeterminant.cpp:321)
+                }
+                if (lane < 2 && m > 0) {
+                    // amplitude *= the moves' ratios of flavour `lane` (SlaterDeterminant.cpp:321)
+                    const int fl = lane;
+                    double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
                     for (int n = 0; n < m; ++n) {
-#pragma unroll
-                        for (int fl = 0; fl < 2; ++fl) {
-                            const double2 q = ws.qv[fl][n];
-                            const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
-                            const double re = mr * q.x - mi * q.y;
-                            const double im = mr * q.y + mi * q.x;
-                            int e;
-                            (void)frexp(fmax(fabs(re), fabs(im)), &e);
-                            s_amp[2 * fl] = ldexp(re, -e);
-                            s_amp[2 * fl + 1] = ldexp(im, -e);
-                            s_amp[4 + fl] += e;
-                        }
+                        const double2 q = ws.qd[fl][n];
+                        const double re = mr * q.x - mi * q.y;
+                        const double im = mr * q.y + mi * q.x;
+                        mr = re;
+                        mi = im;
                     }
+                    int e;
+                    (void)frexp(fmax(fabs(mr), fabs(mi)), &e);
+                    s_amp[2 * fl] = ldexp(mr, -e);
+                    s_amp[2 * fl + 1] = ldexp(mi, -e);
+                    s_amp[4 + fl] += e;
                 }
             }
             __syncthreads();
