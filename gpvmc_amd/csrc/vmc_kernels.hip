@@ -121,6 +121,24 @@ __device__ __forceinline__ double readlane_real(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// amplitude mantissa (mr, mi) *= q.  The mantissa is rescaled by a power of two (exact) only
+// when its binary exponent leaves [-256, 256]; the exponent removed is added to *ex, so
+// amplitude = (mr + i mi) 2^ex throughout (the reference's BigComplex, BigComplex.h)
+__device__ __forceinline__ void amp_mul(double &mr, double &mi, double &ex, double qr, double qi)
+{
+    const double re = mr * qr - mi * qi, im = mr * qi + mi * qr;
+    const int be = (int)((__double_as_longlong(fmax(fabs(re), fabs(im))) >> 52) & 0x7FF) - 1023;
+    if (be > 256 || be < -256) {
+        const double sc = __longlong_as_double((long long)(1023 - be) << 52);  // 2^-be
+        mr = re * sc;
+        mi = im * sc;
+        ex += be;
+    } else {
+        mr = re;
+        mi = im;
+    }
+}
+
 // number of set bits of the label mask m below bit x (x < 64 * NMW)
 template <int NMW> __device__ __forceinline__ int mask_rank(const uint64_t (&m)[NMW], int x)
 {
@@ -641,14 +659,11 @@ sweep_kernel(SweepArgs a)
 #pragma unroll
                     for (int fl = 0; fl < 2; ++fl) {
                         const C q = fl ? q1 : q0;
-                        const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
-                        const double re = mr * (double)q.x - mi * (double)q.y;
-                        const double im = mr * (double)q.y + mi * (double)q.x;
-                        int e;
-                        (void)frexp(fmax(fabs(re), fabs(im)), &e);
-                        s_amp[2 * fl] = ldexp(re, -e);
-                        s_amp[2 * fl + 1] = ldexp(im, -e);
-                        s_amp[4 + fl] += e;
+                        double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1], ex = s_amp[4 + fl];
+                        amp_mul(mr, mi, ex, (double)q.x, (double)q.y);
+                        s_amp[2 * fl] = mr;
+                        s_amp[2 * fl + 1] = mi;
+                        s_amp[4 + fl] = ex;
                     }
                 }
                 // streamed tables: every row is rewritten before the next window reads them
@@ -954,19 +969,14 @@ sweep_kernel(SweepArgs a)
                 if (lane < 2 && m > 0) {
                     // amplitude *= the moves' ratios of flavour `lane` (SlaterDeterminant.cpp:321)
                     const int fl = lane;
-                    double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
+                    double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1], ex = s_amp[4 + fl];
                     for (int n = 0; n < m; ++n) {
                         const double2 q = ws.qd[fl][n];
-                        const double re = mr * q.x - mi * q.y;
-                        const double im = mr * q.y + mi * q.x;
-                        mr = re;
-                        mi = im;
+                        amp_mul(mr, mi, ex, q.x, q.y);
                     }
-                    int e;
-                    (void)frexp(fmax(fabs(mr), fabs(mi)), &e);
-                    s_amp[2 * fl] = ldexp(mr, -e);
-                    s_amp[2 * fl + 1] = ldexp(mi, -e);
-                    s_amp[4 + fl] += e;
+                    s_amp[2 * fl] = mr;
+                    s_amp[2 * fl + 1] = mi;
+                    s_amp[4 + fl] = ex;
                 }
             }
             __syncthreads();
