@@ -35,38 +35,41 @@ using gv::ExcArgs;
 
 using namespace gz;
 
-// LDS layout of one walker (dynamic shared memory; offsets computed by exc_lds_layout)
+// LDS layout of one walker (dynamic shared memory; offsets computed by lds_carve)
 struct Lds {
     int *site;      // [Ns]   (flavour << 16) | label
     int *part;      // [2][nmax] site of each label
     int *cur;       // [2][nmax] orbital in each slot of the current state
     int *slot;      // [2][Ns] slot of each orbital or -1
+    int *qrow;      // [2][Ns] Q row of each orbital or -1
+    int *qorb;      // [2][nqm] orbital of each Q row or -1
+    int *rep;       // [nqm] apply: slot whose unit row replaces a Q row, or -1
     double *vc;     // [2][nmax] complex: c = Phi(cur orbitals, target site)
     double *vy;     // [2][nmax] complex: y = Ai c
-    double *kg;     // [2][nk] complex: rho_m . y per new orbital of the current state
+    double *kg;     // [2][nqm] complex: rho_m . y per new orbital of the current state
     double *ra;     // [nk][2][2][2] complex: (rho_m Ai)[f_n] blocks of every state
     double *alpha;  // [nk] complex: a_k(x') / a_c(x)
-    double *dets;   // [nk][2] complex: per-flavour determinant
     double *a0;     // [nk] complex: amplitudes at the current configuration
     double *hk;     // [nk] complex: ProjHeis h_k
-    double *U;      // [2][3][nmax] complex
-    double *V;      // [2][3][nmax] complex
+    double *kap;    // apply: [nmax + nqm][3] complex update coefficients of one flavour's rows
+    double *dlt;    // apply: [nqm] complex delta of each Q row
+    double *V;      // apply: [3][nmax] complex V rows, then W = B^-1 V
     double *red;    // [16]
     int *ired;      // [16]
 };
 
-__host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk)
+__host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
 {
     size_t b = 0;
-    b += (size_t)Ns * 4 + 2 * (size_t)nmax * 4 * 2 + 2 * (size_t)Ns * 4;
+    b += ((size_t)Ns + 2 * (size_t)nmax * 2 + 2 * (size_t)Ns * 2 + 2 * (size_t)nqm + nqm) * 4;
     b = (b + 15) & ~(size_t)15;
-    b += 16 * ((size_t)2 * nmax * 2 + (size_t)2 * nk + (size_t)nk * 8 + nk + (size_t)nk * 2 + nk + nk +
-               (size_t)2 * 3 * nmax * 2);
+    b += 16 * ((size_t)2 * nmax * 2 + (size_t)2 * nqm + (size_t)nk * 8 + nk + nk + nk +
+               ((size_t)nmax + nqm) * 3 + nqm + (size_t)3 * nmax);
     b += 16 * 8 + 16 * 4;
     return b;
 }
 
-__device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk)
+__device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk, int nqm)
 {
     Lds s;
     int *ip = reinterpret_cast<int *>(base);
@@ -74,18 +77,21 @@ __device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk)
     s.part = ip; ip += 2 * nmax;
     s.cur = ip; ip += 2 * nmax;
     s.slot = ip; ip += 2 * Ns;
+    s.qrow = ip; ip += 2 * Ns;
+    s.qorb = ip; ip += 2 * nqm;
+    s.rep = ip; ip += nqm;
     size_t off = (reinterpret_cast<char *>(ip) - base + 15) & ~(size_t)15;
     double *dp = reinterpret_cast<double *>(base + off);
     s.vc = dp; dp += 2 * (2 * nmax);
     s.vy = dp; dp += 2 * (2 * nmax);
-    s.kg = dp; dp += 2 * (2 * nk);
+    s.kg = dp; dp += 2 * (2 * nqm);
     s.ra = dp; dp += 2 * (nk * 8);
     s.alpha = dp; dp += 2 * nk;
-    s.dets = dp; dp += 2 * (2 * nk);
     s.a0 = dp; dp += 2 * nk;
     s.hk = dp; dp += 2 * nk;
-    s.U = dp; dp += 2 * (2 * 3 * nmax);
-    s.V = dp; dp += 2 * (2 * 3 * nmax);
+    s.kap = dp; dp += 2 * ((nmax + nqm) * 3);
+    s.dlt = dp; dp += 2 * nqm;
+    s.V = dp; dp += 2 * (3 * nmax);
     s.red = dp; dp += 16;
     s.ired = reinterpret_cast<int *>(dp);
     return s;
@@ -95,9 +101,10 @@ template <typename T> struct Walker {
     using C = typename cpx<T>::t;
     const ExcArgs &a;
     Lds s;
-    int w, tid, nt, Ns, n[2], nmax, nk;
+    int w, tid, nt, Ns, n[2], nmax, nk, nq[2], nqm;
     size_t cb;  // channel of the walker times nk: base of its graph arrays
     C *Ai[2];
+    C *Qr[2];   // Q rows: Q[q][f] = (rho_{qorb[q]} Ai)[f], kept up to date with Ai
     const C *tab[2];
 
     __device__ Walker(const ExcArgs &args, char *base, int wk)
@@ -108,15 +115,26 @@ template <typename T> struct Walker {
         n[1] = a.n_dn;
         nmax = n[0] > n[1] ? n[0] : n[1];
         nk = a.nk;
+        nq[0] = a.nq0;
+        nq[1] = a.nq1;
+        nqm = nq[0] > nq[1] ? nq[0] : nq[1];
+        if (nqm < 1) nqm = 1;
         cb = (size_t)(a.chan ? a.chan[w] : 0) * nk;
-        s = lds_carve(base, Ns, nmax, nk);
+        s = lds_carve(base, Ns, nmax, nk, nqm);
         for (int fl = 0; fl < 2; ++fl) {
             Ai[fl] = reinterpret_cast<C *>(a.Ai) + ((size_t)w * 2 + fl) * nmax * nmax;
+            Qr[fl] = reinterpret_cast<C *>(a.Q) + ((size_t)w * 2 + fl) * nqm * nmax;
             tab[fl] = reinterpret_cast<const C *>(a.tab) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
         }
     }
     __device__ zc phi(int fl, int orb, int site) const { return toz(tab[fl][(size_t)orb * Ns + site]); }
     __device__ zc ai(int fl, int r, int f) const { return toz(Ai[fl][(size_t)r * nmax + f]); }
+    // Q row q (an orbital without a row reads as zero; the rows are the new orbitals of the
+    // current state, which every lookup asks for)
+    __device__ zc qv(int fl, int q, int f) const
+    {
+        return q >= 0 ? toz(Qr[fl][(size_t)q * nmax + f]) : zmk(0, 0);
+    }
     __device__ zc lz(const double *p, int i) const { return zmk(p[2 * i], p[2 * i + 1]); }
     __device__ void sz(double *p, int i, zc v) const { p[2 * i] = v.x; p[2 * i + 1] = v.y; }
 
@@ -135,16 +153,8 @@ template <typename T> struct Walker {
     __device__ int n_new(int c, int fl) const { return noff(c, fl)[1] - noff(c, fl)[0]; }
     __device__ int new_orb(int c, int fl, int m) const { return a.new_orb[noff(c, fl)[0] + m]; }
 
-    // (rho_o Ai)[f] = sum_r' Phi(o, site r') Ai[r'][f]
-    __device__ zc rho_ai(int fl, int o, int f) const
-    {
-        zc t = zmk(0, 0);
-        for (int r = 0; r < n[fl]; ++r) t = zadd(t, zmul(phi(fl, o, s.part[fl * nmax + r]), ai(fl, r, f)));
-        return t;
-    }
-
     // RA blocks of every state relative to the current state (after any change of Ai or
-    // of the current state)
+    // of the current state): (rho_{o'_m} Ai)[f_n] is entry f_n of the Q row of o'_m
     __device__ void build_ra()
     {
         const int c = cstate();
@@ -152,7 +162,7 @@ template <typename T> struct Walker {
             const int k = e / 8, fl = (e / 4) & 1, m = (e / 2) & 1, nn = e & 1;
             zc v = zmk(0, 0);
             if (m < hop_n(c, k, fl) && nn < hop_n(c, k, fl))
-                v = rho_ai(fl, hop_to(c, k, fl, m), s.slot[fl * Ns + hop_from(c, k, fl, nn)]);
+                v = qv(fl, s.qrow[fl * Ns + hop_to(c, k, fl, m)], s.slot[fl * Ns + hop_from(c, k, fl, nn)]);
             sz(s.ra, e, v);
         }
         __syncthreads();
@@ -185,7 +195,7 @@ template <typename T> struct Walker {
                 zc t = zmk(0, 0);
                 for (int r = 0; r < n[fl]; ++r)
                     t = zadd(t, zmul(phi(fl, o, s.part[fl * nmax + r]), lz(s.vy, fl * nmax + r)));
-                sz(s.kg, fl * nk + m, t);
+                sz(s.kg, fl * nqm + m, t);
             }
             __syncthreads();
         }
@@ -210,7 +220,7 @@ template <typename T> struct Walker {
                         const int o2 = hop_to(c, k, fl, m), o1 = hop_from(c, k, fl, m);
                         const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
                         const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
-                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nk + hop_new(c, k, fl, m)),
+                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, k, fl, m)),
                                                    zmul(Dm, B[0])), p2s);
                         for (int q = 0; q < R; ++q)
                             B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q),
@@ -218,7 +228,6 @@ template <typename T> struct Walker {
                     }
                     d = det_small(D, B);
                 }
-                sz(s.dets, k * 2 + fl, d);
                 tot = zmul(tot, d);
             }
             sz(s.alpha, k, tot);
@@ -257,95 +266,161 @@ template <typename T> struct Walker {
         __syncthreads();
     }
 
-    // Moves the walker to state kn after eval(mv, ...): Woodbury update of both inverses,
-    // slot bookkeeping, lattice hop.
+    // Moves the walker to state kn after eval(mv, ...): Woodbury update of both inverses
+    // and of the Q rows, slot and Q-row bookkeeping, lattice hop.
+    //   Ai' = Ai - U W,  W = B^-1 V,  U = [y - e_r, Ai[:, f_m]],
+    //   V = [Ai[r, :]; Q_{o'_m} + D_m Ai[r, :] - e_{f_m}]   (InvUpdate, linalg.cpp:402-573)
+    //   Q'_q = rho'_o Ai' with rho'_o = rho_o + delta_o e_r (delta_o = Phi(o, s) - Phi(o, site r)):
+    //   Q'_q = Q_q + delta_o V_0 - sum_a kappa_a W_a,  kappa_0 = KG_q - Phi(o, s) + delta_o y[r],
+    //   kappa_m = Q_q[f_m] + delta_o Ai[r][f_m].  The row of a filled orbital o'_m is taken
+    //   over by the emptied orbital o_m, whose row before the move is e_{f_m} (it was
+    //   occupied): kappa_0 = eps y[r], kappa_m = delta_{m,ms} + eps Ai[r][f_m], delta = eps =
+    //   Phi(o_m, s) - Phi(o_m, site r).  Without a lattice move the delta terms vanish.
     __device__ void apply(bool mv, int ci, int sa, int sb, int cj, int kn)
     {
         const int c = cstate();
+        __shared__ zc s_binv[9];
+        __shared__ zc s_det[2];
         for (int fl = 0; fl < 2; ++fl) {
             const int R = hop_n(c, kn, fl);
             const int r = fl ? cj : ci;
             const int sold = fl ? sb : sa, snew = fl ? sa : sb;
             const int m0 = mv ? 1 : 0, D = R + m0;
-            if (D == 0) continue;
-            double *U = s.U + 2 * (fl * 3 * nmax), *Vv = s.V + 2 * (fl * 3 * nmax);
-            // U columns and V rows, from the old inverse
-            for (int i = tid; i < n[fl]; i += nt) {
-                if (mv) {
-                    zc y = lz(s.vy, fl * nmax + i);
-                    if (i == r) y.x -= 1.0;
-                    sz(U, i, y);
-                    sz(Vv, i, ai(fl, r, i));
-                }
+            const int nf = n[fl], nqf = nq[fl];
+            if (D == 0) {
+                if (tid == 0) s_det[fl] = zmk(1.0, 0.0);
+                continue;
+            }
+            int fmv[2] = {0, 0}, o1v[2] = {0, 0}, o2v[2] = {0, 0};
+            zc Dmv[2] = {zmk(0, 0), zmk(0, 0)};
+            for (int m = 0; m < R; ++m) {
+                o1v[m] = hop_from(c, kn, fl, m);
+                o2v[m] = hop_to(c, kn, fl, m);
+                fmv[m] = s.slot[fl * Ns + o1v[m]];
+                if (mv)
+                    Dmv[m] = zadd(zsub(zsub(phi(fl, o2v[m], snew), phi(fl, o2v[m], sold)), phi(fl, o1v[m], snew)),
+                                  phi(fl, o1v[m], sold));
+            }
+            // V rows from the old inverse and Q rows
+            for (int f = tid; f < nf; f += nt) {
+                if (mv) sz(s.V, f, ai(fl, r, f));
                 for (int m = 0; m < R; ++m) {
-                    const int fm = s.slot[fl * Ns + hop_from(c, kn, fl, m)];
-                    sz(U, (m0 + m) * nmax + i, ai(fl, i, fm));
-                    const int o2 = hop_to(c, kn, fl, m), o1 = hop_from(c, kn, fl, m);
-                    zc v = rho_ai(fl, o2, i);
-                    if (mv) {
-                        const zc Dm = zadd(zsub(zsub(phi(fl, o2, snew), phi(fl, o2, sold)), phi(fl, o1, snew)),
-                                           phi(fl, o1, sold));
-                        v = zadd(v, zmul(Dm, ai(fl, r, i)));
-                    }
-                    if (i == fm) v.x -= 1.0;
-                    sz(Vv, (m0 + m) * nmax + i, v);
+                    zc v = qv(fl, s.qrow[fl * Ns + o2v[m]], f);
+                    if (mv) v = zadd(v, zmul(Dmv[m], ai(fl, r, f)));
+                    if (f == fmv[m]) v.x -= 1.0;
+                    sz(s.V, (m0 + m) * nmax + f, v);
                 }
             }
-        }
-        // capacitance matrices of state kn, rebuilt per flavour from the same inputs
-        __shared__ zc s_binv[2][9];
-        if (tid < 2) {
-            const int fl = tid;
-            const int R = hop_n(c, kn, fl);
-            const int D = R + (mv ? 1 : 0);
-            zc B[9];
-            if (D > 0) {
+            // update coefficients of the inverse's rows (e < nf) and of the Q rows
+            const zc yr = mv ? lz(s.vy, fl * nmax + r) : zmk(0, 0);
+            for (int e = tid; e < nf + nqf; e += nt) {
+                zc k3[3] = {zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+                if (e < nf) {
+                    if (mv) {
+                        zc y = lz(s.vy, fl * nmax + e);
+                        if (e == r) y.x -= 1.0;
+                        k3[0] = y;
+                    }
+                    for (int m = 0; m < R; ++m) k3[m0 + m] = ai(fl, e, fmv[m]);
+                } else {
+                    const int q = e - nf, o = s.qorb[fl * nqm + q];
+                    zc dd = zmk(0, 0);
+                    int rp = -1;
+                    if (o >= 0) {
+                        int ms = -1;
+                        for (int m = 0; m < R; ++m)
+                            if (o2v[m] == o) ms = m;
+                        if (ms >= 0) {
+                            rp = fmv[ms];
+                            const zc eps = mv ? zsub(phi(fl, o1v[ms], snew), phi(fl, o1v[ms], sold)) : zmk(0, 0);
+                            dd = eps;
+                            if (mv) k3[0] = zmul(eps, yr);
+                            for (int m = 0; m < R; ++m) {
+                                zc kv = zmk(m == ms ? 1.0 : 0.0, 0.0);
+                                if (mv) kv = zadd(kv, zmul(eps, ai(fl, r, fmv[m])));
+                                k3[m0 + m] = kv;
+                            }
+                        } else {
+                            if (mv) {
+                                // KG_q = Q_q . c = rho_o . y
+                                zc kg = zmk(0, 0);
+                                for (int f = 0; f < nf; ++f) kg = zadd(kg, zmul(qv(fl, q, f), lz(s.vc, fl * nmax + f)));
+                                const zc pn = phi(fl, o, snew), dq = zsub(pn, phi(fl, o, sold));
+                                dd = dq;
+                                k3[0] = zadd(zsub(kg, pn), zmul(dq, yr));
+                            }
+                            for (int m = 0; m < R; ++m) {
+                                zc kv = qv(fl, q, fmv[m]);
+                                if (mv) kv = zadd(kv, zmul(dd, ai(fl, r, fmv[m])));
+                                k3[m0 + m] = kv;
+                            }
+                        }
+                    }
+                    sz(s.dlt, q, dd);
+                    s.rep[q] = rp;
+                }
+                for (int q3 = 0; q3 < 3; ++q3) sz(s.kap, e * 3 + q3, k3[q3]);
+            }
+            // capacitance matrix of state kn, rebuilt from the same inputs as eval
+            if (tid == 0) {
+                zc B[9];
                 if (!mv) {
                     for (int m = 0; m < R; ++m)
                         for (int q = 0; q < R; ++q) B[m * R + q] = lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q);
                 } else {
-                    const int r = fl ? cj : ci;
-                    const int sold = fl ? sb : sa, snew = fl ? sa : sb;
-                    B[0] = lz(s.vy, fl * nmax + r);
-                    for (int q = 0; q < R; ++q) B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, kn, fl, q)]);
+                    B[0] = yr;
+                    for (int q = 0; q < R; ++q) B[1 + q] = ai(fl, r, fmv[q]);
                     for (int m = 0; m < R; ++m) {
-                        const int o2 = hop_to(c, kn, fl, m), o1 = hop_from(c, kn, fl, m);
-                        const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
-                        const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
-                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nk + hop_new(c, kn, fl, m)), zmul(Dm, B[0])), p2s);
+                        const zc p2s = phi(fl, o2v[m], snew);
+                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, kn, fl, m)), zmul(Dmv[m], B[0])), p2s);
                         for (int q = 0; q < R; ++q)
-                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
+                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q), zmul(Dmv[m], B[1 + q]));
                     }
                 }
-                inv_small(D, B, s_binv[fl]);
+                s_det[fl] = det_small(D, B);
+                inv_small(D, B, s_binv);
             }
-        }
-        __syncthreads();
-        // W = B^-1 V (D x n) into V in place is not safe; form per element:
-        // Ai[i][f] -= sum_a U[a][i] sum_b Binv[a][b] V[b][f]
-        for (int fl = 0; fl < 2; ++fl) {
-            const int R = hop_n(c, kn, fl);
-            const int D = R + (mv ? 1 : 0);
-            if (D == 0) continue;
-            const double *U = s.U + 2 * (fl * 3 * nmax), *Vv = s.V + 2 * (fl * 3 * nmax);
-            for (int e = tid; e < n[fl] * n[fl]; e += nt) {
-                const int i = e / n[fl], f = e - i * n[fl];
-                zc acc = zmk(0, 0);
-                for (int aa = 0; aa < D; ++aa) {
+            __syncthreads();
+            // W = B^-1 V in place; V_0 = Ai[r, :] is kept in this flavour's c vector, which
+            // the coefficients above were the last to read
+            for (int f = tid; f < nf; f += nt) {
+                zc vv[3] = {zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+                for (int b2 = 0; b2 < D; ++b2) vv[b2] = lz(s.V, b2 * nmax + f);
+                if (mv) sz(s.vc, fl * nmax + f, vv[0]);
+                for (int a2 = 0; a2 < D; ++a2) {
                     zc wv = zmk(0, 0);
-                    for (int bb = 0; bb < D; ++bb) wv = zadd(wv, zmul(s_binv[fl][aa * D + bb], lz(Vv, bb * nmax + f)));
-                    acc = zadd(acc, zmul(lz(U, aa * nmax + i), wv));
+                    for (int b2 = 0; b2 < D; ++b2) wv = zadd(wv, zmul(s_binv[a2 * D + b2], vv[b2]));
+                    sz(s.V, a2 * nmax + f, wv);
                 }
-                C &el = Ai[fl][(size_t)i * nmax + f];
-                el.x = (T)((double)el.x - acc.x);
-                el.y = (T)((double)el.y - acc.y);
             }
+            __syncthreads();
+            // rows of the inverse, then the live Q rows
+            for (int e = tid; e < (nf + nqf) * nf; e += nt) {
+                const int row = e / nf, f = e - row * nf;
+                zc acc = zmk(0, 0);
+                for (int a2 = 0; a2 < D; ++a2) acc = zadd(acc, zmul(lz(s.kap, row * 3 + a2), lz(s.V, a2 * nmax + f)));
+                if (row < nf) {
+                    C &el = Ai[fl][(size_t)row * nmax + f];
+                    el.x = (T)((double)el.x - acc.x);
+                    el.y = (T)((double)el.y - acc.y);
+                } else {
+                    const int q = row - nf;
+                    if (s.qorb[fl * nqm + q] < 0) continue;
+                    const int rp = s.rep[q];
+                    zc nv = rp >= 0 ? zmk(f == rp ? 1.0 : 0.0, 0.0) : qv(fl, q, f);
+                    if (mv) nv = zadd(nv, zmul(lz(s.dlt, q), lz(s.vc, fl * nmax + f)));
+                    nv = zsub(nv, acc);
+                    C &el = Qr[fl][(size_t)q * nmax + f];
+                    el.x = (T)nv.x;
+                    el.y = (T)nv.y;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        // amplitude of the new current state (per flavour) and the slot bookkeeping
+        // amplitude of the new current state (per flavour) and the slot / Q-row bookkeeping
         if (tid == 0) {
             for (int fl = 0; fl < 2; ++fl) {
-                const zc d = lz(s.dets, kn * 2 + fl);
+                const zc d = s_det[fl];
                 a.logabs[2 * w + fl] += 0.5 * log(znorm(d));
                 a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
             }
@@ -357,6 +432,10 @@ template <typename T> struct Walker {
                     s.slot[fl * Ns + o1] = -1;
                     s.slot[fl * Ns + o2] = fm;
                     s.cur[fl * nmax + fm] = o2;
+                    const int q = s.qrow[fl * Ns + o2];
+                    s.qrow[fl * Ns + o2] = -1;
+                    s.qrow[fl * Ns + o1] = q;
+                    if (q >= 0) s.qorb[fl * nqm + q] = o1;
                 }
             }
             if (mv) {
@@ -383,19 +462,27 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     Walker<T> wk(a, smem, w);
     const int tid = wk.tid, nt = wk.nt, Ns = wk.Ns, nmax = wk.nmax, nk = wk.nk;
     auto &s = wk.s;
+    const int nqm = wk.nqm;
     for (int i = tid; i < Ns; i += nt) {
         s.site[i] = a.site[(size_t)w * Ns + i];
         s.slot[i] = -1;
         s.slot[Ns + i] = -1;
+        s.qrow[i] = -1;
+        s.qrow[Ns + i] = -1;
     }
     for (int i = tid; i < 2 * nmax; i += nt) {
         s.part[i] = a.part[(size_t)w * 2 * nmax + i];
         s.cur[i] = a.cur[(size_t)w * 2 * nmax + i];
     }
+    for (int i = tid; i < 2 * nqm; i += nt) s.qorb[i] = a.qorb[(size_t)w * 2 * nqm + i];
     __syncthreads();
     for (int i = tid; i < 2 * nmax; i += nt) {
         const int fl = i / nmax, f = i % nmax;
         if (f < wk.n[fl]) s.slot[fl * Ns + s.cur[i]] = f;
+    }
+    for (int i = tid; i < 2 * nqm; i += nt) {
+        const int fl = i / nqm, q = i % nqm;
+        if (q < wk.nq[fl] && s.qorb[i] >= 0) s.qrow[fl * Ns + s.qorb[i]] = q;
     }
     __syncthreads();
     wk.build_ra();
@@ -542,6 +629,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         a.part[(size_t)w * 2 * nmax + i] = s.part[i];
         a.cur[(size_t)w * 2 * nmax + i] = s.cur[i];
     }
+    for (int i = tid; i < 2 * nqm; i += nt) a.qorb[(size_t)w * 2 * nqm + i] = s.qorb[i];
     if (tid == 0) {
         a.weight[w] = S;
         if (a.mode == 0) {
@@ -711,7 +799,9 @@ extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks,
 {
     const int Ns = a->L * a->L;
     const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
-    const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk) + 256;
+    const int nqm = a->nq0 > a->nq1 ? (a->nq0 > 0 ? a->nq0 : 1) : (a->nq1 > 0 ? a->nq1 : 1);
+    if (!a->Q || !a->qorb) return hipErrorInvalidValue;
+    const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk, nqm) + 256;
     if (lds > 160 * 1024 - 4096) return hipErrorInvalidValue;
     if (prec == 0)
         hipLaunchKernelGGL(gx::exc_kernel<double>, dim3(nblocks), dim3(256), lds, st, *a);
@@ -720,7 +810,61 @@ extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks,
     return hipGetLastError();
 }
 
-extern "C" size_t gv_exc_lds_bytes(int Ns, int nmax, int nk) { return gx::exc_lds_bytes(Ns, nmax, nk) + 256; }
+extern "C" size_t gv_exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
+{
+    return gx::exc_lds_bytes(Ns, nmax, nk, nqm < 1 ? 1 : nqm) + 256;
+}
+
+// Q rows of every walker's current state from the FP64 inverse (after Gauss-Jordan): one
+// block per (walker, flavour); qorb[q] = the q-th new orbital of the current state (-1 past
+// them), Q[q][f] = sum_r Phi(qorb[q], site r) Ai64[r][f] in the working precision.
+template <typename T>
+__global__ void __launch_bounds__(256) exc_qbuild_kernel(gv::ExcArgs a, const double2 *Ai64)
+{
+    using C = typename gz::cpx<T>::t;
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int Ns = a.L * a.L;
+    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
+    const int nqm = a.nq0 > a.nq1 ? (a.nq0 > 0 ? a.nq0 : 1) : (a.nq1 > 0 ? a.nq1 : 1);
+    const int n = fl ? a.n_dn : a.n_up, nq = fl ? a.nq1 : a.nq0;
+    const int c = a.cstate[w];
+    const size_t chb = (size_t)(a.chan ? a.chan[w] : 0);
+    const int32_t *off = a.new_off + chb * (2 * a.nk + 1) + c * 2 + fl;
+    const int o0 = off[0], nnew = off[1] - off[0];
+    int32_t *qo = a.qorb + ((size_t)w * 2 + fl) * nqm;
+    for (int q = threadIdx.x; q < nqm; q += blockDim.x) qo[q] = (q < nq && q < nnew) ? a.new_orb[o0 + q] : -1;
+    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
+    const int32_t *part = a.part + ((size_t)w * 2 + fl) * nmax;
+    const double2 *ai = Ai64 + ((size_t)w * 2 + fl) * nmax * nmax;
+    C *Q = reinterpret_cast<C *>(a.Q) + ((size_t)w * 2 + fl) * nqm * nmax;
+    for (int e = threadIdx.x; e < nq * n; e += blockDim.x) {
+        const int q = e / n, f = e - q * n;
+        double vr = 0.0, vi = 0.0;
+        if (q < nnew) {
+            const int o = a.new_orb[o0 + q];
+            for (int r = 0; r < n; ++r) {
+                const double2 p = tab[(size_t)o * Ns + part[r]], x = ai[(size_t)r * nmax + f];
+                vr += p.x * x.x - p.y * x.y;
+                vi += p.x * x.y + p.y * x.x;
+            }
+        }
+        C v;
+        v.x = (T)vr;
+        v.y = (T)vi;
+        Q[(size_t)q * nmax + f] = v;
+    }
+}
+
+extern "C" hipError_t gv_launch_exc_qbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers,
+                                           hipStream_t st)
+{
+    if (!a->Q || !a->qorb) return hipErrorInvalidValue;
+    if (prec == 0)
+        hipLaunchKernelGGL(exc_qbuild_kernel<double>, dim3(2 * nwalkers), dim3(256), 0, st, *a, Ai64);
+    else
+        hipLaunchKernelGGL(exc_qbuild_kernel<float>, dim3(2 * nwalkers), dim3(256), 0, st, *a, Ai64);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st)
 {

@@ -119,6 +119,7 @@ struct gpvmc_ctx {
     bool regk = false;
     int rk_cb = 0, rk_rbw = 0, nq0 = 0, nq1 = 0, rb0 = 0, rb1 = 0;
     void *d_M = nullptr, *d_tabT = nullptr, *d_zscr = nullptr;
+    void *d_Q = nullptr;  // streamed channel kernel: Q rows [W][2][nqm][N]
     int32_t *d_qorb = nullptr;
 };
 
@@ -130,7 +131,7 @@ static void free_ctx(gpvmc_ctx *c)
                     c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork,
                     c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
                     c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
-                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb};
+                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -213,6 +214,7 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     a.hopsign = c->d_hopsign;
     a.chan = c->d_chan;
     a.M = c->d_M;
+    a.Q = c->d_Q;
     a.qorb = c->d_qorb;
     a.tabT = c->d_tabT;
     a.zscr = c->d_zscr;
@@ -287,7 +289,10 @@ static int run_refresh_channel(gpvmc_ctx *c, int mode)
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(gv_launch_exc_build(&a, c->d_work, c->W, c->stream));
     HIPCHK(gv_launch_gj_dyn(&g, c->W, c->stream));
-    if (c->regk) HIPCHK(gv_launch_excr_mbuild(c->prec, &a, c->d_Ai64, c->W, c->stream));
+    if (c->regk)
+        HIPCHK(gv_launch_excr_mbuild(c->prec, &a, c->d_Ai64, c->W, c->stream));
+    else
+        HIPCHK(gv_launch_exc_qbuild(c->prec, &a, c->d_Ai64, c->W, c->stream));
     HIPCHK(launch_exc(c, a));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
@@ -1045,7 +1050,8 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
     const bool regk = cfg->lanes_per_row == 0 &&
                       gv_excr_shape(cfg->precision, (nm + 3) / 4, rb0 + rb1, &kcb, &krbw) &&
                       gv_excr_lds_bytes(cfg->precision, L, nu, nd, nqf[0], nqf[1], rb0, rb1, kcb, nk) <= 160 * 1024;
-    if (!regk && gv_exc_lds_bytes(Ns, nm, nk) > 156 * 1024)
+    const int nqm_all = std::max(1, std::max(nqf[0], nqf[1]));
+    if (!regk && gv_exc_lds_bytes(Ns, nm, nk, nqm_all) > 156 * 1024)
         return fail(GPVMC_ERR_UNSUPPORTED, "channel too large for one workgroup's LDS");
 
     gpvmc_ctx *c = new gpvmc_ctx();
@@ -1098,9 +1104,16 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
         CK(hipMalloc(&c->d_qorb, W * 2 * nqm * sizeof(int32_t)));
         CK(hipMemset(c->d_qorb, 0xff, W * 2 * nqm * sizeof(int32_t)));
         CK(hipMalloc(&c->d_tabT, (size_t)c->G * 2 * Ns * Ns * ts));
-    } else if (c->prec == GPVMC_MIXED) {
-        CK(hipMalloc(&c->d_Ai32, W * 2 * nm * nm * sizeof(float2)));
-        CK(hipMemset(c->d_Ai32, 0, W * 2 * nm * nm * sizeof(float2)));
+    } else {
+        if (c->prec == GPVMC_MIXED) {
+            CK(hipMalloc(&c->d_Ai32, W * 2 * nm * nm * sizeof(float2)));
+            CK(hipMemset(c->d_Ai32, 0, W * 2 * nm * nm * sizeof(float2)));
+        }
+        // Q rows of the streamed kernel, built with the inverse at every init and refresh
+        CK(hipMalloc(&c->d_Q, W * 2 * (size_t)nqm_all * nm * ts));
+        CK(hipMemset(c->d_Q, 0, W * 2 * (size_t)nqm_all * nm * ts));
+        CK(hipMalloc(&c->d_qorb, W * 2 * (size_t)nqm_all * sizeof(int32_t)));
+        CK(hipMemset(c->d_qorb, 0xff, W * 2 * (size_t)nqm_all * sizeof(int32_t)));
     }
     CK(hipMalloc(&c->d_work, W * 2 * nm * nm * sizeof(double2)));
     CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));

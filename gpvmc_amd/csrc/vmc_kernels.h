@@ -103,6 +103,7 @@ struct ExcArgs {
     // Q_fl[q][f] = sum_r Phi_fl(qorb_fl[q], site r) Ai_fl[r][f] (the new orbitals' rows)
     void *M;                  // T2 [W][(rb0 + rb1) * 16][4 * cb] (flavour 0 row blocks first)
     int32_t *qorb;            // [W][2][max(nq0, nq1)] orbital of each Q row
+    void *Q;                  // streamed kernel: T2 [W][2][max(nq0, nq1)][nmax] Q rows, [row][slot]
     const void *tabT;         // T2 [G][2][Ns sites][Ns orbitals] transposed orbital table
     void *zscr;               // T2 [W][16 * ceil(nmax / 16) targets][(rb0 + rb1) * 16] measurement tiles
     int32_t nq0, nq1;         // Q rows per flavour
@@ -165,7 +166,9 @@ hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st
 hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
 // out[g][e] = sum over the walkers idx[off[g] .. off[g+1]) of ss[walker][e], e < per
 hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_t st);
-size_t gv_exc_lds_bytes(int Ns, int nmax, int nk);
+size_t gv_exc_lds_bytes(int Ns, int nmax, int nk, int nqm);
+// Q rows of the streamed channel kernel from the FP64 inverse after Gauss-Jordan
+hipError_t gv_launch_exc_qbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers, hipStream_t st);
 hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st);
 // register-resident channel kernel: the smallest registered shape (column blocks >= cb_min,
