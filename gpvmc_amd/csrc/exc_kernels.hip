@@ -503,38 +503,92 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     const uint64_t ctr0 = a.counter[w];
     int64_t mslot = 0;
     auto measure = [&](bool acc_on) {
-        // ProjHeis::measure on the channel (ProjHeis.cpp:40-154)
+        // ProjHeis::measure on the channel (ProjHeis.cpp:40-154): h_k = sum over bonds of
+        // +a_k/4 (parallel) or -a_k/4 - alpha_k(swap)/2 (antiparallel).  The swap ratios of
+        // every bond come from Y_fl = [Ai_fl; Q_fl] x Phi_fl(current orbitals, sites of the
+        // other flavour's particles): y[r] of moving particle r onto the site of particle t
+        // is Y[r][t], and rho_m . y is Y[n_fl + row of o'_m][t].
         wk.eval(false, 0, 0, 0, 0);
         double S0;
         int kk;
         wk.reduce_alpha(S0, kk);
-        for (int k = tid; k < nk; k += nt) {
-            wk.sz(s.a0, k, wk.lz(s.alpha, k));
-            wk.sz(s.hk, k, zmk(0, 0));
-        }
-        __syncthreads();
-        for (int e = 0; e < 2 * Ns; ++e) {
-            const int4 eg = a.edge[e];
-            const int vi = eg.x, vj = eg.z;
-            const int ii = s.site[vi], ij = s.site[vj];
-            if ((ii >> 16) == (ij >> 16)) {
-                for (int k = tid; k < nk; k += nt) {
-                    zc h = wk.lz(s.hk, k), a0 = wk.lz(s.a0, k);
-                    wk.sz(s.hk, k, zmk(h.x + 0.25 * a0.x, h.y + 0.25 * a0.y));
+        for (int k = tid; k < nk; k += nt) wk.sz(s.a0, k, wk.lz(s.alpha, k));
+        using CT = typename cpx<T>::t;
+        const int rows_max = nmax + nqm;
+        // the chunk of Phi columns lives where apply keeps its coefficients
+        double *cbuf = s.kap;
+        const int CH = (6 * nmax + 4 * nqm) / nmax < 8 ? (6 * nmax + 4 * nqm) / nmax : 8;
+        for (int fl = 0; fl < 2; ++fl) {
+            const int nf = wk.n[fl], no = wk.n[1 - fl], rows = nf + wk.nq[fl];
+            CT *Yf = reinterpret_cast<CT *>(a.Y) + ((size_t)w * 2 + fl) * rows_max * nmax;
+            for (int t0 = 0; t0 < no; t0 += CH) {
+                __syncthreads();
+                for (int e = tid; e < nf * CH; e += nt) {
+                    const int f = e / CH, j = e - f * CH, t = t0 + j;
+                    wk.sz(cbuf, e, t < no ? wk.phi(fl, s.cur[fl * nmax + f], s.part[(1 - fl) * nmax + t]) : zmk(0, 0));
                 }
                 __syncthreads();
-                continue;
+                for (int r = tid; r < rows; r += nt) {
+                    zc acc[8];
+                    for (int j = 0; j < 8; ++j) acc[j] = zmk(0, 0);
+                    for (int f = 0; f < nf; ++f) {
+                        const zc m = r < nf ? wk.ai(fl, r, f) : wk.qv(fl, r - nf, f);
+                        for (int j = 0; j < 8; ++j)
+                            if (j < CH) acc[j] = zadd(acc[j], zmul(m, wk.lz(cbuf, f * CH + j)));
+                    }
+                    for (int j = 0; j < 8; ++j)
+                        if (j < CH && t0 + j < no) {
+                            CT v;
+                            v.x = (T)acc[j].x;
+                            v.y = (T)acc[j].y;
+                            Yf[(size_t)r * nmax + t0 + j] = v;
+                        }
+                }
             }
-            int ci, cj, sa, sb;
-            if ((ii >> 16) == 0) { ci = ii & 0xFFFF; sa = vi; cj = ij & 0xFFFF; sb = vj; }
-            else { ci = ij & 0xFFFF; sa = vj; cj = ii & 0xFFFF; sb = vi; }
-            wk.eval(true, ci, sa, sb, cj);
-            for (int k = tid; k < nk; k += nt) {
-                zc h = wk.lz(s.hk, k), a0 = wk.lz(s.a0, k), ab = wk.lz(s.alpha, k);
-                wk.sz(s.hk, k, zmk(h.x - 0.25 * a0.x - 0.5 * ab.x, h.y - 0.25 * a0.y - 0.5 * ab.y));
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        const int c = wk.cstate();
+        for (int k = tid; k < nk; k += nt) {
+            const zc a0 = wk.lz(s.a0, k);
+            zc h = zmk(0, 0);
+            for (int e = 0; e < 2 * Ns; ++e) {
+                const int4 eg = a.edge[e];
+                const int vi = eg.x, vj = eg.z;
+                const int ii = s.site[vi], ij = s.site[vj];
+                if ((ii >> 16) == (ij >> 16)) {
+                    h = zmk(h.x + 0.25 * a0.x, h.y + 0.25 * a0.y);
+                    continue;
+                }
+                int ci, cj, sa, sb;
+                if ((ii >> 16) == 0) { ci = ii & 0xFFFF; sa = vi; cj = ij & 0xFFFF; sb = vj; }
+                else { ci = ij & 0xFFFF; sa = vj; cj = ii & 0xFFFF; sb = vi; }
+                zc tot = zmk((double)wk.hopsign(c, k), 0.0);
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int R = wk.hop_n(c, k, fl), nf = wk.n[fl];
+                    const int r = fl ? cj : ci, t = fl ? ci : cj;
+                    const int sold = fl ? sb : sa, snew = fl ? sa : sb;
+                    const CT *Yf = reinterpret_cast<const CT *>(a.Y) + ((size_t)w * 2 + fl) * rows_max * nmax;
+                    const int D = R + 1;
+                    zc B[9];
+                    B[0] = toz(Yf[(size_t)r * nmax + t]);
+                    for (int q = 0; q < R; ++q) B[1 + q] = wk.ai(fl, r, s.slot[fl * Ns + wk.hop_from(c, k, fl, q)]);
+                    for (int m = 0; m < R; ++m) {
+                        const int o2 = wk.hop_to(c, k, fl, m), o1 = wk.hop_from(c, k, fl, m);
+                        const zc p2s = wk.phi(fl, o2, snew), p2r = wk.phi(fl, o2, sold);
+                        const zc Dm = zadd(zsub(zsub(p2s, p2r), wk.phi(fl, o1, snew)), wk.phi(fl, o1, sold));
+                        const int qr = s.qrow[fl * Ns + o2];
+                        const zc kgv = qr >= 0 ? toz(Yf[(size_t)(nf + qr) * nmax + t]) : zmk(0, 0);
+                        B[(1 + m) * D] = zsub(zadd(kgv, zmul(Dm, B[0])), p2s);
+                        for (int q = 0; q < R; ++q)
+                            B[(1 + m) * D + 1 + q] = zadd(wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
+                    }
+                    tot = zmul(tot, det_small(D, B));
+                }
+                h = zmk(h.x - 0.25 * a0.x - 0.5 * tot.x, h.y - 0.25 * a0.y - 0.5 * tot.y);
+            }
+            wk.sz(s.hk, k, h);
+        }
+        __syncthreads();
         const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
         if (acc_on) {
             double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
@@ -800,7 +854,7 @@ extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks,
     const int Ns = a->L * a->L;
     const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
     const int nqm = a->nq0 > a->nq1 ? (a->nq0 > 0 ? a->nq0 : 1) : (a->nq1 > 0 ? a->nq1 : 1);
-    if (!a->Q || !a->qorb) return hipErrorInvalidValue;
+    if (!a->Q || !a->qorb || !a->Y) return hipErrorInvalidValue;
     const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk, nqm) + 256;
     if (lds > 160 * 1024 - 4096) return hipErrorInvalidValue;
     if (prec == 0)

@@ -120,6 +120,7 @@ struct gpvmc_ctx {
     int rk_cb = 0, rk_rbw = 0, nq0 = 0, nq1 = 0, rb0 = 0, rb1 = 0;
     void *d_M = nullptr, *d_tabT = nullptr, *d_zscr = nullptr;
     void *d_Q = nullptr;  // streamed channel kernel: Q rows [W][2][nqm][N]
+    void *d_Y = nullptr;  // streamed channel kernel measurement: [W][2][N + nqm][N]
     int32_t *d_qorb = nullptr;
 };
 
@@ -131,7 +132,7 @@ static void free_ctx(gpvmc_ctx *c)
                     c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork,
                     c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
                     c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
-                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q};
+                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q, c->d_Y};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -215,6 +216,7 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     a.chan = c->d_chan;
     a.M = c->d_M;
     a.Q = c->d_Q;
+    a.Y = c->d_Y;
     a.qorb = c->d_qorb;
     a.tabT = c->d_tabT;
     a.zscr = c->d_zscr;
@@ -1114,6 +1116,7 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
         CK(hipMemset(c->d_Q, 0, W * 2 * (size_t)nqm_all * nm * ts));
         CK(hipMalloc(&c->d_qorb, W * 2 * (size_t)nqm_all * sizeof(int32_t)));
         CK(hipMemset(c->d_qorb, 0xff, W * 2 * (size_t)nqm_all * sizeof(int32_t)));
+        CK(hipMalloc(&c->d_Y, W * 2 * ((size_t)nm + nqm_all) * nm * ts));
     }
     CK(hipMalloc(&c->d_work, W * 2 * nm * nm * sizeof(double2)));
     CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));

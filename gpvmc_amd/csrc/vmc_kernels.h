@@ -104,6 +104,8 @@ struct ExcArgs {
     void *M;                  // T2 [W][(rb0 + rb1) * 16][4 * cb] (flavour 0 row blocks first)
     int32_t *qorb;            // [W][2][max(nq0, nq1)] orbital of each Q row
     void *Q;                  // streamed kernel: T2 [W][2][max(nq0, nq1)][nmax] Q rows, [row][slot]
+    void *Y;                  // streamed kernel measurement: T2 [W][2][nmax + nqm][nmax] of
+                              // [Ai; Q] x Phi(current orbitals, sites of the other flavour's particles)
     const void *tabT;         // T2 [G][2][Ns sites][Ns orbitals] transposed orbital table
     void *zscr;               // T2 [W][16 * ceil(nmax / 16) targets][(rb0 + rb1) * 16] measurement tiles
     int32_t nq0, nq1;         // Q rows per flavour
