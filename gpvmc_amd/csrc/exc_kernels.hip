@@ -44,6 +44,7 @@ struct Lds {
     int *qrow;      // [2][Ns] Q row of each orbital or -1
     int *qorb;      // [2][nqm] orbital of each Q row or -1
     int *rep;       // [nqm] apply: slot whose unit row replaces a Q row, or -1
+    int *kqi;       // [nqm] apply: index of each Q row's orbital among the current state's new orbitals
     double *vc;     // [2][nmax] complex: c = Phi(cur orbitals, target site)
     double *vy;     // [2][nmax] complex: y = Ai c
     double *kg;     // [2][nqm] complex: rho_m . y per new orbital of the current state
@@ -61,7 +62,7 @@ struct Lds {
 __host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
 {
     size_t b = 0;
-    b += ((size_t)Ns + 2 * (size_t)nmax * 2 + 2 * (size_t)Ns * 2 + 2 * (size_t)nqm + nqm) * 4;
+    b += ((size_t)Ns + 2 * (size_t)nmax * 2 + 2 * (size_t)Ns * 2 + 2 * (size_t)nqm + 2 * (size_t)nqm) * 4;
     b = (b + 15) & ~(size_t)15;
     b += 16 * ((size_t)2 * nmax * 2 + (size_t)2 * nqm + (size_t)nk * 8 + nk + nk + nk +
                ((size_t)nmax + nqm) * 3 + nqm + (size_t)3 * nmax);
@@ -80,6 +81,7 @@ __device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk, int nqm)
     s.qrow = ip; ip += 2 * Ns;
     s.qorb = ip; ip += 2 * nqm;
     s.rep = ip; ip += nqm;
+    s.kqi = ip; ip += nqm;
     size_t off = (reinterpret_cast<char *>(ip) - base + 15) & ~(size_t)15;
     double *dp = reinterpret_cast<double *>(base + off);
     s.vc = dp; dp += 2 * (2 * nmax);
@@ -311,6 +313,16 @@ template <typename T> struct Walker {
                     sz(s.V, (m0 + m) * nmax + f, v);
                 }
             }
+            // KG of a Q row is the c-product eval formed for that orbital (kg, indexed by the
+            // orbital's place among the current state's new orbitals)
+            for (int q = tid; q < nqf; q += nt) s.kqi[q] = -1;
+            __syncthreads();
+            if (mv)
+                for (int m = tid; m < n_new(c, fl); m += nt) {
+                    const int q = s.qrow[fl * Ns + new_orb(c, fl, m)];
+                    if (q >= 0) s.kqi[q] = m;
+                }
+            __syncthreads();
             // update coefficients of the inverse's rows (e < nf) and of the Q rows
             const zc yr = mv ? lz(s.vy, fl * nmax + r) : zmk(0, 0);
             for (int e = tid; e < nf + nqf; e += nt) {
@@ -344,7 +356,10 @@ template <typename T> struct Walker {
                             if (mv) {
                                 // KG_q = Q_q . c = rho_o . y
                                 zc kg = zmk(0, 0);
-                                for (int f = 0; f < nf; ++f) kg = zadd(kg, zmul(qv(fl, q, f), lz(s.vc, fl * nmax + f)));
+                                if (s.kqi[q] >= 0)
+                                    kg = lz(s.kg, fl * nqm + s.kqi[q]);
+                                else
+                                    for (int f = 0; f < nf; ++f) kg = zadd(kg, zmul(qv(fl, q, f), lz(s.vc, fl * nmax + f)));
                                 const zc pn = phi(fl, o, snew), dq = zsub(pn, phi(fl, o, sold));
                                 dd = dq;
                                 k3[0] = zadd(zsub(kg, pn), zmul(dq, yr));
