@@ -409,25 +409,49 @@ template <typename T> struct Walker {
                 }
             }
             __syncthreads();
-            // rows of the inverse, then the live Q rows
-            for (int e = tid; e < (nf + nqf) * nf; e += nt) {
-                const int row = e / nf, f = e - row * nf;
-                zc acc = zmk(0, 0);
-                for (int a2 = 0; a2 < D; ++a2) acc = zadd(acc, zmul(lz(s.kap, row * 3 + a2), lz(s.V, a2 * nmax + f)));
-                if (row < nf) {
-                    C &el = Ai[fl][(size_t)row * nmax + f];
-                    el.x = (T)((double)el.x - acc.x);
-                    el.y = (T)((double)el.y - acc.y);
-                } else {
-                    const int q = row - nf;
-                    if (s.qorb[fl * nqm + q] < 0) continue;
-                    const int rp = s.rep[q];
-                    zc nv = rp >= 0 ? zmk(f == rp ? 1.0 : 0.0, 0.0) : qv(fl, q, f);
-                    if (mv) nv = zadd(nv, zmul(lz(s.dlt, q), lz(s.vc, fl * nmax + f)));
-                    nv = zsub(nv, acc);
-                    C &el = Qr[fl][(size_t)q * nmax + f];
-                    el.x = (T)nv.x;
-                    el.y = (T)nv.y;
+            // rows of the inverse, then the live Q rows; four elements per thread per pass,
+            // their loads issued before any store
+            {
+                const int tot = (nf + nqf) * nf;
+                for (int e0 = tid; e0 < tot; e0 += 4 * nt) {
+                    C *ep[4];
+                    C ov[4];
+                    int rw[4], fv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int e = e0 + j * nt;
+                        ep[j] = nullptr;
+                        rw[j] = 0;
+                        fv[j] = 0;
+                        if (e < tot) {
+                            const int row = e / nf, f = e - row * nf;
+                            rw[j] = row;
+                            fv[j] = f;
+                            if (row < nf) ep[j] = Ai[fl] + (size_t)row * nmax + f;
+                            else if (s.qorb[fl * nqm + row - nf] >= 0) ep[j] = Qr[fl] + (size_t)(row - nf) * nmax + f;
+                        }
+                        if (ep[j]) ov[j] = *ep[j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (!ep[j]) continue;
+                        const int row = rw[j], f = fv[j];
+                        zc acc = zmk(0, 0);
+                        for (int a2 = 0; a2 < D; ++a2) acc = zadd(acc, zmul(lz(s.kap, row * 3 + a2), lz(s.V, a2 * nmax + f)));
+                        zc nv;
+                        if (row < nf) {
+                            nv = zsub(toz(ov[j]), acc);
+                        } else {
+                            const int q = row - nf, rp = s.rep[q];
+                            nv = rp >= 0 ? zmk(f == rp ? 1.0 : 0.0, 0.0) : toz(ov[j]);
+                            if (mv) nv = zadd(nv, zmul(lz(s.dlt, q), lz(s.vc, fl * nmax + f)));
+                            nv = zsub(nv, acc);
+                        }
+                        C v;
+                        v.x = (T)nv.x;
+                        v.y = (T)nv.y;
+                        *ep[j] = v;
+                    }
                 }
             }
             __syncthreads();
