@@ -185,9 +185,19 @@ template <typename T> struct Walker {
             __syncthreads();
             for (int e = tid; e < n[0] + n[1]; e += nt) {
                 const int fl = e < n[0] ? 0 : 1, r = fl ? e - n[0] : e;
-                zc t = zmk(0, 0);
-                for (int f = 0; f < n[fl]; ++f) t = zadd(t, zmul(ai(fl, r, f), lz(s.vc, fl * nmax + f)));
-                sz(s.vy, fl * nmax + r, t);
+                // four partial sums, four loads in flight per step
+                const C *row = Ai[fl] + (size_t)r * nmax;
+                zc t4[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+                int f = 0;
+                for (; f + 4 <= n[fl]; f += 4) {
+                    C x[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) x[j] = row[f + j];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) t4[j] = zadd(t4[j], zmul(toz(x[j]), lz(s.vc, fl * nmax + f + j)));
+                }
+                for (; f < n[fl]; ++f) t4[0] = zadd(t4[0], zmul(toz(row[f]), lz(s.vc, fl * nmax + f)));
+                sz(s.vy, fl * nmax + r, zadd(zadd(t4[0], t4[1]), zadd(t4[2], t4[3])));
             }
             __syncthreads();
             const int nn0 = n_new(c, 0), nn1 = n_new(c, 1);
@@ -570,8 +580,19 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                 for (int r = tid; r < rows; r += nt) {
                     zc acc[8];
                     for (int j = 0; j < 8; ++j) acc[j] = zmk(0, 0);
-                    for (int f = 0; f < nf; ++f) {
-                        const zc m = r < nf ? wk.ai(fl, r, f) : wk.qv(fl, r - nf, f);
+                    const CT *rowp = r < nf ? wk.Ai[fl] + (size_t)r * nmax : wk.Qr[fl] + (size_t)(r - nf) * nmax;
+                    int f = 0;
+                    for (; f + 2 <= nf; f += 2) {
+                        const CT x0 = rowp[f], x1 = rowp[f + 1];
+                        const zc m0 = toz(x0), m1 = toz(x1);
+                        for (int j = 0; j < 8; ++j)
+                            if (j < CH) {
+                                acc[j] = zadd(acc[j], zmul(m0, wk.lz(cbuf, f * CH + j)));
+                                acc[j] = zadd(acc[j], zmul(m1, wk.lz(cbuf, (f + 1) * CH + j)));
+                            }
+                    }
+                    for (; f < nf; ++f) {
+                        const zc m = toz(rowp[f]);
                         for (int j = 0; j < 8; ++j)
                             if (j < CH) acc[j] = zadd(acc[j], zmul(m, wk.lz(cbuf, f * CH + j)));
                     }
