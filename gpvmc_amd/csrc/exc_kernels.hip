@@ -204,10 +204,18 @@ template <typename T> struct Walker {
             for (int e = tid; e < nn0 + nn1; e += nt) {
                 const int fl = e < nn0 ? 0 : 1, m = fl ? e - nn0 : e;
                 const int o = new_orb(c, fl, m);
-                zc t = zmk(0, 0);
-                for (int r = 0; r < n[fl]; ++r)
-                    t = zadd(t, zmul(phi(fl, o, s.part[fl * nmax + r]), lz(s.vy, fl * nmax + r)));
-                sz(s.kg, fl * nqm + m, t);
+                const C *trow = tab[fl] + (size_t)o * Ns;
+                zc t4[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
+                int r = 0;
+                for (; r + 4 <= n[fl]; r += 4) {
+                    C x[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) x[j] = trow[s.part[fl * nmax + r + j]];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) t4[j] = zadd(t4[j], zmul(toz(x[j]), lz(s.vy, fl * nmax + r + j)));
+                }
+                for (; r < n[fl]; ++r) t4[0] = zadd(t4[0], zmul(toz(trow[s.part[fl * nmax + r]]), lz(s.vy, fl * nmax + r)));
+                sz(s.kg, fl * nqm + m, zadd(zadd(t4[0], t4[1]), zadd(t4[2], t4[3])));
             }
             __syncthreads();
         }
