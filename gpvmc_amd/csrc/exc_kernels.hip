@@ -427,16 +427,16 @@ template <typename T> struct Walker {
                 }
             }
             __syncthreads();
-            // rows of the inverse, then the live Q rows; four elements per thread per pass,
+            // rows of the inverse, then the live Q rows; eight elements per thread per pass,
             // their loads issued before any store
             {
                 const int tot = (nf + nqf) * nf;
-                for (int e0 = tid; e0 < tot; e0 += 4 * nt) {
-                    C *ep[4];
-                    C ov[4];
-                    int rw[4], fv[4];
+                for (int e0 = tid; e0 < tot; e0 += 8 * nt) {
+                    C *ep[8];
+                    C ov[8];
+                    int rw[8], fv[8];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < 8; ++j) {
                         const int e = e0 + j * nt;
                         ep[j] = nullptr;
                         rw[j] = 0;
@@ -451,7 +451,7 @@ template <typename T> struct Walker {
                         if (ep[j]) ov[j] = *ep[j];
                     }
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < 8; ++j) {
                         if (!ep[j]) continue;
                         const int row = rw[j], f = fv[j];
                         zc acc = zmk(0, 0);
