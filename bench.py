@@ -239,8 +239,11 @@ def mixed_batch_leg(seed, sweeps, total=20000, Ls=(8, 10, 12, 16, 20), warmup=1)
     (uniform over Ls x {groundstate, long}); one context per (L, kind) bucket, all swept
     together through one captured hipGraph per sweep (gpvmc_batch_*).  A sweep is L*L steps
     and one measurement per walker (ProjHeis + StagMagnZ for the ground state, ProjHeis
-    H/O for the channels), mixed FP32; an FP64 refresh of every context after warm-up."""
-    from gpvmc_amd import binding, channels, model
+    H/O for the channels), mixed FP32; an FP64 refresh of every context after warm-up.
+    Each bucket's launch configuration (sweep shape and variant, or register-resident vs
+    streamed channel kernel) is picked by timing the candidates on the bucket's walkers
+    (gpvmc_amd.tune.pick_launch)."""
+    from gpvmc_amd import binding, channels, model, tune
     rng = np.random.default_rng(seed)
     Lw = rng.choice(np.array(Ls), size=total)
     kind = rng.integers(0, 2, size=total)
@@ -253,26 +256,43 @@ def mixed_batch_leg(seed, sweeps, total=20000, Ls=(8, 10, 12, 16, 20), warmup=1)
             if n == 0:
                 continue
             tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+            oks = {}
             if kd == 0:
-                b = binding.Batch(L, n, binding.MIXED, n_groups=1, seed=seed + L)
-                b.set_orbitals(0, model.ground_occ(tab))
-                ok = b.init(model.random_spins(L, n, seed + L))
+                def make(lanes_per_row=0):
+                    b = binding.Batch(L, n, binding.MIXED, n_groups=1, seed=seed + L,
+                                      lanes_per_row=lanes_per_row)
+                    b.set_orbitals(0, model.ground_occ(tab))
+                    oks[id(b)] = b.init(model.random_spins(L, n, seed + L))
+                    return b
                 mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ
-                name = "groundstate"
+                cands = tune.launch_candidates(L, binding.MIXED)
             else:
                 # four momenta per size, walker w on q = w mod 4
                 qs = [(0, 0), (L // 2, L // 2), (1, 0), (L // 4, L // 2)]
                 chs = [channels.Channel("long", L, q) for q in qs]
                 nk = max(c.nexc for c in chs)
                 chs = [c for c in chs if c.nexc == nk]
-                b = binding.ChannelBatch(L, n, chs, binding.MIXED, seed=seed + L)
-                b.set_orbital_table(0, tab)
-                ok = b.init(model.random_spins(L, n, seed + 100 + L))
+
+                def make(kernel="auto"):
+                    b = binding.ChannelBatch(L, n, chs, binding.MIXED, seed=seed + L, kernel=kernel)
+                    b.set_orbital_table(0, tab)
+                    oks[id(b)] = b.init(model.random_spins(L, n, seed + 100 + L))
+                    return b
                 mask = binding.MEAS_PROJHEIS
+                cands = tune.launch_candidates(L, binding.MIXED, channel=True)
+            # the autotuner: every launch configuration of the bucket timed on its own walkers
+            b, chosen, hist = tune.pick_launch(make, cands, L * L, meas=L * L, mask=mask)
+            ok = oks[id(b)]
+            if kd == 0:
+                name = "groundstate (lanes_per_row %d, %s)" % (
+                    chosen.get("lanes_per_row", 0),
+                    "windows" if b.sweep_variant == binding.SWEEP_WINDOWS else "one move")
+            else:
                 name = "long (%d states, %s kernel)" % (nk, "register" if b.register_resident else "streamed")
             sw.add(b, L * L, L * L, mask)
             members.append(b)
-            buckets.append({"L": L, "kind": name, "walkers": n, "init_ok": float(np.mean(ok))})
+            buckets.append({"L": L, "kind": name, "walkers": n, "init_ok": float(np.mean(ok)),
+                            "tuning": hist})
     sw.sweep(warmup)
     for b in members:
         b.refresh()

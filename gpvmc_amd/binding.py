@@ -32,6 +32,7 @@ SIGNATURES = {
     "gpvmc_refresh": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_set_sweep_variant": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "gpvmc_sweep_variant": (ctypes.c_int, [_vp]),
+    "gpvmc_sweep_shapes": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32]),
     "gpvmc_walk": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32,
                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gpvmc_measure": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
@@ -97,6 +98,18 @@ def load_library():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sweep_shapes(L, precision):
+    """lanes_per_row of every ground-state sweep shape compiled for (L, precision), the
+    default first (no device needed)."""
+    lib = load_library()
+    n = lib.gpvmc_sweep_shapes(precision, L, None, 0)
+    if n < 0:
+        raise GpvmcError(lib.gpvmc_last_error().decode())
+    out = np.zeros(max(n, 1), np.int32)
+    lib.gpvmc_sweep_shapes(precision, L, _ptr(out), n)
+    return [int(x) for x in out[:n]]
 
 
 class Batch:

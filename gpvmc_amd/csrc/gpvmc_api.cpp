@@ -637,6 +637,18 @@ int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     return GPVMC_OK;
 }
 
+int gpvmc_sweep_shapes(int32_t precision, int32_t L, int32_t *lanes_per_row, int32_t max)
+{
+    if (L < 4 || (L * L) % 2) return fail(GPVMC_ERR_ARG, "L must be >= 4 with L*L even");
+    if (precision != GPVMC_FP64 && precision != GPVMC_MIXED)
+        return fail(GPVMC_ERR_ARG, "precision must be GPVMC_FP64 or GPVMC_MIXED");
+    if (max < 0 || (max > 0 && !lanes_per_row)) return fail(GPVMC_ERR_ARG, "null output");
+    int tmp[32];
+    const int n = gv_sweep_shapes(precision, L * L / 2, tmp, 32);
+    for (int i = 0; i < n && i < max && i < 32; ++i) lanes_per_row[i] = tmp[i];
+    return n;
+}
+
 int gpvmc_set_sweep_variant(gpvmc_ctx *c, int32_t variant)
 {
     if (!c) return fail(GPVMC_ERR_ARG, "null context");

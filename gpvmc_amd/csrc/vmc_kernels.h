@@ -156,6 +156,8 @@ struct GJDynArgs {
 
 extern "C" {
 int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
+/* the lanes-per-row values of every sweep shape of (prec, N), default first; returns the count */
+int gv_sweep_shapes(int prec, int N, int *lpr, int max);
 /* delayed-update windows (several accepted moves per table update) for this shape */
 int gv_sweep_wdu_supported(int prec, int N, int lpr);
 hipError_t gv_launch_sweep_wdu(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks, hipStream_t st);

@@ -1632,6 +1632,17 @@ extern "C" int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthre
     return 0;
 }
 
+extern "C" int gv_sweep_shapes(int prec, int N, int *lpr, int max)
+{
+    int n = 0;
+    for (const auto &e : gv::kSweep)
+        if (e.prec == prec && e.N == N) {
+            if (lpr && n < max) lpr[n] = e.lpr;
+            ++n;
+        }
+    return n;
+}
+
 extern "C" hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
                                       hipStream_t st)
 {

@@ -7,7 +7,13 @@ estimator: the local energy measured on drifted tables differs from the one meas
 fresh tables at the same configuration.  `tune_refresh` finds the longest refresh
 interval (in sweeps) whose energy error per site, averaged over the batch, stays under a
 bound, by doubling the interval until the bound is crossed.
+
+`pick_launch` picks a context's launch configuration (sweep shape and sweep variant for a
+ground-state size, register-resident or streamed kernel for a channel) by timing each
+candidate, built by the caller on the same walkers, and keeps the fastest open.
 """
+import time
+
 import numpy as np
 
 
@@ -41,3 +47,65 @@ def tune_refresh(batch, sweep, bound=1e-7, kmax=64):
             break
         k *= 2
     return best, history
+
+
+def launch_candidates(L, precision, channel=False):
+    """Launch configurations to try: dicts of Batch / ChannelBatch keyword arguments plus
+    the sweep variant ("variant", ground state only)."""
+    from . import binding
+    if channel:
+        return [dict(kernel="auto"), dict(kernel="streamed")]
+    out = []
+    for lpr in binding.sweep_shapes(L, precision):
+        out.append(dict(lanes_per_row=lpr, variant=binding.SWEEP_ONE_MOVE))
+        if precision == binding.MIXED and L == 16 and lpr == 4:
+            out.append(dict(lanes_per_row=lpr, variant=binding.SWEEP_WINDOWS))
+    return out
+
+
+def pick_launch(make, candidates, steps, sweeps=2, meas=0, mask=None):
+    """Builds the context of every candidate with make(**kwargs) (the "variant" key is set
+    on the built context), times `sweeps` walks of `steps` steps (measured every `meas`
+    steps) after one warm-up walk, and returns (the fastest context, still open, its
+    candidate, history).  A single candidate is returned untimed.  A candidate that builds
+    the same kernel as an earlier one (a channel without a register-resident shape) or
+    whose variant the shape lacks is skipped; the others are closed."""
+    from . import binding
+    history = []
+    best = best_c = best_t = None
+    seen = set()
+    for i, cand in enumerate(candidates):
+        kw = {k: v for k, v in cand.items() if k != "variant"}
+        b = make(**kw)
+        key = repr(sorted(kw.items()))
+        if hasattr(b, "register_resident"):
+            key = "register" if b.register_resident else "streamed"
+        if "variant" in cand:
+            try:
+                b.sweep_variant = cand["variant"]
+            except binding.GpvmcError:
+                b.close()
+                continue
+            key += " variant %d" % b.sweep_variant
+        if key in seen:
+            b.close()
+            continue
+        seen.add(key)
+        if len(candidates) == 1:
+            return b, cand, history
+        walk_kw = dict(meas=meas)
+        if mask is not None:
+            walk_kw["mask"] = mask
+        b.walk(steps, **walk_kw)
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            b.walk(steps, **walk_kw)
+        t = (time.perf_counter() - t0) / sweeps
+        history.append(dict(candidate=key, ms_per_sweep=round(1e3 * t, 3)))
+        if best_t is None or t < best_t:
+            if best is not None:
+                best.close()
+            best, best_c, best_t = b, cand, t
+        else:
+            b.close()
+    return best, best_c, history

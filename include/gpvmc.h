@@ -95,6 +95,12 @@ int gpvmc_init_walkers(gpvmc_ctx *ctx, const int32_t *spins, int32_t *ok);
  * that runs (ONE_MOVE or WINDOWS), -1 for a null context.  No counterpart in the reference,
  * whose chain updates the inverse after every accepted move (SlaterDeterminant::Update,
  * SlaterDeterminant.cpp:270-326). */
+/* The lanes_per_row values of the ground-state sweep shapes compiled for (precision, L),
+ * the default first: writes at most `max` of them and returns how many there are (a
+ * negative error code on bad arguments).  For launch-shape tuning; no counterpart in the
+ * reference. */
+int gpvmc_sweep_shapes(int32_t precision, int32_t L, int32_t *lanes_per_row, int32_t max);
+
 #define GPVMC_SWEEP_DEFAULT 0
 #define GPVMC_SWEEP_ONE_MOVE 1
 #define GPVMC_SWEEP_WINDOWS 2
