@@ -67,14 +67,26 @@ def pick_launch(make, candidates, steps, sweeps=2, meas=0, mask=None):
     """Builds the context of every candidate with make(**kwargs) (the "variant" key is set
     on the built context), times `sweeps` walks of `steps` steps (measured every `meas`
     steps) after one warm-up walk, and returns (the fastest context, still open, its
-    candidate, history).  A single candidate is returned untimed.  A candidate that builds
-    the same kernel as an earlier one (a channel without a register-resident shape) or
-    whose variant the shape lacks is skipped; the others are closed."""
+    candidate, history).  A candidate that builds the same kernel as an earlier one (a
+    channel without a register-resident shape) or whose variant the shape lacks is
+    skipped; the others are closed.  When only one distinct candidate remains it is
+    returned untimed."""
     from . import binding
+    walk_kw = dict(meas=meas)
+    if mask is not None:
+        walk_kw["mask"] = mask
+
+    def timed(b):
+        b.walk(steps, **walk_kw)
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            b.walk(steps, **walk_kw)
+        return (time.perf_counter() - t0) / sweeps
+
     history = []
-    best = best_c = best_t = None
+    best = best_c = best_t = best_key = None
     seen = set()
-    for i, cand in enumerate(candidates):
+    for cand in candidates:
         kw = {k: v for k, v in cand.items() if k != "variant"}
         b = make(**kw)
         key = repr(sorted(kw.items()))
@@ -91,21 +103,17 @@ def pick_launch(make, candidates, steps, sweeps=2, meas=0, mask=None):
             b.close()
             continue
         seen.add(key)
-        if len(candidates) == 1:
-            return b, cand, history
-        walk_kw = dict(meas=meas)
-        if mask is not None:
-            walk_kw["mask"] = mask
-        b.walk(steps, **walk_kw)
-        t0 = time.perf_counter()
-        for _ in range(sweeps):
-            b.walk(steps, **walk_kw)
-        t = (time.perf_counter() - t0) / sweeps
+        if best is None:
+            best, best_c, best_key = b, cand, key
+            continue
+        if best_t is None:
+            best_t = timed(best)
+            history.append(dict(candidate=best_key, ms_per_sweep=round(1e3 * best_t, 3)))
+        t = timed(b)
         history.append(dict(candidate=key, ms_per_sweep=round(1e3 * t, 3)))
-        if best_t is None or t < best_t:
-            if best is not None:
-                best.close()
-            best, best_c, best_t = b, cand, t
+        if t < best_t:
+            best.close()
+            best, best_c, best_t, best_key = b, cand, t, key
         else:
             b.close()
     return best, best_c, history

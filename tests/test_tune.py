@@ -67,3 +67,15 @@ def test_pick_launch_keeps_the_fastest_open_and_closes_the_rest():
 def test_pick_launch_single_candidate_is_untimed():
     b, c, hist = tune.pick_launch(lambda: Fake(1.0), [dict()], 10)
     assert hist == [] and c == {} and not b.closed
+
+
+def test_pick_launch_duplicate_kernels_are_untimed():
+    made = []
+
+    def make(kernel="auto"):
+        f = Fake(1.0)
+        f.register_resident = False
+        made.append(f)
+        return f
+    b, c, hist = tune.pick_launch(make, [dict(kernel="auto"), dict(kernel="streamed")], 10)
+    assert hist == [] and c == dict(kernel="auto") and b is made[0] and made[1].closed
