@@ -35,28 +35,67 @@ using gv::ExcArgs;
 
 using namespace gz;
 
+// Explicit address spaces for the walk kernel's pointers: global memory (GAS) and LDS (LAS).
+// Pointers kept in the walker's state are otherwise generic, every access through them a FLAT
+// instruction, which also counts against the LDS counter and so makes each LDS access wait
+// for the global loads in flight.
+#define GAS __attribute__((address_space(1)))
+#define LAS __attribute__((address_space(3)))
+template <typename X> __device__ __forceinline__ X GAS *gptr(X *p) { return (X GAS *)p; }
+// loads and stores of the two- and four-component vector types through global pointers go
+// through clang's native vector types (the HIP vector types copy only from generic memory)
+template <typename X> struct natv;
+template <> struct natv<float2> { typedef float t __attribute__((ext_vector_type(2))); };
+template <> struct natv<double2> { typedef double t __attribute__((ext_vector_type(2))); };
+template <typename X> __device__ __forceinline__ X ldg(const X GAS *p)
+{
+    const typename natv<X>::t v = *reinterpret_cast<const typename natv<X>::t GAS *>(p);
+    X r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+template <typename X> __device__ __forceinline__ void stg(X GAS *p, X v)
+{
+    typename natv<X>::t n;
+    n.x = v.x;
+    n.y = v.y;
+    *reinterpret_cast<typename natv<X>::t GAS *>(p) = n;
+}
+__device__ __forceinline__ int4 ldg4(const int4 GAS *p)
+{
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    const i4 v = *reinterpret_cast<const i4 GAS *>(p);
+    int4 r;
+    r.x = v.x;
+    r.y = v.y;
+    r.z = v.z;
+    r.w = v.w;
+    return r;
+}
+
 // LDS layout of one walker (dynamic shared memory; offsets computed by lds_carve)
 struct Lds {
-    int *site;      // [Ns]   (flavour << 16) | label
-    int *part;      // [2][nmax] site of each label
-    int *cur;       // [2][nmax] orbital in each slot of the current state
-    int *slot;      // [2][Ns] slot of each orbital or -1
-    int *qrow;      // [2][Ns] Q row of each orbital or -1
-    int *qorb;      // [2][nqm] orbital of each Q row or -1
-    int *rep;       // [nqm] apply: slot whose unit row replaces a Q row, or -1
-    int *kqi;       // [nqm] apply: index of each Q row's orbital among the current state's new orbitals
-    double *vc;     // [2][nmax] complex: c = Phi(cur orbitals, target site)
-    double *vy;     // [2][nmax] complex: y = Ai c
-    double *kg;     // [2][nqm] complex: rho_m . y per new orbital of the current state
-    double *ra;     // [nk][2][2][2] complex: (rho_m Ai)[f_n] blocks of every state
-    double *alpha;  // [nk] complex: a_k(x') / a_c(x)
-    double *a0;     // [nk] complex: amplitudes at the current configuration
-    double *hk;     // [nk] complex: ProjHeis h_k
-    double *kap;    // apply: [nmax + nqm][3] complex update coefficients of one flavour's rows
-    double *dlt;    // apply: [nqm] complex delta of each Q row
-    double *V;      // apply: [3][nmax] complex V rows, then W = B^-1 V
-    double *red;    // [16]
-    int *ired;      // [16]
+    int LAS *site;      // [Ns]   (flavour << 16) | label
+    int LAS *part;      // [2][nmax] site of each label
+    int LAS *cur;       // [2][nmax] orbital in each slot of the current state
+    int LAS *slot;      // [2][Ns] slot of each orbital or -1
+    int LAS *qrow;      // [2][Ns] Q row of each orbital or -1
+    int LAS *qorb;      // [2][nqm] orbital of each Q row or -1
+    int LAS *rep;       // [nqm] apply: slot whose unit row replaces a Q row, or -1
+    int LAS *kqi;       // [nqm] apply: index of each Q row's orbital among the current state's new orbitals
+    double LAS *vc;     // [2][nmax] complex: c = Phi(cur orbitals, target site)
+    double LAS *vy;     // [2][nmax] complex: y = Ai c
+    double LAS *kg;     // [2][nqm] complex: rho_m . y per new orbital of the current state
+    double LAS *ra;     // [nk][2][2][2] complex: (rho_m Ai)[f_n] blocks of every state
+    double LAS *alpha;  // [nk] complex: a_k(x') / a_c(x)
+    double LAS *a0;     // [nk] complex: amplitudes at the current configuration
+    double LAS *hk;     // [nk] complex: ProjHeis h_k
+    double LAS *kap;    // apply: [nmax + nqm][3] complex update coefficients of one flavour's rows
+    double LAS *dlt;    // apply: [nqm] complex delta of each Q row
+    double LAS *V;      // apply: [3][nmax] complex V rows, then W = B^-1 V
+    double LAS *red;    // [16]
+    int LAS *ired;      // [16]
 };
 
 __host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
@@ -70,10 +109,10 @@ __host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nq
     return b;
 }
 
-__device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk, int nqm)
+__device__ inline Lds lds_carve(char LAS *base, int Ns, int nmax, int nk, int nqm)
 {
     Lds s;
-    int *ip = reinterpret_cast<int *>(base);
+    int LAS *ip = reinterpret_cast<int LAS *>(base);
     s.site = ip; ip += Ns;
     s.part = ip; ip += 2 * nmax;
     s.cur = ip; ip += 2 * nmax;
@@ -82,8 +121,8 @@ __device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk, int nqm)
     s.qorb = ip; ip += 2 * nqm;
     s.rep = ip; ip += nqm;
     s.kqi = ip; ip += nqm;
-    size_t off = (reinterpret_cast<char *>(ip) - base + 15) & ~(size_t)15;
-    double *dp = reinterpret_cast<double *>(base + off);
+    size_t off = (reinterpret_cast<char LAS *>(ip) - base + 15) & ~(size_t)15;
+    double LAS *dp = reinterpret_cast<double LAS *>(base + off);
     s.vc = dp; dp += 2 * (2 * nmax);
     s.vy = dp; dp += 2 * (2 * nmax);
     s.kg = dp; dp += 2 * (2 * nqm);
@@ -95,7 +134,7 @@ __device__ inline Lds lds_carve(char *base, int Ns, int nmax, int nk, int nqm)
     s.dlt = dp; dp += 2 * nqm;
     s.V = dp; dp += 2 * (3 * nmax);
     s.red = dp; dp += 16;
-    s.ired = reinterpret_cast<int *>(dp);
+    s.ired = reinterpret_cast<int LAS *>(dp);
     return s;
 }
 
@@ -105,11 +144,11 @@ template <typename T> struct Walker {
     Lds s;
     int w, tid, nt, Ns, n[2], nmax, nk, nq[2], nqm;
     size_t cb;  // channel of the walker times nk: base of its graph arrays
-    C *Ai[2];
-    C *Qr[2];   // Q rows: Q[q][f] = (rho_{qorb[q]} Ai)[f], kept up to date with Ai
-    const C *tab[2];
+    C GAS *Ai[2];
+    C GAS *Qr[2];   // Q rows: Q[q][f] = (rho_{qorb[q]} Ai)[f], kept up to date with Ai
+    const C GAS *tab[2];
 
-    __device__ Walker(const ExcArgs &args, char *base, int wk)
+    __device__ __forceinline__ Walker(const ExcArgs &args, char *base, int wk)
         : a(args), w(wk), tid(threadIdx.x), nt(blockDim.x)
     {
         Ns = a.L * a.L;
@@ -121,43 +160,51 @@ template <typename T> struct Walker {
         nq[1] = a.nq1;
         nqm = nq[0] > nq[1] ? nq[0] : nq[1];
         if (nqm < 1) nqm = 1;
-        cb = (size_t)(a.chan ? a.chan[w] : 0) * nk;
-        s = lds_carve(base, Ns, nmax, nk, nqm);
-        for (int fl = 0; fl < 2; ++fl) {
-            Ai[fl] = reinterpret_cast<C *>(a.Ai) + ((size_t)w * 2 + fl) * nmax * nmax;
-            Qr[fl] = reinterpret_cast<C *>(a.Q) + ((size_t)w * 2 + fl) * nqm * nmax;
-            tab[fl] = reinterpret_cast<const C *>(a.tab) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
-        }
+        cb = (size_t)(a.chan ? gptr(a.chan)[w] : 0) * nk;
+        s = lds_carve((char LAS *)base, Ns, nmax, nk, nqm);
+        Ai[0] = gptr(reinterpret_cast<C *>(a.Ai)) + (size_t)w * 2 * nmax * nmax;
+        Ai[1] = Ai[0] + (size_t)nmax * nmax;
+        Qr[0] = gptr(reinterpret_cast<C *>(a.Q)) + (size_t)w * 2 * nqm * nmax;
+        Qr[1] = Qr[0] + (size_t)nqm * nmax;
+        tab[0] = gptr(reinterpret_cast<const C *>(a.tab)) + (size_t)gptr(a.group)[w] * 2 * Ns * Ns;
+        tab[1] = tab[0] + (size_t)Ns * Ns;
     }
-    __device__ zc phi(int fl, int orb, int site) const { return toz(tab[fl][(size_t)orb * Ns + site]); }
-    __device__ zc ai(int fl, int r, int f) const { return toz(Ai[fl][(size_t)r * nmax + f]); }
+    // per-flavour members by a select rather than a runtime index, so that the walker's
+    // state stays in registers
+    __device__ int nfl(int fl) const { return fl ? n[1] : n[0]; }
+    __device__ int nql(int fl) const { return fl ? nq[1] : nq[0]; }
+    __device__ C GAS *aip(int fl) const { return fl ? Ai[1] : Ai[0]; }
+    __device__ C GAS *qrp(int fl) const { return fl ? Qr[1] : Qr[0]; }
+    __device__ const C GAS *tbp(int fl) const { return fl ? tab[1] : tab[0]; }
+    __device__ zc phi(int fl, int orb, int site) const { return toz(ldg(tbp(fl) + (size_t)orb * Ns + site)); }
+    __device__ zc ai(int fl, int r, int f) const { return toz(ldg(aip(fl) + (size_t)r * nmax + f)); }
     // Q row q (an orbital without a row reads as zero; the rows are the new orbitals of the
     // current state, which every lookup asks for)
     __device__ zc qv(int fl, int q, int f) const
     {
-        return q >= 0 ? toz(Qr[fl][(size_t)q * nmax + f]) : zmk(0, 0);
+        return q >= 0 ? toz(ldg(qrp(fl) + (size_t)q * nmax + f)) : zmk(0, 0);
     }
-    __device__ zc lz(const double *p, int i) const { return zmk(p[2 * i], p[2 * i + 1]); }
-    __device__ void sz(double *p, int i, zc v) const { p[2 * i] = v.x; p[2 * i + 1] = v.y; }
+    __device__ zc lz(const double LAS *p, int i) const { return zmk(p[2 * i], p[2 * i + 1]); }
+    __device__ void sz(double LAS *p, int i, zc v) const { p[2 * i] = v.x; p[2 * i + 1] = v.y; }
 
-    __device__ int cstate() const { return a.cstate[w]; }
+    __device__ int cstate() const { return gptr(a.cstate)[w]; }
     __device__ size_t ck(int c, int k) const { return (cb + c) * nk + k; }
-    __device__ int hop_n(int c, int k, int fl) const { return a.hop_n[ck(c, k) * 2 + fl]; }
-    __device__ int hop_from(int c, int k, int fl, int h) const { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2]; }
-    __device__ int hop_to(int c, int k, int fl, int h) const { return a.hop_orb[((ck(c, k) * 2 + fl) * 2 + h) * 2 + 1]; }
+    __device__ int hop_n(int c, int k, int fl) const { return gptr(a.hop_n)[ck(c, k) * 2 + fl]; }
+    __device__ int hop_from(int c, int k, int fl, int h) const { return gptr(a.hop_orb)[((ck(c, k) * 2 + fl) * 2 + h) * 2]; }
+    __device__ int hop_to(int c, int k, int fl, int h) const { return gptr(a.hop_orb)[((ck(c, k) * 2 + fl) * 2 + h) * 2 + 1]; }
     __device__ int hop_new(int c, int k, int fl, int h) const  // index into the new orbitals of (c, fl)
     {
-        return a.hop_new[(ck(c, k) * 2 + fl) * 2 + h];
+        return gptr(a.hop_new)[(ck(c, k) * 2 + fl) * 2 + h];
     }
-    __device__ int hopsign(int c, int k) const { return a.hopsign[ck(c, k)]; }
+    __device__ int hopsign(int c, int k) const { return gptr(a.hopsign)[ck(c, k)]; }
     // new_off holds 2 nk + 1 offsets per channel
-    __device__ const int32_t *noff(int c, int fl) const { return a.new_off + (cb / nk) * (2 * nk + 1) + c * 2 + fl; }
+    __device__ const int32_t GAS *noff(int c, int fl) const { return gptr(a.new_off) + (cb / nk) * (2 * nk + 1) + c * 2 + fl; }
     __device__ int n_new(int c, int fl) const { return noff(c, fl)[1] - noff(c, fl)[0]; }
-    __device__ int new_orb(int c, int fl, int m) const { return a.new_orb[noff(c, fl)[0] + m]; }
+    __device__ int new_orb(int c, int fl, int m) const { return gptr(a.new_orb)[noff(c, fl)[0] + m]; }
 
     // RA blocks of every state relative to the current state (after any change of Ai or
     // of the current state): (rho_{o'_m} Ai)[f_n] is entry f_n of the Q row of o'_m
-    __device__ void build_ra()
+    __device__ __forceinline__ void build_ra()
     {
         const int c = cstate();
         for (int e = tid; e < nk * 8; e += nt) {
@@ -173,49 +220,59 @@ template <typename T> struct Walker {
     // Ratios of every state for the swap moving up particle ci to site sb and down
     // particle cj to site sa (mv = true), or at the current configuration (mv = false):
     // fills alpha[k] and dets[k][fl]; leaves c, y, kg in LDS for apply().
-    __device__ void eval(bool mv, int ci, int sa, int sb, int cj)
+    __device__ __forceinline__ void eval(bool mv, int ci, int sa, int sb, int cj)
     {
         const int c = cstate();
         if (mv) {
             for (int fl = 0; fl < 2; ++fl) {
                 const int tgt = fl ? sa : sb;
-                for (int f = tid; f < n[fl]; f += nt)
+                for (int f = tid; f < nfl(fl); f += nt)
                     sz(s.vc, fl * nmax + f, phi(fl, s.cur[fl * nmax + f], tgt));
             }
             __syncthreads();
-            for (int e = tid; e < n[0] + n[1]; e += nt) {
-                const int fl = e < n[0] ? 0 : 1, r = fl ? e - n[0] : e;
-                // four partial sums, four loads in flight per step
-                const C *row = Ai[fl] + (size_t)r * nmax;
-                zc t4[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
-                int f = 0;
-                for (; f + 4 <= n[fl]; f += 4) {
-                    C x[4];
+            // y = Ai c and KG = rho . y: sixteen lanes per row (f or r = l16 + 16 j, j < 16
+            // as nmax <= 256, a row's sixteen consecutive elements in one cache line), every
+            // load of a lane's share of the row issued before the first product, the partial
+            // sums reduced across the sixteen lanes
+            const int l16 = tid & 15, g16 = tid >> 4, ng16 = nt >> 4;
+            for (int e = g16; e < n[0] + n[1]; e += ng16) {
+                const int fl = e < n[0] ? 0 : 1, r = fl ? e - n[0] : e, nf = nfl(fl);
+                const C GAS *row = aip(fl) + (size_t)r * nmax;
+                C x[16];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) x[j] = row[f + j];
+                for (int j = 0; j < 16; ++j)
+                    if (l16 + 16 * j < nf) x[j] = ldg(row + l16 + 16 * j);
+                zc t2[2] = {zmk(0, 0), zmk(0, 0)};
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) t4[j] = zadd(t4[j], zmul(toz(x[j]), lz(s.vc, fl * nmax + f + j)));
+                for (int j = 0; j < 16; ++j)
+                    if (l16 + 16 * j < nf) t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), lz(s.vc, fl * nmax + l16 + 16 * j)));
+                zc t = zadd(t2[0], t2[1]);
+                for (int o = 8; o > 0; o >>= 1) {
+                    t.x += __shfl_xor(t.x, o, 16);
+                    t.y += __shfl_xor(t.y, o, 16);
                 }
-                for (; f < n[fl]; ++f) t4[0] = zadd(t4[0], zmul(toz(row[f]), lz(s.vc, fl * nmax + f)));
-                sz(s.vy, fl * nmax + r, zadd(zadd(t4[0], t4[1]), zadd(t4[2], t4[3])));
+                if (l16 == 0) sz(s.vy, fl * nmax + r, t);
             }
             __syncthreads();
             const int nn0 = n_new(c, 0), nn1 = n_new(c, 1);
-            for (int e = tid; e < nn0 + nn1; e += nt) {
-                const int fl = e < nn0 ? 0 : 1, m = fl ? e - nn0 : e;
+            for (int e = g16; e < nn0 + nn1; e += ng16) {
+                const int fl = e < nn0 ? 0 : 1, m = fl ? e - nn0 : e, nf = nfl(fl);
                 const int o = new_orb(c, fl, m);
-                const C *trow = tab[fl] + (size_t)o * Ns;
-                zc t4[4] = {zmk(0, 0), zmk(0, 0), zmk(0, 0), zmk(0, 0)};
-                int r = 0;
-                for (; r + 4 <= n[fl]; r += 4) {
-                    C x[4];
+                const C GAS *trow = tbp(fl) + (size_t)o * Ns;
+                C x[16];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) x[j] = trow[s.part[fl * nmax + r + j]];
+                for (int j = 0; j < 16; ++j)
+                    if (l16 + 16 * j < nf) x[j] = ldg(trow + s.part[fl * nmax + l16 + 16 * j]);
+                zc t2[2] = {zmk(0, 0), zmk(0, 0)};
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) t4[j] = zadd(t4[j], zmul(toz(x[j]), lz(s.vy, fl * nmax + r + j)));
+                for (int j = 0; j < 16; ++j)
+                    if (l16 + 16 * j < nf) t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), lz(s.vy, fl * nmax + l16 + 16 * j)));
+                zc t = zadd(t2[0], t2[1]);
+                for (int o2 = 8; o2 > 0; o2 >>= 1) {
+                    t.x += __shfl_xor(t.x, o2, 16);
+                    t.y += __shfl_xor(t.y, o2, 16);
                 }
-                for (; r < n[fl]; ++r) t4[0] = zadd(t4[0], zmul(toz(trow[s.part[fl * nmax + r]]), lz(s.vy, fl * nmax + r)));
-                sz(s.kg, fl * nqm + m, zadd(zadd(t4[0], t4[1]), zadd(t4[2], t4[3])));
+                if (l16 == 0) sz(s.kg, fl * nqm + m, t);
             }
             __syncthreads();
         }
@@ -223,32 +280,38 @@ template <typename T> struct Walker {
             zc tot = zmk((double)hopsign(c, k), 0.0);
             for (int fl = 0; fl < 2; ++fl) {
                 const int R = hop_n(c, k, fl);
+                // the capacitance matrix padded to 3 x 3 with the identity (constant indices)
                 zc B[9];
-                zc d;
+                eye3(B);
                 if (!mv) {
-                    for (int m = 0; m < R; ++m)
-                        for (int q = 0; q < R; ++q) B[m * R + q] = lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q);
-                    d = det_small(R, B);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (m < R && q < R) B[m * 3 + q] = lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q);
                 } else {
                     const int r = fl ? cj : ci;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
-                    const int D = R + 1;
                     B[0] = lz(s.vy, fl * nmax + r);
-                    for (int q = 0; q < R; ++q)
-                        B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, k, fl, q)]);
-                    for (int m = 0; m < R; ++m) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (q < R) B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, k, fl, q)]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        if (m >= R) continue;
                         const int o2 = hop_to(c, k, fl, m), o1 = hop_from(c, k, fl, m);
                         const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
                         const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
-                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, k, fl, m)),
+                        B[(1 + m) * 3] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, k, fl, m)),
                                                    zmul(Dm, B[0])), p2s);
-                        for (int q = 0; q < R; ++q)
-                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q),
-                                                          zmul(Dm, B[1 + q]));
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (q < R)
+                                B[(1 + m) * 3 + 1 + q] = zadd(lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q),
+                                                              zmul(Dm, B[1 + q]));
                     }
-                    d = det_small(D, B);
                 }
-                tot = zmul(tot, d);
+                tot = zmul(tot, det3(B));
             }
             sz(s.alpha, k, tot);
         }
@@ -256,7 +319,7 @@ template <typename T> struct Walker {
     }
 
     // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element)
-    __device__ void reduce_alpha(double &sum, int &kmax)
+    __device__ __forceinline__ void reduce_alpha(double &sum, int &kmax)
     {
         double ls = 0.0, bv = -1.0;
         int bk = nk;
@@ -296,7 +359,7 @@ template <typename T> struct Walker {
     //   over by the emptied orbital o_m, whose row before the move is e_{f_m} (it was
     //   occupied): kappa_0 = eps y[r], kappa_m = delta_{m,ms} + eps Ai[r][f_m], delta = eps =
     //   Phi(o_m, s) - Phi(o_m, site r).  Without a lattice move the delta terms vanish.
-    __device__ void apply(bool mv, int ci, int sa, int sb, int cj, int kn)
+    __device__ __forceinline__ void apply(bool mv, int ci, int sa, int sb, int cj, int kn)
     {
         const int c = cstate();
         __shared__ zc s_binv[9];
@@ -306,14 +369,16 @@ template <typename T> struct Walker {
             const int r = fl ? cj : ci;
             const int sold = fl ? sb : sa, snew = fl ? sa : sb;
             const int m0 = mv ? 1 : 0, D = R + m0;
-            const int nf = n[fl], nqf = nq[fl];
+            const int nf = nfl(fl), nqf = nql(fl);
             if (D == 0) {
                 if (tid == 0) s_det[fl] = zmk(1.0, 0.0);
                 continue;
             }
             int fmv[2] = {0, 0}, o1v[2] = {0, 0}, o2v[2] = {0, 0};
             zc Dmv[2] = {zmk(0, 0), zmk(0, 0)};
-            for (int m = 0; m < R; ++m) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                if (m >= R) continue;
                 o1v[m] = hop_from(c, kn, fl, m);
                 o2v[m] = hop_to(c, kn, fl, m);
                 fmv[m] = s.slot[fl * Ns + o1v[m]];
@@ -324,7 +389,9 @@ template <typename T> struct Walker {
             // V rows from the old inverse and Q rows
             for (int f = tid; f < nf; f += nt) {
                 if (mv) sz(s.V, f, ai(fl, r, f));
-                for (int m = 0; m < R; ++m) {
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    if (m >= R) continue;
                     zc v = qv(fl, s.qrow[fl * Ns + o2v[m]], f);
                     if (mv) v = zadd(v, zmul(Dmv[m], ai(fl, r, f)));
                     if (f == fmv[m]) v.x -= 1.0;
@@ -351,21 +418,27 @@ template <typename T> struct Walker {
                         if (e == r) y.x -= 1.0;
                         k3[0] = y;
                     }
-                    for (int m = 0; m < R; ++m) k3[m0 + m] = ai(fl, e, fmv[m]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                        if (m < R) k3[m0 + m] = ai(fl, e, fmv[m]);
                 } else {
                     const int q = e - nf, o = s.qorb[fl * nqm + q];
                     zc dd = zmk(0, 0);
                     int rp = -1;
                     if (o >= 0) {
                         int ms = -1;
-                        for (int m = 0; m < R; ++m)
-                            if (o2v[m] == o) ms = m;
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+                            if (m < R && o2v[m] == o) ms = m;
                         if (ms >= 0) {
-                            rp = fmv[ms];
-                            const zc eps = mv ? zsub(phi(fl, o1v[ms], snew), phi(fl, o1v[ms], sold)) : zmk(0, 0);
+                            rp = ms ? fmv[1] : fmv[0];
+                            const int o1s = ms ? o1v[1] : o1v[0];
+                            const zc eps = mv ? zsub(phi(fl, o1s, snew), phi(fl, o1s, sold)) : zmk(0, 0);
                             dd = eps;
                             if (mv) k3[0] = zmul(eps, yr);
-                            for (int m = 0; m < R; ++m) {
+#pragma unroll
+                            for (int m = 0; m < 2; ++m) {
+                                if (m >= R) continue;
                                 zc kv = zmk(m == ms ? 1.0 : 0.0, 0.0);
                                 if (mv) kv = zadd(kv, zmul(eps, ai(fl, r, fmv[m])));
                                 k3[m0 + m] = kv;
@@ -382,7 +455,9 @@ template <typename T> struct Walker {
                                 dd = dq;
                                 k3[0] = zadd(zsub(kg, pn), zmul(dq, yr));
                             }
-                            for (int m = 0; m < R; ++m) {
+#pragma unroll
+                            for (int m = 0; m < 2; ++m) {
+                                if (m >= R) continue;
                                 zc kv = qv(fl, q, fmv[m]);
                                 if (mv) kv = zadd(kv, zmul(dd, ai(fl, r, fmv[m])));
                                 k3[m0 + m] = kv;
@@ -392,84 +467,147 @@ template <typename T> struct Walker {
                     sz(s.dlt, q, dd);
                     s.rep[q] = rp;
                 }
+#pragma unroll
                 for (int q3 = 0; q3 < 3; ++q3) sz(s.kap, e * 3 + q3, k3[q3]);
             }
             // capacitance matrix of state kn, rebuilt from the same inputs as eval
+            // (padded to 3 x 3 with the identity: its inverse's leading D x D block is B^-1)
             if (tid == 0) {
                 zc B[9];
+                eye3(B);
                 if (!mv) {
-                    for (int m = 0; m < R; ++m)
-                        for (int q = 0; q < R; ++q) B[m * R + q] = lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (m < R && q < R) B[m * 3 + q] = lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q);
                 } else {
                     B[0] = yr;
-                    for (int q = 0; q < R; ++q) B[1 + q] = ai(fl, r, fmv[q]);
-                    for (int m = 0; m < R; ++m) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (q < R) B[1 + q] = ai(fl, r, fmv[q]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        if (m >= R) continue;
                         const zc p2s = phi(fl, o2v[m], snew);
-                        B[(1 + m) * D] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, kn, fl, m)), zmul(Dmv[m], B[0])), p2s);
-                        for (int q = 0; q < R; ++q)
-                            B[(1 + m) * D + 1 + q] = zadd(lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q), zmul(Dmv[m], B[1 + q]));
+                        B[(1 + m) * 3] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, kn, fl, m)), zmul(Dmv[m], B[0])), p2s);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (q < R)
+                                B[(1 + m) * 3 + 1 + q] = zadd(lz(s.ra, ((kn * 2 + fl) * 2 + m) * 2 + q), zmul(Dmv[m], B[1 + q]));
                     }
                 }
-                s_det[fl] = det_small(D, B);
-                inv_small(D, B, s_binv);
+                s_det[fl] = det3(B);
+                inv3(B, s_binv);
             }
             __syncthreads();
             // W = B^-1 V in place; V_0 = Ai[r, :] is kept in this flavour's c vector, which
             // the coefficients above were the last to read
             for (int f = tid; f < nf; f += nt) {
                 zc vv[3] = {zmk(0, 0), zmk(0, 0), zmk(0, 0)};
-                for (int b2 = 0; b2 < D; ++b2) vv[b2] = lz(s.V, b2 * nmax + f);
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2)
+                    if (b2 < D) vv[b2] = lz(s.V, b2 * nmax + f);
                 if (mv) sz(s.vc, fl * nmax + f, vv[0]);
-                for (int a2 = 0; a2 < D; ++a2) {
+#pragma unroll
+                for (int a2 = 0; a2 < 3; ++a2) {
+                    if (a2 >= D) continue;
                     zc wv = zmk(0, 0);
-                    for (int b2 = 0; b2 < D; ++b2) wv = zadd(wv, zmul(s_binv[a2 * D + b2], vv[b2]));
+#pragma unroll
+                    for (int b2 = 0; b2 < 3; ++b2)
+                        if (b2 < D) wv = zadd(wv, zmul(s_binv[a2 * 3 + b2], vv[b2]));
                     sz(s.V, a2 * nmax + f, wv);
                 }
             }
             __syncthreads();
-            // rows of the inverse, then the live Q rows; eight elements per thread per pass,
-            // their loads issued before any store
+            // rows of the inverse, then the live Q rows: a wave per row, lane l holding columns
+            // f = min(l + 64 j, nf - 1) (nmax <= 256; lanes past the last column repeat it and
+            // store the same value), W and V_0 kept in registers across the wave's rows,
+            // arithmetic in the storage precision.  The loop body has no branch around a load
+            // or a store, so that the next batch's loads stay in flight during this batch's
+            // arithmetic and stores: a row without data (a Q row without orbital, or past the
+            // last row) reads and writes the walker's measurement buffer, which every
+            // measurement rewrites before reading it.
             {
-                const int tot = (nf + nqf) * nf;
-                for (int e0 = tid; e0 < tot; e0 += 8 * nt) {
-                    C *ep[8];
-                    C ov[8];
-                    int rw[8], fv[8];
+                const int lane = tid & 63, nw = nt >> 6, rows = nf + nqf;
+                int fc[4];
+                C wr[3][4], v0[4];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int e = e0 + j * nt;
-                        ep[j] = nullptr;
-                        rw[j] = 0;
-                        fv[j] = 0;
-                        if (e < tot) {
-                            const int row = e / nf, f = e - row * nf;
-                            rw[j] = row;
-                            fv[j] = f;
-                            if (row < nf) ep[j] = Ai[fl] + (size_t)row * nmax + f;
-                            else if (s.qorb[fl * nqm + row - nf] >= 0) ep[j] = Qr[fl] + (size_t)(row - nf) * nmax + f;
-                        }
-                        if (ep[j]) ov[j] = *ep[j];
-                    }
+                for (int j = 0; j < 4; ++j) {
+                    const int f = lane + 64 * j < nf ? lane + 64 * j : nf - 1;
+                    fc[j] = f;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (!ep[j]) continue;
-                        const int row = rw[j], f = fv[j];
-                        zc acc = zmk(0, 0);
-                        for (int a2 = 0; a2 < D; ++a2) acc = zadd(acc, zmul(lz(s.kap, row * 3 + a2), lz(s.V, a2 * nmax + f)));
-                        zc nv;
-                        if (row < nf) {
-                            nv = zsub(toz(ov[j]), acc);
-                        } else {
-                            const int q = row - nf, rp = s.rep[q];
-                            nv = rp >= 0 ? zmk(f == rp ? 1.0 : 0.0, 0.0) : toz(ov[j]);
-                            if (mv) nv = zadd(nv, zmul(lz(s.dlt, q), lz(s.vc, fl * nmax + f)));
-                            nv = zsub(nv, acc);
-                        }
-                        C v;
-                        v.x = (T)nv.x;
-                        v.y = (T)nv.y;
-                        *ep[j] = v;
+                    for (int a2 = 0; a2 < 3; ++a2) {
+                        wr[a2][j].x = a2 < D ? (T)s.V[2 * (a2 * nmax + f)] : (T)0;
+                        wr[a2][j].y = a2 < D ? (T)s.V[2 * (a2 * nmax + f) + 1] : (T)0;
                     }
+                    v0[j].x = mv ? (T)s.vc[2 * (fl * nmax + f)] : (T)0;
+                    v0[j].y = mv ? (T)s.vc[2 * (fl * nmax + f) + 1] : (T)0;
+                }
+                C GAS *dump = gptr(reinterpret_cast<C *>(a.Y)) + ((size_t)w * 2 + fl) * (nmax + nqm) * nmax;
+                // batches of RB rows per wave (row = r0 + h nw), software-pipelined: the next
+                // batch's loads are issued before this batch's arithmetic and stores
+                constexpr int RB = 4;
+                const int wid = __builtin_amdgcn_readfirstlane(tid >> 6), step = RB * nw;
+                auto row_ptr = [&](int row) -> C GAS * {
+                    if (row < nf) return aip(fl) + (size_t)row * nmax;
+                    if (row < rows && s.qorb[fl * nqm + row - nf] >= 0) return qrp(fl) + (size_t)(row - nf) * nmax;
+                    return dump + (size_t)(row % (nmax + nqm)) * nmax;
+                };
+                auto issue = [&](int r0, C GAS *(&p)[RB], C (&v)[RB][4]) {
+#pragma unroll
+                    for (int h = 0; h < RB; ++h) {
+                        p[h] = row_ptr(r0 + h * nw);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) v[h][j] = ldg(p[h] + fc[j]);
+                    }
+                };
+                auto finish = [&](int r0, C GAS *(&p)[RB], C (&v)[RB][4]) {
+#pragma unroll
+                    for (int h = 0; h < RB; ++h) {
+                        const int row = r0 + h * nw;
+                        const bool qr = row >= nf;
+                        const int qi = qr ? (row - nf < nqm ? row - nf : nqm - 1) : 0;
+                        const int kr = row < nf + nqm ? row : nf + nqm - 1;
+                        C kp[3];
+#pragma unroll
+                        for (int a2 = 0; a2 < 3; ++a2) {
+                            kp[a2].x = (T)s.kap[2 * (kr * 3 + a2)];
+                            kp[a2].y = (T)s.kap[2 * (kr * 3 + a2) + 1];
+                        }
+                        const int rpl = qr ? s.rep[qi] : -1;
+                        C dl;
+                        dl.x = qr && mv ? (T)s.dlt[2 * qi] : (T)0;
+                        dl.y = qr && mv ? (T)s.dlt[2 * qi + 1] : (T)0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            T ax = (T)0, ay = (T)0;
+#pragma unroll
+                            for (int a2 = 0; a2 < 3; ++a2) {
+                                ax += kp[a2].x * wr[a2][j].x - kp[a2].y * wr[a2][j].y;
+                                ay += kp[a2].x * wr[a2][j].y + kp[a2].y * wr[a2][j].x;
+                            }
+                            C nv = v[h][j];
+                            if (rpl >= 0) {
+                                nv.x = fc[j] == rpl ? (T)1 : (T)0;
+                                nv.y = (T)0;
+                            }
+                            nv.x += dl.x * v0[j].x - dl.y * v0[j].y;
+                            nv.y += dl.x * v0[j].y + dl.y * v0[j].x;
+                            nv.x -= ax;
+                            nv.y -= ay;
+                            stg(p[h] + fc[j], nv);
+                        }
+                    }
+                };
+                C GAS *pa[RB], *pb[RB];
+                C va[RB][4], vb[RB][4];
+                issue(wid, pa, va);
+                for (int r0 = wid; r0 < rows; r0 += 2 * step) {
+                    issue(r0 + step, pb, vb);
+                    finish(r0, pa, va);
+                    issue(r0 + 2 * step, pa, va);
+                    finish(r0 + step, pb, vb);
                 }
             }
             __syncthreads();
@@ -478,8 +616,8 @@ template <typename T> struct Walker {
         if (tid == 0) {
             for (int fl = 0; fl < 2; ++fl) {
                 const zc d = s_det[fl];
-                a.logabs[2 * w + fl] += 0.5 * log(znorm(d));
-                a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
+                gptr(a.logabs)[2 * w + fl] += 0.5 * log(znorm(d));
+                gptr(a.phase)[2 * w + fl] = remainder(gptr(a.phase)[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
             }
             for (int fl = 0; fl < 2; ++fl) {
                 const int R = hop_n(c, kn, fl);
@@ -501,7 +639,7 @@ template <typename T> struct Walker {
                 s.part[ci] = sb;
                 s.part[nmax + cj] = sa;
             }
-            a.cstate[w] = kn;
+            gptr(a.cstate)[w] = kn;
         }
         __syncthreads();
     }
@@ -515,31 +653,31 @@ template <typename T>
 __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
 {
     extern __shared__ __align__(16) char smem[];
-    const int w = a.order ? a.order[blockIdx.x] : (int)blockIdx.x;
+    const int w = a.order ? gptr(a.order)[blockIdx.x] : (int)blockIdx.x;
     Walker<T> wk(a, smem, w);
     const int tid = wk.tid, nt = wk.nt, Ns = wk.Ns, nmax = wk.nmax, nk = wk.nk;
     auto &s = wk.s;
     const int nqm = wk.nqm;
     for (int i = tid; i < Ns; i += nt) {
-        s.site[i] = a.site[(size_t)w * Ns + i];
+        s.site[i] = gptr(a.site)[(size_t)w * Ns + i];
         s.slot[i] = -1;
         s.slot[Ns + i] = -1;
         s.qrow[i] = -1;
         s.qrow[Ns + i] = -1;
     }
     for (int i = tid; i < 2 * nmax; i += nt) {
-        s.part[i] = a.part[(size_t)w * 2 * nmax + i];
-        s.cur[i] = a.cur[(size_t)w * 2 * nmax + i];
+        s.part[i] = gptr(a.part)[(size_t)w * 2 * nmax + i];
+        s.cur[i] = gptr(a.cur)[(size_t)w * 2 * nmax + i];
     }
-    for (int i = tid; i < 2 * nqm; i += nt) s.qorb[i] = a.qorb[(size_t)w * 2 * nqm + i];
+    for (int i = tid; i < 2 * nqm; i += nt) s.qorb[i] = gptr(a.qorb)[(size_t)w * 2 * nqm + i];
     __syncthreads();
     for (int i = tid; i < 2 * nmax; i += nt) {
         const int fl = i / nmax, f = i % nmax;
-        if (f < wk.n[fl]) s.slot[fl * Ns + s.cur[i]] = f;
+        if (f < wk.nfl(fl)) s.slot[fl * Ns + s.cur[i]] = f;
     }
     for (int i = tid; i < 2 * nqm; i += nt) {
         const int fl = i / nqm, q = i % nqm;
-        if (q < wk.nq[fl] && s.qorb[i] >= 0) s.qrow[fl * Ns + s.qorb[i]] = q;
+        if (q < wk.nql(fl) && s.qorb[i] >= 0) s.qrow[fl * Ns + s.qorb[i]] = q;
     }
     __syncthreads();
     wk.build_ra();
@@ -557,7 +695,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     }
     int64_t n_noop = 0, n_acc = 0, n_rej = 0;
     double sacc[6] = {0, 0, 0, 0, 0, 0};  // [0] n_meas, [1] sum ln w
-    const uint64_t ctr0 = a.counter[w];
+    const uint64_t ctr0 = gptr(a.counter)[w];
     int64_t mslot = 0;
     auto measure = [&](bool acc_on) {
         // ProjHeis::measure on the channel (ProjHeis.cpp:40-154): h_k = sum over bonds of
@@ -573,11 +711,11 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         using CT = typename cpx<T>::t;
         const int rows_max = nmax + nqm;
         // the chunk of Phi columns lives where apply keeps its coefficients
-        double *cbuf = s.kap;
+        double LAS *cbuf = s.kap;
         const int CH = (6 * nmax + 4 * nqm) / nmax < 8 ? (6 * nmax + 4 * nqm) / nmax : 8;
         for (int fl = 0; fl < 2; ++fl) {
-            const int nf = wk.n[fl], no = wk.n[1 - fl], rows = nf + wk.nq[fl];
-            CT *Yf = reinterpret_cast<CT *>(a.Y) + ((size_t)w * 2 + fl) * rows_max * nmax;
+            const int nf = wk.nfl(fl), no = wk.nfl(1 - fl), rows = nf + wk.nql(fl);
+            CT GAS *Yf = gptr(reinterpret_cast<CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
             for (int t0 = 0; t0 < no; t0 += CH) {
                 __syncthreads();
                 for (int e = tid; e < nf * CH; e += nt) {
@@ -588,10 +726,10 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                 for (int r = tid; r < rows; r += nt) {
                     zc acc[8];
                     for (int j = 0; j < 8; ++j) acc[j] = zmk(0, 0);
-                    const CT *rowp = r < nf ? wk.Ai[fl] + (size_t)r * nmax : wk.Qr[fl] + (size_t)(r - nf) * nmax;
+                    const CT GAS *rowp = r < nf ? wk.aip(fl) + (size_t)r * nmax : wk.qrp(fl) + (size_t)(r - nf) * nmax;
                     int f = 0;
                     for (; f + 2 <= nf; f += 2) {
-                        const CT x0 = rowp[f], x1 = rowp[f + 1];
+                        const CT x0 = ldg(rowp + f), x1 = ldg(rowp + f + 1);
                         const zc m0 = toz(x0), m1 = toz(x1);
                         for (int j = 0; j < 8; ++j)
                             if (j < CH) {
@@ -600,7 +738,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                             }
                     }
                     for (; f < nf; ++f) {
-                        const zc m = toz(rowp[f]);
+                        const zc m = toz(ldg(rowp + f));
                         for (int j = 0; j < 8; ++j)
                             if (j < CH) acc[j] = zadd(acc[j], zmul(m, wk.lz(cbuf, f * CH + j)));
                     }
@@ -609,7 +747,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                             CT v;
                             v.x = (T)acc[j].x;
                             v.y = (T)acc[j].y;
-                            Yf[(size_t)r * nmax + t0 + j] = v;
+                            stg(Yf + (size_t)r * nmax + t0 + j, v);
                         }
                 }
             }
@@ -620,7 +758,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
             const zc a0 = wk.lz(s.a0, k);
             zc h = zmk(0, 0);
             for (int e = 0; e < 2 * Ns; ++e) {
-                const int4 eg = a.edge[e];
+                const int4 eg = ldg4(gptr(a.edge) + e);
                 const int vi = eg.x, vj = eg.z;
                 const int ii = s.site[vi], ij = s.site[vj];
                 if ((ii >> 16) == (ij >> 16)) {
@@ -632,58 +770,67 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                 else { ci = ij & 0xFFFF; sa = vj; cj = ii & 0xFFFF; sb = vi; }
                 zc tot = zmk((double)wk.hopsign(c, k), 0.0);
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = wk.hop_n(c, k, fl), nf = wk.n[fl];
+                    const int R = wk.hop_n(c, k, fl), nf = wk.nfl(fl);
                     const int r = fl ? cj : ci, t = fl ? ci : cj;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
-                    const CT *Yf = reinterpret_cast<const CT *>(a.Y) + ((size_t)w * 2 + fl) * rows_max * nmax;
-                    const int D = R + 1;
-                    zc B[9];
-                    B[0] = toz(Yf[(size_t)r * nmax + t]);
-                    for (int q = 0; q < R; ++q) B[1 + q] = wk.ai(fl, r, s.slot[fl * Ns + wk.hop_from(c, k, fl, q)]);
-                    for (int m = 0; m < R; ++m) {
+                    const CT GAS *Yf = gptr(reinterpret_cast<const CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
+                    zc B[9];  // padded to 3 x 3 with the identity
+                    eye3(B);
+                    B[0] = toz(ldg(Yf + (size_t)r * nmax + t));
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (q < R) B[1 + q] = wk.ai(fl, r, s.slot[fl * Ns + wk.hop_from(c, k, fl, q)]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        if (m >= R) continue;
                         const int o2 = wk.hop_to(c, k, fl, m), o1 = wk.hop_from(c, k, fl, m);
                         const zc p2s = wk.phi(fl, o2, snew), p2r = wk.phi(fl, o2, sold);
                         const zc Dm = zadd(zsub(zsub(p2s, p2r), wk.phi(fl, o1, snew)), wk.phi(fl, o1, sold));
                         const int qr = s.qrow[fl * Ns + o2];
-                        const zc kgv = qr >= 0 ? toz(Yf[(size_t)(nf + qr) * nmax + t]) : zmk(0, 0);
-                        B[(1 + m) * D] = zsub(zadd(kgv, zmul(Dm, B[0])), p2s);
-                        for (int q = 0; q < R; ++q)
-                            B[(1 + m) * D + 1 + q] = zadd(wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
+                        const zc kgv = qr >= 0 ? toz(ldg(Yf + (size_t)(nf + qr) * nmax + t)) : zmk(0, 0);
+                        B[(1 + m) * 3] = zsub(zadd(kgv, zmul(Dm, B[0])), p2s);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (q < R)
+                                B[(1 + m) * 3 + 1 + q] = zadd(wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
                     }
-                    tot = zmul(tot, det_small(D, B));
+                    tot = zmul(tot, det3(B));
                 }
                 h = zmk(h.x - 0.25 * a0.x - 0.5 * tot.x, h.y - 0.25 * a0.y - 0.5 * tot.y);
             }
             wk.sz(s.hk, k, h);
         }
         __syncthreads();
-        const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
+        const double lw = 2.0 * (gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1]) + log(S0);
         if (acc_on) {
-            double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
+            double2 GAS *HO = gptr(reinterpret_cast<double2 *>(a.ho)) + (size_t)w * 2 * nk * nk;
             for (int e = tid; e < nk * nk; e += nt) {
                 const int k1 = e / nk, k2 = e % nk;
                 const zc a1 = wk.lz(s.a0, k1);
                 const zc h2 = wk.lz(s.hk, k2), a2 = wk.lz(s.a0, k2);
                 const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
-                HO[e].x += hv.x / S0;
-                HO[e].y += hv.y / S0;
-                HO[(size_t)nk * nk + e].x += ov.x / S0;
-                HO[(size_t)nk * nk + e].y += ov.y / S0;
+                double2 hs = ldg(HO + e), os = ldg(HO + (size_t)nk * nk + e);
+                hs.x += hv.x / S0;
+                hs.y += hv.y / S0;
+                os.x += ov.x / S0;
+                os.y += ov.y / S0;
+                stg(HO + e, hs);
+                stg(HO + (size_t)nk * nk + e, os);
             }
             if (tid == 0) { sacc[0] += 1; sacc[1] += lw; }
         }
-        if (tid == 0 && a.tr_logw) a.tr_logw[(size_t)w * a.n_meas_slots + mslot] = lw;
+        if (tid == 0 && a.tr_logw) gptr(a.tr_logw)[(size_t)w * a.n_meas_slots + mslot] = lw;
         ++mslot;
         __syncthreads();
     };
     if (a.mode == 4) {
         // amplitudes of every state at the current configuration, as ln|a_k| and arg a_k
-        const double la = a.logabs[2 * w] + a.logabs[2 * w + 1], ph = a.phase[2 * w] + a.phase[2 * w + 1];
+        const double la = gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1], ph = gptr(a.phase)[2 * w] + gptr(a.phase)[2 * w + 1];
         for (int k = tid; k < nk; k += nt) {
             const zc al = wk.lz(s.alpha, k);
             const double nrm = znorm(al);
-            a.amp_out[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
-            a.amp_out[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
+            gptr(a.amp_out)[((size_t)w * nk + k) * 2] = nrm > 0.0 ? la + 0.5 * log(nrm) : -INFINITY;
+            gptr(a.amp_out)[((size_t)w * nk + k) * 2 + 1] = remainder(ph + atan2(al.y, al.x), 2 * M_PI);
         }
     } else if (a.mode == 1) {
         measure(false);
@@ -693,7 +840,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
             int v1, ed;
             double u3;
             if (a.uniforms) {
-                const double *up = a.uniforms + ((size_t)w * a.n_steps + st) * 4;
+                const double GAS *up = gptr(a.uniforms) + ((size_t)w * a.n_steps + st) * 4;
                 v1 = (int)(Ns * up[0]);
                 v1 = v1 < Ns ? v1 : Ns - 1;
                 ed = (int)(4 * up[1]);
@@ -707,7 +854,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                 ed = (int)(cc[1] >> 30);
                 u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
             }
-            const int v2 = a.nbr[v1 * 4 + ed];
+            const int v2 = gptr(a.nbr)[v1 * 4 + ed];
             const int i1 = s.site[v1], i2 = s.site[v2];
             const bool noop = (i1 >> 16) == (i2 >> 16);
             double ratio = 1.0;
@@ -737,26 +884,26 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
             }
             if (tid == 0) {
                 const size_t ti = (size_t)w * a.n_steps + st;
-                if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
-                if (a.tr_accept) a.tr_accept[ti] = acc ? 1 : 0;
-                if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+                if (a.tr_noop) gptr(a.tr_noop)[ti] = noop ? 1 : 0;
+                if (a.tr_accept) gptr(a.tr_accept)[ti] = acc ? 1 : 0;
+                if (a.tr_ratio) gptr(a.tr_ratio)[ti] = ratio;
             }
             if (a.meas > 0 && (a.meas == 1 || st % a.meas == 0)) measure(true);
         }
     }
     // write back
     __syncthreads();
-    for (int i = tid; i < Ns; i += nt) a.site[(size_t)w * Ns + i] = s.site[i];
+    for (int i = tid; i < Ns; i += nt) gptr(a.site)[(size_t)w * Ns + i] = s.site[i];
     for (int i = tid; i < 2 * nmax; i += nt) {
-        a.part[(size_t)w * 2 * nmax + i] = s.part[i];
-        a.cur[(size_t)w * 2 * nmax + i] = s.cur[i];
+        gptr(a.part)[(size_t)w * 2 * nmax + i] = s.part[i];
+        gptr(a.cur)[(size_t)w * 2 * nmax + i] = s.cur[i];
     }
-    for (int i = tid; i < 2 * nqm; i += nt) a.qorb[(size_t)w * 2 * nqm + i] = s.qorb[i];
+    for (int i = tid; i < 2 * nqm; i += nt) gptr(a.qorb)[(size_t)w * 2 * nqm + i] = s.qorb[i];
     if (tid == 0) {
-        a.weight[w] = S;
+        gptr(a.weight)[w] = S;
         if (a.mode == 0) {
-            a.counter[w] = ctr0 + (uint64_t)a.n_steps;
-            double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
+            gptr(a.counter)[w] = ctr0 + (uint64_t)a.n_steps;
+            double GAS *ac = gptr(a.acc) + (size_t)w * gv::ACC_STRIDE;
             ac[3] += sacc[0];
             ac[7] += (double)a.n_steps;
             ac[8] += (double)n_acc;
@@ -923,6 +1070,7 @@ extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks,
     const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
     const int nqm = a->nq0 > a->nq1 ? (a->nq0 > 0 ? a->nq0 : 1) : (a->nq1 > 0 ? a->nq1 : 1);
     if (!a->Q || !a->qorb || !a->Y) return hipErrorInvalidValue;
+    if (nmax > 256) return hipErrorInvalidValue;  // the update loop holds four column chunks of 64 per lane
     const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk, nqm) + 256;
     if (lds > 160 * 1024 - 4096) return hipErrorInvalidValue;
     if (prec == 0)
