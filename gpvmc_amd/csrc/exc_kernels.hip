@@ -98,13 +98,21 @@ struct Lds {
     int LAS *ired;      // [16]
 };
 
+// complex elements of the apply coefficients' area, which the measurement reuses for its
+// bond list (two ints per bond, 2 Ns bonds)
+__host__ __device__ inline size_t kap_elems(int Ns, int nmax, int nqm)
+{
+    const size_t k = ((size_t)nmax + nqm) * 3;
+    return k > (size_t)Ns ? k : (size_t)Ns;
+}
+
 __host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
 {
     size_t b = 0;
     b += ((size_t)Ns + 2 * (size_t)nmax * 2 + 2 * (size_t)Ns * 2 + 2 * (size_t)nqm + 2 * (size_t)nqm) * 4;
     b = (b + 15) & ~(size_t)15;
     b += 16 * ((size_t)2 * nmax * 2 + (size_t)2 * nqm + (size_t)nk * 8 + nk + nk + nk +
-               ((size_t)nmax + nqm) * 3 + nqm + (size_t)3 * nmax);
+               kap_elems(Ns, nmax, nqm) + nqm + (size_t)3 * nmax);
     b += 16 * 8 + 16 * 4;
     return b;
 }
@@ -130,7 +138,7 @@ __device__ inline Lds lds_carve(char LAS *base, int Ns, int nmax, int nk, int nq
     s.alpha = dp; dp += 2 * nk;
     s.a0 = dp; dp += 2 * nk;
     s.hk = dp; dp += 2 * nk;
-    s.kap = dp; dp += 2 * ((nmax + nqm) * 3);
+    s.kap = dp; dp += 2 * kap_elems(Ns, nmax, nqm);
     s.dlt = dp; dp += 2 * nqm;
     s.V = dp; dp += 2 * (3 * nmax);
     s.red = dp; dp += 16;
@@ -231,48 +239,85 @@ template <typename T> struct Walker {
             }
             __syncthreads();
             // y = Ai c and KG = rho . y: sixteen lanes per row (f or r = l16 + 16 j, j < 16
-            // as nmax <= 256, a row's sixteen consecutive elements in one cache line), every
-            // load of a lane's share of the row issued before the first product, the partial
-            // sums reduced across the sixteen lanes
+            // as nmax <= 256, a row's sixteen consecutive elements in one cache line), the
+            // partial sums reduced across the sixteen lanes.  Loads are unconditional (lanes
+            // past the row's end repeat its last element, which their product masks out) and
+            // the next row's loads are issued before this row's products, so that two rows
+            // per group stay in flight.
             const int l16 = tid & 15, g16 = tid >> 4, ng16 = nt >> 4;
-            for (int e = g16; e < n[0] + n[1]; e += ng16) {
-                const int fl = e < n[0] ? 0 : 1, r = fl ? e - n[0] : e, nf = nfl(fl);
-                const C GAS *row = aip(fl) + (size_t)r * nmax;
-                C x[16];
+            {
+                const int rows = n[0] + n[1];
+                auto ld = [&](int e, C (&x)[16]) {
+                    const int ec = e < rows ? e : rows - 1;
+                    const int fl = ec < n[0] ? 0 : 1, r = fl ? ec - n[0] : ec, nf = nfl(fl);
+                    const C GAS *row = aip(fl) + (size_t)r * nmax;
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (l16 + 16 * j < nf) x[j] = ldg(row + l16 + 16 * j);
-                zc t2[2] = {zmk(0, 0), zmk(0, 0)};
+                    for (int j = 0; j < 16; ++j) x[j] = ldg(row + (l16 + 16 * j < nf ? l16 + 16 * j : nf - 1));
+                };
+                auto fin = [&](int e, const C (&x)[16]) {
+                    const int ec = e < rows ? e : rows - 1;
+                    const int fl = ec < n[0] ? 0 : 1, r = fl ? ec - n[0] : ec, nf = nfl(fl);
+                    zc t2[2] = {zmk(0, 0), zmk(0, 0)};
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (l16 + 16 * j < nf) t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), lz(s.vc, fl * nmax + l16 + 16 * j)));
-                zc t = zadd(t2[0], t2[1]);
-                for (int o = 8; o > 0; o >>= 1) {
-                    t.x += __shfl_xor(t.x, o, 16);
-                    t.y += __shfl_xor(t.y, o, 16);
+                    for (int j = 0; j < 16; ++j) {
+                        const int f = l16 + 16 * j < nf ? l16 + 16 * j : nf - 1;
+                        const zc cv = l16 + 16 * j < nf ? lz(s.vc, fl * nmax + f) : zmk(0, 0);
+                        t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), cv));
+                    }
+                    zc t = zadd(t2[0], t2[1]);
+                    for (int o = 8; o > 0; o >>= 1) {
+                        t.x += __shfl_xor(t.x, o, 16);
+                        t.y += __shfl_xor(t.y, o, 16);
+                    }
+                    if (l16 == 0 && e < rows) sz(s.vy, fl * nmax + r, t);
+                };
+                C xa[16], xb[16];
+                ld(g16, xa);
+                for (int e = g16; e < rows; e += 2 * ng16) {
+                    ld(e + ng16, xb);
+                    fin(e, xa);
+                    ld(e + 2 * ng16, xa);
+                    fin(e + ng16, xb);
                 }
-                if (l16 == 0) sz(s.vy, fl * nmax + r, t);
             }
             __syncthreads();
-            const int nn0 = n_new(c, 0), nn1 = n_new(c, 1);
-            for (int e = g16; e < nn0 + nn1; e += ng16) {
-                const int fl = e < nn0 ? 0 : 1, m = fl ? e - nn0 : e, nf = nfl(fl);
-                const int o = new_orb(c, fl, m);
-                const C GAS *trow = tbp(fl) + (size_t)o * Ns;
-                C x[16];
+            {
+                const int nn0 = n_new(c, 0), nn1 = n_new(c, 1), rows = nn0 + nn1;
+                auto ld = [&](int e, C (&x)[16]) {
+                    const int ec = e < rows ? (e > 0 ? e : 0) : (rows > 0 ? rows - 1 : 0);
+                    const int fl = ec < nn0 ? 0 : 1, m = fl ? ec - nn0 : ec, nf = nfl(fl);
+                    const C GAS *trow = tbp(fl) + (size_t)(rows > 0 ? new_orb(c, fl, m) : 0) * Ns;
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (l16 + 16 * j < nf) x[j] = ldg(trow + s.part[fl * nmax + l16 + 16 * j]);
-                zc t2[2] = {zmk(0, 0), zmk(0, 0)};
+                    for (int j = 0; j < 16; ++j)
+                        x[j] = ldg(trow + s.part[fl * nmax + (l16 + 16 * j < nf ? l16 + 16 * j : nf - 1)]);
+                };
+                auto fin = [&](int e, const C (&x)[16]) {
+                    const int ec = e < rows ? e : (rows > 0 ? rows - 1 : 0);
+                    const int fl = ec < nn0 ? 0 : 1, m = fl ? ec - nn0 : ec, nf = nfl(fl);
+                    zc t2[2] = {zmk(0, 0), zmk(0, 0)};
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (l16 + 16 * j < nf) t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), lz(s.vy, fl * nmax + l16 + 16 * j)));
-                zc t = zadd(t2[0], t2[1]);
-                for (int o2 = 8; o2 > 0; o2 >>= 1) {
-                    t.x += __shfl_xor(t.x, o2, 16);
-                    t.y += __shfl_xor(t.y, o2, 16);
+                    for (int j = 0; j < 16; ++j) {
+                        const int r = l16 + 16 * j < nf ? l16 + 16 * j : nf - 1;
+                        const zc yv = l16 + 16 * j < nf ? lz(s.vy, fl * nmax + r) : zmk(0, 0);
+                        t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), yv));
+                    }
+                    zc t = zadd(t2[0], t2[1]);
+                    for (int o2 = 8; o2 > 0; o2 >>= 1) {
+                        t.x += __shfl_xor(t.x, o2, 16);
+                        t.y += __shfl_xor(t.y, o2, 16);
+                    }
+                    if (l16 == 0 && e < rows) sz(s.kg, fl * nqm + m, t);
+                };
+                if (rows > 0) {
+                    C xa[16], xb[16];
+                    ld(g16, xa);
+                    for (int e = g16; e < rows; e += 2 * ng16) {
+                        ld(e + ng16, xb);
+                        fin(e, xa);
+                        ld(e + 2 * ng16, xa);
+                        fin(e + ng16, xb);
+                    }
                 }
-                if (l16 == 0) sz(s.kg, fl * nqm + m, t);
             }
             __syncthreads();
         }
@@ -710,11 +755,14 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         for (int k = tid; k < nk; k += nt) wk.sz(s.a0, k, wk.lz(s.alpha, k));
         using CT = typename cpx<T>::t;
         const int rows_max = nmax + nqm;
-        // the chunk of Phi columns lives where apply keeps its coefficients
+        // the chunk of Phi columns, then the bond list, live where apply keeps its coefficients
         double LAS *cbuf = s.kap;
         const int CH = (6 * nmax + 4 * nqm) / nmax < 8 ? (6 * nmax + 4 * nqm) / nmax : 8;
+        // Q rows of Y: a thread per row, the row read in blocks of 16 elements with the next
+        // block's loads in flight (unconditional loads, the last element repeated past the
+        // row's end and masked out of the products)
         for (int fl = 0; fl < 2; ++fl) {
-            const int nf = wk.nfl(fl), no = wk.nfl(1 - fl), rows = nf + wk.nql(fl);
+            const int nf = wk.nfl(fl), no = wk.nfl(1 - fl), nqf = wk.nql(fl);
             CT GAS *Yf = gptr(reinterpret_cast<CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
             for (int t0 = 0; t0 < no; t0 += CH) {
                 __syncthreads();
@@ -723,54 +771,147 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                     wk.sz(cbuf, e, t < no ? wk.phi(fl, s.cur[fl * nmax + f], s.part[(1 - fl) * nmax + t]) : zmk(0, 0));
                 }
                 __syncthreads();
-                for (int r = tid; r < rows; r += nt) {
+                for (int q = tid; q < nqf; q += nt) {
+                    const CT GAS *rowp = wk.qrp(fl) + (size_t)q * nmax;
                     zc acc[8];
+#pragma unroll
                     for (int j = 0; j < 8; ++j) acc[j] = zmk(0, 0);
-                    const CT GAS *rowp = r < nf ? wk.aip(fl) + (size_t)r * nmax : wk.qrp(fl) + (size_t)(r - nf) * nmax;
-                    int f = 0;
-                    for (; f + 2 <= nf; f += 2) {
-                        const CT x0 = ldg(rowp + f), x1 = ldg(rowp + f + 1);
-                        const zc m0 = toz(x0), m1 = toz(x1);
-                        for (int j = 0; j < 8; ++j)
-                            if (j < CH) {
-                                acc[j] = zadd(acc[j], zmul(m0, wk.lz(cbuf, f * CH + j)));
-                                acc[j] = zadd(acc[j], zmul(m1, wk.lz(cbuf, (f + 1) * CH + j)));
-                            }
-                    }
-                    for (; f < nf; ++f) {
-                        const zc m = toz(ldg(rowp + f));
-                        for (int j = 0; j < 8; ++j)
-                            if (j < CH) acc[j] = zadd(acc[j], zmul(m, wk.lz(cbuf, f * CH + j)));
+                    auto ld = [&](int f0, CT (&x)[16]) {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) x[j] = ldg(rowp + (f0 + j < nf ? f0 + j : nf - 1));
+                    };
+                    auto mac = [&](int f0, const CT (&x)[16]) {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const int f = f0 + j < nf ? f0 + j : nf - 1;
+                            const zc m = f0 + j < nf ? toz(x[j]) : zmk(0, 0);
+#pragma unroll
+                            for (int jj = 0; jj < 8; ++jj)
+                                acc[jj] = zadd(acc[jj], zmul(m, wk.lz(cbuf, f * CH + (jj < CH ? jj : CH - 1))));
+                        }
+                    };
+                    CT xa[16], xb[16];
+                    ld(0, xa);
+                    for (int f0 = 0; f0 < nf; f0 += 32) {
+                        ld(f0 + 16, xb);
+                        mac(f0, xa);
+                        ld(f0 + 32, xa);
+                        mac(f0 + 16, xb);
                     }
                     for (int j = 0; j < 8; ++j)
                         if (j < CH && t0 + j < no) {
                             CT v;
                             v.x = (T)acc[j].x;
                             v.y = (T)acc[j].y;
-                            stg(Yf + (size_t)r * nmax + t0 + j, v);
+                            stg(Yf + (size_t)(nf + q) * nmax + t0 + j, v);
                         }
                 }
             }
         }
         __syncthreads();
+        // antiparallel bonds in bond order (a ballot prefix sum per round of nt bonds):
+        // bl[2 b] = ci | cj << 16, bl[2 b + 1] = sa | sb << 16 (up particle ci at site sa,
+        // down particle cj at site sb)
+        __shared__ int s_wcnt[16];
+        int LAS *bl = reinterpret_cast<int LAS *>(cbuf);
+        int nb = 0;
+        {
+            const int lane = tid & 63, wid = tid >> 6, nwv = (nt + 63) >> 6;
+            for (int e0 = 0; e0 < 2 * Ns; e0 += nt) {
+                const int e = e0 + tid;
+                bool ap = false;
+                int v0 = 0, v1 = 0;
+                if (e < 2 * Ns) {
+                    const int4 eg = ldg4(gptr(a.edge) + e);
+                    const int vi = eg.x, vj = eg.z;
+                    const int ii = s.site[vi], ij = s.site[vj];
+                    ap = (ii >> 16) != (ij >> 16);
+                    if ((ii >> 16) == 0) { v0 = (ii & 0xFFFF) | ((ij & 0xFFFF) << 16); v1 = vi | (vj << 16); }
+                    else { v0 = (ij & 0xFFFF) | ((ii & 0xFFFF) << 16); v1 = vj | (vi << 16); }
+                }
+                const unsigned long long bm = __ballot(ap);
+                const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+                if (lane == 0) s_wcnt[wid] = __popcll(bm);
+                __syncthreads();
+                int off = nb, tot = 0;
+                for (int q = 0; q < nwv; ++q) {
+                    if (q < wid) off += s_wcnt[q];
+                    tot += s_wcnt[q];
+                }
+                if (ap) {
+                    bl[2 * (off + pre)] = v0;
+                    bl[2 * (off + pre) + 1] = v1;
+                }
+                nb += tot;
+                __syncthreads();
+            }
+        }
+        // the inverse rows' entries Y[r][t] the bonds read: y[r] of moving particle r (flavour
+        // fl) onto the site of the other flavour's particle t, a sixteen-lane group per (bond,
+        // flavour)
+        {
+            const int l16 = tid & 15, g16 = tid >> 4, ng16 = nt >> 4;
+            for (int e = g16; e < 2 * nb; e += ng16) {
+                const int b = e >> 1, fl = e & 1, nf = wk.nfl(fl);
+                const int v0 = bl[2 * b], v1 = bl[2 * b + 1];
+                const int r = fl ? v0 >> 16 : v0 & 0xFFFF, t = fl ? v0 & 0xFFFF : v0 >> 16;
+                const int snew = fl ? v1 & 0xFFFF : v1 >> 16;
+                const CT GAS *row = wk.aip(fl) + (size_t)r * nmax;
+                CT x[16], pv[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int f = l16 + 16 * j < nf ? l16 + 16 * j : nf - 1;
+                    x[j] = ldg(row + f);
+                    pv[j] = ldg(wk.tbp(fl) + (size_t)s.cur[fl * nmax + f] * Ns + snew);
+                }
+                zc t2[2] = {zmk(0, 0), zmk(0, 0)};
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (l16 + 16 * j < nf) t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), toz(pv[j])));
+                zc tt = zadd(t2[0], t2[1]);
+                for (int o = 8; o > 0; o >>= 1) {
+                    tt.x += __shfl_xor(tt.x, o, 16);
+                    tt.y += __shfl_xor(tt.y, o, 16);
+                }
+                if (l16 == 0) {
+                    CT v;
+                    v.x = (T)tt.x;
+                    v.y = (T)tt.y;
+                    stg(gptr(reinterpret_cast<CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax + (size_t)r * nmax + t, v);
+                }
+            }
+        }
+        __syncthreads();
+        // h_k: a thread per state, the state's hop data held in registers across the bonds
         const int c = wk.cstate();
+        const int npar = 2 * Ns - nb;
         for (int k = tid; k < nk; k += nt) {
             const zc a0 = wk.lz(s.a0, k);
-            zc h = zmk(0, 0);
-            for (int e = 0; e < 2 * Ns; ++e) {
-                const int4 eg = ldg4(gptr(a.edge) + e);
-                const int vi = eg.x, vj = eg.z;
-                const int ii = s.site[vi], ij = s.site[vj];
-                if ((ii >> 16) == (ij >> 16)) {
-                    h = zmk(h.x + 0.25 * a0.x, h.y + 0.25 * a0.y);
-                    continue;
+            int Rv[2], o1v[2][2], o2v[2][2], fqv[2][2], qrv[2][2];
+            zc rav[2][2][2];
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                Rv[fl] = wk.hop_n(c, k, fl);
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const bool on = m < Rv[fl];
+                    o1v[fl][m] = on ? wk.hop_from(c, k, fl, m) : 0;
+                    o2v[fl][m] = on ? wk.hop_to(c, k, fl, m) : 0;
+                    fqv[fl][m] = on ? s.slot[fl * Ns + o1v[fl][m]] : 0;
+                    qrv[fl][m] = on ? s.qrow[fl * Ns + o2v[fl][m]] : -1;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) rav[fl][m][q] = wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q);
                 }
-                int ci, cj, sa, sb;
-                if ((ii >> 16) == 0) { ci = ii & 0xFFFF; sa = vi; cj = ij & 0xFFFF; sb = vj; }
-                else { ci = ij & 0xFFFF; sa = vj; cj = ii & 0xFFFF; sb = vi; }
-                zc tot = zmk((double)wk.hopsign(c, k), 0.0);
+            }
+            const double hsg = (double)wk.hopsign(c, k);
+            zc h = zmk(0, 0);
+            for (int b = 0; b < nb; ++b) {
+                const int v0 = bl[2 * b], v1 = bl[2 * b + 1];
+                const int ci = v0 & 0xFFFF, cj = v0 >> 16, sa = v1 & 0xFFFF, sb = v1 >> 16;
+                zc tot = zmk(hsg, 0.0);
+#pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = wk.hop_n(c, k, fl), nf = wk.nfl(fl);
+                    const int R = Rv[fl], nf = wk.nfl(fl);
                     const int r = fl ? cj : ci, t = fl ? ci : cj;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
                     const CT GAS *Yf = gptr(reinterpret_cast<const CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
@@ -779,25 +920,26 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                     B[0] = toz(ldg(Yf + (size_t)r * nmax + t));
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = wk.ai(fl, r, s.slot[fl * Ns + wk.hop_from(c, k, fl, q)]);
+                        if (q < R) B[1 + q] = wk.ai(fl, r, fqv[fl][q]);
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
                         if (m >= R) continue;
-                        const int o2 = wk.hop_to(c, k, fl, m), o1 = wk.hop_from(c, k, fl, m);
+                        const int o2 = o2v[fl][m], o1 = o1v[fl][m];
                         const zc p2s = wk.phi(fl, o2, snew), p2r = wk.phi(fl, o2, sold);
                         const zc Dm = zadd(zsub(zsub(p2s, p2r), wk.phi(fl, o1, snew)), wk.phi(fl, o1, sold));
-                        const int qr = s.qrow[fl * Ns + o2];
+                        const int qr = qrv[fl][m];
                         const zc kgv = qr >= 0 ? toz(ldg(Yf + (size_t)(nf + qr) * nmax + t)) : zmk(0, 0);
                         B[(1 + m) * 3] = zsub(zadd(kgv, zmul(Dm, B[0])), p2s);
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
-                            if (q < R)
-                                B[(1 + m) * 3 + 1 + q] = zadd(wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q), zmul(Dm, B[1 + q]));
+                            if (q < R) B[(1 + m) * 3 + 1 + q] = zadd(rav[fl][m][q], zmul(Dm, B[1 + q]));
                     }
                     tot = zmul(tot, det3(B));
                 }
                 h = zmk(h.x - 0.25 * a0.x - 0.5 * tot.x, h.y - 0.25 * a0.y - 0.5 * tot.y);
             }
+            // parallel bonds: +a_k / 4 each
+            h = zmk(h.x + 0.25 * npar * a0.x, h.y + 0.25 * npar * a0.y);
             wk.sz(s.hk, k, h);
         }
         __syncthreads();
