@@ -155,6 +155,7 @@ template <typename T> struct Walker {
     C GAS *Ai[2];
     C GAS *Qr[2];   // Q rows: Q[q][f] = (rho_{qorb[q]} Ai)[f], kept up to date with Ai
     const C GAS *tab[2];
+    const C GAS *tabT[2];  // [site][orbital]
 
     __device__ __forceinline__ Walker(const ExcArgs &args, char *base, int wk)
         : a(args), w(wk), tid(threadIdx.x), nt(blockDim.x)
@@ -176,6 +177,8 @@ template <typename T> struct Walker {
         Qr[1] = Qr[0] + (size_t)nqm * nmax;
         tab[0] = gptr(reinterpret_cast<const C *>(a.tab)) + (size_t)gptr(a.group)[w] * 2 * Ns * Ns;
         tab[1] = tab[0] + (size_t)Ns * Ns;
+        tabT[0] = gptr(reinterpret_cast<const C *>(a.tabT)) + (size_t)gptr(a.group)[w] * 2 * Ns * Ns;
+        tabT[1] = tabT[0] + (size_t)Ns * Ns;
     }
     // per-flavour members by a select rather than a runtime index, so that the walker's
     // state stays in registers
@@ -184,6 +187,7 @@ template <typename T> struct Walker {
     __device__ C GAS *aip(int fl) const { return fl ? Ai[1] : Ai[0]; }
     __device__ C GAS *qrp(int fl) const { return fl ? Qr[1] : Qr[0]; }
     __device__ const C GAS *tbp(int fl) const { return fl ? tab[1] : tab[0]; }
+    __device__ const C GAS *tTp(int fl) const { return fl ? tabT[1] : tabT[0]; }
     __device__ zc phi(int fl, int orb, int site) const { return toz(ldg(tbp(fl) + (size_t)orb * Ns + site)); }
     __device__ zc ai(int fl, int r, int f) const { return toz(ldg(aip(fl) + (size_t)r * nmax + f)); }
     // Q row q (an orbital without a row reads as zero; the rows are the new orbitals of the
@@ -281,42 +285,40 @@ template <typename T> struct Walker {
                 }
             }
             __syncthreads();
+            // KG_m = rho_{o_m} . y over the particles, a thread per new orbital o_m of the current
+            // state, from the transposed orbital table: for particle r every lane reads the
+            // row of its site, at its own orbital's column, in blocks of 16 particles with the
+            // next block's loads in flight (particles past a flavour's count repeat its last
+            // one, masked out of the sums)
             {
                 const int nn0 = n_new(c, 0), nn1 = n_new(c, 1), rows = nn0 + nn1;
-                auto ld = [&](int e, C (&x)[16]) {
-                    const int ec = e < rows ? (e > 0 ? e : 0) : (rows > 0 ? rows - 1 : 0);
+                for (int e0 = 0; e0 < rows; e0 += nt) {
+                    const int e = e0 + tid, ec = e < rows ? e : rows - 1;
                     const int fl = ec < nn0 ? 0 : 1, m = fl ? ec - nn0 : ec, nf = nfl(fl);
-                    const C GAS *trow = tbp(fl) + (size_t)(rows > 0 ? new_orb(c, fl, m) : 0) * Ns;
-#pragma unroll
-                    for (int j = 0; j < 16; ++j)
-                        x[j] = ldg(trow + s.part[fl * nmax + (l16 + 16 * j < nf ? l16 + 16 * j : nf - 1)]);
-                };
-                auto fin = [&](int e, const C (&x)[16]) {
-                    const int ec = e < rows ? e : (rows > 0 ? rows - 1 : 0);
-                    const int fl = ec < nn0 ? 0 : 1, m = fl ? ec - nn0 : ec, nf = nfl(fl);
+                    const C GAS *col = tTp(fl) + new_orb(c, fl, m);
                     zc t2[2] = {zmk(0, 0), zmk(0, 0)};
+                    auto ld = [&](int r0, C (&x)[16]) {
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int r = l16 + 16 * j < nf ? l16 + 16 * j : nf - 1;
-                        const zc yv = l16 + 16 * j < nf ? lz(s.vy, fl * nmax + r) : zmk(0, 0);
-                        t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), yv));
-                    }
-                    zc t = zadd(t2[0], t2[1]);
-                    for (int o2 = 8; o2 > 0; o2 >>= 1) {
-                        t.x += __shfl_xor(t.x, o2, 16);
-                        t.y += __shfl_xor(t.y, o2, 16);
-                    }
-                    if (l16 == 0 && e < rows) sz(s.kg, fl * nqm + m, t);
-                };
-                if (rows > 0) {
+                        for (int j = 0; j < 16; ++j)
+                            x[j] = ldg(col + (size_t)s.part[fl * nmax + (r0 + j < nf ? r0 + j : nf - 1)] * Ns);
+                    };
+                    auto fin = [&](int r0, const C (&x)[16]) {
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const int r = r0 + j < nf ? r0 + j : nf - 1;
+                            const zc yv = r0 + j < nf ? lz(s.vy, fl * nmax + r) : zmk(0, 0);
+                            t2[j & 1] = zadd(t2[j & 1], zmul(toz(x[j]), yv));
+                        }
+                    };
                     C xa[16], xb[16];
-                    ld(g16, xa);
-                    for (int e = g16; e < rows; e += 2 * ng16) {
-                        ld(e + ng16, xb);
-                        fin(e, xa);
-                        ld(e + 2 * ng16, xa);
-                        fin(e + ng16, xb);
+                    ld(0, xa);
+                    for (int r0 = 0; r0 < nmax; r0 += 32) {
+                        ld(r0 + 16, xb);
+                        fin(r0, xa);
+                        ld(r0 + 32, xa);
+                        fin(r0 + 16, xb);
                     }
+                    if (e < rows) sz(s.kg, fl * nqm + m, zadd(t2[0], t2[1]));
                 }
             }
             __syncthreads();
@@ -1211,7 +1213,7 @@ extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks,
     const int Ns = a->L * a->L;
     const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
     const int nqm = a->nq0 > a->nq1 ? (a->nq0 > 0 ? a->nq0 : 1) : (a->nq1 > 0 ? a->nq1 : 1);
-    if (!a->Q || !a->qorb || !a->Y) return hipErrorInvalidValue;
+    if (!a->Q || !a->qorb || !a->Y || !a->tabT) return hipErrorInvalidValue;
     if (nmax > 256) return hipErrorInvalidValue;  // the update loop holds four column chunks of 64 per lane
     const size_t lds = gx::exc_lds_bytes(Ns, nmax, a->nk, nqm) + 256;
     if (lds > 160 * 1024 - 4096) return hipErrorInvalidValue;

@@ -1117,7 +1117,6 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
         CK(hipMalloc(&c->d_zscr, W * zt * rows * ts));
         CK(hipMalloc(&c->d_qorb, W * 2 * nqm * sizeof(int32_t)));
         CK(hipMemset(c->d_qorb, 0xff, W * 2 * nqm * sizeof(int32_t)));
-        CK(hipMalloc(&c->d_tabT, (size_t)c->G * 2 * Ns * Ns * ts));
     } else {
         if (c->prec == GPVMC_MIXED) {
             CK(hipMalloc(&c->d_Ai32, W * 2 * nm * nm * sizeof(float2)));
@@ -1130,6 +1129,8 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
         CK(hipMemset(c->d_qorb, 0xff, W * 2 * (size_t)nqm_all * sizeof(int32_t)));
         CK(hipMalloc(&c->d_Y, W * 2 * ((size_t)nm + nqm_all) * nm * ts));
     }
+    // [G][fl][site][orbital] orbital table of both channel kernels
+    CK(hipMalloc(&c->d_tabT, (size_t)c->G * 2 * Ns * Ns * ts));
     CK(hipMalloc(&c->d_work, W * 2 * nm * nm * sizeof(double2)));
     CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));
     CK(hipMalloc(&c->d_part, W * 2 * nm * sizeof(int32_t)));
@@ -1194,7 +1195,7 @@ int gpvmc_set_orbital_table(gpvmc_ctx *c, int32_t group, const double *table)
         for (size_t i = 0; i < n; ++i) f[i] = make_float2((float)table[2 * i], (float)table[2 * i + 1]);
         HIPCHK(hipMemcpy(c->d_tab32 + (size_t)group * n, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
     }
-    if (c->regk) {
+    if (c->channel) {
         // [fl][site][orbital] copy in the working precision
         const size_t Ns = c->Ns;
         if (c->prec == GPVMC_FP64) {
