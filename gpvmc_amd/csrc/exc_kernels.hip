@@ -99,11 +99,15 @@ struct Lds {
 };
 
 // complex elements of the apply coefficients' area, which the measurement reuses for its
-// bond list (two ints per bond, 2 Ns bonds)
+// tiles of Phi and its bond list (two ints per bond, 2 Ns bonds)
 __host__ __device__ inline size_t kap_elems(int Ns, int nmax, int nqm)
 {
-    const size_t k = ((size_t)nmax + nqm) * 3;
-    return k > (size_t)Ns ? k : (size_t)Ns;
+    size_t k = ((size_t)nmax + nqm) * 3;
+    k = k > (size_t)Ns ? k : (size_t)Ns;
+    // the measurement's 16-column tile of Phi ((nmax + 3) x 16 complex FP32) spans this area,
+    // dlt and V
+    const size_t t = 5 * (size_t)nmax + 24;
+    return k > t ? k : t;
 }
 
 __host__ __device__ inline size_t exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
@@ -760,6 +764,80 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         // the chunk of Phi columns, then the bond list, live where apply keeps its coefficients
         double LAS *cbuf = s.kap;
         const int CH = (6 * nmax + 4 * nqm) / nmax < 8 ? (6 * nmax + 4 * nqm) / nmax : 8;
+        if constexpr (sizeof(T) == 4) {
+            // Q rows of Y on MFMA (v_mfma_f32_16x16x4f32): per 16 targets a tile of
+            // Phi(current orbitals, target sites) in LDS (the B operand, [f][target], zero past
+            // the last orbital and target), each wave a 16-row block of Q at a time (the A
+            // operand straight from HBM, lane l holding row l % 16 and column l / 16 of each
+            // 16 x 4 step, eight steps' loads in flight); the complex product as four real
+            // products, Yr = Qr Phir - Qi Phii, Yi = Qr Phii + Qi Phir.  Output element v of
+            // lane l is row 4 (l / 16) + v, target l % 16.
+            typedef float v4 __attribute__((ext_vector_type(4)));
+            const int lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
+            const int li = lane & 15, kq = lane >> 4;
+            float LAS *ct = reinterpret_cast<float LAS *>(cbuf);
+            for (int fl = 0; fl < 2; ++fl) {
+                const int nf = wk.nfl(fl), no = wk.nfl(1 - fl), nqf = wk.nql(fl), nf4 = (nf + 3) & ~3;
+                CT GAS *Yf = gptr(reinterpret_cast<CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
+                const int nqb = (nqf + 15) >> 4, nks = nf4 >> 2;
+                for (int t0 = 0; t0 < no; t0 += 16) {
+                    __syncthreads();
+                    for (int e = tid; e < nf4 * 16; e += nt) {
+                        const int f = e >> 4, tg = t0 + (e & 15);
+                        CT v;
+                        v.x = v.y = 0.f;
+                        if (f < nf && tg < no)
+                            v = ldg(wk.tTp(fl) + (size_t)s.part[(1 - fl) * nmax + tg] * Ns + s.cur[fl * nmax + f]);
+                        ct[2 * e] = v.x;
+                        ct[2 * e + 1] = v.y;
+                    }
+                    __syncthreads();
+                    for (int qb = wid; qb < nqb; qb += nw) {
+                        const int q = qb * 16 + li;
+                        const CT GAS *arow = wk.qrp(fl) + (size_t)(q < nqf ? q : nqf - 1) * nmax;
+                        v4 zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+                        auto lda = [&](int k0, CT (&x)[8]) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                const int f = 4 * (k0 + j) + kq;
+                                x[j] = ldg(arow + (f < nf ? f : nf - 1));
+                            }
+                        };
+                        auto mm = [&](int k0, const CT (&x)[8]) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                const int f = 4 * (k0 + j) + kq;
+                                const int fb = f < nf4 ? f : nf4 - 1;
+                                const float bx = f < nf4 ? ct[2 * (fb * 16 + li)] : 0.f;
+                                const float by = f < nf4 ? ct[2 * (fb * 16 + li) + 1] : 0.f;
+                                zr = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, bx, zr, 0, 0, 0);
+                                zr = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, -by, zr, 0, 0, 0);
+                                zi = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].x, by, zi, 0, 0, 0);
+                                zi = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j].y, bx, zi, 0, 0, 0);
+                            }
+                        };
+                        CT xa[8], xb[8];
+                        lda(0, xa);
+                        for (int k0 = 0; k0 < nks; k0 += 16) {
+                            lda(k0 + 8, xb);
+                            mm(k0, xa);
+                            lda(k0 + 16, xa);
+                            mm(k0 + 8, xb);
+                        }
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int row = qb * 16 + 4 * kq + v, tg = t0 + li;
+                            if (row < nqf && tg < no) {
+                                CT y;
+                                y.x = zr[v];
+                                y.y = zi[v];
+                                stg(Yf + (size_t)(nf + row) * nmax + tg, y);
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
         // Q rows of Y: a thread per row, the row read in blocks of 16 elements with the next
         // block's loads in flight (unconditional loads, the last element repeated past the
         // row's end and masked out of the products)
@@ -809,6 +887,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                         }
                 }
             }
+        }
         }
         __syncthreads();
         // antiparallel bonds in bond order (a ballot prefix sum per round of nt bonds):
