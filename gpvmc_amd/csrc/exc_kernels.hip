@@ -213,6 +213,11 @@ template <typename T> struct Walker {
         return gptr(a.hop_new)[(ck(c, k) * 2 + fl) * 2 + h];
     }
     __device__ int hopsign(int c, int k) const { return gptr(a.hopsign)[ck(c, k)]; }
+    // hop orbitals clamped to a valid index, for loads issued before hop_n is known (their
+    // results are masked where h >= hop_n)
+    __device__ int hop_from_c(int c, int k, int fl, int h) const { return oclamp(hop_from(c, k, fl, h)); }
+    __device__ int hop_to_c(int c, int k, int fl, int h) const { return oclamp(hop_to(c, k, fl, h)); }
+    __device__ int oclamp(int o) const { return o < 0 ? 0 : (o < Ns ? o : Ns - 1); }
     // new_off holds 2 nk + 1 offsets per channel
     __device__ const int32_t GAS *noff(int c, int fl) const { return gptr(a.new_off) + (cb / nk) * (2 * nk + 1) + c * 2 + fl; }
     __device__ int n_new(int c, int fl) const { return noff(c, fl)[1] - noff(c, fl)[0]; }
@@ -223,12 +228,14 @@ template <typename T> struct Walker {
     __device__ __forceinline__ void build_ra()
     {
         const int c = cstate();
+        // every load issued unconditionally (clamped indices), the entries outside the hop
+        // count masked afterwards
         for (int e = tid; e < nk * 8; e += nt) {
             const int k = e / 8, fl = (e / 4) & 1, m = (e / 2) & 1, nn = e & 1;
-            zc v = zmk(0, 0);
-            if (m < hop_n(c, k, fl) && nn < hop_n(c, k, fl))
-                v = qv(fl, s.qrow[fl * Ns + hop_to(c, k, fl, m)], s.slot[fl * Ns + hop_from(c, k, fl, nn)]);
-            sz(s.ra, e, v);
+            const int R = hop_n(c, k, fl);
+            const int q = s.qrow[fl * Ns + hop_to_c(c, k, fl, m)], f = s.slot[fl * Ns + hop_from_c(c, k, fl, nn)];
+            const zc v = toz(ldg(qrp(fl) + (size_t)(q >= 0 ? q : 0) * nmax + (f >= 0 ? f : 0)));
+            sz(s.ra, e, m < R && nn < R && q >= 0 ? v : zmk(0, 0));
         }
         __syncthreads();
     }
@@ -331,7 +338,8 @@ template <typename T> struct Walker {
             zc tot = zmk((double)hopsign(c, k), 0.0);
             for (int fl = 0; fl < 2; ++fl) {
                 const int R = hop_n(c, k, fl);
-                // the capacitance matrix padded to 3 x 3 with the identity (constant indices)
+                // the capacitance matrix padded to 3 x 3 with the identity (constant indices);
+                // every load is issued before R is known (clamped indices) and masked after
                 zc B[9];
                 eye3(B);
                 if (!mv) {
@@ -343,18 +351,34 @@ template <typename T> struct Walker {
                 } else {
                     const int r = fl ? cj : ci;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
+                    int o1[2], o2[2], hn[2];
+                    zc aq[2], p2s[2], p2r[2], p1s[2], p1r[2], kgv[2];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        o1[m] = hop_from_c(c, k, fl, m);
+                        o2[m] = hop_to_c(c, k, fl, m);
+                        const int h = hop_new(c, k, fl, m);  // o'_m's place among the new orbitals
+                        hn[m] = h < 0 ? 0 : (h < nqm ? h : nqm - 1);
+                    }
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        const int f = s.slot[fl * Ns + o1[m]];
+                        aq[m] = ai(fl, r, f >= 0 ? f : 0);
+                        p2s[m] = phi(fl, o2[m], snew);
+                        p2r[m] = phi(fl, o2[m], sold);
+                        p1s[m] = phi(fl, o1[m], snew);
+                        p1r[m] = phi(fl, o1[m], sold);
+                        kgv[m] = lz(s.kg, fl * nqm + hn[m]);
+                    }
                     B[0] = lz(s.vy, fl * nmax + r);
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = ai(fl, r, s.slot[fl * Ns + hop_from(c, k, fl, q)]);
+                        if (q < R) B[1 + q] = aq[q];
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
                         if (m >= R) continue;
-                        const int o2 = hop_to(c, k, fl, m), o1 = hop_from(c, k, fl, m);
-                        const zc p2s = phi(fl, o2, snew), p2r = phi(fl, o2, sold);
-                        const zc Dm = zadd(zsub(zsub(p2s, p2r), phi(fl, o1, snew)), phi(fl, o1, sold));
-                        B[(1 + m) * 3] = zsub(zadd(lz(s.kg, fl * nqm + hop_new(c, k, fl, m)),
-                                                   zmul(Dm, B[0])), p2s);
+                        const zc Dm = zadd(zsub(zsub(p2s[m], p2r[m]), p1s[m]), p1r[m]);
+                        B[(1 + m) * 3] = zsub(zadd(kgv[m], zmul(Dm, B[0])), p2s[m]);
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
                             if (q < R)
@@ -976,9 +1000,10 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
                     const bool on = m < Rv[fl];
-                    o1v[fl][m] = on ? wk.hop_from(c, k, fl, m) : 0;
-                    o2v[fl][m] = on ? wk.hop_to(c, k, fl, m) : 0;
-                    fqv[fl][m] = on ? s.slot[fl * Ns + o1v[fl][m]] : 0;
+                    o1v[fl][m] = wk.hop_from_c(c, k, fl, m);
+                    o2v[fl][m] = wk.hop_to_c(c, k, fl, m);
+                    const int fq = s.slot[fl * Ns + o1v[fl][m]];
+                    fqv[fl][m] = fq >= 0 ? fq : 0;
                     qrv[fl][m] = on ? s.qrow[fl * Ns + o2v[fl][m]] : -1;
 #pragma unroll
                     for (int q = 0; q < 2; ++q) rav[fl][m][q] = wk.lz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + q);
@@ -996,21 +1021,30 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                     const int r = fl ? cj : ci, t = fl ? ci : cj;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
                     const CT GAS *Yf = gptr(reinterpret_cast<const CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
+                    // every load issued before the hop count selects (clamped indices)
+                    zc aq[2], p2s[2], p2r[2], p1s[2], p1r[2], kq[2];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        aq[m] = wk.ai(fl, r, fqv[fl][m]);
+                        p2s[m] = wk.phi(fl, o2v[fl][m], snew);
+                        p2r[m] = wk.phi(fl, o2v[fl][m], sold);
+                        p1s[m] = wk.phi(fl, o1v[fl][m], snew);
+                        p1r[m] = wk.phi(fl, o1v[fl][m], sold);
+                        const int qr = qrv[fl][m];
+                        const zc kv = toz(ldg(Yf + (size_t)(nf + (qr >= 0 ? qr : 0)) * nmax + t));
+                        kq[m] = qr >= 0 ? kv : zmk(0, 0);
+                    }
                     zc B[9];  // padded to 3 x 3 with the identity
                     eye3(B);
                     B[0] = toz(ldg(Yf + (size_t)r * nmax + t));
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = wk.ai(fl, r, fqv[fl][q]);
+                        if (q < R) B[1 + q] = aq[q];
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
                         if (m >= R) continue;
-                        const int o2 = o2v[fl][m], o1 = o1v[fl][m];
-                        const zc p2s = wk.phi(fl, o2, snew), p2r = wk.phi(fl, o2, sold);
-                        const zc Dm = zadd(zsub(zsub(p2s, p2r), wk.phi(fl, o1, snew)), wk.phi(fl, o1, sold));
-                        const int qr = qrv[fl][m];
-                        const zc kgv = qr >= 0 ? toz(ldg(Yf + (size_t)(nf + qr) * nmax + t)) : zmk(0, 0);
-                        B[(1 + m) * 3] = zsub(zadd(kgv, zmul(Dm, B[0])), p2s);
+                        const zc Dm = zadd(zsub(zsub(p2s[m], p2r[m]), p1s[m]), p1r[m]);
+                        B[(1 + m) * 3] = zsub(zadd(kq[m], zmul(Dm, B[0])), p2s[m]);
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
                             if (q < R) B[(1 + m) * 3 + 1 + q] = zadd(rav[fl][m][q], zmul(Dm, B[1 + q]));
