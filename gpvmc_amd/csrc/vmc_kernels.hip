@@ -220,6 +220,15 @@ template <typename T, int N, int LPR> constexpr size_t wdu_lds_bytes()
 //    applies the rank-1 update (its rows go through LDS, one more barrier), and the next
 //    window starts after it.  Without an accepted move the window simply advances.
 // Every decision is the one MetroMC::Step takes on the same uniforms, in the same order.
+// slots per pipelined chunk of a streamed row: the largest divisor of the lane's slot count
+// that is at most 12
+constexpr int stream_chunk(int cpt)
+{
+    for (int d = 12; d > 1; --d)
+        if (cpt % d == 0) return d;
+    return 1;
+}
+
 template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
@@ -628,10 +637,10 @@ sweep_kernel(SweepArgs a)
                     const C nc = cmk<C>(-coef.x, -coef.y);
                     if (has_row) {
                         const T *sr = &s_row[fl][c][0][0], *si = &s_row[fl][c][1][0];
-#pragma unroll UNR
-                        for (int k = 0; k < CPT; ++k) {
-                            const T vr = sr[k], vi = si[k];
-                            if constexpr (RES) {
+                        if constexpr (RES) {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
+                                const T vr = sr[k], vi = si[k];
                                 T er = gr[fl][k], ei = gi[fl][k];
                                 er = fma(nc.x, vr, er);
                                 er = fma(-nc.y, vi, er);
@@ -639,9 +648,31 @@ sweep_kernel(SweepArgs a)
                                 ei = fma(nc.y, vr, ei);
                                 gr[fl][k] = er;
                                 gi[fl][k] = ei;
-                            } else {
-                                C &e = gat(fl, row, c + LPR * k);
-                                e = cfma(nc, cmk<C>(vr, vi), e);
+                            }
+                        } else {
+                            // streamed rows in chunks of SK slots, fully unrolled: the next
+                            // chunk's loads are issued before this chunk's stores, so that the
+                            // loads do not wait behind the stores
+                            constexpr int SK = stream_chunk(CPT), NCH = CPT / SK;
+                            C ea[SK], eb[SK];
+                            auto ldk = [&](int k0, C (&e)[SK]) {
+#pragma unroll
+                                for (int j = 0; j < SK; ++j) e[j] = gat(fl, row, c + LPR * (k0 + j));
+                            };
+                            auto upk = [&](int k0, C (&e)[SK]) {
+#pragma unroll
+                                for (int j = 0; j < SK; ++j) {
+                                    const int k = k0 + j;
+                                    gat(fl, row, c + LPR * k) = cfma(nc, cmk<C>(sr[k], si[k]), e[j]);
+                                }
+                            };
+                            ldk(0, ea);
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch += 2) {
+                                if (ch + 1 < NCH) ldk((ch + 1) * SK, eb);
+                                upk(ch * SK, ea);
+                                if (ch + 2 < NCH) ldk((ch + 2) * SK, ea);
+                                if (ch + 1 < NCH) upk((ch + 1) * SK, eb);
                             }
                         }
                     }
