@@ -603,15 +603,34 @@ sweep_kernel(SweepArgs a)
                     const int rsel = fl ? cj_j : ci_j;
                     if (row == rsel) {
                         T *dr = &s_row[fl][c][0][0], *di = &s_row[fl][c][1][0];
-#pragma unroll UNR
-                        for (int k = 0; k < CPT; ++k) {
-                            if constexpr (RES) {
+                        if constexpr (RES) {
+#pragma unroll
+                            for (int k = 0; k < CPT; ++k) {
                                 dr[k] = gr[fl][k];
                                 di[k] = gi[fl][k];
-                            } else {
-                                const C e = gat(fl, row, c + LPR * k);
-                                dr[k] = e.x;
-                                di[k] = e.y;
+                            }
+                        } else {
+                            // streamed row in chunks, the next chunk's loads in flight
+                            constexpr int SK = stream_chunk(CPT), NCH = CPT / SK;
+                            C ea[SK], eb[SK];
+                            auto ldk = [&](int k0, C (&e)[SK]) {
+#pragma unroll
+                                for (int j = 0; j < SK; ++j) e[j] = gat(fl, row, c + LPR * (k0 + j));
+                            };
+                            auto stk = [&](int k0, const C (&e)[SK]) {
+#pragma unroll
+                                for (int j = 0; j < SK; ++j) {
+                                    dr[k0 + j] = e[j].x;
+                                    di[k0 + j] = e[j].y;
+                                }
+                            };
+                            ldk(0, ea);
+#pragma unroll
+                            for (int ch = 0; ch < NCH; ch += 2) {
+                                if (ch + 1 < NCH) ldk((ch + 1) * SK, eb);
+                                stk(ch * SK, ea);
+                                if (ch + 2 < NCH) ldk((ch + 2) * SK, ea);
+                                if (ch + 1 < NCH) stk((ch + 1) * SK, eb);
                             }
                         }
                         if (c == (fl ? ls1 : ls0)) {
