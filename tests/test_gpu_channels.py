@@ -210,3 +210,36 @@ def test_l12_trans_matches_oracle(W, nsteps):
         rels.append(np.abs(out["ratio"][w][:upto][live] - o["ratio"][:upto][live]) / o["ratio"][:upto][live])
     rel = np.concatenate(rels)
     assert len(rel) > 10 and np.median(rel) < 1e-3
+
+
+@pytest.mark.parametrize("L", [16, 20])
+def test_streamed_mixed_measurement_matches_fp64(L):
+    """The streamed kernel's FP32 measurement (the Q-row product on MFMA) against its FP64
+    one (on the VALU) at the sizes only the streamed kernel runs (L = 16: 128 Q rows, L = 20:
+    200, a partial block of 16): same walkers, same uniforms, one step and one measurement
+    after init; H/O within 5e-4 of their scale wherever both took the same decision."""
+    W, seed = 4, 2020 + L
+    ch = channels.Channel("long", L, (1, 0))
+    tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+    spins = model.random_spins(L, W, seed)
+    res = {}
+    for prec in (binding.FP64, binding.MIXED):
+        say("case streamed measurement L=%d" % L, "fp64" if prec == binding.FP64 else "mixed")
+        with binding.ChannelBatch(L, W, ch, prec, seed=seed, kernel="streamed") as b:
+            assert not b.register_resident
+            b.set_orbital_table(0, tab)
+            ok = b.init(spins)
+            out = b.walk(1, meas=1, trace=True)
+            ho, cs, _ = b.channel_sums()
+            res[prec] = (ok, out["accept"], ho, cs)
+    ok64, acc64, ho64, cs64 = res[binding.FP64]
+    ok32, acc32, ho32, cs32 = res[binding.MIXED]
+    n = 0
+    for w in range(W):
+        if not (ok64[w] and ok32[w]) or acc64[w][0] != acc32[w][0] or cs64[w] != cs32[w]:
+            continue
+        scale = np.abs(ho64[w]).max()
+        assert scale > 0
+        np.testing.assert_allclose(ho32[w], ho64[w], rtol=0, atol=5e-4 * scale)
+        n += 1
+    assert n >= W // 2
