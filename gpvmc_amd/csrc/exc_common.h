@@ -26,6 +26,24 @@ __device__ __forceinline__ zc zdiv(zc a, zc b)
 }
 template <typename C> __device__ __forceinline__ zc toz(C v) { return zmk((double)v.x, (double)v.y); }
 
+// running product of complex factors as mantissa (mr, mi) times 2^ex: the mantissa is
+// rescaled by a power of two only when its exponent leaves [-256, 256], so that a walk needs
+// no logarithm per accepted move
+__device__ __forceinline__ void zamp_mul(double &mr, double &mi, double &ex, double qr, double qi)
+{
+    const double re = mr * qr - mi * qi, im = mr * qi + mi * qr;
+    const int be = (int)((__double_as_longlong(fmax(fabs(re), fabs(im))) >> 52) & 0x7FF) - 1023;
+    if (be > 256 || be < -256) {
+        const double sc = __longlong_as_double((long long)(1023 - be) << 52);  // 2^-be
+        mr = re * sc;
+        mi = im * sc;
+        ex += be;
+    } else {
+        mr = re;
+        mi = im;
+    }
+}
+
 __device__ inline zc det_small(int n, const zc *B)  // n <= 3, row-major n x n
 {
     if (n == 0) return zmk(1.0, 0.0);

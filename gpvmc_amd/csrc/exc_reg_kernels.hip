@@ -171,6 +171,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     zc *const s_red = reinterpret_cast<zc *>(smem + lo.red);
     int *const s_ired = reinterpret_cast<int *>(smem + lo.ired);
     char *const s_un = smem + lo.un;
+    // thread 0: the product of the accepted moves' determinant ratios per flavour, mantissa
+    // [0..3] and binary exponent [4..5], folded into logabs / phase when the launch ends (in
+    // the spare bytes past the carved layout)
+    double *const s_amp = reinterpret_cast<double *>(smem + lo.total);
     // the current state's row of the hop graph, resolved to Q rows and slots:
     // hn[k][fl] hops, hto/hfr[k][fl][m] orbitals, hq[k][fl][m] Q row of hto, hs[k][fl][m]
     // slot of hfr, hsg[k] HopSign
@@ -657,8 +661,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         if (tid == 0) {
             for (int fl = 0; fl < 2; ++fl) {
                 const zc d = s_dets[kn * 2 + fl];
-                a.logabs[2 * w + fl] += 0.5 * log(znorm(d));
-                a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
+                zamp_mul(s_amp[2 * fl], s_amp[2 * fl + 1], s_amp[4 + fl], d.x, d.y);
             }
             for (int fl = 0; fl < 2; ++fl)
                 for (int m = 0; m < RK(fl); ++m) {
@@ -679,12 +682,19 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 s_part[nmax + cj] = sa;
             }
         }
+        // the hop graph's row of the current state changes only with the state (the slot
+        // and Q-row bookkeeping above moves nothing when kn is the current state)
+        const bool moved_state = kn != cs;
         cs = kn;
         __syncthreads();
-        cache_hops();
+        if (moved_state) cache_hops();
         build_ra();
     };
 
+    if (tid == 0) {
+        s_amp[0] = 1.0; s_amp[1] = 0.0; s_amp[2] = 1.0; s_amp[3] = 0.0;
+        s_amp[4] = 0.0; s_amp[5] = 0.0;
+    }
     cache_hops();
     build_ra();
     double S;
@@ -928,6 +938,15 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     }
     for (int i = tid; i < 2 * nqm; i += NT) a.qorb[(size_t)w * 2 * nqm + i] = s_qorb[i];
     if (tid == 0) {
+        if (MK == 0) {
+            // ln|det| and arg det per flavour += the launch's product of ratios
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                const double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1];
+                a.logabs[2 * w + fl] += log(hypot(mr, mi)) + s_amp[4 + fl] * 0.69314718055994531;
+                a.phase[2 * w + fl] = remainder(a.phase[2 * w + fl] + atan2(mi, mr), 2 * M_PI);
+            }
+        }
         a.cstate[w] = cs;
         a.weight[w] = S;
         if (MK == 1 && a.mode == 5) {
