@@ -312,6 +312,54 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             __syncthreads();
         }
     };
+    // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element) from each
+    // thread's partials: wave butterflies, then one barrier and a pass over the waves'
+    // results.  The results alternate between two halves of s_red, so the next reduction
+    // writes the other half; the one after it comes after this one's barrier in every thread.
+    int rpar = 0;
+    auto reduce_partials = [&](double ls, double bv, int bk, double &sum, int &kmax) __attribute__((always_inline)) {
+        for (int o = 32; o > 0; o >>= 1) {
+            ls += __shfl_xor(ls, o);
+            const double ov = __shfl_xor(bv, o);
+            const int ok = __shfl_xor(bk, o);
+            if (ov > bv || (ov == bv && ok < bk)) {
+                bv = ov;
+                bk = ok;
+            }
+        }
+        zc *const rd = s_red + rpar * NW;
+        int *const ri = s_ired + rpar * NW;
+        rpar ^= 1;
+        if (lane == 0) {
+            rd[wv] = zmk(ls, bv);
+            ri[wv] = bk;
+        }
+        __syncthreads();
+        sum = 0.0;
+        bv = -1.0;
+        bk = nk;
+        for (int q = 0; q < NW; ++q) {
+            sum += rd[q].x;
+            if (rd[q].y > bv || (rd[q].y == bv && ri[q] < bk)) {
+                bv = rd[q].y;
+                bk = ri[q];
+            }
+        }
+        kmax = __builtin_amdgcn_readfirstlane(bk);
+    };
+    auto reduce_alpha = [&](double &sum, int &kmax) __attribute__((always_inline)) {
+        double ls = 0.0, bv = -1.0;
+        int bk = nk;
+        for (int k = tid; k < nk; k += NT) {
+            const double v = znorm(s_alpha[k]);
+            ls += v;
+            if (v > bv) {
+                bv = v;
+                bk = k;
+            }
+        }
+        reduce_partials(ls, bv, bk, sum, kmax);
+    };
     // amplitudes of every state at the current configuration, relative to the current state
     auto eval0 = [&]() __attribute__((always_inline)) {
         for (int k = tid; k < nk; k += NT) {
@@ -355,7 +403,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     };
     // ratios of every state for the swap moving up particle ci to sb and down particle
     // cj to sa: c vectors, one GEMV per flavour, the moving rows, the capacitances
-    auto eval_mv = [&](int ci, int sa, int sb, int cj) __attribute__((always_inline)) {
+    auto eval_mv = [&](int ci, int sa, int sb, int cj, double &S2, int &kn) __attribute__((always_inline)) {
         for (int t = tid; t < 2 * NC; t += NT) {
             const int fl = t >= NC ? 1 : 0, f = t - fl * NC;
             C2 v = mk2<C2>(0.0, 0.0);
@@ -406,6 +454,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 }
         }
         __syncthreads();
+        // with the reduction's partials (its barrier also publishes alpha and the dets)
+        double ls = 0.0, bv = -1.0;
+        int bk = nk;
         for (int k = tid; k < nk; k += NT) {
             zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
@@ -417,47 +468,14 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 tot = zmul(tot, d);
             }
             s_alpha[k] = tot;
-        }
-        __syncthreads();
-    };
-    // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element)
-    auto reduce_alpha = [&](double &sum, int &kmax) __attribute__((always_inline)) {
-        double ls = 0.0, bv = -1.0;
-        int bk = nk;
-        for (int k = tid; k < nk; k += NT) {
-            const double v = znorm(s_alpha[k]);
+            const double v = znorm(tot);
             ls += v;
             if (v > bv) {
                 bv = v;
                 bk = k;
             }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            ls += __shfl_xor(ls, o);
-            const double ov = __shfl_xor(bv, o);
-            const int ok = __shfl_xor(bk, o);
-            if (ov > bv || (ov == bv && ok < bk)) {
-                bv = ov;
-                bk = ok;
-            }
-        }
-        if (lane == 0) {
-            s_red[wv] = zmk(ls, bv);
-            s_ired[wv] = bk;
-        }
-        __syncthreads();
-        sum = 0.0;
-        bv = -1.0;
-        bk = nk;
-        for (int q = 0; q < NW; ++q) {
-            sum += s_red[q].x;
-            if (s_red[q].y > bv || (s_red[q].y == bv && s_ired[q] < bk)) {
-                bv = s_red[q].y;
-                bk = s_ired[q];
-            }
-        }
-        kmax = __builtin_amdgcn_readfirstlane(bk);
-        __syncthreads();
+        reduce_partials(ls, bv, bk, S2, kn);
     };
     // move to state kn after eval0 (mv false) or eval_mv (mv true)
     auto apply = [&](bool mv, int ci, int sa, int sb, int cj, int kn) __attribute__((always_inline)) {
@@ -899,8 +917,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                         cj = i1 & 0xFFFF;
                         sb = v1;
                     }
-                    eval_mv(ci, sa, sb, cj);
-                    reduce_alpha(S2, kn);
+                    eval_mv(ci, sa, sb, cj, S2, kn);
                     ratio = S2 / S;
                     acc = ratio > 1.0 ? (u3 < 1.0) : (u3 < ratio);  // MetroMC::YesNo
                     acc = __builtin_amdgcn_readfirstlane(acc ? 1 : 0) != 0;
