@@ -98,19 +98,23 @@ __device__ __forceinline__ zc det3(const zc *B)
 __device__ __forceinline__ void inv3(const zc *B, zc *Bi)
 {
     const zc d = det3(B);
+    // one reciprocal of the determinant, nine products (a division per cofactor would put
+    // eighteen FP64 divides on the one lane that runs this)
+    const double rn = 1.0 / (d.x * d.x + d.y * d.y);
+    const zc di = zmk(d.x * rn, -d.y * rn);
     auto cof = [&](int r0, int r1, int c0, int c1) {
         return zsub(zmul(B[r0 * 3 + c0], B[r1 * 3 + c1]), zmul(B[r0 * 3 + c1], B[r1 * 3 + c0]));
     };
     // Bi[i][j] = cofactor C[j][i] / d
-    Bi[0] = zdiv(cof(1, 2, 1, 2), d);
-    Bi[3] = zdiv(zsub(zmk(0, 0), cof(1, 2, 0, 2)), d);
-    Bi[6] = zdiv(cof(1, 2, 0, 1), d);
-    Bi[1] = zdiv(zsub(zmk(0, 0), cof(0, 2, 1, 2)), d);
-    Bi[4] = zdiv(cof(0, 2, 0, 2), d);
-    Bi[7] = zdiv(zsub(zmk(0, 0), cof(0, 2, 0, 1)), d);
-    Bi[2] = zdiv(cof(0, 1, 1, 2), d);
-    Bi[5] = zdiv(zsub(zmk(0, 0), cof(0, 1, 0, 2)), d);
-    Bi[8] = zdiv(cof(0, 1, 0, 1), d);
+    Bi[0] = zmul(cof(1, 2, 1, 2), di);
+    Bi[3] = zmul(zsub(zmk(0, 0), cof(1, 2, 0, 2)), di);
+    Bi[6] = zmul(cof(1, 2, 0, 1), di);
+    Bi[1] = zmul(zsub(zmk(0, 0), cof(0, 2, 1, 2)), di);
+    Bi[4] = zmul(cof(0, 2, 0, 2), di);
+    Bi[7] = zmul(zsub(zmk(0, 0), cof(0, 2, 0, 1)), di);
+    Bi[2] = zmul(cof(0, 1, 1, 2), di);
+    Bi[5] = zmul(zsub(zmk(0, 0), cof(0, 1, 0, 2)), di);
+    Bi[8] = zmul(cof(0, 1, 0, 1), di);
 }
 __device__ __forceinline__ void eye3(zc *B)
 {
