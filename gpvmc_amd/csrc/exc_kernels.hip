@@ -150,6 +150,12 @@ __device__ inline Lds lds_carve(char LAS *base, int Ns, int nmax, int nk, int nq
     return s;
 }
 
+// ln|product| of a mantissa/exponent pair per flavour (both flavours summed)
+__device__ __forceinline__ double amp_log(const double LAS *m)
+{
+    return log(hypot(m[0], m[1])) + log(hypot(m[2], m[3])) + (m[4] + m[5]) * 0.69314718055994531;
+}
+
 template <typename T> struct Walker {
     using C = typename cpx<T>::t;
     const ExcArgs &a;
@@ -160,6 +166,9 @@ template <typename T> struct Walker {
     C GAS *Qr[2];   // Q rows: Q[q][f] = (rho_{qorb[q]} Ai)[f], kept up to date with Ai
     const C GAS *tab[2];
     const C GAS *tabT[2];  // [site][orbital]
+    // thread 0: product of the accepted moves' determinant ratios per flavour since the
+    // launch began, mantissa [0..3] and binary exponent [4..5] (LDS)
+    double LAS *amp;
 
     __device__ __forceinline__ Walker(const ExcArgs &args, char *base, int wk)
         : a(args), w(wk), tid(threadIdx.x), nt(blockDim.x)
@@ -691,8 +700,11 @@ template <typename T> struct Walker {
         if (tid == 0) {
             for (int fl = 0; fl < 2; ++fl) {
                 const zc d = s_det[fl];
-                gptr(a.logabs)[2 * w + fl] += 0.5 * log(znorm(d));
-                gptr(a.phase)[2 * w + fl] = remainder(gptr(a.phase)[2 * w + fl] + atan2(d.y, d.x), 2 * M_PI);
+                double mr = amp[2 * fl], mi = amp[2 * fl + 1], ex = amp[4 + fl];
+                zamp_mul(mr, mi, ex, d.x, d.y);
+                amp[2 * fl] = mr;
+                amp[2 * fl + 1] = mi;
+                amp[4 + fl] = ex;
             }
             for (int fl = 0; fl < 2; ++fl) {
                 const int R = hop_n(c, kn, fl);
@@ -730,6 +742,12 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     extern __shared__ __align__(16) char smem[];
     const int w = a.order ? gptr(a.order)[blockIdx.x] : (int)blockIdx.x;
     Walker<T> wk(a, smem, w);
+    __shared__ double s_amp[6];
+    wk.amp = (double LAS *)s_amp;
+    if (threadIdx.x == 0) {
+        wk.amp[0] = 1.0; wk.amp[1] = 0.0; wk.amp[2] = 1.0; wk.amp[3] = 0.0;
+        wk.amp[4] = 0.0; wk.amp[5] = 0.0;
+    }
     const int tid = wk.tid, nt = wk.nt, Ns = wk.Ns, nmax = wk.nmax, nk = wk.nk;
     auto &s = wk.s;
     const int nqm = wk.nqm;
@@ -1058,7 +1076,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
             wk.sz(s.hk, k, h);
         }
         __syncthreads();
-        const double lw = 2.0 * (gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1]) + log(S0);
+        const double lw = 2.0 * (gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1] + amp_log(wk.amp)) + log(S0);
         if (acc_on) {
             double2 GAS *HO = gptr(reinterpret_cast<double2 *>(a.ho)) + (size_t)w * 2 * nk * nk;
             for (int e = tid; e < nk * nk; e += nt) {
@@ -1082,7 +1100,8 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     };
     if (a.mode == 4) {
         // amplitudes of every state at the current configuration, as ln|a_k| and arg a_k
-        const double la = gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1], ph = gptr(a.phase)[2 * w] + gptr(a.phase)[2 * w + 1];
+        const double la = gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1] + amp_log(wk.amp);
+        const double ph = gptr(a.phase)[2 * w] + gptr(a.phase)[2 * w + 1] + atan2(wk.amp[1], wk.amp[0]) + atan2(wk.amp[3], wk.amp[2]);
         for (int k = tid; k < nk; k += nt) {
             const zc al = wk.lz(s.alpha, k);
             const double nrm = znorm(al);
@@ -1157,6 +1176,12 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     }
     for (int i = tid; i < 2 * nqm; i += nt) gptr(a.qorb)[(size_t)w * 2 * nqm + i] = s.qorb[i];
     if (tid == 0) {
+        // ln|det| and arg det per flavour += the launch's product of ratios
+        for (int fl = 0; fl < 2; ++fl) {
+            const double mr = wk.amp[2 * fl], mi = wk.amp[2 * fl + 1];
+            gptr(a.logabs)[2 * w + fl] += log(hypot(mr, mi)) + wk.amp[4 + fl] * 0.69314718055994531;
+            gptr(a.phase)[2 * w + fl] = remainder(gptr(a.phase)[2 * w + fl] + atan2(mi, mr), 2 * M_PI);
+        }
         gptr(a.weight)[w] = S;
         if (a.mode == 0) {
             gptr(a.counter)[w] = ctr0 + (uint64_t)a.n_steps;
