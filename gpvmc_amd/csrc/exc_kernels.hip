@@ -1079,18 +1079,44 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
         const double lw = 2.0 * (gptr(a.logabs)[2 * w] + gptr(a.logabs)[2 * w + 1] + amp_log(wk.amp)) + log(S0);
         if (acc_on) {
             double2 GAS *HO = gptr(reinterpret_cast<double2 *>(a.ho)) + (size_t)w * 2 * nk * nk;
-            for (int e = tid; e < nk * nk; e += nt) {
-                const int k1 = e / nk, k2 = e % nk;
-                const zc a1 = wk.lz(s.a0, k1);
-                const zc h2 = wk.lz(s.hk, k2), a2 = wk.lz(s.a0, k2);
-                const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
-                double2 hs = ldg(HO + e), os = ldg(HO + (size_t)nk * nk + e);
-                hs.x += hv.x / S0;
-                hs.y += hv.y / S0;
-                os.x += ov.x / S0;
-                os.y += ov.y / S0;
-                stg(HO + e, hs);
-                stg(HO + (size_t)nk * nk + e, os);
+            // H and O sums in batches of four elements per thread, the next batch's loads
+            // issued before this batch's stores (elements past the end repeat the last one,
+            // which its own thread stores with the value it computed)
+            const int ne = nk * nk;
+            constexpr int HB = 4;
+            auto hld = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) {
+#pragma unroll
+                for (int j = 0; j < HB; ++j) {
+                    const int e = e0 + j * nt < ne ? e0 + j * nt : ne - 1;
+                    hs[j] = ldg(HO + e);
+                    os[j] = ldg(HO + (size_t)nk * nk + e);
+                }
+            };
+            auto hst = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) {
+#pragma unroll
+                for (int j = 0; j < HB; ++j) {
+                    const int e = e0 + j * nt;
+                    if (e >= ne) continue;
+                    const int k1 = e / nk, k2 = e - k1 * nk;
+                    const zc a1 = wk.lz(s.a0, k1);
+                    const zc h2 = wk.lz(s.hk, k2), a2 = wk.lz(s.a0, k2);
+                    const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
+                    hs[j].x += hv.x / S0;
+                    hs[j].y += hv.y / S0;
+                    os[j].x += ov.x / S0;
+                    os[j].y += ov.y / S0;
+                    stg(HO + e, hs[j]);
+                    stg(HO + (size_t)nk * nk + e, os[j]);
+                }
+            };
+            double2 ha[HB], oa[HB], hb[HB], ob[HB];
+            const int stride = HB * nt;
+            hld(tid, ha, oa);
+            for (int e0 = tid; e0 < ne; e0 += 2 * stride) {
+                hld(e0 + stride, hb, ob);
+                hst(e0, ha, oa);
+                hld(e0 + 2 * stride, ha, oa);
+                hst(e0 + stride, hb, ob);
             }
             if (tid == 0) { sacc[0] += 1; sacc[1] += lw; }
         }
