@@ -1029,40 +1029,53 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
             }
             const double hsg = (double)wk.hopsign(c, k);
             zc h = zmk(0, 0);
-            for (int b = 0; b < nb; ++b) {
-                const int v0 = bl[2 * b], v1 = bl[2 * b + 1];
+            // the bond's loads (all issued before the hop counts select, clamped indices), and
+            // the next bond's issued before this one's determinants
+            struct BondIn {
+                CT y0[2], aq[2][2], p2s[2][2], p2r[2][2], p1s[2][2], p1r[2][2], kq[2][2];
+            };
+            auto bld = [&](int b, BondIn &in) {
+                const int bc = b < nb ? b : nb - 1;
+                const int v0 = bl[2 * bc], v1 = bl[2 * bc + 1];
                 const int ci = v0 & 0xFFFF, cj = v0 >> 16, sa = v1 & 0xFFFF, sb = v1 >> 16;
-                zc tot = zmk(hsg, 0.0);
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = Rv[fl], nf = wk.nfl(fl);
+                    const int nf = wk.nfl(fl);
                     const int r = fl ? cj : ci, t = fl ? ci : cj;
                     const int sold = fl ? sb : sa, snew = fl ? sa : sb;
                     const CT GAS *Yf = gptr(reinterpret_cast<const CT *>(a.Y)) + ((size_t)w * 2 + fl) * rows_max * nmax;
-                    // every load issued before the hop count selects (clamped indices)
-                    zc aq[2], p2s[2], p2r[2], p1s[2], p1r[2], kq[2];
+                    const CT GAS *tf = wk.tbp(fl);
+                    in.y0[fl] = ldg(Yf + (size_t)r * nmax + t);
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
-                        aq[m] = wk.ai(fl, r, fqv[fl][m]);
-                        p2s[m] = wk.phi(fl, o2v[fl][m], snew);
-                        p2r[m] = wk.phi(fl, o2v[fl][m], sold);
-                        p1s[m] = wk.phi(fl, o1v[fl][m], snew);
-                        p1r[m] = wk.phi(fl, o1v[fl][m], sold);
+                        in.aq[fl][m] = ldg(wk.aip(fl) + (size_t)r * nmax + fqv[fl][m]);
+                        in.p2s[fl][m] = ldg(tf + (size_t)o2v[fl][m] * Ns + snew);
+                        in.p2r[fl][m] = ldg(tf + (size_t)o2v[fl][m] * Ns + sold);
+                        in.p1s[fl][m] = ldg(tf + (size_t)o1v[fl][m] * Ns + snew);
+                        in.p1r[fl][m] = ldg(tf + (size_t)o1v[fl][m] * Ns + sold);
                         const int qr = qrv[fl][m];
-                        const zc kv = toz(ldg(Yf + (size_t)(nf + (qr >= 0 ? qr : 0)) * nmax + t));
-                        kq[m] = qr >= 0 ? kv : zmk(0, 0);
+                        in.kq[fl][m] = ldg(Yf + (size_t)(nf + (qr >= 0 ? qr : 0)) * nmax + t);
                     }
+                }
+            };
+            auto bfin = [&](const BondIn &in) {
+                zc tot = zmk(hsg, 0.0);
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int R = Rv[fl];
                     zc B[9];  // padded to 3 x 3 with the identity
                     eye3(B);
-                    B[0] = toz(ldg(Yf + (size_t)r * nmax + t));
+                    B[0] = toz(in.y0[fl]);
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = aq[q];
+                        if (q < R) B[1 + q] = toz(in.aq[fl][q]);
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
                         if (m >= R) continue;
-                        const zc Dm = zadd(zsub(zsub(p2s[m], p2r[m]), p1s[m]), p1r[m]);
-                        B[(1 + m) * 3] = zsub(zadd(kq[m], zmul(Dm, B[0])), p2s[m]);
+                        const zc p2s = toz(in.p2s[fl][m]);
+                        const zc Dm = zadd(zsub(zsub(p2s, toz(in.p2r[fl][m])), toz(in.p1s[fl][m])), toz(in.p1r[fl][m]));
+                        const zc kq = qrv[fl][m] >= 0 ? toz(in.kq[fl][m]) : zmk(0, 0);
+                        B[(1 + m) * 3] = zsub(zadd(kq, zmul(Dm, B[0])), p2s);
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
                             if (q < R) B[(1 + m) * 3 + 1 + q] = zadd(rav[fl][m][q], zmul(Dm, B[1 + q]));
@@ -1070,6 +1083,16 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
                     tot = zmul(tot, det3(B));
                 }
                 h = zmk(h.x - 0.25 * a0.x - 0.5 * tot.x, h.y - 0.25 * a0.y - 0.5 * tot.y);
+            };
+            if (nb > 0) {
+                BondIn ia, ib;
+                bld(0, ia);
+                for (int b = 0; b < nb; b += 2) {
+                    bld(b + 1, ib);
+                    bfin(ia);
+                    bld(b + 2, ia);
+                    if (b + 1 < nb) bfin(ib);
+                }
             }
             // parallel bonds: +a_k / 4 each
             h = zmk(h.x + 0.25 * npar * a0.x, h.y + 0.25 * npar * a0.y);
