@@ -772,44 +772,83 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 __syncthreads();
             }
         }
-        // every antiparallel bond's ratio of every state
+        // antiparallel bonds in bond order (a ballot prefix sum per round of NT bonds) in the
+        // area the contraction tiles used: bl[2 b] = ci | cj << 16, bl[2 b + 1] = sa | sb << 16
+        // (up particle ci at site sa, down particle cj at site sb)
+        int *const bl = reinterpret_cast<int *>(s_un);
+        int nb = 0;
+        for (int e0 = 0; e0 < 2 * Ns; e0 += NT) {
+            const int e = e0 + tid;
+            bool ap = false;
+            int v0 = 0, v1 = 0;
+            if (e < 2 * Ns) {
+                const int4 eg = a.edge[e];
+                const int ii = s_site[eg.x], ij = s_site[eg.z];
+                ap = (ii >> 16) != (ij >> 16);
+                if ((ii >> 16) == 0) { v0 = (ii & 0xFFFF) | ((ij & 0xFFFF) << 16); v1 = eg.x | (eg.z << 16); }
+                else { v0 = (ij & 0xFFFF) | ((ii & 0xFFFF) << 16); v1 = eg.z | (eg.x << 16); }
+            }
+            const unsigned long long bm = __ballot(ap);
+            const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
+            if (lane == 0) s_ired[wv] = __popcll(bm);
+            __syncthreads();
+            int off = nb, tot = 0;
+            for (int q = 0; q < NW; ++q) {
+                if (q < wv) off += s_ired[q];
+                tot += s_ired[q];
+            }
+            if (ap) {
+                bl[2 * (off + pre)] = v0;
+                bl[2 * (off + pre) + 1] = v1;
+            }
+            nb += tot;
+            __syncthreads();
+        }
+        // every antiparallel bond's ratio of every state: a thread per (state, group of
+        // bonds), every load issued before the hop counts select (the cached hop data holds
+        // valid indices past the hop count), the next bond's loads in flight
         const int ngrp = z.ngrp;
         for (int e2 = tid; e2 < nk * ngrp; e2 += NT) {
             const int k = e2 % nk, gq = e2 / nk;
-            zc hs = zmk(0.0, 0.0);
-            for (int e = gq; e < 2 * Ns; e += ngrp) {
-                const int4 eg = a.edge[e];
-                const int ii = s_site[eg.x], ij = s_site[eg.z];
-                if ((ii >> 16) == (ij >> 16)) continue;
-                int ci, cj, sa, sb;
-                if ((ii >> 16) == 0) {
-                    ci = ii & 0xFFFF;
-                    sa = eg.x;
-                    cj = ij & 0xFFFF;
-                    sb = eg.z;
-                } else {
-                    ci = ij & 0xFFFF;
-                    sa = eg.z;
-                    cj = ii & 0xFFFF;
-                    sb = eg.x;
+            struct BondIn {
+                C2 y0[2], aq[2][2], kq[2][2];
+                zc p2s[2][2];
+            };
+            auto bld = [&](int b, BondIn &in) __attribute__((always_inline)) {
+                const int bc = b < nb ? b : nb - 1;
+                const int v0 = bl[2 * bc], v1 = bl[2 * bc + 1];
+                const int ci = v0 & 0xFFFF, cj = v0 >> 16, sa = v1 & 0xFFFF, sb = v1 >> 16;
+#pragma unroll
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int r = fl ? cj : ci, tg = fl ? ci : cj;
+                    const int snew = fl ? sa : sb, base = fl ? RB0 * 16 : 0;
+                    const C2 *zcol = Zg + (size_t)tg * RS + base;
+                    in.y0[fl] = zcol[r];
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        in.aq[fl][m] = Mg[(size_t)(base + r) * NC + s_hs[(k * 2 + fl) * 2 + m]];
+                        const int o2 = s_hto[(k * 2 + fl) * 2 + m];
+                        in.p2s[fl][m] = phi(fl, o2 >= 0 ? o2 : 0, snew);
+                        in.kq[fl][m] = zcol[NF(fl) + s_hq[(k * 2 + fl) * 2 + m]];
+                    }
                 }
+            };
+            zc hs = zmk(0.0, 0.0);
+            auto bfin = [&](const BondIn &in) __attribute__((always_inline)) {
                 zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = s_hn[k * 2 + fl], r = fl ? cj : ci, tg = fl ? ci : cj;
-                    const int snew = fl ? sa : sb, base = fl ? RB0 * 16 : 0;
-                    const C2 *zcol = Zg + (size_t)tg * RS + base;
+                    const int R = s_hn[k * 2 + fl];
                     zc B[9];
                     eye3(B);
-                    B[0] = toz(zcol[r]);
+                    B[0] = toz(in.y0[fl]);
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (q < R) B[1 + q] = toz(Mg[(size_t)(base + r) * NC + s_hs[(k * 2 + fl) * 2 + q]]);
+                        if (q < R) B[1 + q] = toz(in.aq[fl][q]);
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
                         if (m < R) {
-                            const int o2 = s_hto[(k * 2 + fl) * 2 + m];
-                            B[(1 + m) * 3] = zsub(toz(zcol[NF(fl) + s_hq[(k * 2 + fl) * 2 + m]]), phi(fl, o2, snew));
+                            B[(1 + m) * 3] = zsub(toz(in.kq[fl][m]), in.p2s[fl][m]);
 #pragma unroll
                             for (int q = 0; q < 2; ++q)
                                 if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
@@ -817,6 +856,16 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     tot = zmul(tot, det3(B));
                 }
                 hs = zadd(hs, tot);
+            };
+            if (gq < nb) {
+                BondIn ia, ib;
+                bld(gq, ia);
+                for (int b = gq; b < nb; b += 2 * ngrp) {
+                    bld(b + ngrp, ib);
+                    bfin(ia);
+                    bld(b + 2 * ngrp, ia);
+                    if (b + ngrp < nb) bfin(ib);
+                }
             }
             s_hpart[gq * nk + k] = hs;
         }
