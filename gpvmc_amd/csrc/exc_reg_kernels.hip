@@ -892,14 +892,42 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
         if (acc_on) {
             double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
-            for (int e = tid; e < nk * nk; e += NT) {
-                const int k1 = e / nk, k2 = e - k1 * nk;
-                const zc a1 = s_a0[k1], h2 = s_alpha[k2], a2 = s_a0[k2];
-                const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
-                HO[e].x += hv.x / S0;
-                HO[e].y += hv.y / S0;
-                HO[(size_t)nk * nk + e].x += ov.x / S0;
-                HO[(size_t)nk * nk + e].y += ov.y / S0;
+            // in batches of four elements per thread, the next batch's loads issued before this
+            // batch's stores (elements past the end repeat the last one and are not stored)
+            const int ne = nk * nk;
+            constexpr int HB = 4;
+            auto hld = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int j = 0; j < HB; ++j) {
+                    const int e = e0 + j * NT < ne ? e0 + j * NT : ne - 1;
+                    hs[j] = HO[e];
+                    os[j] = HO[(size_t)nk * nk + e];
+                }
+            };
+            auto hst = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int j = 0; j < HB; ++j) {
+                    const int e = e0 + j * NT;
+                    if (e >= ne) continue;
+                    const int k1 = e / nk, k2 = e - k1 * nk;
+                    const zc a1 = s_a0[k1], h2 = s_alpha[k2], a2 = s_a0[k2];
+                    const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
+                    hs[j].x += hv.x / S0;
+                    hs[j].y += hv.y / S0;
+                    os[j].x += ov.x / S0;
+                    os[j].y += ov.y / S0;
+                    HO[e] = hs[j];
+                    HO[(size_t)nk * nk + e] = os[j];
+                }
+            };
+            double2 ha[HB], oa[HB], hb[HB], ob[HB];
+            constexpr int stride = HB * NT;
+            hld(tid, ha, oa);
+            for (int e0 = tid; e0 < ne; e0 += 2 * stride) {
+                hld(e0 + stride, hb, ob);
+                hst(e0, ha, oa);
+                hld(e0 + 2 * stride, ha, oa);
+                hst(e0 + stride, hb, ob);
             }
             if (tid == 0) {
                 sacc[0] += 1;
