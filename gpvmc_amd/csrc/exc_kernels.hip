@@ -239,12 +239,35 @@ template <typename T> struct Walker {
         const int c = cstate();
         // every load issued unconditionally (clamped indices), the entries outside the hop
         // count masked afterwards
-        for (int e = tid; e < nk * 8; e += nt) {
-            const int k = e / 8, fl = (e / 4) & 1, m = (e / 2) & 1, nn = e & 1;
-            const int R = hop_n(c, k, fl);
-            const int q = s.qrow[fl * Ns + hop_to_c(c, k, fl, m)], f = s.slot[fl * Ns + hop_from_c(c, k, fl, nn)];
-            const zc v = toz(ldg(qrp(fl) + (size_t)(q >= 0 ? q : 0) * nmax + (f >= 0 ? f : 0)));
-            sz(s.ra, e, m < R && nn < R && q >= 0 ? v : zmk(0, 0));
+        // a thread per state: its hop data once, then its eight Q-row entries together
+        for (int k = tid; k < nk; k += nt) {
+            int R[2], q[2][2], f[2][2];
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                R[fl] = hop_n(c, k, fl);
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    q[fl][m] = s.qrow[fl * Ns + hop_to_c(c, k, fl, m)];
+                    f[fl][m] = s.slot[fl * Ns + hop_from_c(c, k, fl, m)];
+                }
+            }
+            C v[2][2][2];
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 2; ++nn)
+                        v[fl][m][nn] = ldg(qrp(fl) + (size_t)(q[fl][m] >= 0 ? q[fl][m] : 0) * nmax +
+                                           (f[fl][nn] >= 0 ? f[fl][nn] : 0));
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int nn = 0; nn < 2; ++nn)
+                        sz(s.ra, ((k * 2 + fl) * 2 + m) * 2 + nn,
+                           m < R[fl] && nn < R[fl] && q[fl][m] >= 0 ? toz(v[fl][m][nn]) : zmk(0, 0));
         }
         __syncthreads();
     }
