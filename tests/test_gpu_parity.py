@@ -4,8 +4,9 @@ Every test calls the HIP path through the C ABI (gpvmc_amd.binding) and compares
 with the C oracle fed the same uniform stream.  Bars:
   * FP64 mode: identical no-op/accept sequences and configurations (bit-exact integer
     state); weight ratios per `ratios_match` (every |q_up q_dn| within 1e-8 relative +
-    1e-12 absolute, median relative error <= 1e-10, and for well-conditioned walks
-    >= 99.9% of ratios above 1e-12 within 1e-10 and all within 1e-9); inverses within
+    1e-12 absolute, median relative error <= 1e-10, and on the reference streams at L = 4
+    and 8 every ratio above 1e-12 within 1e-10; the reference's own per-step ratios
+    within 1e-10 at L = 4 and 8 and 5e-9 at L = 6); inverses within
     1e-9; ratio tables within 1e-8 of the oracle's Ai Phi; ln|amp| within 1e-9; E_loc per
     `eloc_close` (1e-10 absolute or relative, 1e-6 relative next to a node); m_z exact;
   * mixed FP32 mode, refreshed every sweep: median relative error of |q_up q_dn| <= 1e-5
@@ -50,7 +51,9 @@ def ratios_match(gpu, orc, live, well_conditioned=False):
     on sqrt(r) = |q_up q_dn|.  Bars:
       every step            |d sqrt(r)| <= 1e-8 sqrt(r) + 1e-12
       median over steps     relative error <= 1e-10
-      well-conditioned      >= 99.9% of ratios above 1e-12 within 1e-10, all within 1e-9."""
+      well-conditioned      every ratio above 1e-12 within 1e-10 (the reference streams at
+                            L = 4 and 8; L = 6 keeps the per-step envelope, its two correct
+                            FP64 evaluations differing by up to 1.5e-9)."""
     g, o = gpu[live], orc[live]
     sg, so = np.sqrt(g), np.sqrt(o)
     assert (np.abs(sg - so) <= 1e-8 * so + 1e-12).all()
@@ -58,8 +61,8 @@ def ratios_match(gpu, orc, live, well_conditioned=False):
     assert np.median(rel) <= 1e-10
     if well_conditioned:
         big = o > 1e-12
-        assert (rel[big] <= 1e-10).mean() >= 0.999
-        assert (rel[big] <= 1e-9).all()
+        say("  ratios above 1e-12: max relative error %.2e over %d steps" % (rel[big].max(), big.sum()))
+        assert (rel[big] <= 1e-10).all()
 
 
 def eloc_close(a, b):
@@ -140,6 +143,21 @@ def test_fp64_replays_reference_trajectory(name):
     live = ~o["noop"].astype(bool)
     assert np.array_equal(o["noop"], out["noop"][0])
     ratios_match(out["ratio"][0], o["ratio"], live, well_conditioned=(L != 6))
+    # the reference's own per-step weight ratios (the first len(g["ratio"]) steps).  They
+    # carry its own ~1e-10 rounding: the predicted weight uses q from zgemv
+    # (linalg.cpp:146) and the weight it divides by q from zgemm (linalg.cpp:434), both
+    # through base-10 BigDouble rescaling; the oracle meets the same 1e-9 bar against them.
+    # Moves into exact zeros give ratios of pure rounding (1e-60): an absolute floor.
+    nr = len(g["ratio"])
+    lr = live[:nr]
+    r_ref, r_gpu = g["ratio"][lr], out["ratio"][0][:nr][lr]
+    rel_ref = np.abs(r_gpu - r_ref) / np.maximum(r_ref, 1e-300)
+    say("  vs the reference's ratios: median %.2e max %.2e over %d live steps"
+        % (np.median(rel_ref), rel_ref[r_ref > 1e-12].max(), lr.sum()))
+    # L = 4 and 8: 3e-14 and 2e-12 measured; L = 6 (near-degenerate orbitals, where two
+    # correct FP64 evaluations differ by up to 1.5e-9): 1.6e-9 measured
+    bar = 5e-9 if L == 6 else 1e-10
+    assert (np.abs(r_gpu - r_ref) <= bar * r_ref + 1e-40).all()
 
 
 @pytest.mark.parametrize("L,W,nsteps,meas", [(4, 16, 2000, 16), (8, 32, 3000, 64), (6, 8, 1500, 36)])
