@@ -54,6 +54,9 @@ SIGNATURES = {
     "gpvmc_set_orbital_table": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_read_channel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
     "gpvmc_kernel_time": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "gpvmc_det_inv": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
+                                     _vp]),
+    "gpvmc_det_inv_workspace_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "gpvmc_reset_kernel_time": (ctypes.c_int, [_vp]),
 }
 
@@ -110,6 +113,27 @@ def sweep_shapes(L, precision):
     out = np.zeros(max(n, 1), np.int32)
     lib.gpvmc_sweep_shapes(precision, L, _ptr(out), n)
     return [int(x) for x in out[:n]]
+
+
+def _check_rc(lib, rc):
+    if rc != 0:
+        raise GpvmcError("gpvmc error %d: %s" % (rc, lib.gpvmc_last_error().decode()))
+
+
+def det_inv_workspace_bytes(n):
+    """bytes of device workspace gpvmc_det_inv uses per n x n matrix (0: none)"""
+    lib = load_library()
+    b = ctypes.c_int64()
+    _check_rc(lib, lib.gpvmc_det_inv_workspace_bytes(n, ctypes.byref(b)))
+    return b.value
+
+
+def det_inv(n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr, singular_ptr, stream=None, device=0):
+    """Batched linalg::DetInv on device memory (gpvmc_det_inv): every argument after
+    `batch` is a device address (int); `stream` a hipStream_t handle or None."""
+    lib = load_library()
+    _check_rc(lib, lib.gpvmc_det_inv(device, n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr, singular_ptr,
+                                     stream))
 
 
 class Batch:

@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-I" + CSRC,
 ]
 
-SOURCES = ("vmc_kernels.hip", "exc_kernels.hip", "exc_reg_kernels.hip", "gpvmc_api.cpp")
+SOURCES = ("vmc_kernels.hip", "exc_kernels.hip", "exc_reg_kernels.hip", "detinv_kernels.hip", "gpvmc_api.cpp")
 
 
 def sources():

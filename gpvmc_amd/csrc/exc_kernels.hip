@@ -1268,155 +1268,7 @@ __global__ void __launch_bounds__(256) exc_kernel(ExcArgs a)
     }
 }
 
-// The current state's Slater matrices of every walker, A[fl][f][r] = Phi(cur orbital of
-// slot f, site of label r), into the FP64 working buffer of the streamed Gauss-Jordan.
-__global__ void __launch_bounds__(256) exc_build_kernel(ExcArgs a, double2 *work)
-{
-    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
-    const int Ns = a.L * a.L;
-    const int n = fl ? a.n_dn : a.n_up;
-    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
-    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
-    const int *cur = a.cur + (size_t)w * 2 * nmax + (size_t)fl * nmax;
-    const int *part = a.part + (size_t)w * 2 * nmax + (size_t)fl * nmax;
-    double2 *M = work + ((size_t)w * 2 + fl) * nmax * nmax;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int f = e / n, r = e % n;
-        M[(size_t)f * nmax + r] = tab[(size_t)cur[f] * Ns + part[r]];
-    }
-}
-
-// Runtime-size FP64 Gauss-Jordan with implicit partial pivoting on a prebuilt matrix
-// (the algorithm of gj_kernel: pivot = largest |re|+|im| among unpivoted rows, lowest row
-// on ties; linalg.cpp:9-44).  One block per (walker, flavour), one row per thread.
-__global__ void __launch_bounds__(256) gj_dyn_kernel(gv::GJDynArgs a)
-{
-    using C = double2;
-    extern __shared__ __align__(16) char smem[];
-    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
-    const int n = fl ? a.n_dn : a.n_up;
-    const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    constexpr int NW = 4;
-    C *s_prow = reinterpret_cast<C *>(smem);                      // [2][NW][nmax]
-    unsigned long long *s_ckey = reinterpret_cast<unsigned long long *>(s_prow + 2 * NW * nmax);  // [2][NW]
-    int *s_perm = reinterpret_cast<int *>(s_ckey + 2 * NW);      // [nmax]
-    int *s_invp = s_perm + nmax;                                  // [nmax]
-    int *s_flag = s_invp + nmax;                                  // [1]
-    unsigned char *s_seen = reinterpret_cast<unsigned char *>(s_flag + 1);
-    const int row = tid;
-    const bool has_row = row < n;
-    for (int i = tid; i < nmax; i += 256) s_invp[i] = 0;
-    if (tid == 0) *s_flag = 0;
-    __syncthreads();
-    C *M = a.work + ((size_t)w * 2 + fl) * nmax * nmax;
-    bool pivoted = !has_row;
-    C dm = make_double2(1.0, 0.0);
-    int de = 0;
-    int par = 0;
-    for (int k = 0; k < n; ++k) {
-        const C bk = has_row ? M[(size_t)row * nmax + k] : make_double2(0.0, 0.0);
-        const float mag = (float)(fabs(bk.x) + fabs(bk.y));
-        unsigned long long key = 0ull;
-        if (!pivoted) key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long ok2 = __shfl_xor(key, o);
-            key = ok2 > key ? ok2 : key;
-        }
-        const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
-        if (has_row && !pivoted && key != 0ull && row == wrow)
-            for (int j = 0; j < n; ++j) s_prow[(par * NW + wid) * nmax + j] = M[(size_t)row * nmax + j];
-        if (lane == 0) s_ckey[par * NW + wid] = key;
-        __syncthreads();
-        int bw = 0;
-        unsigned long long bkey = s_ckey[par * NW];
-        for (int q = 1; q < NW; ++q) {
-            const unsigned long long v = s_ckey[par * NW + q];
-            if (v > bkey) { bkey = v; bw = q; }
-        }
-        const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
-        const C *prow = s_prow + (size_t)(par * NW + bw) * nmax;
-        const C piv = prow[k];
-        const double np = piv.x * piv.x + piv.y * piv.y;
-        C ainv;
-        if (bkey != 0ull && np > 0.0) {
-            ainv = make_double2(piv.x / np, -piv.y / np);
-        } else {
-            ainv = make_double2(0.0, 0.0);
-            if (tid == 0) *s_flag = 1;
-        }
-        if (tid == 0) {
-            s_perm[k] = (p >= 0 && p < n) ? p : 0;
-            if (p >= 0 && p < n) s_invp[p] = k;
-            const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
-            int e;
-            (void)frexp(fmax(fabs(re), fabs(im)), &e);
-            dm = make_double2(ldexp(re, -e), ldexp(im, -e));
-            de += e;
-        }
-        if (has_row) {
-            const bool me = (row == p);
-            const double bax = bk.x * ainv.x - bk.y * ainv.y, bay = bk.x * ainv.y + bk.y * ainv.x;
-            for (int j = 0; j < n; ++j) {
-                const C rj = prow[j];
-                C &m = M[(size_t)row * nmax + j];
-                if (me) {
-                    m = (j == k) ? ainv : make_double2(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
-                } else if (j == k) {
-                    m = make_double2(bax, bay);
-                } else {
-                    m.x = fma(-bax, rj.x, m.x);
-                    m.x = fma(bay, rj.y, m.x);
-                    m.y = fma(-bax, rj.y, m.y);
-                    m.y = fma(-bay, rj.x, m.y);
-                }
-            }
-            if (me) pivoted = true;
-        }
-        par ^= 1;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        int sgn = 1;
-        for (int i = 0; i < n; ++i) s_seen[i] = 0;
-        for (int i = 0; i < n; ++i) {
-            if (s_seen[i]) continue;
-            int len = 0, j = i;
-            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
-            if ((len & 1) == 0) sgn = -sgn;
-        }
-        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
-        double ph = atan2(dm.y, dm.x);
-        if (sgn < 0) ph += M_PI;
-        a.logabs[2 * w + fl] = la;
-        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
-        a.status[2 * w + fl] = *s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
-    }
-    C *out = a.Ai64 + ((size_t)w * 2 + fl) * nmax * nmax;
-    float2 *out32 = a.Ai32 ? a.Ai32 + ((size_t)w * 2 + fl) * nmax * nmax : nullptr;
-    if (has_row) {
-        int orow = s_invp[row];
-        orow = orow < 0 ? 0 : (orow < n ? orow : n - 1);
-        for (int j = 0; j < n; ++j) {
-            int oc = s_perm[j];
-            oc = oc < 0 ? 0 : (oc < n ? oc : n - 1);
-            const C v = M[(size_t)row * nmax + j];
-            out[(size_t)orow * nmax + oc] = v;
-            if (out32) out32[(size_t)orow * nmax + oc] = make_float2((float)v.x, (float)v.y);
-        }
-    }
-}
-
 }  // namespace gx
-
-extern "C" hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st)
-{
-    const int nmax = a->n_up > a->n_dn ? a->n_up : a->n_dn;
-    if (nmax > 256) return hipErrorInvalidValue;
-    const size_t lds = (size_t)2 * 4 * nmax * sizeof(double2) + 2 * 4 * 8 + (size_t)2 * nmax * 4 + 16 + nmax + 16;
-    hipLaunchKernelGGL(gx::gj_dyn_kernel, dim3(2 * nwalkers), dim3(256), lds, st, *a);
-    return hipGetLastError();
-}
 
 extern "C" hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_t st)
 {
@@ -1439,14 +1291,21 @@ extern "C" size_t gv_exc_lds_bytes(int Ns, int nmax, int nk, int nqm)
     return gx::exc_lds_bytes(Ns, nmax, nk, nqm < 1 ? 1 : nqm) + 256;
 }
 
-// Q rows of every walker's current state from the FP64 inverse (after Gauss-Jordan): one
-// block per (walker, flavour); qorb[q] = the q-th new orbital of the current state (-1 past
-// them), Q[q][f] = sum_r Phi(qorb[q], site r) Ai64[r][f] in the working precision.
+// Q rows of every walker's current state from the FP64 inverse: one block per (walker,
+// flavour); qorb[q] = the q-th new orbital of the current state (-1 past them) and
+// Q[q][f] = sum_r Phi(qorb[q], site r) Ai64[r][f] in the working precision, a complex
+// (nq x n) x (n x n) product in FP64.  Output tiles of 64 x 64, each thread a 4 x 4
+// micro-tile; the factors pass through LDS in chunks of 16 along r.
 template <typename T>
 __global__ void __launch_bounds__(256) exc_qbuild_kernel(gv::ExcArgs a, const double2 *Ai64)
 {
     using C = typename gz::cpx<T>::t;
+    constexpr int TQ = 64, TF = 64, KC = 16;
+    __shared__ double2 s_rho[KC][TQ + 1];  // Phi(orbital of row q, site of label r), [r][q]
+    __shared__ double2 s_ai[KC][TF];       // Ai64[r][f]
+    __shared__ int s_orb[256], s_site[256];
     const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int tid = threadIdx.x;
     const int Ns = a.L * a.L;
     const int nmax = a.n_up > a.n_dn ? a.n_up : a.n_dn;
     const int nqm = a.nq0 > a.nq1 ? (a.nq0 > 0 ? a.nq0 : 1) : (a.nq1 > 0 ? a.nq1 : 1);
@@ -1455,27 +1314,71 @@ __global__ void __launch_bounds__(256) exc_qbuild_kernel(gv::ExcArgs a, const do
     const size_t chb = (size_t)(a.chan ? a.chan[w] : 0);
     const int32_t *off = a.new_off + chb * (2 * a.nk + 1) + c * 2 + fl;
     const int o0 = off[0], nnew = off[1] - off[0];
+    const int nrow = nq < nnew ? nq : nnew;  // rows with an orbital; the rest are zero
     int32_t *qo = a.qorb + ((size_t)w * 2 + fl) * nqm;
-    for (int q = threadIdx.x; q < nqm; q += blockDim.x) qo[q] = (q < nq && q < nnew) ? a.new_orb[o0 + q] : -1;
-    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
+    for (int q = tid; q < nqm; q += blockDim.x) qo[q] = (q < nq && q < nnew) ? a.new_orb[o0 + q] : -1;
     const int32_t *part = a.part + ((size_t)w * 2 + fl) * nmax;
+    for (int i = tid; i < 256; i += blockDim.x) {
+        s_orb[i] = i < nrow ? a.new_orb[o0 + i] : 0;
+        s_site[i] = i < n ? part[i] : 0;
+    }
+    __syncthreads();
+    const double2 *tab = reinterpret_cast<const double2 *>(a.tab64) + ((size_t)a.group[w] * 2 + fl) * Ns * Ns;
     const double2 *ai = Ai64 + ((size_t)w * 2 + fl) * nmax * nmax;
     C *Q = reinterpret_cast<C *>(a.Q) + ((size_t)w * 2 + fl) * nqm * nmax;
-    for (int e = threadIdx.x; e < nq * n; e += blockDim.x) {
-        const int q = e / n, f = e - q * n;
-        double vr = 0.0, vi = 0.0;
-        if (q < nnew) {
-            const int o = a.new_orb[o0 + q];
-            for (int r = 0; r < n; ++r) {
-                const double2 p = tab[(size_t)o * Ns + part[r]], x = ai[(size_t)r * nmax + f];
-                vr += p.x * x.x - p.y * x.y;
-                vi += p.x * x.y + p.y * x.x;
+    const int tq = (tid / 16) * 4, tf = (tid % 16) * 4;  // micro-tile origin inside the output tile
+    for (int q0 = 0; q0 < nq; q0 += TQ) {
+        for (int f0 = 0; f0 < n; f0 += TF) {
+            double2 acc[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+            for (int k0 = 0; k0 < n; k0 += KC) {
+                // stage: 64 x 16 of each factor, four elements per thread
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int idx = tid + 256 * e;  // 0 .. 1023
+                    const int qq = idx / KC, rr = idx % KC;
+                    const int q = q0 + qq, r = k0 + rr;
+                    s_rho[rr][qq] = (q < nrow && r < n) ? tab[(size_t)s_orb[q] * Ns + s_site[r]] : make_double2(0.0, 0.0);
+                    const int rk = idx / TF, ff = idx % TF;
+                    const int r2 = k0 + rk, f = f0 + ff;
+                    s_ai[rk][ff] = (r2 < n && f < n) ? ai[(size_t)r2 * nmax + f] : make_double2(0.0, 0.0);
+                }
+                __syncthreads();
+#pragma unroll 4
+                for (int kk = 0; kk < KC; ++kk) {
+                    double2 x[4], y[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) x[i] = s_rho[kk][tq + i];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) y[j] = s_ai[kk][tf + j];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            acc[i][j].x = fma(x[i].x, y[j].x, acc[i][j].x);
+                            acc[i][j].x = fma(-x[i].y, y[j].y, acc[i][j].x);
+                            acc[i][j].y = fma(x[i].x, y[j].y, acc[i][j].y);
+                            acc[i][j].y = fma(x[i].y, y[j].x, acc[i][j].y);
+                        }
+                }
+                __syncthreads();
             }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int q = q0 + tq + i, f = f0 + tf + j;
+                    if (q < nq && f < n) {
+                        C v;
+                        v.x = (T)acc[i][j].x;
+                        v.y = (T)acc[i][j].y;
+                        Q[(size_t)q * nmax + f] = v;
+                    }
+                }
         }
-        C v;
-        v.x = (T)vr;
-        v.y = (T)vi;
-        Q[(size_t)q * nmax + f] = v;
     }
 }
 
@@ -1483,6 +1386,7 @@ extern "C" hipError_t gv_launch_exc_qbuild(int prec, const gv::ExcArgs *a, const
                                            hipStream_t st)
 {
     if (!a->Q || !a->qorb) return hipErrorInvalidValue;
+    if (a->nq0 > 256 || a->nq1 > 256 || a->n_up > 256 || a->n_dn > 256) return hipErrorInvalidValue;
     if (prec == 0)
         hipLaunchKernelGGL(exc_qbuild_kernel<double>, dim3(2 * nwalkers), dim3(256), 0, st, *a, Ai64);
     else
@@ -1490,8 +1394,3 @@ extern "C" hipError_t gv_launch_exc_qbuild(int prec, const gv::ExcArgs *a, const
     return hipGetLastError();
 }
 
-extern "C" hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st)
-{
-    hipLaunchKernelGGL(gx::exc_build_kernel, dim3(2 * nwalkers), dim3(256), 0, st, *a, work);
-    return hipGetLastError();
-}

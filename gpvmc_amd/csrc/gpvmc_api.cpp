@@ -98,7 +98,7 @@ struct gpvmc_ctx {
     size_t scratch_bytes = 0;
     // StatSpinStruct sums [W][ss_block(Ns)], allocated on first request
     double *d_ss = nullptr;
-    // working matrices of the streamed Gauss-Jordan variant (large N only)
+    // working matrices of the streamed FP64 inversion (N > 128 only)
     double2 *d_gjwork = nullptr;
     // channel (exciton) contexts: N is the larger flavour size, part is [W][2][N]
     bool channel = false;
@@ -276,21 +276,37 @@ static hipError_t launch_exc_walk(gpvmc_ctx *c, gv::ExcArgs a, int64_t n_steps, 
 
 // channel contexts: FP64 Gauss-Jordan of the current state's matrices, then the weight
 // bookkeeping (mode 2: also the choice of the dominant state, mode 3: weight only)
-static int run_refresh_channel(gpvmc_ctx *c, int mode)
+// FP64 inversion of every walker's Slater matrices (both flavours), the role of
+// SlaterDeterminant::Init / linalg::DetInv; mode 0 ground state, 1 channel
+static gv::DIArgs detinv_args(gpvmc_ctx *c, int mode)
 {
-    gv::ExcArgs a = exc_args(c, mode);
-    gv::GJDynArgs g{};
-    g.work = c->d_work;
+    gv::DIArgs g{};
+    g.mode = mode;
+    g.n_up = c->channel ? c->n_up : c->N;
+    g.n_dn = c->channel ? c->n_dn : c->N;
+    g.nstride = c->N;
+    g.Ns = c->Ns;
+    g.ld = gv_detinv_ld(c->N);
+    g.part = c->d_part;
+    g.cur = c->d_cur;
+    g.group = c->d_group;
+    g.occ = c->d_occ64;
+    g.tab = c->d_tab64;
     g.Ai64 = c->d_Ai64;
-    g.Ai32 = (c->prec == GPVMC_MIXED && !c->regk) ? c->d_Ai32 : nullptr;
+    g.Ai32 = (c->channel && c->prec == GPVMC_MIXED && !c->regk) ? c->d_Ai32 : nullptr;
     g.logabs = c->d_logabs;
     g.phase = c->d_phase;
     g.status = c->d_status;
-    g.n_up = c->n_up;
-    g.n_dn = c->n_dn;
+    g.work = c->channel ? c->d_work : c->d_gjwork;
+    return g;
+}
+
+static int run_refresh_channel(gpvmc_ctx *c, int mode)
+{
+    gv::ExcArgs a = exc_args(c, mode);
+    const gv::DIArgs g = detinv_args(c, 1);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(gv_launch_exc_build(&a, c->d_work, c->W, c->stream));
-    HIPCHK(gv_launch_gj_dyn(&g, c->W, c->stream));
+    HIPCHK(gv_launch_detinv(&g, 2 * c->W, c->stream));
     if (c->regk)
         HIPCHK(gv_launch_excr_mbuild(c->prec, &a, c->d_Ai64, c->W, c->stream));
     else
@@ -309,18 +325,10 @@ static int run_refresh_channel(gpvmc_ctx *c, int mode)
 // G = Ai Phi_other in FP64, stored in the working precision.
 static int run_refresh(gpvmc_ctx *c, double *drift_dev)
 {
-    gv::GJArgs a{};
-    a.Ai = c->d_Ai64;
-    a.part = c->d_part;
-    a.occ = c->d_occ64;
-    a.group = c->d_group;
-    a.logabs = c->d_logabs;
-    a.phase = c->d_phase;
-    a.status = c->d_status;
-    if (const size_t wb = gv_gj_work_bytes(c->N)) {
-        if (!c->d_gjwork) HIPCHK(hipMalloc(&c->d_gjwork, wb * (size_t)c->W));
-        a.work = c->d_gjwork;
+    if (const size_t wb = gv_detinv_work_bytes(c->N)) {
+        if (!c->d_gjwork) HIPCHK(hipMalloc(&c->d_gjwork, wb * 2 * (size_t)c->W));
     }
+    const gv::DIArgs a = detinv_args(c, 0);
     gv::GBArgs b{};
     b.G = c->d_G;
     b.Ai = c->d_Ai64;
@@ -329,7 +337,7 @@ static int run_refresh(gpvmc_ctx *c, double *drift_dev)
     b.group = c->d_group;
     b.drift = drift_dev;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(gv_launch_gj(c->N, &a, c->W, c->stream));
+    HIPCHK(gv_launch_detinv(&a, 2 * c->W, c->stream));
     HIPCHK(gv_launch_gbuild(c->prec, c->N, &b, c->W, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
@@ -477,6 +485,8 @@ static int walk_channel(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t ma
 }
 
 extern "C" {
+
+int gv_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
 
 const char *gpvmc_last_error(void) { return g_err.c_str(); }
 
@@ -1131,7 +1141,7 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
     }
     // [G][fl][site][orbital] orbital table of both channel kernels
     CK(hipMalloc(&c->d_tabT, (size_t)c->G * 2 * Ns * Ns * ts));
-    CK(hipMalloc(&c->d_work, W * 2 * nm * nm * sizeof(double2)));
+    if (const size_t wb = gv_detinv_work_bytes(nm)) CK(hipMalloc(&c->d_work, wb * 2 * W));
     CK(hipMalloc(&c->d_site, W * Ns * sizeof(int32_t)));
     CK(hipMalloc(&c->d_part, W * 2 * nm * sizeof(int32_t)));
     CK(hipMalloc(&c->d_cur, W * 2 * nm * sizeof(int32_t)));

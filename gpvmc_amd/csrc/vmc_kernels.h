@@ -47,17 +47,6 @@ struct SweepArgs {
 // (both receive -ratio of the same swap), spp and smm vanish for two flavours.
 __host__ __device__ constexpr size_t ss_block(size_t Ns) { return 3 * Ns * Ns + Ns + 1; }
 
-struct GJArgs {
-    double2 *Ai;              // [W][2][N][N] FP64 inverse, Ai[fl][label][slot]
-    const int32_t *part;      // [W][2][N]
-    const double2 *occ;       // [G][2][Ns][N] FP64
-    const int32_t *group;     // [W]
-    double *logabs;           // [W][2]
-    double *phase;            // [W][2]
-    int32_t *status;          // [W][2]
-    double2 *work;            // [W][2][N][N] working matrices of the streamed variant, or null
-};
-
 struct GBArgs {
     void *G;                  // OUT2 [W][2][N][N]
     const double2 *Ai;        // [W][2][N][N] FP64
@@ -142,14 +131,29 @@ __device__ __forceinline__ void philox4x32_10_dev(uint32_t c[4], uint32_t k0, ui
     }
 }
 
-// runtime-size FP64 Gauss-Jordan on prebuilt matrices (channel contexts)
-struct GJDynArgs {
-    double2 *work;            // [W][2][nmax][nmax] A[slot][label] in, destroyed
-    double2 *Ai64;            // [W][2][nmax][nmax] out, Ai[label][slot]
-    float2 *Ai32;             // same in FP32, or null
-    double *logabs, *phase;   // [W][2]
-    int32_t *status;          // [W][2] 1 = singular
-    int32_t n_up, n_dn;
+// FP64 inversion with log-determinant (detinv_kernels.hip), one block per matrix.
+// Modes 0/1: block 2w + fl inverts flavour fl of walker w, A(f, r) = Phi(orbital of slot f,
+// site of label r) from the occupied-orbital table (mode 0, ground state: slot f holds
+// orbital column f of occ) or from the full table and the slots' orbitals (mode 1,
+// channels); out Ai[w][fl][label][slot] with row length nstride.  Mode 2: block b inverts
+// the caller's column-major matrix A[b] (element (i, j) at i + j n) into the same layout.
+struct DIArgs {
+    int32_t mode;
+    int32_t n_up, n_dn;       // matrix size of flavour 0 / 1 (mode 2: n_up for every matrix)
+    int32_t nstride;          // modes 0/1: row length of part, cur and the outputs
+    int32_t Ns;               // sites (modes 0/1)
+    int32_t ld;               // leading dimension of `work` (streamed kernel, n > 128)
+    const int32_t *part;      // [W][2][nstride] site of each label
+    const int32_t *cur;       // [W][2][nstride] orbital in each slot (mode 1)
+    const int32_t *group;     // [W]
+    const double2 *occ;       // mode 0: [G][2][Ns][n]
+    const double2 *tab;       // mode 1: [G][2][Ns][Ns] Phi[fl][orbital][site]
+    const double2 *A;         // mode 2: [batch][n][n] column-major
+    double2 *Ai64;            // out, FP64
+    float2 *Ai32;             // modes 0/1: the same in FP32, or null
+    double *logabs, *phase;   // [blocks] ln|det|, arg det
+    int32_t *status;          // [blocks] 1 = singular (|det|^2 < 1e-9 or a zero pivot)
+    double2 *work;            // [blocks][n][ld] working matrices of the streamed kernel, or null
 };
 
 }  // namespace gv
@@ -164,17 +168,20 @@ hipError_t gv_launch_sweep_wdu(int prec, int N, int lpr, const gv::SweepArgs *a,
 hipError_t gv_launch_sweep(int prec, int N, int lpr, const gv::SweepArgs *a, int nblocks,
                            hipStream_t st);
 int gv_refresh_supported(int prec, int N);
-// bytes of GJArgs::work the refresh of N x N matrices needs per walker (0: none)
-size_t gv_gj_work_bytes(int N);
-hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st);
 hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st);
 // out[g][e] = sum over the walkers idx[off[g] .. off[g+1]) of ss[walker][e], e < per
 hipError_t gv_launch_exc(int prec, const gv::ExcArgs *a, int nblocks, hipStream_t st);
 size_t gv_exc_lds_bytes(int Ns, int nmax, int nk, int nqm);
 // Q rows of the streamed channel kernel from the FP64 inverse after Gauss-Jordan
 hipError_t gv_launch_exc_qbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers, hipStream_t st);
-hipError_t gv_launch_exc_build(const gv::ExcArgs *a, double2 *work, int nwalkers, hipStream_t st);
-hipError_t gv_launch_gj_dyn(const gv::GJDynArgs *a, int nwalkers, hipStream_t st);
+// records the message returned by gpvmc_last_error and returns code (gpvmc_api.cpp)
+int gv_fail(int code, const char *msg);
+// FP64 inversion with log-determinant: 1 if n is supported, the work bytes per block
+// (0 when the register-resident kernel covers n) and the work buffer's leading dimension
+int gv_detinv_supported(int n);
+size_t gv_detinv_work_bytes(int n);
+int gv_detinv_ld(int n);
+hipError_t gv_launch_detinv(const gv::DIArgs *a, int nblocks, hipStream_t st);
 // register-resident channel kernel: the smallest registered shape (column blocks >= cb_min,
 // row-block slots >= row_blocks) of a precision (1 = found)
 int gv_excr_shape(int prec, int cb_min, int row_blocks, int *cb, int *rbw);

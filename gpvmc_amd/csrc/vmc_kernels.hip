@@ -1127,316 +1127,6 @@ sweep_kernel(SweepArgs a)
 }
 
 // ---------------------------------------------------------------------------------
-// FP64 Gauss-Jordan inversion with partial pivoting, one block per (walker, flavour).
-// Builds A(f, r) = Phi(orb_f, site(r)) from the configuration, applies N Jordan exchanges
-// with the pivot row of each column chosen as the largest |re|+|im| among unpivoted rows
-// (the zgetrf choice, linalg.cpp:16), and writes Ai[label][slot] = (A^-1)(label, slot) in
-// FP64.  ln|det| and arg(det) include the permutation sign (linalg.cpp:17-22);
-// status[w*2+fl] = 1 when |det|^2 < 1e-9 (linalg.cpp:23) or a pivot vanished.
-template <int N, int LPR>
-__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
-gj_kernel(GJArgs a)
-{
-    using C = double2;
-    constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    constexpr int NW = NT / 64;
-    constexpr int CPT = N / LPR;
-    constexpr int NS = 2 * N;
-    static_assert(N % LPR == 0, "LPR must divide N");
-
-    __shared__ int s_part[N];
-    __shared__ C s_prow[2][NW][N];     // [parity][wave] best candidate row of that wave
-    __shared__ unsigned long long s_ckey[2][NW];
-    __shared__ int s_perm[N];          // pivot row chosen for column k
-    __shared__ int s_invp[N];          // column at which row i was pivoted
-    __shared__ int s_flag;
-    __shared__ unsigned char s_seen[N];
-
-    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int row = tid / LPR, c = tid % LPR;
-    const bool has_row = row < N;
-
-    for (int i = tid; i < N; i += NT) {
-        s_part[i] = a.part[(size_t)w * 2 * N + (size_t)fl * N + i];
-        s_invp[i] = 0;
-    }
-    if (tid == 0) s_flag = 0;
-    __syncthreads();
-
-    const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
-    C b[CPT];
-    if (has_row) {
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) b[k] = occ[(size_t)s_part[c + LPR * k] * N + row];
-    } else {
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) b[k] = make_double2(0.0, 0.0);
-    }
-    bool pivoted = !has_row;
-    // determinant as a binary mantissa/exponent in thread 0 (no logarithm per pivot)
-    C dm = make_double2(1.0, 0.0);
-    int de = 0;
-    int par = 0;
-    const int gbase = tid & ~(LPR - 1);
-
-    // Column k = ck + LPR*kk of the matrix lives in lane ck of each row group, slot kk.
-    // Each row's register array is rotated left by one slot after every LPR columns, so
-    // the current column is always slot 0 (a compile-time register index).
-    for (int kk = 0; kk < CPT; ++kk) {
-        for (int ck = 0; ck < LPR; ++ck) {
-            const int k = ck + LPR * kk;
-            C bk;
-            bk.x = __shfl(b[0].x, gbase + ck);
-            bk.y = __shfl(b[0].y, gbase + ck);
-            // pivot search: largest |re|+|im| among unpivoted rows, lowest row on ties;
-            // key = magnitude bits (non-negative float, order-preserving) : ~row
-            const float mag = (float)(fabs(bk.x) + fabs(bk.y));
-            unsigned long long key = 0ull;
-            if (!pivoted)
-                key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long ok2 = __shfl_xor(key, o);
-                key = ok2 > key ? ok2 : key;
-            }
-            const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
-            // the wave's best row publishes itself as that wave's candidate
-            if (has_row && !pivoted && key != 0ull && row == wrow) {
-#pragma unroll
-                for (int t = 0; t < CPT; ++t) {
-                    const int ot = t + kk < CPT ? t + kk : t + kk - CPT;
-                    s_prow[par][wid][c + LPR * ot] = b[t];
-                }
-            }
-            if (lane == 0) s_ckey[par][wid] = key;
-            __syncthreads();
-            int bw = 0;
-            unsigned long long bkey = s_ckey[par][0];
-#pragma unroll
-            for (int q = 1; q < NW; ++q) {
-                const unsigned long long v = s_ckey[par][q];
-                if (v > bkey) { bkey = v; bw = q; }
-            }
-            const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
-            const C piv = s_prow[par][bw][k];
-            const double np = piv.x * piv.x + piv.y * piv.y;
-            C ainv;
-            if (bkey != 0ull && np > 0.0) {
-                ainv = make_double2(piv.x / np, -piv.y / np);
-            } else {
-                ainv = make_double2(0.0, 0.0);
-                if (tid == 0) s_flag = 1;
-            }
-            if (tid == 0) {
-                s_perm[k] = (p >= 0 && p < N) ? p : 0;
-                if (p >= 0 && p < N) s_invp[p] = k;
-                const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
-                int e;
-                (void)frexp(fmax(fabs(re), fabs(im)), &e);
-                dm = make_double2(ldexp(re, -e), ldexp(im, -e));
-                de += e;
-            }
-            if (has_row) {
-                const bool me = (row == p);
-                const C bka = cmul(bk, ainv);
-#pragma unroll
-                for (int t = 0; t < CPT; ++t) {
-                    const int ot = t + kk < CPT ? t + kk : t + kk - CPT;
-                    const int j = c + LPR * ot;
-                    const C rj = s_prow[par][bw][j];
-                    if (me) {
-                        b[t] = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
-                    } else {
-                        b[t] = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, b[t]);
-                    }
-                }
-                if (me) pivoted = true;
-            }
-            par ^= 1;
-        }
-        // rotate left by one slot
-        const C first = b[0];
-#pragma unroll
-        for (int t = 0; t + 1 < CPT; ++t) b[t] = b[t + 1];
-        b[CPT - 1] = first;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        // sign of the permutation k -> perm[k] by cycle counting
-        int sgn = 1;
-        for (int i = 0; i < N; ++i) s_seen[i] = 0;
-        for (int i = 0; i < N; ++i) {
-            if (s_seen[i]) continue;
-            int len = 0, j = i;
-            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
-            if ((len & 1) == 0) sgn = -sgn;
-        }
-        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
-        double ph = atan2(dm.y, dm.x);
-        if (sgn < 0) ph += M_PI;
-        const int sing = s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
-        a.logabs[2 * w + fl] = la;
-        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
-        a.status[2 * w + fl] = sing;
-    }
-    // A^-1[invp[i]][perm[j]] = B[i][j]  (Jordan exchange with implicit pivoting)
-    C *out = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
-    if (has_row) {
-        int orow = s_invp[row];
-        orow = orow < 0 ? 0 : (orow < N ? orow : N - 1);  // in range even for a NaN matrix
-#pragma unroll
-        for (int t = 0; t < CPT; ++t) {
-            int oc = s_perm[c + LPR * t];
-            oc = oc < 0 ? 0 : (oc < N ? oc : N - 1);
-            out[(size_t)orow * N + oc] = b[t];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------
-// The same Gauss-Jordan inversion for matrices too large for a CU's registers
-// (N = 200: 640 KiB per matrix in FP64).  Thread (row, c) keeps its part of the working
-// row in global memory (GJArgs::work, read and written only by that thread); the pivot
-// row of each step is snapshotted in LDS as in gj_kernel, so the pivot search, the
-// arithmetic and the output permutation are those of gj_kernel.
-template <int N, int LPR>
-__global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
-gj_stream_kernel(GJArgs a)
-{
-    using C = double2;
-    constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    constexpr int NW = NT / 64;
-    constexpr int CPT = N / LPR;
-    constexpr int NS = 2 * N;
-    static_assert(N % LPR == 0, "LPR must divide N");
-
-    __shared__ int s_part[N];
-    __shared__ C s_prow[2][NW][N];
-    __shared__ unsigned long long s_ckey[2][NW];
-    __shared__ int s_perm[N];
-    __shared__ int s_invp[N];
-    __shared__ int s_flag;
-    __shared__ unsigned char s_seen[N];
-
-    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int row = tid / LPR, c = tid % LPR;
-    const bool has_row = row < N;
-
-    for (int i = tid; i < N; i += NT) {
-        s_part[i] = a.part[(size_t)w * 2 * N + (size_t)fl * N + i];
-        s_invp[i] = 0;
-    }
-    if (tid == 0) s_flag = 0;
-    __syncthreads();
-
-    const C *occ = a.occ + (size_t)a.group[w] * 2 * NS * N + (size_t)fl * NS * N;
-    C *M = a.work + ((size_t)w * 2 + fl) * N * N;     // M[row][j], row-major
-    if (has_row) {
-        for (int t = 0; t < CPT; ++t) {
-            const int j = c + LPR * t;
-            M[(size_t)row * N + j] = occ[(size_t)s_part[j] * N + row];
-        }
-    }
-    bool pivoted = !has_row;
-    C dm = make_double2(1.0, 0.0);
-    int de = 0;
-    int par = 0;
-    for (int k = 0; k < N; ++k) {
-        const C bk = has_row ? M[(size_t)row * N + k] : make_double2(0.0, 0.0);
-        const float mag = (float)(fabs(bk.x) + fabs(bk.y));
-        unsigned long long key = 0ull;
-        if (!pivoted)
-            key = ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned int)(~row);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long ok2 = __shfl_xor(key, o);
-            key = ok2 > key ? ok2 : key;
-        }
-        const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
-        if (has_row && !pivoted && key != 0ull && row == wrow) {
-            for (int t = 0; t < CPT; ++t) {
-                const int j = c + LPR * t;
-                s_prow[par][wid][j] = M[(size_t)row * N + j];
-            }
-        }
-        if (lane == 0) s_ckey[par][wid] = key;
-        __syncthreads();
-        int bw = 0;
-        unsigned long long bkey = s_ckey[par][0];
-        for (int q = 1; q < NW; ++q) {
-            const unsigned long long v = s_ckey[par][q];
-            if (v > bkey) { bkey = v; bw = q; }
-        }
-        const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
-        const C piv = s_prow[par][bw][k];
-        const double np = piv.x * piv.x + piv.y * piv.y;
-        C ainv;
-        if (bkey != 0ull && np > 0.0) {
-            ainv = make_double2(piv.x / np, -piv.y / np);
-        } else {
-            ainv = make_double2(0.0, 0.0);
-            if (tid == 0) s_flag = 1;
-        }
-        if (tid == 0) {
-            s_perm[k] = (p >= 0 && p < N) ? p : 0;
-            if (p >= 0 && p < N) s_invp[p] = k;
-            const double re = dm.x * piv.x - dm.y * piv.y, im = dm.x * piv.y + dm.y * piv.x;
-            int e;
-            (void)frexp(fmax(fabs(re), fabs(im)), &e);
-            dm = make_double2(ldexp(re, -e), ldexp(im, -e));
-            de += e;
-        }
-        if (has_row) {
-            const bool me = (row == p);
-            const C bka = cmul(bk, ainv);
-            for (int t = 0; t < CPT; ++t) {
-                const int j = c + LPR * t;
-                const C rj = s_prow[par][bw][j];
-                C &m = M[(size_t)row * N + j];
-                if (me) {
-                    m = (j == k) ? ainv : cmk<C>(-(rj.x * ainv.x - rj.y * ainv.y), -(rj.x * ainv.y + rj.y * ainv.x));
-                } else {
-                    m = (j == k) ? bka : cfma(cmk<C>(-bka.x, -bka.y), rj, m);
-                }
-            }
-            if (me) pivoted = true;
-        }
-        par ^= 1;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int sgn = 1;
-        for (int i = 0; i < N; ++i) s_seen[i] = 0;
-        for (int i = 0; i < N; ++i) {
-            if (s_seen[i]) continue;
-            int len = 0, j = i;
-            while (!s_seen[j]) { s_seen[j] = 1; j = s_perm[j]; ++len; }
-            if ((len & 1) == 0) sgn = -sgn;
-        }
-        const double la = log(hypot(dm.x, dm.y)) + de * 0.69314718055994531;
-        double ph = atan2(dm.y, dm.x);
-        if (sgn < 0) ph += M_PI;
-        const int sing = s_flag || (2.0 * la < -20.72326583694641);  // ln(1e-9)
-        a.logabs[2 * w + fl] = la;
-        a.phase[2 * w + fl] = remainder(ph, 2 * M_PI);
-        a.status[2 * w + fl] = sing;
-    }
-    C *out = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
-    if (has_row) {
-        int orow = s_invp[row];
-        orow = orow < 0 ? 0 : (orow < N ? orow : N - 1);
-        for (int t = 0; t < CPT; ++t) {
-            const int j = c + LPR * t;
-            int oc = s_perm[j];
-            oc = oc < 0 ? 0 : (oc < N ? oc : N - 1);
-            out[(size_t)orow * N + oc] = M[(size_t)row * N + j];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------
 // Ratio tables from the FP64 inverses, one block per (walker, flavour):
 // G_fl[i][j] = sum_f Ai_fl[i][f] Phi_fl(orb_f, site of particle j of the other flavour),
 // accumulated in FP64 over LDS-staged chunks of FB orbitals and written as OUT.  If
@@ -1569,23 +1259,6 @@ static hipError_t launch_sweep_wdu_t(const SweepArgs &a, int nblocks, hipStream_
     return hipGetLastError();
 }
 
-template <int N, int LPR>
-static hipError_t launch_gj_t(const GJArgs &a, int nwalkers, hipStream_t st)
-{
-    constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    hipLaunchKernelGGL((gj_kernel<N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
-    return hipGetLastError();
-}
-
-template <int N, int LPR>
-static hipError_t launch_gj_stream_t(const GJArgs &a, int nwalkers, hipStream_t st)
-{
-    constexpr int NT = ((N * LPR + 63) / 64) * 64;
-    if (!a.work) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gj_stream_kernel<N, LPR>), dim3(2 * nwalkers), dim3(NT), 0, st, a);
-    return hipGetLastError();
-}
-
 template <typename OUT, int N, int LPR>
 static hipError_t launch_gb_t(const GBArgs &a, int nwalkers, hipStream_t st)
 {
@@ -1601,10 +1274,6 @@ struct SweepEntry {
 struct SweepWduEntry {
     int prec, N, lpr;
     hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
-};
-struct GJEntry {
-    int N, lpr, streamed;
-    hipError_t (*fn)(const GJArgs &, int, hipStream_t);
 };
 struct GBEntry {
     int prec, N, lpr;
@@ -1634,15 +1303,6 @@ static const SweepEntry kSweep[] = {
 };
 static const SweepWduEntry kSweepWdu[] = {
     {1, 128, 4, launch_sweep_wdu_t<float, 128, 4>},
-};
-static const GJEntry kGJ[] = {
-    {8, 8, 0, launch_gj_t<8, 8>},
-    {18, 2, 0, launch_gj_t<18, 2>},
-    {32, 4, 0, launch_gj_t<32, 4>},
-    {50, 2, 0, launch_gj_t<50, 2>},
-    {72, 4, 0, launch_gj_t<72, 4>},
-    {128, 4, 0, launch_gj_t<128, 4>},
-    {200, 2, 1, launch_gj_stream_t<200, 2>},
 };
 static const GBEntry kGB[] = {
     {0, 8, 8, launch_gb_t<double, 8, 8>},
@@ -1718,27 +1378,10 @@ extern "C" hipError_t gv_launch_sweep_wdu(int prec, int N, int lpr, const gv::Sw
 
 extern "C" int gv_refresh_supported(int prec, int N)
 {
-    int ok = 0;
-    for (const auto &e : gv::kGJ)
-        if (e.N == N) ok = 1;
-    if (!ok) return 0;
+    if (!gv_detinv_supported(N)) return 0;
     for (const auto &e : gv::kGB)
         if (e.prec == prec && e.N == N) return 1;
     return 0;
-}
-
-extern "C" size_t gv_gj_work_bytes(int N)
-{
-    for (const auto &e : gv::kGJ)
-        if (e.N == N) return e.streamed ? (size_t)2 * N * N * sizeof(double2) : 0;
-    return 0;
-}
-
-extern "C" hipError_t gv_launch_gj(int N, const gv::GJArgs *a, int nwalkers, hipStream_t st)
-{
-    for (const auto &e : gv::kGJ)
-        if (e.N == N) return e.fn(*a, nwalkers, st);
-    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int nwalkers, hipStream_t st)

@@ -223,6 +223,22 @@ int gpvmc_read_inverse(gpvmc_ctx *ctx, int32_t walker, double *ai);
  * SlaterDeterminant::VirtUpdate computes for every single swap (SlaterDeterminant.cpp:96-268). */
 int gpvmc_read_ratio_table(gpvmc_ctx *ctx, int32_t walker, double *g);
 
+/* ---- dense linear algebra on caller device memory --------------------------------
+ * Batched linalg::DetInv (linalg.cpp:9-44): for b < batch, the inverse of the complex
+ * n x n matrix a[b] (column-major, element (i, j) at a[b][i + j n], complex interleaved)
+ * into ai[b] in the same layout, logabs[b] = ln|det|, phase[b] = arg det and singular[b] =
+ * 1 when |det|^2 < 1e-9 (the reference's test) or a pivot vanishes.  Gauss-Jordan
+ * elimination with partial pivoting (largest |re| + |im| of the column, the zgetrf choice)
+ * in FP64.  All array arguments are device pointers; `stream` is a hipStream_t (NULL: the
+ * null stream) and the call returns once the work is queued on it.  1 <= n <= 256.  For
+ * n > 128 the kernel works in a device workspace of gpvmc_det_inv_workspace_bytes(n) per
+ * matrix, which the library allocates on first need and keeps (grown as needed) for later
+ * calls on the device; calls that overlap on one device share it and race. */
+int gpvmc_det_inv(int32_t device, int32_t n, int32_t batch, const double *a, double *ai,
+                  double *logabs, double *phase, int32_t *singular, void *stream);
+/* *bytes = workspace bytes per matrix of gpvmc_det_inv at size n (0 when none is needed) */
+int gpvmc_det_inv_workspace_bytes(int32_t n, int64_t *bytes);
+
 /* Cumulative device time of the sweep / refresh kernels (HIP events on the context's
  * stream) and the number of launches since creation or the last reset. */
 int gpvmc_kernel_time(gpvmc_ctx *ctx, double *sweep_ms, int64_t *sweep_launches,

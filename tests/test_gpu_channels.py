@@ -239,7 +239,11 @@ def test_streamed_mixed_measurement_matches_fp64(L):
     ok32, acc32, ho32, cs32 = res[binding.MIXED]
     n = 0
     for w in range(W):
-        if not (ok64[w] and ok32[w]) or acc64[w][0] != acc32[w][0] or cs64[w] != cs32[w]:
+        # H and O are sums of conj(a_k1) h_k2 / w over all states: the same configuration
+        # gives the same sums whichever state the walker keeps as its current one (random
+        # starts can hold two states of equal |a_k|, where FP32 and FP64 rounding pick
+        # different ones)
+        if not (ok64[w] and ok32[w]) or acc64[w][0] != acc32[w][0]:
             continue
         scale = np.abs(ho64[w]).max()
         assert scale > 0
