@@ -57,6 +57,8 @@ SIGNATURES = {
     "gpvmc_det_inv": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
     "gpvmc_det_inv_workspace_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "gpvmc_det_inv_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, _vp,
+                                             _vp, _vp]),
     "gpvmc_reset_kernel_time": (ctypes.c_int, [_vp]),
 }
 
@@ -134,6 +136,13 @@ def det_inv(n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr, singular_ptr, stream
     lib = load_library()
     _check_rc(lib, lib.gpvmc_det_inv(device, n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr, singular_ptr,
                                      stream))
+
+
+def det_inv_noalloc(n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr, singular_ptr, stream=None, device=0):
+    """gpvmc_det_inv with its workspace in the library's fixed device array (no allocation)"""
+    lib = load_library()
+    _check_rc(lib, lib.gpvmc_det_inv_noalloc(device, n, batch, a_ptr, ai_ptr, logabs_ptr, phase_ptr,
+                                             singular_ptr, stream))
 
 
 class Batch:

@@ -644,3 +644,58 @@ extern "C" int gpvmc_det_inv_workspace_bytes(int32_t n, int64_t *bytes)
     *bytes = (int64_t)gv_detinv_work_bytes(n);
     return GPVMC_OK;
 }
+
+// ---- allocation-free variant ------------------------------------------------------------
+// For callers that cannot allocate device memory (code under graph capture, serving loops):
+// the streamed kernel's workspace is a fixed file-scope device array.  Each device holds
+// its own copy until the process ends; its address is looked up once per device and used
+// in place of the array's host-side name.
+
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // 64 MiB per device
+
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scr_mu;
+void *g_scr_ptr[kMaxDevices];  // device address of `scratch` on each device
+
+// the device address of `scratch` on `device` (current), looked up on the first call
+int scratch_ptr(int32_t device, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return gv_fail(GPVMC_ERR_HIP, (std::string("scratch address: ") + hipGetErrorString(e)).c_str());
+        g_scr_ptr[device] = p;
+    }
+    *out = g_scr_ptr[device];
+    return GPVMC_OK;
+}
+
+}  // namespace
+
+extern "C" int gpvmc_det_inv_noalloc(int32_t device, int32_t n, int32_t batch, const double *a, double *ai,
+                                     double *logabs, double *phase, int32_t *singular, void *stream)
+{
+    int rc = detinv_check(device, n, batch, a, ai, logabs, phase, singular);
+    if (rc) return rc;
+    const size_t per = gv_detinv_work_bytes(n);
+    if (per * (size_t)batch > kNoAllocScratchBytes)
+        return gv_fail(GPVMC_ERR_ARG,
+                       ("gpvmc_det_inv_noalloc: " + std::to_string(batch) + " matrices of size " + std::to_string(n) +
+                        " need " + std::to_string(per * (size_t)batch) + " bytes of workspace, above its fixed " +
+                        std::to_string(kNoAllocScratchBytes) + "; at most " +
+                        std::to_string(kNoAllocScratchBytes / per) + " per call at this size")
+                           .c_str());
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return gv_fail(GPVMC_ERR_HIP, (std::string("hipSetDevice: ") + hipGetErrorString(e)).c_str());
+    void *work = nullptr;
+    if (per) {
+        rc = scratch_ptr(device, &work);
+        if (rc) return rc;
+    }
+    return detinv_launch(n, batch, a, ai, logabs, phase, singular, work, stream);
+}

@@ -238,6 +238,16 @@ int gpvmc_det_inv(int32_t device, int32_t n, int32_t batch, const double *a, dou
                   double *logabs, double *phase, int32_t *singular, void *stream);
 /* *bytes = workspace bytes per matrix of gpvmc_det_inv at size n (0 when none is needed) */
 int gpvmc_det_inv_workspace_bytes(int32_t n, int64_t *bytes);
+/* gpvmc_det_inv without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The workspace of n > 128 is a fixed 64 MiB device array per
+ * device, held until the process ends: at most floor(64 MiB / workspace bytes) matrices per
+ * call (81 at n = 200, 64 at n = 256); larger batches fail with GPVMC_ERR_ARG before any
+ * work is queued.  Every call on a device shares that array, so calls that overlap on one
+ * device (other streams, other threads) race; the kernel fills its workspace before
+ * reading it, so nothing carries over between calls. */
+int gpvmc_det_inv_noalloc(int32_t device, int32_t n, int32_t batch, const double *a, double *ai,
+                          double *logabs, double *phase, int32_t *singular, void *stream);
 
 /* Cumulative device time of the sweep / refresh kernels (HIP events on the context's
  * stream) and the number of launches since creation or the last reset. */
