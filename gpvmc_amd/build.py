@@ -62,16 +62,25 @@ HOST_LIB = os.path.join(PKG, "lib", "libgpvmc_host.so")
 HOST_EXE = os.path.join(PKG, "lib", "vmc_host")
 
 
+# HDF5 (C library and its high-level API) of the image, for the reference's output files
+HDF5 = os.environ.get("GPVMC_HDF5", "/opt/conda")
+
+
 def build_host(verbose=False):
     """The C++ host classes (include/gpvmc_host.hpp) as libgpvmc_host.so over
-    libgpvmc_hip.so, and the vmc_host driver; plain C++ (no device code)."""
+    libgpvmc_hip.so and HDF5, and the vmc_host driver; plain C++ (no device code)."""
     cxx = os.environ.get("CXX", "g++")
     libdir = os.path.dirname(LIB)
     host = os.path.join(PKG, "host")
     common = ["-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include")]
+    h5lib = os.path.join(HDF5, "lib")
     cmds = [
-        [cxx] + common + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp", os.path.join(host, "gpvmc_host.cpp"),
-                          "-L" + libdir, "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
+        [cxx] + common + ["-I" + os.path.join(HDF5, "include"), "-fPIC", "-shared", "-o", HOST_LIB + ".tmp",
+                          os.path.join(host, "gpvmc_host.cpp"), "-L" + libdir, "-lgpvmc_hip",
+                          os.path.join(h5lib, "libhdf5_hl.so"), os.path.join(h5lib, "libhdf5.so"),
+                          # the system library directory before HDF5's: the system libstdc++ (the
+                          # one ROCm needs), not the older one next to HDF5
+                          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/usr/lib/x86_64-linux-gnu", "-Wl,-rpath," + h5lib],
         [cxx] + common + ["-o", HOST_EXE + ".tmp", os.path.join(host, "vmc_host.cpp"), "-L" + libdir,
                           "-lgpvmc_host", "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
     ]

@@ -3,9 +3,14 @@
 // states and their hop graph, initial configurations, and the calls into libgpvmc_hip.so.
 #include "gpvmc_host.hpp"
 
+#include <hdf5.h>
+#include <hdf5_hl.h>
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <ctime>
 #include <utility>
 
 namespace gpvmc {
@@ -78,6 +83,7 @@ StagFluxWaveFunction::StagFluxWaveFunction(int L, double phi, double neel, doubl
     if (L < 2 || (L * L) % 2) throw Error("L must be >= 2 with L*L even");
     const std::vector<QN> qn = magnetic_zone(L, bcx, bcy);
     if ((int)qn.size() != Ns_) throw Error("magnetic zone enumeration does not give L*L orbitals");
+    for (const QN &q : qn) qn_.insert(qn_.end(), {q.kx, q.ky, q.band});
     // the orbital table (StagFluxWaveFunction.cpp:51-79, 138-163)
     const double pi = std::acos(-1.0), ph = phi * pi;
     table_.assign((size_t)2 * Ns_ * Ns_, 0.0);
@@ -241,7 +247,45 @@ gpvmc_channel StagFluxWaveFunction::GraphABI() const
     return ch;
 }
 
-std::vector<int32_t> RanInit(int L, int n_walkers, int n_up, uint64_t seed)
+std::vector<int32_t> RanInitDraw(int L, int n_up, const std::function<double()> &uniform, int64_t *used)
+{
+    const int Ns = L * L;
+    if (n_up < 0 || n_up > Ns) throw Error("RanInit: n_up out of range");
+    std::vector<int32_t> s(Ns, -1);
+    int64_t n_used = 0;
+    for (int np = 0; np != n_up;) {
+        const double u = uniform();
+        ++n_used;
+        const int v = (int)(u * Ns);
+        if (v < 0 || v >= Ns) throw Error("RanInit: uniform outside [0, 1)");
+        if (s[v] < 0) {
+            s[v] = 0;
+            ++np;
+        }
+    }
+    for (int v = 0; v < Ns; ++v)
+        if (s[v] < 0) s[v] = 1;
+    if (used) *used = n_used;
+    return s;
+}
+
+std::vector<int32_t> RanInit(int L, int n_walkers, int n_up, uint64_t seed, int attempt)
+{
+    const int Ns = L * L;
+    std::vector<int32_t> out((size_t)n_walkers * Ns);
+    for (int w = 0; w < n_walkers; ++w) {
+        uint32_t i = 0;
+        const std::vector<int32_t> s = RanInitDraw(L, n_up, [&]() {
+            uint32_t c[4] = {i++, (uint32_t)attempt, (uint32_t)w, 2u};
+            philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+            return ((double)c[0] + 0.5) * 0x1p-32;
+        });
+        std::copy(s.begin(), s.end(), out.begin() + (size_t)w * Ns);
+    }
+    return out;
+}
+
+std::vector<int32_t> RandomSpins(int L, int n_walkers, int n_up, uint64_t seed)
 {
     const int Ns = L * L;
     std::vector<int32_t> out((size_t)n_walkers * Ns);
@@ -511,8 +555,40 @@ void MetroMC::Walk(int64_t len, int64_t meas)
     std::vector<double> logw, mz;
     if (n_meas && (traces & TraceLogW)) logw.resize((size_t)W * n_meas);
     if (n_meas && (traces & TraceStagMagn)) mz.resize((size_t)W * n_meas * 2);
-    check(gpvmc_walk(amp.Context(), len, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     mz.empty() ? nullptr : mz.data(), logw.empty() ? nullptr : logw.data()));
+    const bool channel = amp.GetWaveFunction().GetChannel() != StagFluxWaveFunction::GroundState;
+    if (channel && !mz.empty()) {
+        // a channel context measures ProjHeis and OverlapTrack on the device but has no m_z
+        // trace: walk each measured step on its own, then read the configuration it measured
+        // (StagMagnTrack.cpp:17-50, m_z = (1/Ns) sum_v e^{i pi (x+y)} (2 up_v - 1)); the
+        // Philox counters run on across the calls, so the chain is the one a single call walks
+        const int L = amp.GetWaveFunction().GetL(), Ns = L * L;
+        std::vector<double> lw1(W);
+        int64_t s = 0, m = 0;
+        while (s < len) {
+            const bool measured = meas == 1 || s % meas == 0;
+            const int64_t seg = measured ? 1 : std::min<int64_t>(len - s, meas - s % meas);
+            check(gpvmc_walk(amp.Context(), seg, measured ? 1 : 0, mask, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, measured && !logw.empty() ? lw1.data() : nullptr));
+            if (measured) {
+                const std::vector<int32_t> sp = amp.Spins();
+                for (int w = 0; w < W; ++w) {
+                    double mr = 0;
+                    for (int v = 0; v < Ns; ++v) {
+                        const int x = v % L, y = v / L;
+                        mr += ((x + y) % 2 ? -1.0 : 1.0) * (sp[(size_t)w * Ns + v] == 0 ? 1.0 : -1.0);
+                    }
+                    mz[((size_t)w * n_meas + m) * 2] = mr / Ns;
+                    mz[((size_t)w * n_meas + m) * 2 + 1] = 0.0;
+                    if (!logw.empty()) logw[(size_t)w * n_meas + m] = lw1[w];
+                }
+                ++m;
+            }
+            s += seg;
+        }
+    } else {
+        check(gpvmc_walk(amp.Context(), len, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr,
+                         mz.empty() ? nullptr : mz.data(), logw.empty() ? nullptr : logw.data()));
+    }
     if (n_meas && traces)
         for (Quantity *q : quantities_)
             if (q->Traces()) q->TakeTraces(W, n_meas, logw.empty() ? nullptr : logw.data(), mz.empty() ? nullptr : mz.data());
@@ -588,6 +664,7 @@ void FileManager::Write()
     for (auto &it : streams_) {
         std::vector<std::vector<Dataset>> &rec = data_[it.first];
         if (rec.empty()) rec.resize(n_chains_);
+        std::vector<int> written(n_chains_);
         for (int c = 0; c < n_chains_; ++c) {
             // a stream with nothing saved since its dataset closed (Stat() == 0) rewrites that
             // dataset, attributes included, as the reference does
@@ -599,8 +676,10 @@ void FileManager::Write()
             d.cols = ms.Cols();
             d.mat = ms.Mat();
             for (const auto &a : dattr_) d.attr[a.first] = a.second[c];
+            written[c] = (int)ds.size() - 1;
             if (ms.Stat() == stat_per_sample_) ms.Stat() = 0;
         }
+        if (!prefix_.empty()) WriteHDF5(it.first, rec, written);
     }
 }
 
@@ -619,71 +698,180 @@ const std::vector<FileManager::Dataset> &FileManager::Datasets(const std::string
 }
 
 namespace {
-std::string json_str(const std::string &s)
-{
-    std::string o = "\"";
-    for (char ch : s) {
-        if (ch == '"' || ch == '\\') o += '\\';
-        if ((unsigned char)ch < 0x20) {
-            char b[8];
-            std::snprintf(b, sizeof b, "\\u%04x", ch);
-            o += b;
-        } else {
-            o += ch;
-        }
+
+// closes an HDF5 id when it leaves scope
+struct H5Id {
+    hid_t id;
+    herr_t (*close)(hid_t);
+    ~H5Id()
+    {
+        if (id >= 0) close(id);
     }
-    return o + "\"";
+};
+
+bool file_exists(const std::string &path)
+{
+    struct stat sb;
+    return stat(path.c_str(), &sb) == 0;
 }
 
-void json_num(FILE *f, double v)
+void root_attributes(hid_t f, const std::map<std::string, double> &num, const std::map<std::string, std::string> &str,
+                     const std::string &type)
 {
-    if (std::isfinite(v))
-        std::fprintf(f, "%.17g", v);
-    else
-        std::fprintf(f, "null");
+    for (const auto &a : num)
+        if (H5LTset_attribute_double(f, "/", a.first.c_str(), &a.second, 1) < 0)
+            throw Error("HDF5: cannot set attribute " + a.first);
+    for (const auto &a : str)
+        if (H5LTset_attribute_string(f, "/", a.first.c_str(), a.second.c_str()) < 0)
+            throw Error("HDF5: cannot set attribute " + a.first);
+    if (H5LTset_attribute_string(f, "/", "type", type.c_str()) < 0) throw Error("HDF5: cannot set attribute type");
 }
+
+hid_t open_group(hid_t f, const std::string &g)
+{
+    if (H5Lexists(f, g.c_str(), H5P_DEFAULT) > 0) return H5Gopen2(f, g.c_str(), H5P_DEFAULT);
+    return H5Gcreate2(f, g.c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+}
+
 }  // namespace
 
-void FileManager::WriteJSON(const std::string &dir, const std::string &prefix) const
+void FileManager::SetOutput(const std::string &dir, const std::string &prefix)
 {
-    for (const auto &q : data_) {
-        const std::string path = dir + "/" + prefix + "-" + q.first + ".json";
-        FILE *f = std::fopen(path.c_str(), "w");
-        if (!f) throw Error("FileManager::WriteJSON: cannot open " + path);
-        // root attributes, then "type" = the quantity (FileManager.cpp:474-490)
-        std::fprintf(f, "{\"attributes\": {");
-        for (const auto &a : fattr_num_) {
-            std::fprintf(f, "%s: ", json_str(a.first).c_str());
-            json_num(f, a.second);
-            std::fprintf(f, ", ");
+    dir_ = dir;
+    prefix_ = prefix;
+}
+
+std::string FileManager::RankGroup(int c) const
+{
+    return "/rank-" + std::to_string(n_chains_ == 1 ? 0 : c + 1);
+}
+
+int64_t FileManager::WriteSimple(const std::string &name) const
+{
+    if (prefix_.empty()) throw Error("FileManager::WriteSimple: no output set");
+    const std::string path = dir_ + "/" + prefix_ + "-" + name + ".h5";
+    const hid_t f = H5Fcreate(path.c_str(), H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) throw Error("Could not create file \"" + path + "\".");
+    root_attributes(f, fattr_num_, fattr_str_, name);
+    return (int64_t)f;
+}
+
+void FileManager::WriteHDF5(const std::string &name, const std::vector<std::vector<Dataset>> &rec,
+                            const std::vector<int> &written) const
+{
+    const std::string path = dir_ + "/" + prefix_ + "-" + name + ".h5";
+    hid_t f;
+    if (!file_exists(path)) {
+        f = H5Fcreate(path.c_str(), H5F_ACC_EXCL, H5P_DEFAULT, H5P_DEFAULT);
+        if (f >= 0) {
+            H5Id guard{f, H5Fclose};
+            root_attributes(f, fattr_num_, fattr_str_, name);
+            const std::time_t t = std::time(nullptr);
+            if (H5LTset_attribute_string(f, "/", "date", std::asctime(std::localtime(&t))) < 0)
+                throw Error("HDF5: cannot set attribute date");
+            guard.id = -1;
         }
-        for (const auto &a : fattr_str_)
-            std::fprintf(f, "%s: %s, ", json_str(a.first).c_str(), json_str(a.second).c_str());
-        std::fprintf(f, "\"type\": %s}", json_str(q.first).c_str());
-        for (int c = 0; c < n_chains_; ++c) {
-            std::fprintf(f, ",\n\"rank-%d\": {", c);
-            const std::vector<Dataset> &ds = q.second[c];
-            for (size_t k = 0; k < ds.size(); ++k) {
-                std::fprintf(f, "%s\"data-%zu\": {\"shape\": [%d, %d], \"attr\": {", k ? ", " : "", k, ds[k].rows,
-                             ds[k].cols);
-                bool fa = true;
-                for (const auto &a : ds[k].attr) {
-                    std::fprintf(f, "%s%s: ", fa ? "" : ", ", json_str(a.first).c_str());
-                    json_num(f, a.second);
-                    fa = false;
-                }
-                std::fprintf(f, "}, \"data\": [");
-                for (size_t e = 0; e < ds[k].mat.size(); ++e) {
-                    if (e) std::fputc(',', f);
-                    json_num(f, ds[k].mat[e]);
-                }
-                std::fprintf(f, "]}");
-            }
-            std::fprintf(f, "}");
-        }
-        std::fprintf(f, "}\n");
-        if (std::fclose(f)) throw Error("FileManager::WriteJSON: write failed for " + path);
+    } else {
+        f = H5Fopen(path.c_str(), H5F_ACC_RDWR, H5P_DEFAULT);
     }
+    if (f < 0) throw Error("Could not create nor open file \"" + path + "\".");
+    H5Id file{f, H5Fclose};
+    for (int c = 0; c < n_chains_; ++c) {
+        const std::string g = RankGroup(c);
+        H5Id grp{open_group(f, g), H5Gclose};
+        if (grp.id < 0) throw Error("HDF5: cannot create group " + g + " in " + path);
+        const int k = written[c];
+        const Dataset &d = rec[c][k];
+        const std::string dn = g + "/data-" + std::to_string(k);
+        const hsize_t dims[2] = {(hsize_t)d.rows, (hsize_t)d.cols};
+        herr_t e;
+        if (H5Lexists(f, dn.c_str(), H5P_DEFAULT) <= 0) {
+            e = H5LTmake_dataset_double(f, dn.c_str(), 2, dims, d.mat.data());
+        } else {
+            H5Id did{H5Dopen2(f, dn.c_str(), H5P_DEFAULT), H5Dclose};
+            e = did.id < 0 ? -1 : H5Dwrite(did.id, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, d.mat.data());
+        }
+        if (e < 0) throw Error("Could not write dataset in file \"" + path + "\".");
+        for (const auto &a : d.attr)
+            if (H5LTset_attribute_double(f, dn.c_str(), a.first.c_str(), &a.second, 1) < 0)
+                throw Error("HDF5: cannot set attribute " + a.first);
+    }
+}
+
+void StagFluxWaveFunction::Save(const FileManager &fm) const
+{
+    H5Id f{(hid_t)fm.WriteSimple("WaveFunction"), H5Fclose};
+    for (int fl = 0; fl < 2; ++fl) {
+        // states_<fl>[e][s] = 1 where state e occupies orbital s of flavour fl
+        std::vector<int> fock((size_t)nk_ * Ns_, 0);
+        for (int e = 0; e < nk_; ++e) {
+            const std::vector<int32_t> &o = states_[e];
+            const int off = fl ? npt_[0] : 0;
+            for (int i = 0; i < npt_[fl]; ++i) fock[(size_t)e * Ns_ + o[off + i]] = 1;
+        }
+        const std::string name = "states_" + std::to_string(fl);
+        const hsize_t dims[2] = {(hsize_t)nk_, (hsize_t)Ns_};
+        if (H5LTmake_dataset_int(f.id, name.c_str(), 2, dims, fock.data()) < 0)
+            throw Error("HDF5: cannot write " + name);
+        H5Id ds{H5Dopen2(f.id, name.c_str(), H5P_DEFAULT), H5Dclose};
+        const hsize_t dim = (hsize_t)Ns_;
+        H5Id sp{H5Screate_simple(1, &dim, nullptr), H5Sclose};
+        const char *keys[3] = {"kx", "ky", "band"};
+        for (int q = 0; q < 3; ++q) {
+            std::vector<unsigned int> v(Ns_);
+            for (int i = 0; i < Ns_; ++i) v[i] = (unsigned int)qn_[3 * i + q];
+            H5Id at{H5Acreate2(ds.id, keys[q], H5T_STD_U32BE, sp.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose};
+            if (at.id < 0 || H5Awrite(at.id, H5T_NATIVE_UINT, v.data()) < 0)
+                throw Error(std::string("HDF5: cannot write attribute ") + keys[q]);
+        }
+    }
+}
+
+void SaveFockState(const FileManager &fm, int L, const std::vector<int32_t> &spins)
+{
+    const int Ns = L * L, W = fm.NChains();
+    if ((int64_t)spins.size() != (int64_t)W * Ns) throw Error("SaveFockState: spins must hold every chain");
+    H5Id f{(hid_t)fm.WriteSimple("FockState"), H5Fclose};
+    for (int fl = 0; fl < 2; ++fl)
+        for (int c = 0; c < W; ++c) {
+            std::vector<int> occ(Ns);
+            for (int v = 0; v < Ns; ++v) occ[v] = spins[(size_t)c * Ns + v] == fl ? 1 : 0;
+            H5Id grp{open_group(f.id, fm.RankGroup(c)), H5Gclose};
+            const hsize_t dim = (hsize_t)Ns;
+            if (grp.id < 0 || H5LTmake_dataset_int(grp.id, ("flavour-" + std::to_string(fl)).c_str(), 1, &dim,
+                                                    occ.data()) < 0)
+                throw Error("HDF5: cannot write the FockState of chain " + std::to_string(c));
+        }
+}
+
+std::vector<int32_t> LoadFockState(const std::string &path, int L, int n_chains)
+{
+    const int Ns = L * L;
+    H5Id f{H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose};
+    if (f.id < 0) throw Error("cannot open spin state file " + path);
+    FileManager names(n_chains);
+    std::vector<int32_t> spins((size_t)n_chains * Ns, -1);
+    for (int c = 0; c < n_chains; ++c)
+        for (int fl = 0; fl < 2; ++fl) {
+            const std::string dn = names.RankGroup(c) + "/flavour-" + std::to_string(fl);
+            int rank = 0;
+            hsize_t dim = 0;
+            if (H5Lexists(f.id, names.RankGroup(c).c_str(), H5P_DEFAULT) <= 0 ||
+                H5Lexists(f.id, dn.c_str(), H5P_DEFAULT) <= 0 || H5LTget_dataset_ndims(f.id, dn.c_str(), &rank) < 0 ||
+                rank != 1 || H5LTget_dataset_info(f.id, dn.c_str(), &dim, nullptr, nullptr) < 0 || (int)dim != Ns)
+                throw Error("spin state file " + path + " has no " + std::to_string(Ns) + "-site dataset " + dn);
+            std::vector<int> occ(Ns);
+            if (H5LTread_dataset_int(f.id, dn.c_str(), occ.data()) < 0) throw Error("cannot read " + dn);
+            for (int v = 0; v < Ns; ++v)
+                if (occ[v]) {
+                    int32_t &s = spins[(size_t)c * Ns + v];
+                    if (s >= 0) throw Error("spin state file " + path + ": site occupied twice in " + dn);
+                    s = fl;
+                }
+        }
+    for (int32_t s : spins)
+        if (s < 0) throw Error("spin state file " + path + ": a site is empty");
+    return spins;
 }
 
 }  // namespace gpvmc
@@ -729,12 +917,38 @@ extern "C" int gpvmc_host_channel(int32_t L, int32_t channel, int32_t qx, int32_
     }
 }
 
-// RanInit's configurations
-extern "C" int gpvmc_host_raninit(int32_t L, int32_t n_walkers, int32_t n_up, uint64_t seed, int32_t *spins)
+// Fisher-Yates configurations (the Python host's model.random_spins)
+extern "C" int gpvmc_host_random_spins(int32_t L, int32_t n_walkers, int32_t n_up, uint64_t seed, int32_t *spins)
 {
-    const std::vector<int32_t> s = gpvmc::RanInit(L, n_walkers, n_up, seed);
+    const std::vector<int32_t> s = gpvmc::RandomSpins(L, n_walkers, n_up, seed);
     std::copy(s.begin(), s.end(), spins);
     return 0;
+}
+
+// RanInit's configurations from Philox (vmc_host's draw for attempt `attempt`)
+extern "C" int gpvmc_host_raninit(int32_t L, int32_t n_walkers, int32_t n_up, uint64_t seed, int32_t attempt,
+                                  int32_t *spins)
+{
+    const std::vector<int32_t> s = gpvmc::RanInit(L, n_walkers, n_up, seed, attempt);
+    std::copy(s.begin(), s.end(), spins);
+    return 0;
+}
+
+// RanInit's draw on a given uniform stream u[0 .. nu): returns the uniforms consumed, -1
+// when the stream runs out
+extern "C" int64_t gpvmc_host_raninit_stream(int32_t L, int32_t n_up, const double *u, int64_t nu, int32_t *spins)
+{
+    try {
+        int64_t i = 0, used = 0;
+        const std::vector<int32_t> s = gpvmc::RanInitDraw(L, n_up, [&]() {
+            if (i >= nu) throw gpvmc::Error("out of uniforms");
+            return u[i++];
+        }, &used);
+        std::copy(s.begin(), s.end(), spins);
+        return used;
+    } catch (const std::exception &) {
+        return -1;
+    }
 }
 
 // OverlapTrack's w_m / mean(w) from ln w_m
@@ -777,6 +991,105 @@ extern "C" int gpvmc_host_filemanager_script(int32_t n_chains, int32_t stat_per_
                 stats[(size_t)c * max_ds + k] = ds[k].attr.at("statistics");
             }
         }
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+// The FileManager / WaveFunction::Save / SaveFockState script of the reference harness's
+// h5layout mode, through these classes: prefix 7 (ground state at L, phi, neel; a
+// ProjHeis-named stream written 4 times, 2 saves per sample; the FockState of `spins`)
+// and prefix 8 (the transverse exciton's WaveFunction file at q = (1, 2)).
+extern "C" int gpvmc_host_h5_script(const char *dir, int32_t L, double phi, double neel, const int32_t *spins)
+{
+    try {
+        {
+            gpvmc::FileManager fm(1);
+            fm.SetOutput(dir, "7");
+            fm.StatPerSample() = 2;
+            fm.FileAttribute("L", (double)L);
+            fm.FileAttribute("phi", phi);
+            fm.FileAttribute("channel", std::string("groundstate"));
+            gpvmc::StagFluxWaveFunction wav(L, phi, neel);
+            wav.Save(fm);
+            for (int s = 0; s < 4; ++s) {
+                const double m[4] = {0.25 * s - 1.0, 1.0 / (s + 3.0), -0.5 * s, 2.0 + s};
+                fm.FileStream("ProjHeis", 0).Write(2, 2, m);
+                fm.DataAttribute("rej", 0.1 * s);
+                fm.DataAttribute("time", 0.0);
+                fm.DataAttribute("statistics", (double)((s % 2 + 1) * 100));
+                fm.Write();
+            }
+            gpvmc::SaveFockState(fm, L, std::vector<int32_t>(spins, spins + L * L));
+        }
+        {
+            gpvmc::FileManager fm(1);
+            fm.SetOutput(dir, "8");
+            fm.FileAttribute("L", (double)L);
+            fm.FileAttribute("channel", std::string("trans"));
+            gpvmc::StagFluxWaveFunction wav(L, phi, neel, 0.0, 1.0, 1.0, gpvmc::StagFluxWaveFunction::TransExciton, 1, 2);
+            wav.Save(fm);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+// A double dataset of an HDF5 file: writes at most `max` values, returns how many it has
+// (-1 when it cannot be read)
+extern "C" int64_t gpvmc_host_h5_read(const char *path, const char *dataset, double *out, int64_t max)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int64_t n = -1;
+    int rank = 0;
+    if (H5Lexists(f, dataset, H5P_DEFAULT) > 0 && H5LTget_dataset_ndims(f, dataset, &rank) >= 0 && rank <= 4) {
+        hsize_t dims[4] = {1, 1, 1, 1};
+        if (H5LTget_dataset_info(f, dataset, dims, nullptr, nullptr) >= 0) {
+            n = 1;
+            for (int i = 0; i < rank; ++i) n *= (int64_t)dims[i];
+            if (n <= max && H5LTread_dataset_double(f, dataset, out) < 0) n = -1;
+        }
+    }
+    H5Fclose(f);
+    return n;
+}
+
+// A scalar double attribute of an object of an HDF5 file (0 on success)
+extern "C" int gpvmc_host_h5_attr(const char *path, const char *object, const char *attr, double *out)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    const int rc = H5LTget_attribute_double(f, object, attr, out) < 0 ? -1 : 0;
+    H5Fclose(f);
+    return rc;
+}
+
+// FockState files for tests: write spins[w][site] of n_chains chains to
+// <dir>/<prefix>-FockState.h5, or read them back (the --spinstate reader)
+extern "C" int gpvmc_host_fockstate_write(const char *dir, const char *prefix, int32_t L, int32_t n_chains,
+                                          const int32_t *spins)
+{
+    try {
+        gpvmc::FileManager fm(n_chains);
+        fm.SetOutput(dir, prefix);
+        gpvmc::SaveFockState(fm, L, std::vector<int32_t>(spins, spins + (size_t)n_chains * L * L));
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+extern "C" int gpvmc_host_fockstate_read(const char *path, int32_t L, int32_t n_chains, int32_t *spins)
+{
+    try {
+        const std::vector<int32_t> s = gpvmc::LoadFockState(path, L, n_chains);
+        std::copy(s.begin(), s.end(), spins);
         return 0;
     } catch (const std::exception &e) {
         g_host_error = e.what();

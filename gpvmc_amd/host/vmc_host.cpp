@@ -7,8 +7,12 @@
 //       (vmc.cpp:69-94, ArgParse's --key=value / --flag syntax) and its sample / save loop
 //       (vmc.cpp:295-317); every walker is one chain (the reference's rank).  Extra flags:
 //       --walkers (default 1) and --precision=fp64|mixed (default fp64).  Each quantity's
-//       records go to <dir>/<prefix>-<Name>.json (FileManager::WriteJSON), the prefix from
-//       --prefix or, when it is negative, from <dir>/.runfile (FileManager.cpp:20-64).
+//       records go to <dir>/<prefix>-<Name>.h5 (/rank-<r>/data-<k> with rej, time and
+//       statistics, FileManager.cpp:465-551), the prefix from --prefix or, when it is
+//       negative, from <dir>/.runfile (FileManager.cpp:20-64); <prefix>-WaveFunction.h5 at
+//       the start (WaveFunction.cpp:85-137) and <prefix>-FockState.h5 at the end
+//       (State.cpp:117-166); --spinstate=<FockState file> starts from its configurations
+//       (vmc.cpp:178-220).
 //   vmc_host L walkers phi neel channel(0 ground, 1 trans, 2 long) qx qy therm_sweeps
 //       samples meas_interval precision(0 FP64, 1 mixed) seed
 //       one measured walk; prints one JSON line of per-walker means.
@@ -148,12 +152,8 @@ int run_flags(int argc, char **argv)
     else
         throw gpvmc::Error("precision must be fp64 or mixed");
     if (f.num["jr"] != 0.0) throw gpvmc::Error("--jr != 0 is not supported (the reference's vmc does not use it)");
-    if (!f.str["spinstate"].empty())
-        throw gpvmc::Error("--spinstate reads a FockState HDF5 file, which this build does not read");
     if (f.bo["Neel_init"] && ch == gpvmc::StagFluxWaveFunction::TransExciton)
         throw gpvmc::Error("--Neel_init fills L*L/2 sites per flavour; the trans channel has L*L/2 +- 1");
-    if (f.bo["track_stagmagn"] && ch != gpvmc::StagFluxWaveFunction::GroundState)
-        throw gpvmc::Error("--track_stagmagn is measured in ground-state contexts only");
     const size_t samples = (size_t)f.num["samples"], saves = (size_t)f.num["samples_saves"],
                  stat = (size_t)f.num["samples_saves_stat"], therm = (size_t)f.num["therm"];
     size_t meas = (size_t)f.num["meas_interv"];
@@ -161,7 +161,10 @@ int run_flags(int argc, char **argv)
     const uint64_t seed = (uint64_t)f.num["seed"];
     std::printf("seed=%llu\n", (unsigned long long)seed);
 
+    // FileManager and its root attributes (vmc.cpp:101-117)
     gpvmc::FileManager fm(W);
+    const int num = run_number(f.str["dir"], (int)f.num["prefix"]);
+    fm.SetOutput(f.str["dir"], std::to_string(num));
     fm.StatPerSample() = (int)saves;
     for (const auto &a : f.bo) fm.FileAttribute(a.first, a.second ? 1.0 : 0.0);
     for (const auto &a : f.num) fm.FileAttribute(a.first, a.second);
@@ -169,12 +172,27 @@ int run_flags(int argc, char **argv)
     fm.FileAttribute("stagflux_wav", 1.0);
     fm.FileAttribute("sfpnxphz_wav", 0.0);
     fm.FileAttribute("sfpnzphx_wav", 0.0);
+    fm.FileAttribute("gitversion", "gpvmc_amd-" + std::to_string(gpvmc_version()));
 
     gpvmc::StagFluxWaveFunction wav(L, f.num["phi"], f.num["neel"], f.num["neel_exp"], f.num["phase_shift_x"],
                                     f.num["phase_shift_y"], ch, (int)f.num["qx"], (int)f.num["qy"]);
+    wav.Save(fm);
     gpvmc::SlaterDeterminant amp(wav, W, prec, seed);
     std::vector<int32_t> spins, ok;
-    if (f.bo["Neel_init"]) {
+    if (!f.str["spinstate"].empty()) {
+        // vmc.cpp:178-220: the configurations of a FockState file, labels in site order
+        spins = gpvmc::LoadFockState(f.str["spinstate"], L, W);
+        for (int w = 0; w < W; ++w) {
+            int nu = 0;
+            for (int v = 0; v < L * L; ++v) nu += spins[(size_t)w * L * L + v] == 0;
+            if (nu != wav.GetNpt(0))
+                throw gpvmc::Error("--spinstate: chain " + std::to_string(w) + " has " + std::to_string(nu) +
+                                   " particles of flavour 0, the channel " + std::to_string(wav.GetNpt(0)));
+        }
+        ok = amp.Init(spins);
+        for (int x : ok)
+            if (!x) throw gpvmc::Error("--spinstate: a configuration of the file has a zero amplitude");
+    } else if (f.bo["Neel_init"]) {
         // vmc.cpp:163-177: flavour 0 on the sites with x + y odd
         spins.resize((size_t)W * L * L);
         for (int w = 0; w < W; ++w)
@@ -184,15 +202,15 @@ int run_flags(int argc, char **argv)
         for (int x : ok)
             if (!x) throw gpvmc::Error("the Neel configuration has a zero amplitude");
     } else {
-        // vmc.cpp:222-241: up to 10 RanInit draws per chain (draw t from seed + 1000 t)
-        spins = gpvmc::RanInit(L, W, wav.GetNpt(0), seed);
+        // vmc.cpp:222-241: up to 10 RanInit draws per chain (draw t on Philox attempt t)
+        spins = gpvmc::RanInit(L, W, wav.GetNpt(0), seed, 0);
         ok = amp.Init(spins);
         for (int tries = 1;; ++tries) {
             bool all = true;
             for (int x : ok) all = all && x;
             if (all) break;
             if (tries == 10) throw gpvmc::Error("could not find a non-singular starting configuration.");
-            const std::vector<int32_t> fresh = gpvmc::RanInit(L, W, wav.GetNpt(0), seed + 1000ull * tries);
+            const std::vector<int32_t> fresh = gpvmc::RanInit(L, W, wav.GetNpt(0), seed, tries);
             for (int w = 0; w < W; ++w)
                 if (!ok[w]) std::copy(fresh.begin() + (size_t)w * L * L, fresh.begin() + (size_t)(w + 1) * L * L,
                                       spins.begin() + (size_t)w * L * L);
@@ -213,7 +231,6 @@ int run_flags(int argc, char **argv)
     if (f.bo["track_stagmagn"]) mc.AddQuantity(&mtrack);
     if (f.bo["track_overlap"]) mc.AddQuantity(&otrack);
     if (therm) mc.Walk((int64_t)(therm * L * L), 0);
-    const int num = run_number(f.str["dir"], (int)f.num["prefix"]);
     double walk_s = 0;
     for (size_t sample = 0; sample < samples; ++sample)
         for (size_t m = 0; m < saves; ++m) {
@@ -228,7 +245,7 @@ int run_flags(int argc, char **argv)
             if (f.num["verbose"] > 0)
                 std::printf("sample %zu save %zu: rejection %.6f\n", sample, m, mc.Rejection());
         }
-    fm.WriteJSON(f.str["dir"], std::to_string(num));
+    gpvmc::SaveFockState(fm, L, amp.Spins());  // State::Save at the end of the run (vmc.cpp:329)
     std::printf("{\"prefix\": %d, \"chains\": %d, \"rejection\": %.15g, \"quantities\": [", num, W, mc.Rejection());
     const std::vector<std::string> names = fm.Names();
     for (size_t i = 0; i < names.size(); ++i) std::printf("%s\"%s\"", i ? ", " : "", names[i].c_str());
@@ -263,14 +280,14 @@ int main(int argc, char **argv)
                                         qx, qy);
         gpvmc::SlaterDeterminant amp(wav, W, prec, seed);
         // singular starts are redrawn, as vmc.cpp:230-241 does: walker w's configuration
-        // comes from RanInit with seed + 1000 * (its try)
-        std::vector<int32_t> spins = gpvmc::RanInit(L, W, wav.GetNpt(0), seed);
+        // comes from the Fisher-Yates draw of seed + 1000 * (its try)
+        std::vector<int32_t> spins = gpvmc::RandomSpins(L, W, wav.GetNpt(0), seed);
         std::vector<int32_t> ok = amp.Init(spins);
         for (int tries = 1; tries < 50; ++tries) {
             bool all = true;
             for (int x : ok) all = all && x;
             if (all) break;
-            const std::vector<int32_t> fresh = gpvmc::RanInit(L, W, wav.GetNpt(0), seed + 1000ull * tries);
+            const std::vector<int32_t> fresh = gpvmc::RandomSpins(L, W, wav.GetNpt(0), seed + 1000ull * tries);
             for (int w = 0; w < W; ++w)
                 if (!ok[w]) std::copy(fresh.begin() + (size_t)w * L * L, fresh.begin() + (size_t)(w + 1) * L * L,
                                       spins.begin() + (size_t)w * L * L);

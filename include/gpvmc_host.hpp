@@ -29,6 +29,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -68,20 +69,37 @@ class StagFluxWaveFunction {
     int HopSign(int c, int k) const { return hopsign_[(size_t)c * nk_ + k]; }
     /* the graph as the C ABI takes it (pointers into this object) */
     gpvmc_channel GraphABI() const;
+    /* quantum numbers (kx, ky, band) of orbital f at [3 f .. 3 f + 2] (the fock2qn table,
+     * StagFluxWaveFunction.cpp:24-41) */
+    const std::vector<int32_t> &QuantumNumbers() const { return qn_; }
+    /* WaveFunction::Save (WaveFunction.cpp:85-137): <dir>/<prefix>-WaveFunction.h5 with, per
+     * flavour fl, the int dataset states_<fl>[state][orbital] = 1 where the channel state
+     * occupies the orbital, and on it the unsigned attributes kx, ky, band[orbital] */
+    void Save(const class FileManager &fm) const;
 
   private:
     int L_, Ns_, N_, nk_;
     int npt_[2];
     Channel ch_;
+    std::vector<int32_t> qn_;
     std::vector<std::complex<double>> table_;
     std::vector<std::vector<int32_t>> states_;
     std::vector<int32_t> state_orb_, hop_n_, hop_orb_, hopsign_;
 };
 
-/* Initial configurations: LatticeState::RanInit's role (LatticeState.cpp:37-82), a
- * Fisher-Yates shuffle per walker drawn from Philox4x32-10 (key seed, counter
- * (i, 0, w, 1)); n_up particles of flavour 0.  spins[w][site] in {0, 1}. */
-std::vector<int32_t> RanInit(int L, int n_walkers, int n_up, uint64_t seed);
+/* LatticeState::RanInit (LatticeState.cpp:37-82) for two flavours with one internal state
+ * per site: flavour 0 takes the site v = floor(u Ns) of each uniform u until n_up distinct
+ * sites are taken (a site already taken is drawn again), then flavour 1 fills every
+ * remaining site.  Returns spins[site] in {0, 1}; *used (may be null) receives the number of
+ * uniforms consumed. */
+std::vector<int32_t> RanInitDraw(int L, int n_up, const std::function<double()> &uniform, int64_t *used = nullptr);
+/* RanInitDraw for a batch: walker w's uniforms from Philox4x32-10, key seed, counter
+ * (i, attempt, w, 2) for its i-th uniform, u = (x + 1/2) 2^-32 in (0, 1) (the open interval
+ * of RanGen's uniform_pos, RanGen.cpp:86).  spins[w][site]. */
+std::vector<int32_t> RanInit(int L, int n_walkers, int n_up, uint64_t seed, int attempt = 0);
+/* Half-filled configurations by a Fisher-Yates shuffle per walker from Philox4x32-10 (key
+ * seed, counter (i, 0, w, 1)): the Python host's model.random_spins. */
+std::vector<int32_t> RandomSpins(int L, int n_walkers, int n_up, uint64_t seed);
 
 /* log-magnitude and phase of an amplitude (the reference's BigComplex m_amp) */
 struct LogAmp {
@@ -147,10 +165,7 @@ class MatStream {
  * when the stream's Stat() is 1, else over the last one; a stream whose Stat() reaches
  * StatPerSample() restarts at its next Write (one dataset per sample of StatPerSample()
  * saves).  The data attributes set before a Write (rej, time, statistics) go onto every
- * dataset it writes.  The reference writes each quantity's records to <dir>/<prefix>-<name>.h5
- * as /rank-<r>/data-<k> with the root attributes and "type" = name; WriteJSON writes the same
- * tree to <dir>/<prefix>-<name>.json ("attributes", "rank-<r>": {"data-<k>": {"shape",
- * "attr", "data" row-major}}). */
+ * dataset it writes; with SetOutput they also go to <dir>/<prefix>-<name>.h5. */
 class FileManager {
   public:
     struct Dataset {
@@ -159,6 +174,20 @@ class FileManager {
         std::map<std::string, double> attr;
     };
     explicit FileManager(int n_chains);
+    /* HDF5 output: every later Write() also stores the datasets it writes in
+     * <dir>/<prefix>-<name>.h5, as the reference's FileManager::Write (FileManager.cpp:465-551):
+     * a new file gets the root attributes, "type" = name and "date"; chain c's datasets go
+     * to group RankGroup(c) as data-<k> (2-D doubles) with the data attributes. */
+    void SetOutput(const std::string &dir, const std::string &prefix);
+    const std::string &Dir() const { return dir_; }
+    const std::string &Prefix() const { return prefix_; }
+    /* FileManager::WriteSimple (FileManager.cpp:555-575): creates <dir>/<prefix>-<name>.h5
+     * with the root attributes and "type"; returns the open HDF5 file id (the caller closes
+     * it with H5Fclose) */
+    int64_t WriteSimple(const std::string &name) const;
+    /* group of chain c: "/rank-0" for a single chain (the reference without MPI), else
+     * "/rank-<c + 1>" (the calculation ranks 1 .. W of an MPI run, rank 0 being its writer) */
+    std::string RankGroup(int c) const;
     int NChains() const { return n_chains_; }
     MatStream &FileStream(const std::string &name, int chain);
     int &StatPerSample() { return stat_per_sample_; }
@@ -170,16 +199,26 @@ class FileManager {
     void Write();
     std::vector<std::string> Names() const;
     const std::vector<Dataset> &Datasets(const std::string &name, int chain) const;
-    void WriteJSON(const std::string &dir, const std::string &prefix) const;
 
   private:
+    void WriteHDF5(const std::string &name, const std::vector<std::vector<Dataset>> &rec,
+                   const std::vector<int> &written) const;
     int n_chains_, stat_per_sample_ = 1;
+    std::string dir_, prefix_;
     std::map<std::string, std::vector<MatStream>> streams_;
     std::map<std::string, std::vector<std::vector<Dataset>>> data_;
     std::map<std::string, std::vector<double>> dattr_;
     std::map<std::string, double> fattr_num_;
     std::map<std::string, std::string> fattr_str_;
 };
+
+/* State::Save (State.cpp:117-166): <dir>/<prefix>-FockState.h5 with, for chain c, the int
+ * datasets RankGroup(c)/flavour-<fl>[site] = 1 where flavour fl occupies the site.
+ * spins[c][site] in {0, 1}. */
+void SaveFockState(const FileManager &fm, int L, const std::vector<int32_t> &spins);
+/* The configurations of a FockState file for n_chains chains (--spinstate, vmc.cpp:178-220):
+ * chain c reads RankGroup(c) of a FileManager with n_chains chains. */
+std::vector<int32_t> LoadFockState(const std::string &path, int L, int n_chains);
 
 /* per-measurement traces a quantity asks MetroMC::Walk to record */
 enum Trace : uint32_t { TraceLogW = 1u, TraceStagMagn = 2u };

@@ -7,10 +7,12 @@
 //  * RanGen is built with -DUSE_RNG_STD (RanGen.cpp:46-47,114-115): uniform() is
 //    std::rand()/RAND_MAX, so the stream is reproducible from the seed through glibc's
 //    rand() on any machine with the same libc.
-//  * zdotu_sub.f (Fortran) cannot be compiled here (no Fortran compiler); its symbol is
-//    left unresolved at link time.  Only reference paths that never call it are run:
-//    Init/DetInv, ground-state VirtUpdate/DetUpdate (nc=1, rr=0), Update/InvUpdate,
-//    LatticeStepper, MetroMC::Step, the excitation graph and HopSign.
+//  * zdotu_sub.f (the Fortran shim over BLAS zdotu) is compiled with the LLVM flang of
+//    the ROCm toolchain.  flang returns COMPLEX functions in registers, as the GNU
+//    convention does, so the harness selects MKL's GNU interface layer before any BLAS call
+//    (MKL's default layer returns them through a hidden argument).
+//  * FileManager, ProjHeis and the Quantity classes are the reference's own; HDF5 is the
+//    image's (/opt/conda).
 //
 // Modes (all output is one JSON object on stdout):
 //   lattice L
@@ -18,6 +20,11 @@
 //   walk L phi_in_units_of_pi neel neel_exp seed nsteps snap_every
 //   excgraph L channel(trans|long) qx qy
 //   time L phi_in_units_of_pi neel seed nsteps   (wall time of nsteps MetroMC::Step)
+//   h5layout dir L phi_in_units_of_pi neel seed   (files written by FileManager, WaveFunction::Save
+//                                                and State::Save; the data written on stdout)
+//   excwalk channel L qx qy phi_in_units_of_pi neel seed nsteps meas
+//                                               (a channel chain with ProjHeis every meas steps)
+//   eloc L phi_in_units_of_pi neel seed nsteps meas   (ground-state ProjHeis E_loc along a chain)
 #include <cstdio>
 #include <ctime>
 #include <cstdlib>
@@ -43,8 +50,15 @@
 #include "LatticeStepper.h"
 #include "MetroMC.h"
 #include "RanGen.h"
+#include "ProjHeis.h"
+#include "FileManager.h"
 #undef private
 #undef protected
+#include <hdf5.h>
+
+// MKL's interface-layer switch (mkl_rt): LP64 integers, GNU convention for complex returns
+extern "C" int MKL_Set_Interface_Layer(int);
+static const int kMklLp64Gnu = 0 + 2;
 
 using namespace std;
 
@@ -312,10 +326,170 @@ static int mode_time(int L, double phi, double neel, int seed, long nsteps)
     return 0;
 }
 
+// FileManager::Write / WriteSimple, WaveFunction::Save and State::Save into `dir` (prefix 7
+// for the ground state at (L, phi, neel), prefix 8 for the transverse exciton at q = (1, 2)):
+// RanInit from the glibc stream of `seed` (LatticeState.cpp:37-82), a ProjHeis-named
+// stream written 4 times with 2 saves per sample and the rej/time/statistics attributes.
+static int mode_h5layout(const string& dir, int L, double phi, double neel, int seed)
+{
+    size_t N = L * L / 2;
+    RanGen::srand(seed);
+    SquareLattice slat(L, L);
+    {
+        FileManager fm(dir, 7);
+        fm.StatPerSample() = 2;
+        fm.FileAttribute("L", double(L));
+        fm.FileAttribute("phi", phi);
+        fm.FileAttribute("channel", string("groundstate"));
+        StagFluxGroundState wav(&fm, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0});
+        wav.Save();
+        LatticeState sp(&fm, &slat, {N, N}, {1, 1});
+        vector<vector<size_t>> pop = {{N}, {N}};
+        sp.RanInit(pop);
+        printf("{\"fock\":");
+        pr_fock(&sp);
+        printf(",\"saves\":[");
+        for(int s=0; s<4; ++s){
+            double m[4] = {0.25 * s - 1.0, 1.0 / (s + 3.0), -0.5 * s, 2.0 + s};
+            fm.FileStream("ProjHeis").Write(2, 2, m);
+            fm.DataAttribute("rej", 0.1 * s);
+            fm.DataAttribute("time", 0.0);
+            fm.DataAttribute("statistics", double((s % 2 + 1) * 100));
+            fm.Write();
+            printf("%s[%.17g,%.17g,%.17g,%.17g]", s ? "," : "", m[0], m[1], m[2], m[3]);
+        }
+        printf("]");
+        sp.Save();
+    }
+    {
+        FileManager fm(dir, 8);
+        fm.FileAttribute("L", double(L));
+        fm.FileAttribute("channel", string("trans"));
+        StagFluxTransExciton wav(&fm, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0}, {1, 2});
+        wav.Save();
+    }
+    printf("}\n");
+    return 0;
+}
+
+// vmc.cpp:221-259 start-up (RanInit retries, then the kmax state) on a channel, then
+// MetroMC::Step nsteps times with ProjHeis::measure after step s when s % meas == 0
+// (MetroMC.cpp:45-103, ProjHeis.cpp:40-153): per-step predicted weight ratios, moves, the
+// starting configuration and the H/O sums.  The reference's own DetUpdate/InvUpdate run,
+// crossing terms from the A of the last Init included (linalg.cpp:196,320,457).
+static int mode_excwalk(const string& channel, int L, size_t qx, size_t qy, double phi, double neel, int seed,
+                        long nsteps, long meas)
+{
+    size_t N = L * L / 2;
+    RanGen::srand(seed);
+    SquareLattice slat(L, L);
+    vector<size_t> npt = {N, N};
+    if(channel == "trans") npt = {N + 1, N - 1};
+    LatticeState sp(nullptr, &slat, npt, {1, 1});
+    WaveFunction* wav = nullptr;
+    if(channel == "trans")
+        wav = new StagFluxTransExciton(nullptr, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0}, {qx, qy});
+    else
+        wav = new StagFluxLongExciton(nullptr, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0}, {qx, qy});
+    SlaterDeterminant amp(&sp, wav);
+    vector<vector<size_t>> pop = {{npt[0]}, {npt[1]}};
+    int tc = 0;
+    while(amp.Amp() == 0.0 && tc < 10){ sp.RanInit(pop); amp.Init(); tc++; }
+    if(tc == 10){ fprintf(stderr, "no regular start\n"); return 1; }
+    BigDouble maxq = norm(amp.Amp());
+    size_t kmax = wav->GetCurrentStateIndex();
+    for(size_t k=0; k<wav->GetNExc(); ++k){
+        wav->Hop(k);
+        amp.Init();
+        if(norm(amp.Amp()) > maxq){ kmax = k; maxq = norm(amp.Amp()); }
+    }
+    wav->Hop(kmax);
+    amp.Init();
+    LatticeStepper step(&sp, wav, &amp);
+    MetroMC varmc(&step, nullptr);
+    ProjHeis heis(&step, nullptr, &slat);
+    heis.init();
+    printf("{\"channel\":\"%s\",\"L\":%d,\"q\":[%zu,%zu],\"seed\":%d,\"nsteps\":%ld,\"meas\":%ld,"
+           "\"init_attempts\":%d,\"init_fock\":", channel.c_str(), L, qx, qy, seed, nsteps, meas, tc);
+    pr_fock(&sp);
+    vector<double> ratio(nsteps);
+    vector<int> moved(nsteps);
+    for(long s=0; s<nsteps; ++s){
+        vector<size_t> before = sp.Getfs()[0];
+        BigDouble w0 = varmc.m_state_weight;
+        varmc.Step(true);
+        ratio[s] = double(step.m_prev_weight / w0);
+        moved[s] = (before != sp.Getfs()[0]);
+        if(meas > 0 && s % meas == 0) heis.measure();
+    }
+    printf(",\"ratio\":[");
+    for(long s=0; s<nsteps; ++s) printf("%s%.17g", s ? "," : "", ratio[s]);
+    printf("],\"moved\":[");
+    for(long s=0; s<nsteps; ++s) printf("%s%d", s ? "," : "", moved[s]);
+    printf("],\"n_meas\":%zu,\"ho\":[", heis.m_meas);
+    for(size_t i=0; i<heis.m_vals.size(); ++i)
+        printf("%s%.17g,%.17g", i ? "," : "", real(heis.m_vals[i]), imag(heis.m_vals[i]));
+    printf("],\"final_fock\":");
+    pr_fock(&sp);
+    printf("}\n");
+    delete wav;
+    return 0;
+}
+
+// ground-state ProjHeis E_loc per site along a chain (vmc.cpp start-up, MetroMC::Step, and
+// ProjHeis::measure after step s when s % meas == 0): the per-measurement values and the
+// configurations they were measured at
+static int mode_eloc(int L, double phi, double neel, int seed, long nsteps, long meas)
+{
+    size_t N = L * L / 2;
+    RanGen::srand(seed);
+    SquareLattice slat(L, L);
+    LatticeState sp(nullptr, &slat, {N, N}, {1, 1});
+    StagFluxGroundState wav(nullptr, L, L, phi * M_PI, neel, 0.0, {1.0, 1.0});
+    SlaterDeterminant amp(&sp, &wav);
+    vector<vector<size_t>> pop = {{N}, {N}};
+    int tc = 0;
+    while(amp.Amp() == 0.0 && tc < 10){ sp.RanInit(pop); amp.Init(); tc++; }
+    if(tc == 10){ fprintf(stderr, "no regular start\n"); return 1; }
+    LatticeStepper step(&sp, &wav, &amp);
+    MetroMC varmc(&step, nullptr);
+    ProjHeis heis(&step, nullptr, &slat);
+    printf("{\"L\":%d,\"seed\":%d,\"nsteps\":%ld,\"meas\":%ld,\"eloc\":[", L, seed, nsteps, meas);
+    vector<vector<int>> confs;
+    int m = 0;
+    for(long s=0; s<nsteps; ++s){
+        varmc.Step(true);
+        if(meas > 0 && s % meas == 0){
+            heis.init();
+            heis.measure();
+            printf("%s[%.17g,%.17g]", m++ ? "," : "", real(heis.m_vals[0]), imag(heis.m_vals[0]));
+            vector<int> f(sp.GetNsites());
+            for(size_t v=0; v<sp.GetNsites(); ++v) f[v] = (sp.Getfs()[0][v] != sp.GetNpt()[0]) ? 0 : 1;
+            confs.push_back(f);
+        }
+    }
+    printf("],\"confs\":[");
+    for(size_t t=0; t<confs.size(); ++t){
+        printf("%s[", t ? "," : "");
+        for(size_t v=0; v<confs[t].size(); ++v) printf("%s%d", v ? "," : "", confs[t][v]);
+        printf("]");
+    }
+    printf("]}\n");
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if(argc < 2){ fprintf(stderr, "usage: see header\n"); return 2; }
+    MKL_Set_Interface_Layer(kMklLp64Gnu);
     string mode = argv[1];
+    if(mode == "h5layout" && argc == 7)
+        return mode_h5layout(argv[2], atoi(argv[3]), atof(argv[4]), atof(argv[5]), atoi(argv[6]));
+    if(mode == "excwalk" && argc == 11)
+        return mode_excwalk(argv[2], atoi(argv[3]), (size_t)atoi(argv[4]), (size_t)atoi(argv[5]), atof(argv[6]),
+                            atof(argv[7]), atoi(argv[8]), atol(argv[9]), atol(argv[10]));
+    if(mode == "eloc" && argc == 8)
+        return mode_eloc(atoi(argv[2]), atof(argv[3]), atof(argv[4]), atoi(argv[5]), atol(argv[6]), atol(argv[7]));
     if(mode == "lattice" && argc == 3) return mode_lattice(atoi(argv[2]));
     if(mode == "orbitals" && argc == 8)
         return mode_orbitals(atoi(argv[2]), atof(argv[3]), atof(argv[4]), atof(argv[5]),

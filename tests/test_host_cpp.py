@@ -22,9 +22,27 @@ def host_lib():
     lib = ctypes.CDLL(build.HOST_LIB)
     lib.gpvmc_host_orbital_table.argtypes = [ctypes.c_int32] + [ctypes.c_double] * 5 + [ctypes.c_void_p]
     lib.gpvmc_host_channel.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 7
+    lib.gpvmc_host_random_spins.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                            ctypes.c_void_p]
     lib.gpvmc_host_raninit.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
-                                       ctypes.c_void_p]
+                                       ctypes.c_int32, ctypes.c_void_p]
+    lib.gpvmc_host_h5_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]
+    lib.gpvmc_host_h5_read.restype = ctypes.c_int64
+    lib.gpvmc_host_h5_attr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p]
     return lib
+
+
+def h5_read(lib, path, name, n_max=1 << 20):
+    out = np.zeros(n_max)
+    n = lib.gpvmc_host_h5_read(str(path).encode(), name.encode(), p(out), n_max)
+    assert n >= 0, (path, name)
+    return out[:n]
+
+
+def h5_attr(lib, path, obj, attr):
+    v = ctypes.c_double()
+    assert lib.gpvmc_host_h5_attr(str(path).encode(), obj.encode(), attr.encode(), ctypes.byref(v)) == 0, (obj, attr)
+    return v.value
 
 
 def p(a):
@@ -71,11 +89,11 @@ def test_channel_graph_equals_python(name, L, q):
                     assert (ho[c, k, fl, h, 0], ho[c, k, fl, h, 1]) == (a, b)
 
 
-def test_raninit_equals_python():
+def test_random_spins_equals_python():
     lib = host_lib()
     L, W, seed = 8, 5, 12345
     out = np.zeros((W, L * L), np.int32)
-    lib.gpvmc_host_raninit(L, W, L * L // 2, seed, p(out))
+    lib.gpvmc_host_random_spins(L, W, L * L // 2, seed, p(out))
     assert np.array_equal(out, model.random_spins(L, W, seed))
 
 
@@ -143,7 +161,7 @@ def test_host_trans_channel_walk_equals_binding():
 
     def draw(s):
         spins = np.zeros((W, L * L), np.int32)
-        lib.gpvmc_host_raninit(L, W, ch.npt[0], s, p(spins))
+        lib.gpvmc_host_random_spins(L, W, ch.npt[0], s, p(spins))
         return spins
 
     b = binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed)
@@ -245,14 +263,27 @@ def test_host_flag_driver_sample_loop_equals_binding(tmp_path):
     assert r.returncode == 0, r.stderr
     summary = json.loads(r.stdout.strip().splitlines()[-1])
     assert summary["prefix"] == 3
-    files = {q: json.load(open(tmp_path / f"3-{q}.json")) for q in summary["quantities"]}
+    lib = host_lib()
+    files = {q: tmp_path / f"3-{q}.h5" for q in summary["quantities"]}
     assert set(files) == {"ProjHeis", "StagMagnZ", "OverlapTrack", "StagMagnTrack"}
-    assert files["ProjHeis"]["attributes"]["type"] == "ProjHeis"
-    assert files["ProjHeis"]["attributes"]["L"] == L and files["ProjHeis"]["attributes"]["channel"] == "groundstate"
+    assert all(f.exists() for f in files.values())
+    assert h5_attr(lib, files["ProjHeis"], "/", "L") == L
+    assert (tmp_path / "3-WaveFunction.h5").exists() and (tmp_path / "3-FockState.h5").exists()
 
+    # the start: RanInit draws on Philox attempts 0, 1, ... for the singular chains
+    def draw(attempt):
+        spins = np.zeros((W, L * L), np.int32)
+        lib.gpvmc_host_raninit(L, W, L * L // 2, seed, attempt, p(spins))
+        return spins
     b = binding.Batch(L, W, binding.FP64, n_groups=1, seed=seed)
     b.set_orbitals(0, model.ground_occ(model.orbital_table(L, phi * np.pi, neel)))
-    ok = init_redrawing(b, lambda s: model.random_spins(L, W, s), seed)
+    spins = draw(0)
+    ok = b.init(spins)
+    for t in range(1, 10):
+        if ok.all():
+            break
+        spins[~ok] = draw(t)[~ok]
+        ok = b.init(spins)
     assert ok.all()
     b.walk(therm * L * L)
     mask = binding.MEAS_PROJHEIS | binding.MEAS_STAGMAGNZ | binding.MEAS_OVERLAP
@@ -280,11 +311,54 @@ def test_host_flag_driver_sample_loop_equals_binding(tmp_path):
         for q, f in files.items():
             want = np.mean([ps[q] for ps in per_save], axis=0)
             for c in range(W):
-                d = f[f"rank-{c}"][f"data-{sample}"]
-                assert d["shape"] == shapes[q]
-                assert d["attr"]["statistics"] == saves * stat
-                assert abs(d["attr"]["rej"] - rejected[c] / steps[c]) < 1e-15
-                np.testing.assert_allclose(np.array(d["data"]), want[c], rtol=1e-11, atol=1e-12)
+                # a batch of W chains is an MPI run's calculation ranks 1 .. W
+                name = f"/rank-{c + 1}/data-{sample}"
+                d = h5_read(lib, f, name)
+                assert len(d) == shapes[q][0] * shapes[q][1]
+                assert h5_attr(lib, f, name, "statistics") == saves * stat
+                assert abs(h5_attr(lib, f, name, "rej") - rejected[c] / steps[c]) < 1e-15
+                np.testing.assert_allclose(d, np.ravel(want[c]), rtol=1e-11, atol=1e-12)
     for f in files.values():
-        assert all(len(f[f"rank-{c}"]) == samples for c in range(W))
+        assert lib.gpvmc_host_h5_read(str(f).encode(), f"/rank-1/data-{samples}".encode(), None, 0) == -1
+    # the final configurations in FockState.h5 (State::Save)
+    spins_f = b.state()[0]
+    for c in range(W):
+        for fl in range(2):
+            occ = h5_read(lib, tmp_path / "3-FockState.h5", f"/rank-{c + 1}/flavour-{fl}")
+            assert np.array_equal(occ.astype(np.int32), (spins_f[c] == fl).astype(np.int32))
     b.close()
+
+
+@pytest.mark.gpu
+def test_host_track_stagmagn_in_a_channel(tmp_path):
+    """--track_stagmagn in the trans channel (vmc.cpp:278-281): the walk is taken one measured
+    step at a time so that the host can read each measured configuration; the chain, its
+    ProjHeis H/O and OverlapTrack records are the ones of the same run without the flag, and
+    every m_z is the staggered magnetisation of a configuration with N+1 up and N-1 down
+    spins ((1/Ns) sum_v (-1)^(x+y) (2 up_v - 1), a multiple of 2/Ns)."""
+    if not os.path.exists(build.HOST_EXE):
+        pytest.skip("vmc_host not built")
+    print("case host track_stagmagn trans", flush=True)
+    L, W, stat = 4, 2, 6
+    base = [build.HOST_EXE, f"--L={L}", f"--walkers={W}", "--channel=trans", "--qx=1", "--qy=2", "--phi=0.25",
+            "--neel=0.1", "--seed=5", "--therm=2", "--samples=1", f"--samples_saves_stat={stat}", "--meas_projheis",
+            "--track_overlap", f"--dir={tmp_path}", "--verbose=0"]
+    for pre, extra in (("1", []), ("2", ["--track_stagmagn"])):
+        r = subprocess.run(base + [f"--prefix={pre}"] + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+    lib = host_lib()
+    for c in range(W):
+        for q in ("ProjHeis", "OverlapTrack"):
+            a = h5_read(lib, tmp_path / f"1-{q}.h5", f"/rank-{c + 1}/data-0")
+            b = h5_read(lib, tmp_path / f"2-{q}.h5", f"/rank-{c + 1}/data-0")
+            np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-14)
+        mz = h5_read(lib, tmp_path / "2-StagMagnTrack.h5", f"/rank-{c + 1}/data-0").reshape(-1, 2)
+        assert mz.shape == (stat, 2)
+        assert np.all(mz[:, 1] == 0.0)
+        k = mz[:, 0] * L * L / 2
+        assert np.allclose(k, np.round(k), atol=1e-12) and np.all(np.abs(mz[:, 0]) <= 1.0)
+    for c in range(W):
+        for fl in range(2):
+            name = f"/rank-{c + 1}/flavour-{fl}"
+            assert np.array_equal(h5_read(lib, tmp_path / "1-FockState.h5", name),
+                                  h5_read(lib, tmp_path / "2-FockState.h5", name))
