@@ -83,6 +83,21 @@ static void pr_fock(const LatticeState* sp)
     printf("]");
 }
 
+// Number of values the glibc stream of `seed` has given so far (the RanGen of a USE_RNG_STD
+// build draws std::rand()): draw the next value, re-seed, count the draws up to its first
+// occurrence, then leave the stream where it was.  A 31-bit value recurring within the few
+// thousand draws of a start-up is the only way to miscount; a caller checks the chain.
+static long stream_position(int seed)
+{
+    int next = std::rand();
+    std::srand(seed);
+    long n = 0;
+    while(std::rand() != next) ++n;
+    std::srand(seed);
+    for(long i=0; i<n; ++i) std::rand();
+    return n;
+}
+
 static void pr_labels(const LatticeState* sp, size_t fl)
 {
     printf("[");
@@ -409,8 +424,10 @@ static int mode_excwalk(const string& channel, int L, size_t qx, size_t qy, doub
     MetroMC varmc(&step, nullptr);
     ProjHeis heis(&step, nullptr, &slat);
     heis.init();
+    const long pos = stream_position(seed);
     printf("{\"channel\":\"%s\",\"L\":%d,\"q\":[%zu,%zu],\"seed\":%d,\"nsteps\":%ld,\"meas\":%ld,"
-           "\"init_attempts\":%d,\"init_fock\":", channel.c_str(), L, qx, qy, seed, nsteps, meas, tc);
+           "\"init_attempts\":%d,\"stream_pos\":%ld,\"kmax\":%zu,\"init_fock\":", channel.c_str(), L, qx, qy,
+           seed, nsteps, meas, tc, pos, kmax);
     pr_fock(&sp);
     vector<double> ratio(nsteps);
     vector<int> moved(nsteps);

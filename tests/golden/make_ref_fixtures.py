@@ -1,12 +1,13 @@
 """Writes the golden fixtures under tests/golden/ from the reference-built harness.
 
 Run in the build container only (it needs /root/reference to build oracle/_ref):
-    make -C oracle ref && python tests/golden/make_ref_fixtures.py
+    make -C oracle ref && python tests/golden/make_ref_fixtures.py [base] [exciton] [eloc]
 
 The harness (oracle/ref_harness.cpp) drives the reference's own compiled classes; the
 fixtures are plain data (.npz, no pickles): lattice edge order, orbital tables,
 RanInit configurations, per-step weight ratios and moves of MetroMC::Step, snapshots,
-inverse matrices and excitation graphs.  The uniform stream of each walk is glibc's
+inverse matrices and excitation graphs; exciton chains with their ProjHeis H/O sums and
+ground-state E_loc along a chain.  The uniform stream of each walk is glibc's
 rand()/RAND_MAX from the stored seed (the reference's USE_RNG_STD generator); the first
 values of that stream are stored too so a libc mismatch is detected.
 """
@@ -27,7 +28,8 @@ def run(*args):
     env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
     out = subprocess.run([HARNESS] + [str(a) for a in args], check=True,
                          capture_output=True, text=True, env=env).stdout
-    return json.loads(out)
+    # the reference prints its own warnings (a singular DetInv of a channel state) on stdout
+    return json.loads([line for line in out.splitlines() if line.startswith("{")][-1])
 
 
 def cplx(flat):
@@ -126,9 +128,52 @@ def excgraph(L, channel, qx, qy):
         block_n=len(blk))
 
 
-def main():
+def no_rand_max(seed, need):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(ctypes.c_uint(seed))
+    assert all(libc.rand() != 2147483647 for _ in range(need)), "stream hits RAND_MAX"
+
+
+def excwalk(channel, L, qx, qy, phi, neel, seed, nsteps, meas):
+    """The reference's exciton chain (its DetUpdate/InvUpdate with the crossing terms taken
+    from the A of the last Init), ProjHeis H/O sums every `meas` steps."""
+    d = run("excwalk", channel, L, qx, qy, phi, neel, seed, nsteps, meas)
+    assert d["init_attempts"] == 1, "a retried start leaves ghost labels in the reference's fock"
+    no_rand_max(seed, d["stream_pos"] + 4 * nsteps)
+    ho = cplx(d["ho"])
+    np.savez_compressed(
+        os.path.join(HERE, "excwalk_%s_L%d_q%d_%d_s%d.npz" % (channel, L, qx, qy, seed)),
+        channel=channel, L=L, q=np.array([qx, qy]), phi=phi, neel=neel, seed=seed, nsteps=nsteps,
+        meas=meas, glibc_head=glibc_head(seed), stream_pos=d["stream_pos"], kmax=d["kmax"],
+        init_spins=np.array(d["init_fock"], np.int8), ratio=np.array(d["ratio"], np.float64),
+        moved=np.packbits(np.array(d["moved"], np.uint8)), n_meas=d["n_meas"],
+        ho=ho.reshape(2, -1, int(round(np.sqrt(len(ho) / 2)))),
+        final_spins=np.array(d["final_fock"], np.int8))
+
+
+def eloc(L, phi, neel, seed, nsteps, meas):
+    """ProjHeis::measure of the ground state (E_loc per site) at the configurations of a chain"""
+    d = run("eloc", L, phi, neel, seed, nsteps, meas)
+    e = np.array(d["eloc"], np.float64)
+    np.savez_compressed(
+        os.path.join(HERE, "eloc_L%d_s%d.npz" % (L, seed)),
+        L=L, phi=phi, neel=neel, seed=seed, eloc=e[:, 0] + 1j * e[:, 1],
+        confs=np.array(d["confs"], np.int8))
+
+
+def main(groups):
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
+    if "exciton" in groups:
+        excwalk("trans", 4, 1, 2, 0.25, 0.1, 12345, 4000, 16)
+        excwalk("long", 4, 2, 2, 0.25, 0.1, 4242, 4000, 16)
+        excwalk("trans", 6, 1, 2, 0.25, 0.1, 31, 2000, 36)
+        excwalk("long", 6, 2, 0, 0.25, 0.1, 33, 2000, 36)
+    if "eloc" in groups:
+        eloc(4, 0.25, 0.1, 12345, 2000, 16)
+        eloc(6, 0.1667, 0.05, 4242, 3600, 36)
+    if "base" not in groups:
+        return
     for L in (4, 6, 8):
         lattice(L)
     orbitals(4, 0.25, 0.1, 0.0, 1.0, 1.0, "L4_a")
@@ -146,4 +191,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # groups: base (lattice, orbitals, walks, excitation graphs), exciton, eloc; default all
+    main(sys.argv[1:] or ["base", "exciton", "eloc"])

@@ -160,6 +160,25 @@ def test_fp64_replays_reference_trajectory(name):
     assert (np.abs(r_gpu - r_ref) <= bar * r_ref + 1e-40).all()
 
 
+@pytest.mark.parametrize("name", ["eloc_L4_s12345.npz", "eloc_L6_s4242.npz"])
+def test_fp64_eloc_equals_reference_projheis(name):
+    """The reference's ProjHeis::measure (ProjHeis.cpp:40-158) at the configurations of its own
+    chain against gpvmc_measure on a batch holding those configurations: E_loc per site within
+    1e-11 relative (1e-11 absolute below 1; the reference's inverse carries its chain's
+    Sherman-Morrison rounding, ours a fresh FP64 inversion)."""
+    g = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    L = int(g["L"])
+    say("case fp64 eloc vs reference", name)
+    occ = occ_for(L, float(g["phi"]), float(g["neel"]), 0.0)
+    confs = g["confs"].astype(np.int32)
+    with binding.Batch(L, len(confs), binding.FP64) as b:
+        b.set_orbitals(0, occ)
+        assert b.init(confs).all()
+        e = b.measure()["eloc"]
+    ref = g["eloc"]
+    assert (np.abs(e - ref) <= 1e-11 * np.maximum(1.0, np.abs(ref))).all(), np.abs(e - ref).max()
+
+
 @pytest.mark.parametrize("L,W,nsteps,meas", [(4, 16, 2000, 16), (8, 32, 3000, 64), (6, 8, 1500, 36)])
 def test_fp64_philox_batch_matches_oracle(L, W, nsteps, meas):
     say("case fp64 philox batch", L, W, nsteps, meas)

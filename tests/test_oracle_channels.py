@@ -6,8 +6,12 @@ or_exc_local, or_exc_walk):
   * at L=4 their ratios between states equal those of second-quantised site-basis
     amplitudes (tests/exact.py), which fixes every relative sign;
   * h_k / a_k equals the exact local energy (H psi_k)(alpha) / psi_k(alpha) for every
-    state of the trans and long channels on sampled configurations.
+    state of the trans and long channels on sampled configurations;
+  * or_exc_walk replays the reference's own exciton chains (fixtures excwalk_*.npz, from
+    the reference compiled in oracle/_ref): same moves, weight ratios and H/O sums.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -15,6 +19,7 @@ import exact
 import oracle_ref as O
 from gpvmc_amd import channels, model
 
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = [("trans", 4, (1, 2)), ("long", 4, (0, 0)), ("long", 4, (1, 0)), ("trans", 6, (2, 3))]
 
 
@@ -100,3 +105,38 @@ def test_exc_amps_and_local_energy_exact_L4(name, q):
             assert abs(h[k] / a[k] - eex[k, c]) < 1e-8 * max(1.0, abs(eex[k, c]))
             checked += 1
     assert checked > 100
+
+
+EXCWALKS = ["excwalk_trans_L4_q1_2_s12345.npz", "excwalk_long_L4_q2_2_s4242.npz",
+            "excwalk_trans_L6_q1_2_s31.npz", "excwalk_long_L6_q2_0_s33.npz"]
+
+
+@pytest.mark.parametrize("name", EXCWALKS)
+def test_exc_walk_equals_reference_chain(name):
+    """The reference's own exciton chain (its DetUpdate/InvUpdate, whose crossing terms read
+    the A of the last Init, linalg.cpp:196,320,457; ProjHeis H/O, ProjHeis.cpp:98-153), run on
+    its glibc stream from its start: the oracle's exact chain (every amplitude by LU) on the
+    same uniforms makes the same moves, its weight ratios agree within 1e-11 relative and its
+    H/O sums within 1e-11 of their scale.  Ratios of moves onto vanishing amplitudes
+    (1e-28 .. 1e-32, rounding noise on both sides) have an absolute floor of 1e-20.
+
+    The stale crossing terms cancel: the reference adds them as a block-column operation
+    [B11; B21] += [B12; B22] X, X = A_cross - C_cross, on the capacitance B, and InvUpdate
+    applies the same X to A^-1 C, so det(B) and the Woodbury product do not depend on X."""
+    g = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    L, n, meas = int(g["L"]), int(g["nsteps"]), int(g["meas"])
+    ch = channels.Channel(str(g["channel"]), L, tuple(int(x) for x in g["q"]))
+    tab = model.orbital_table(L, float(g["phi"]) * np.pi, float(g["neel"]))
+    pos = int(g["stream_pos"])
+    u = O.glibc_uniforms(int(g["seed"]), pos + 4 * n)[pos:]
+    w = O.ExcWalker(L, tab, ch, g["init_spins"].astype(np.int32))
+    o = w.walk(u, meas=meas)
+    moved = np.unpackbits(g["moved"])[:n].astype(bool)
+    assert np.array_equal(moved, (o["accept"] & ~o["noop"]).astype(bool))
+    r_ref = g["ratio"]
+    assert (np.abs(r_ref - o["ratio"]) <= 1e-11 * o["ratio"] + 1e-20).all()
+    assert o["n_meas"] == int(g["n_meas"])
+    ho = g["ho"].reshape(2 * ch.nexc, ch.nexc)
+    scale = np.abs(ho).max()
+    assert np.abs(o["HO"] - ho).max() <= 1e-11 * scale
+    assert np.array_equal(w.spins(), g["final_spins"].astype(np.int32))
