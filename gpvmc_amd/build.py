@@ -60,6 +60,7 @@ def build(verbose=False):
 
 HOST_LIB = os.path.join(PKG, "lib", "libgpvmc_host.so")
 HOST_EXE = os.path.join(PKG, "lib", "vmc_host")
+CONCAT_EXE = os.path.join(PKG, "lib", "concatenate")
 
 
 # HDF5 (C library and its high-level API) of the image, for the reference's output files
@@ -68,7 +69,8 @@ HDF5 = os.environ.get("GPVMC_HDF5", "/opt/conda")
 
 def build_host(verbose=False):
     """The C++ host classes (include/gpvmc_host.hpp) as libgpvmc_host.so over
-    libgpvmc_hip.so and HDF5, and the vmc_host driver; plain C++ (no device code)."""
+    libgpvmc_hip.so and HDF5, the vmc_host driver and the concatenate tool; plain C++ (no
+    device code)."""
     cxx = os.environ.get("CXX", "g++")
     libdir = os.path.dirname(LIB)
     host = os.path.join(PKG, "host")
@@ -76,15 +78,18 @@ def build_host(verbose=False):
     h5lib = os.path.join(HDF5, "lib")
     cmds = [
         [cxx] + common + ["-I" + os.path.join(HDF5, "include"), "-fPIC", "-shared", "-o", HOST_LIB + ".tmp",
-                          os.path.join(host, "gpvmc_host.cpp"), "-L" + libdir, "-lgpvmc_hip",
+                          os.path.join(host, "gpvmc_host.cpp"), os.path.join(host, "h5merge.cpp"),
+                          "-L" + libdir, "-lgpvmc_hip",
                           os.path.join(h5lib, "libhdf5_hl.so"), os.path.join(h5lib, "libhdf5.so"),
                           # the system library directory before HDF5's: the system libstdc++ (the
                           # one ROCm needs), not the older one next to HDF5
                           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/usr/lib/x86_64-linux-gnu", "-Wl,-rpath," + h5lib],
         [cxx] + common + ["-o", HOST_EXE + ".tmp", os.path.join(host, "vmc_host.cpp"), "-L" + libdir,
                           "-lgpvmc_host", "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
+        [cxx] + common + ["-o", CONCAT_EXE + ".tmp", os.path.join(host, "concatenate.cpp"), "-L" + libdir,
+                          "-lgpvmc_host", "-lgpvmc_hip", "-Wl,-rpath,$ORIGIN"],
     ]
-    for cmd, out in zip(cmds, (HOST_LIB, HOST_EXE)):
+    for cmd, out in zip(cmds, (HOST_LIB, HOST_EXE, CONCAT_EXE)):
         if verbose:
             print(" ".join(cmd), flush=True)
         if out == HOST_EXE:
@@ -94,6 +99,7 @@ def build_host(verbose=False):
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("building %s failed" % os.path.basename(out))
     os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    os.replace(CONCAT_EXE + ".tmp", CONCAT_EXE)
 
 
 if __name__ == "__main__":

@@ -1096,3 +1096,63 @@ extern "C" int gpvmc_host_fockstate_read(const char *path, int32_t L, int32_t n_
         return -1;
     }
 }
+
+// The sample grouping of the merge: sample_of[i] and rank_in_sample[i] of record i (its
+// sample and its place among that sample's members), totals[s] of each sample
+extern "C" int gpvmc_host_bunch_records(int32_t n, const int32_t *statistics, int32_t nsamp, int32_t *sample_of,
+                                        int32_t *rank_in_sample, int32_t *totals)
+{
+    try {
+        const gpvmc::RecordBunches b = gpvmc::BunchRecords(nsamp, std::vector<int>(statistics, statistics + n));
+        for (size_t s = 0; s < b.members.size(); ++s) {
+            totals[s] = b.statistics[s];
+            for (size_t i = 0; i < b.members[s].size(); ++i) {
+                sample_of[b.members[s][i]] = (int32_t)s;
+                rank_in_sample[b.members[s][i]] = (int32_t)i;
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+// gpvmc::Concatenate over n_in input paths
+extern "C" int gpvmc_host_concatenate(int32_t n_in, const char *const *in, const char *out, int32_t nsamp)
+{
+    try {
+        gpvmc::Concatenate(std::vector<std::string>(in, in + n_in), out, nsamp);
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+// Records for tests of the merge: a FileManager of n_chains chains writing `name` into
+// <dir>/<prefix>-<name>.h5, one dataset per write (rows x cols doubles per chain, data
+// [write][chain][rows*cols]) with the data attribute statistics[write]; root attributes
+// "L" = L and "channel" = "groundstate"
+extern "C" int gpvmc_host_records_write(const char *dir, const char *prefix, const char *name, int32_t L,
+                                        int32_t n_chains, int32_t n_writes, int32_t rows, int32_t cols,
+                                        const double *data, const double *statistics)
+{
+    try {
+        gpvmc::FileManager fm(n_chains);
+        fm.SetOutput(dir, prefix);
+        fm.FileAttribute("L", (double)L);
+        fm.FileAttribute("channel", std::string("groundstate"));
+        const size_t mn = (size_t)rows * cols;
+        for (int k = 0; k < n_writes; ++k) {
+            for (int c = 0; c < n_chains; ++c)
+                fm.FileStream(name, c).Write(rows, cols, data + ((size_t)k * n_chains + c) * mn);
+            fm.DataAttribute("statistics", statistics[k]);
+            fm.Write();
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}

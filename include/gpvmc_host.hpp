@@ -220,6 +220,27 @@ void SaveFockState(const FileManager &fm, int L, const std::vector<int32_t> &spi
  * chain c reads RankGroup(c) of a FileManager with n_chains chains. */
 std::vector<int32_t> LoadFockState(const std::string &path, int L, int n_chains);
 
+/* The sample grouping of tools/concatenate.cpp:107-133: records (datasets) with the given
+ * "statistics" are split into nsamp samples of about equal total statistics.  The record
+ * indices are sorted by statistics (std::sort, so equal statistics keep the order the
+ * reference's sort gives them); while more than 2 nsamp + 1 remain, the smallest and the
+ * largest go together into sample n (n cycling 0 .. nsamp-1); each one left over, largest
+ * first, goes into the first sample of smallest total so far.  Returns the members of each
+ * sample (in the order they were added) and each sample's total. */
+struct RecordBunches {
+    std::vector<std::vector<int>> members;
+    std::vector<int> statistics;
+};
+RecordBunches BunchRecords(int nsamp, const std::vector<int> &statistics);
+/* tools/concatenate.cpp:26-233: merges the /rank-r/data-k records of several output files
+ * of one quantity (ranks from 1 up, data sets from 0 up, as long as they exist; file order,
+ * then rank, then k) into `out` (created; an existing file is an error): the root attributes
+ * of the first file, and /rank-1/data-s = the plain mean of sample s's records (summed in
+ * member order; the first member's datatype and shape) with the int attribute "statistics"
+ * = the sample's total.  nsamp = 0 makes one sample per record.  Errors (no record, more
+ * samples than records, records of different shapes) throw. */
+void Concatenate(const std::vector<std::string> &in, const std::string &out, int nsamp);
+
 /* per-measurement traces a quantity asks MetroMC::Walk to record */
 enum Trace : uint32_t { TraceLogW = 1u, TraceStagMagn = 2u };
 
