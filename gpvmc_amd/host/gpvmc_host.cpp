@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <ctime>
 #include <utility>
 
@@ -1130,19 +1132,41 @@ extern "C" int gpvmc_host_concatenate(int32_t n_in, const char *const *in, const
     }
 }
 
-// Records for tests of the merge: a FileManager of n_chains chains writing `name` into
-// <dir>/<prefix>-<name>.h5, one dataset per write (rows x cols doubles per chain, data
-// [write][chain][rows*cols]) with the data attribute statistics[write]; root attributes
-// "L" = L and "channel" = "groundstate"
-extern "C" int gpvmc_host_records_write(const char *dir, const char *prefix, const char *name, int32_t L,
+// Root attributes "key=value;key=value" (a value that reads whole as a number is numeric,
+// otherwise a string) on a FileManager
+static void set_root_attributes(gpvmc::FileManager &fm, const char *spec)
+{
+    std::string all = spec ? spec : "";
+    size_t pos = 0;
+    while (pos < all.size()) {
+        size_t end = all.find(';', pos);
+        if (end == std::string::npos) end = all.size();
+        const std::string item = all.substr(pos, end - pos);
+        const size_t eq = item.find('=');
+        if (eq == std::string::npos || eq == 0) throw gpvmc::Error("bad root attribute \"" + item + "\"");
+        const std::string key = item.substr(0, eq), val = item.substr(eq + 1);
+        char *stop = nullptr;
+        const double x = std::strtod(val.c_str(), &stop);
+        if (!val.empty() && stop && *stop == 0)
+            fm.FileAttribute(key, x);
+        else
+            fm.FileAttribute(key, val);
+        pos = end + 1;
+    }
+}
+
+// Records for tests of the merge and of the post-processing: a FileManager of n_chains
+// chains writing `name` into <dir>/<prefix>-<name>.h5, one dataset per write (rows x cols
+// doubles per chain, data [write][chain][rows*cols]) with the data attribute
+// statistics[write], and the root attributes of root_attrs
+extern "C" int gpvmc_host_records_write(const char *dir, const char *prefix, const char *name, const char *root_attrs,
                                         int32_t n_chains, int32_t n_writes, int32_t rows, int32_t cols,
                                         const double *data, const double *statistics)
 {
     try {
         gpvmc::FileManager fm(n_chains);
         fm.SetOutput(dir, prefix);
-        fm.FileAttribute("L", (double)L);
-        fm.FileAttribute("channel", std::string("groundstate"));
+        set_root_attributes(fm, root_attrs);
         const size_t mn = (size_t)rows * cols;
         for (int k = 0; k < n_writes; ++k) {
             for (int c = 0; c < n_chains; ++c)
@@ -1155,4 +1179,139 @@ extern "C" int gpvmc_host_records_write(const char *dir, const char *prefix, con
         g_host_error = e.what();
         return -1;
     }
+}
+
+// WaveFunction::Save of a channel into <dir>/<prefix>-WaveFunction.h5 with the root
+// attributes of root_attrs (channel: 0 ground state, 1 trans, 2 long; phi in units of pi)
+extern "C" int gpvmc_host_wavefunction_save(const char *dir, const char *prefix, const char *root_attrs, int32_t L,
+                                            double phi, double neel, int32_t channel, int32_t qx, int32_t qy)
+{
+    try {
+        gpvmc::FileManager fm(1);
+        fm.SetOutput(dir, prefix);
+        set_root_attributes(fm, root_attrs);
+        gpvmc::StagFluxWaveFunction wav(L, phi, neel, 0.0, 1.0, 1.0, (gpvmc::StagFluxWaveFunction::Channel)channel,
+                                        qx, qy);
+        wav.Save(fm);
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_error = e.what();
+        return -1;
+    }
+}
+
+// ---- HDF5 inspection for host-side post-processing (the image has no h5py) ----------------
+namespace {
+
+herr_t collect_link(hid_t, const char *name, const H5L_info_t *, void *op)
+{
+    static_cast<std::vector<std::string> *>(op)->push_back(name);
+    return 0;
+}
+
+herr_t collect_attr(hid_t, const char *name, const H5A_info_t *, void *op)
+{
+    static_cast<std::vector<std::string> *>(op)->push_back(name);
+    return 0;
+}
+
+// names joined by '\n' into buf (cap bytes with the terminating zero); returns the bytes the
+// whole list needs, so a caller can retry with a larger buffer
+int64_t put_names(const std::vector<std::string> &names, char *buf, int64_t cap)
+{
+    std::string all;
+    for (size_t i = 0; i < names.size(); ++i) all += (i ? "\n" : "") + names[i];
+    if (buf && cap > 0) {
+        const size_t n = std::min<size_t>(all.size(), (size_t)cap - 1);
+        std::copy(all.begin(), all.begin() + n, buf);
+        buf[n] = 0;
+    }
+    return (int64_t)all.size() + 1;
+}
+
+}  // namespace
+
+// The members of a group (in name order): returns the buffer size the list needs, -1 when
+// the file or the group cannot be opened
+extern "C" int64_t gpvmc_host_h5_children(const char *path, const char *group, char *buf, int64_t cap)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    std::vector<std::string> names;
+    const herr_t rc = H5Literate_by_name(f, group, H5_INDEX_NAME, H5_ITER_INC, nullptr, collect_link, &names,
+                                          H5P_DEFAULT);
+    H5Fclose(f);
+    return rc < 0 ? -1 : put_names(names, buf, cap);
+}
+
+// The shape of a dataset: its rank (dims[0 .. rank) written, at most max_rank), -1 on error
+extern "C" int32_t gpvmc_host_h5_shape(const char *path, const char *dataset, int64_t *dims, int32_t max_rank)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rank = -1;
+    if (H5Lexists(f, dataset, H5P_DEFAULT) > 0 && H5LTget_dataset_ndims(f, dataset, &rank) >= 0 && rank <= max_rank) {
+        std::vector<hsize_t> d(std::max(rank, 1));
+        if (H5LTget_dataset_info(f, dataset, d.data(), nullptr, nullptr) < 0)
+            rank = -1;
+        else
+            for (int i = 0; i < rank; ++i) dims[i] = (int64_t)d[i];
+    } else {
+        rank = -1;
+    }
+    H5Fclose(f);
+    return rank;
+}
+
+// The attribute names of an object ("/" for the file), as gpvmc_host_h5_children
+extern "C" int64_t gpvmc_host_h5_attr_names(const char *path, const char *object, char *buf, int64_t cap)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    std::vector<std::string> names;
+    const herr_t rc = H5Aiterate_by_name(f, object, H5_INDEX_NAME, H5_ITER_INC, nullptr, collect_attr, &names,
+                                         H5P_DEFAULT);
+    H5Fclose(f);
+    return rc < 0 ? -1 : put_names(names, buf, cap);
+}
+
+// An attribute's value: a string attribute is copied into str (strcap bytes with the zero)
+// and -2 is returned; a numeric one is converted to doubles (at most maxn into num) and its
+// element count is returned; -1 on error
+extern "C" int64_t gpvmc_host_h5_attr_value(const char *path, const char *object, const char *name, double *num,
+                                            int64_t maxn, char *str, int64_t strcap)
+{
+    const hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int64_t out = -1;
+    const hid_t a = H5Aopen_by_name(f, object, name, H5P_DEFAULT, H5P_DEFAULT);
+    if (a >= 0) {
+        const hid_t type = H5Aget_type(a), space = H5Aget_space(a);
+        const hssize_t n = H5Sget_simple_extent_npoints(space);
+        if (H5Tget_class(type) == H5T_STRING) {
+            const size_t len = H5Tget_size(type);
+            if (!H5Tis_variable_str(type) && n == 1) {
+                std::vector<char> s(len + 1, 0);
+                if (H5Aread(a, type, s.data()) >= 0) {
+                    const size_t k = std::min<size_t>(std::strlen(s.data()), strcap > 0 ? (size_t)strcap - 1 : 0);
+                    if (str && strcap > 0) {
+                        std::copy(s.data(), s.data() + k, str);
+                        str[k] = 0;
+                    }
+                    out = -2;
+                }
+            }
+        } else if (n >= 0) {
+            std::vector<double> v((size_t)std::max<hssize_t>(n, 1));
+            if (H5Aread(a, H5T_NATIVE_DOUBLE, v.data()) >= 0) {
+                std::copy(v.begin(), v.begin() + std::min<int64_t>(n, maxn), num);
+                out = n;
+            }
+        }
+        H5Sclose(space);
+        H5Tclose(type);
+        H5Aclose(a);
+    }
+    H5Fclose(f);
+    return out;
 }

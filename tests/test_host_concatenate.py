@@ -31,8 +31,8 @@ def host_lib():
     if not os.path.exists(build.HOST_LIB):
         pytest.skip("libgpvmc_host.so not built")
     lib = ctypes.CDLL(build.HOST_LIB)
-    lib.gpvmc_host_records_write.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p] + \
-        [ctypes.c_int32] * 5 + [ctypes.c_void_p, ctypes.c_void_p]
+    lib.gpvmc_host_records_write.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_int32] * 4 + \
+        [ctypes.c_void_p, ctypes.c_void_p]
     lib.gpvmc_host_bunch_records.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p]
     lib.gpvmc_host_concatenate.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]
@@ -53,8 +53,9 @@ def write_inputs(lib, d, files, chains, writes, stats, rows=2, cols=4, seed=0):
     data = rng.normal(size=(files, writes, chains, rows * cols))
     for f in range(files):
         st = np.ascontiguousarray(stats[f], np.float64)
-        assert lib.gpvmc_host_records_write(str(d).encode(), str(f + 1).encode(), b"ProjHeis", 8, chains, writes,
-                                            rows, cols, p(np.ascontiguousarray(data[f])), p(st)) == 0, \
+        assert lib.gpvmc_host_records_write(str(d).encode(), str(f + 1).encode(), b"ProjHeis",
+                                            b"L=8;channel=groundstate", chains, writes, rows, cols,
+                                            p(np.ascontiguousarray(data[f])), p(st)) == 0, \
             lib.gpvmc_host_last_error()
     return data.transpose(0, 2, 1, 3)
 

@@ -362,3 +362,36 @@ def test_host_track_stagmagn_in_a_channel(tmp_path):
             name = f"/rank-{c + 1}/flavour-{fl}"
             assert np.array_equal(h5_read(lib, tmp_path / "1-FockState.h5", name),
                                   h5_read(lib, tmp_path / "2-FockState.h5", name))
+
+
+@pytest.mark.gpu
+def test_postproc_reads_vmc_host_output(tmp_path):
+    """The post-processing (gpvmc_amd/postproc.py) on files vmc_host wrote: S(q) from the
+    StatSpinStruct records obeys the per-configuration identities S^zz(0) = 0 (S^z_tot = 0)
+    and sum_q S^zz(q) = N/4, and the trans channel's eigen system from its ProjHeis and
+    WaveFunction files satisfies H v = E O v with real eigenvalues and spectral weights
+    summing to s^H O s."""
+    if not os.path.exists(build.HOST_EXE):
+        pytest.skip("vmc_host not built")
+    from gpvmc_amd import postproc as P
+    print("case postproc on vmc_host output", flush=True)
+    L, W = 4, 4
+    common = [build.HOST_EXE, f"--L={L}", f"--walkers={W}", "--phi=0.25", "--neel=0.1", "--seed=21", "--therm=4",
+              "--samples=2", "--samples_saves_stat=8", f"--dir={tmp_path}", "--verbose=0"]
+    r = subprocess.run(common + ["--prefix=1", "--meas_statspinstruct"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    (sxx, syy, szz), _ = P.get_stat_spin_struct(str(tmp_path / "1-StatSpinStruct.h5"), 2)
+    for s in range(2):
+        assert abs(szz[s][0, 0]) < 1e-12
+        assert abs(szz[s].sum() - L * L / 4) < 1e-12
+    r = subprocess.run(common + ["--prefix=2", "--meas_projheis", "--channel=trans", "--qx=1", "--qy=2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    f = str(tmp_path / "2-ProjHeis.h5")
+    H, Om, E, V = P.get_eig_sys(f, 2)
+    wts = P.get_sq_ampl(f, 2, O=Om, evecs=V)
+    s = P.spinops(P.get_attr(f))[1]
+    for k in range(2):
+        assert np.isfinite(E[k]).all()
+        np.testing.assert_allclose(H[k] @ V[k], Om[k] @ V[k] @ np.diag(E[k]), atol=1e-8 * np.abs(H[k]).max())
+        np.testing.assert_allclose(wts[k].sum(), (s.conj() @ Om[k] @ s).real, rtol=1e-8)
