@@ -29,6 +29,7 @@ SIGNATURES = {
     "gpvmc_set_orbitals": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "gpvmc_set_groups": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_init_walkers": (ctypes.c_int, [_vp, _vp, _vp]),
+    "gpvmc_init_random": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int32, _vp, _vp]),
     "gpvmc_refresh": (ctypes.c_int, [_vp, _vp]),
     "gpvmc_set_sweep_variant": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "gpvmc_sweep_variant": (ctypes.c_int, [_vp]),
@@ -195,6 +196,13 @@ class Batch:
         ok = np.zeros(self.W, np.int32)
         self._check(self.lib.gpvmc_init_walkers(self.h, _ptr(s), _ptr(ok)))
         return ok.astype(bool)
+
+    def init_random(self, seed, max_attempts=10):
+        """the random start with redraws on the device (gpvmc_init_random): (ok, attempts)"""
+        ok = np.zeros(self.W, np.int32)
+        att = np.zeros(self.W, np.int32)
+        self._check(self.lib.gpvmc_init_random(self.h, seed, max_attempts, _ptr(ok), _ptr(att)))
+        return ok.astype(bool), att
 
     def refresh(self, drift=False):
         d = np.zeros(self.W) if drift else None

@@ -83,6 +83,7 @@ struct gpvmc_ctx {
     double2 *d_Ai64 = nullptr;    // FP64 inverse of the last init/refresh
     int32_t *d_site = nullptr, *d_part = nullptr, *d_group = nullptr, *d_order = nullptr;
     int32_t *d_nbr = nullptr, *d_status = nullptr;
+    int32_t *d_flag = nullptr;    // one int: gv_launch_raninit's incomplete-draw flag
     int4 *d_edge = nullptr;
     double *d_logabs = nullptr, *d_phase = nullptr, *d_acc = nullptr, *d_drift = nullptr;
     uint64_t *d_counter = nullptr;
@@ -132,7 +133,8 @@ static void free_ctx(gpvmc_ctx *c)
                     c->d_counter, c->d_occ64, c->d_occ32, c->d_scratch, c->d_ss, c->d_gjwork,
                     c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
                     c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
-                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q, c->d_Y};
+                    c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q, c->d_Y,
+                    c->d_flag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -348,6 +350,40 @@ static int run_refresh(gpvmc_ctx *c, double *drift_dev)
     return GPVMC_OK;
 }
 
+// SlaterDeterminant::Init after the configurations (d_site, d_part) are on the device:
+// cleared accumulators and counters, the FP64 inversion, the ratio tables (ground state) or
+// the current-state choice (channels: every walker starts from state 0 of its channel,
+// slots in increasing orbital order, and moves to its dominant state); st receives the
+// status pair of every walker
+static int init_uploaded(gpvmc_ctx *c, std::vector<int32_t> &st)
+{
+    const int W = c->W;
+    int rc = GPVMC_OK;
+    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
+    if (c->channel) {
+        const int nm = c->N, npt[2] = {c->n_up, c->n_dn};
+        std::vector<int32_t> cur((size_t)W * 2 * nm, 0);
+        for (int w = 0; w < W; ++w) {
+            const int32_t *so = c->state_orb.data() + (size_t)c->chans[w] * c->nk * (c->n_up + c->n_dn);
+            for (int fl = 0; fl < 2; ++fl)
+                for (int f = 0; f < npt[fl]; ++f)
+                    cur[(size_t)w * 2 * nm + (size_t)fl * nm + f] = so[(fl ? c->n_up : 0) + f];
+        }
+        HIPCHK(hipMemcpy(c->d_cur, cur.data(), cur.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemsetAsync(c->d_cstate, 0, (size_t)W * sizeof(int32_t), c->stream));
+        HIPCHK(hipMemsetAsync(c->d_ho, 0, (size_t)W * 4 * c->nk * c->nk * sizeof(double), c->stream));
+        rc = run_refresh_channel(c, 2);
+    } else {
+        rc = clear_statspin(c);
+        if (!rc) rc = run_refresh(c, nullptr);
+    }
+    if (rc) return rc;
+    st.assign((size_t)W * 2, 0);
+    HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return GPVMC_OK;
+}
+
 static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
 {
     if (!spins) return fail(GPVMC_ERR_ARG, "null spins");
@@ -356,7 +392,7 @@ static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
             return fail(GPVMC_ERR_STATE, "orbital table of group " + std::to_string(c->groups[w]) + " not set");
     const int Ns = c->Ns, nm = c->N, W = c->W;
     const int npt[2] = {c->n_up, c->n_dn};
-    std::vector<int32_t> site((size_t)W * Ns), part((size_t)W * 2 * nm, 0), cur((size_t)W * 2 * nm, 0);
+    std::vector<int32_t> site((size_t)W * Ns), part((size_t)W * 2 * nm, 0);
     for (int w = 0; w < W; ++w) {
         int cnt[2] = {0, 0};
         for (int v = 0; v < Ns; ++v) {
@@ -368,25 +404,13 @@ static int init_channel(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
             part[(size_t)w * 2 * nm + (size_t)fl * nm + cnt[fl]] = v;
             cnt[fl]++;
         }
-        // the current state starts as state 0 of the walker's channel, slots in increasing
-        // orbital order
-        const int32_t *so = c->state_orb.data() + (size_t)c->chans[w] * c->nk * Ns;
-        for (int fl = 0; fl < 2; ++fl)
-            for (int f = 0; f < npt[fl]; ++f)
-                cur[(size_t)w * 2 * nm + (size_t)fl * nm + f] = so[(fl ? c->n_up : 0) + f];
     }
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipMemcpy(c->d_site, site.data(), site.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_cur, cur.data(), cur.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(c->d_cstate, 0, (size_t)W * sizeof(int32_t), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_ho, 0, (size_t)W * 4 * c->nk * c->nk * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
-    int rc = run_refresh_channel(c, 2);
+    std::vector<int32_t> st;
+    int rc = init_uploaded(c, st);
     if (rc) return rc;
-    std::vector<int32_t> st((size_t)W * 2);
-    HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (ok)
         for (int w = 0; w < W; ++w) ok[w] = (st[2 * w] == 0 && st[2 * w + 1] == 0) ? 1 : 0;
     c->initialised = true;
@@ -633,16 +657,61 @@ int gpvmc_init_walkers(gpvmc_ctx *c, const int32_t *spins, int32_t *ok)
     HIPCHK(hipSetDevice(c->cfg.device));
     HIPCHK(hipMemcpy(c->d_site, site.data(), site.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_part, part.data(), part.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(c->d_acc, 0, (size_t)W * gv::ACC_STRIDE * sizeof(double), c->stream));
-    HIPCHK(hipMemsetAsync(c->d_counter, 0, (size_t)W * sizeof(uint64_t), c->stream));
-    int rc = clear_statspin(c);
+    std::vector<int32_t> st;
+    const int rc = init_uploaded(c, st);
     if (rc) return rc;
-    rc = run_refresh(c, nullptr);
-    if (rc) return rc;
-    std::vector<int32_t> st((size_t)W * 2);
-    HIPCHK(hipMemcpy(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (ok)
         for (int w = 0; w < W; ++w) ok[w] = (st[2 * w] == 0 && st[2 * w + 1] == 0) ? 1 : 0;
+    c->initialised = true;
+    return GPVMC_OK;
+}
+
+int gpvmc_init_random(gpvmc_ctx *c, uint64_t seed, int32_t max_attempts, int32_t *ok, int32_t *attempts)
+{
+    if (!c) return fail(GPVMC_ERR_ARG, "null context");
+    if (max_attempts < 1 || max_attempts > 1000000) return fail(GPVMC_ERR_ARG, "max_attempts must be in [1, 1e6]");
+    for (int w = 0; w < c->W; ++w)
+        if (!c->have_orb[c->groups[w]])
+            return fail(GPVMC_ERR_STATE, "orbitals of group " + std::to_string(c->groups[w]) + " not set");
+    const int W = c->W;
+    HIPCHK(hipSetDevice(c->cfg.device));
+    if (!c->d_flag) HIPCHK(hipMalloc(&c->d_flag, sizeof(int32_t)));
+    gv::RIArgs a{};
+    a.W = W;
+    a.Ns = c->Ns;
+    a.nstride = c->N;
+    a.n_up = c->channel ? c->n_up : c->N;
+    a.n_dn = c->channel ? c->n_dn : c->N;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.site = c->d_site;
+    a.part = c->d_part;
+    a.incomplete = c->d_flag;
+    std::vector<int32_t> st, used(W, 1);
+    for (int t = 0; t < max_attempts; ++t) {
+        // attempt 0 draws every walker, later ones only the walkers still singular
+        a.attempt = t;
+        a.status = t ? c->d_status : nullptr;
+        HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int32_t), c->stream));
+        HIPCHK(gv_launch_raninit(&a, c->stream));
+        int32_t incomplete = 0;
+        HIPCHK(hipMemcpyAsync(&incomplete, c->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (incomplete) return fail(GPVMC_ERR_STATE, "a RanInit draw did not complete within 2^20 uniforms");
+        const int rc = init_uploaded(c, st);
+        if (rc) return rc;
+        bool all = true;
+        for (int w = 0; w < W; ++w)
+            if (st[2 * w] || st[2 * w + 1]) {
+                all = false;
+                if (t + 1 < max_attempts) used[w] = t + 2;
+            }
+        if (all) break;
+    }
+    for (int w = 0; w < W; ++w) {
+        if (ok) ok[w] = (st[2 * w] == 0 && st[2 * w + 1] == 0) ? 1 : 0;
+        if (attempts) attempts[w] = used[w];
+    }
     c->initialised = true;
     return GPVMC_OK;
 }

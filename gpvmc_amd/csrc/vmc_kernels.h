@@ -156,9 +156,25 @@ struct DIArgs {
     double2 *work;            // [blocks][n][ld] working matrices of the streamed kernel, or null
 };
 
+// LatticeState::RanInit (LatticeState.cpp:37-82) on the device, one thread per walker:
+// flavour 0 takes site floor(u Ns) of each uniform u until n_up distinct sites are taken,
+// flavour 1 the rest; uniform i of walker w at attempt t is Philox4x32-10 (key seed, counter
+// (i, t, w, 2)), u = (x0 + 1/2) 2^-32.  Labels in increasing site order (State::InitFock):
+// site[w][v] = (flavour << 16) | label, part[w][fl * nstride + label] = v, unused part
+// entries 0.  With status non-null only the walkers whose status pair is not (0, 0) draw.
+// A walker whose draw stops at the cap of 2^20 uniforms sets *incomplete to 1.
+struct RIArgs {
+    int32_t W, Ns, nstride, n_up, n_dn, attempt;
+    uint32_t key0, key1;
+    const int32_t *status;    // [W][2] or null
+    int32_t *site, *part;
+    int32_t *incomplete;      // one int, cleared by the caller
+};
+
 }  // namespace gv
 
 extern "C" {
+hipError_t gv_launch_raninit(const gv::RIArgs *a, hipStream_t st);
 int gv_sweep_supported(int prec, int N, int lpr, int *res, int *nthreads);
 /* the lanes-per-row values of every sweep shape of (prec, N), default first; returns the count */
 int gv_sweep_shapes(int prec, int N, int *lpr, int max);

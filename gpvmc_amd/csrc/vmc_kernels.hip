@@ -1390,3 +1390,52 @@ extern "C" hipError_t gv_launch_gbuild(int prec, int N, const gv::GBArgs *a, int
         if (e.prec == prec && e.N == N) return e.fn(*a, nwalkers, st);
     return hipErrorInvalidValue;
 }
+
+namespace gv {
+
+// One thread per walker; the rejection draw and the label pass touch only the walker's own
+// site row (Ns ints) and part row, so the walkers are independent.  A draw that has not
+// found n_up sites after 2^20 uniforms (probability far below 2^-1000 for n_up < Ns) is
+// reported through *incomplete.
+__global__ void __launch_bounds__(64) raninit_kernel(RIArgs a)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.W) return;
+    if (a.status && a.status[2 * w] == 0 && a.status[2 * w + 1] == 0) return;
+    int32_t *site = a.site + (size_t)w * a.Ns;
+    int32_t *part = a.part + (size_t)w * 2 * a.nstride;
+    for (int v = 0; v < a.Ns; ++v) site[v] = -1;
+    uint32_t i = 0;
+    int np = 0;
+    while (np != a.n_up && i < (1u << 20)) {
+        uint32_t c[4] = {i++, (uint32_t)a.attempt, (uint32_t)w, 2u};
+        philox4x32_10_dev(c, a.key0, a.key1);
+        const double u = ((double)c[0] + 0.5) * 0x1p-32;
+        const int v = (int)(u * a.Ns);
+        if (site[v] < 0) {
+            site[v] = 0;
+            ++np;
+        }
+    }
+    if (np != a.n_up) *a.incomplete = 1;
+    int cnt[2] = {0, 0};
+    for (int v = 0; v < a.Ns; ++v) {
+        const int fl = site[v] == 0 ? 0 : 1;
+        const int lab = cnt[fl]++;
+        site[v] = (fl << 16) | lab;
+        if (lab < a.nstride) part[fl * a.nstride + lab] = v;
+    }
+    for (int fl = 0; fl < 2; ++fl)
+        for (int lab = cnt[fl]; lab < a.nstride; ++lab) part[fl * a.nstride + lab] = 0;
+}
+
+}  // namespace gv
+
+extern "C" hipError_t gv_launch_raninit(const gv::RIArgs *a, hipStream_t st)
+{
+    if (a->W <= 0 || a->Ns <= 0 || a->n_up < 0 || a->n_up >= a->Ns || a->n_up > a->nstride ||
+        a->Ns - a->n_up > a->nstride || !a->site || !a->part || !a->incomplete)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gv::raninit_kernel, dim3((a->W + 63) / 64), dim3(64), 0, st, *a);
+    return hipGetLastError();
+}

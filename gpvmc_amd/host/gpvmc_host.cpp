@@ -355,6 +355,14 @@ std::vector<int32_t> SlaterDeterminant::Init(const std::vector<int32_t> &spins)
     return ok;
 }
 
+std::vector<int32_t> SlaterDeterminant::InitRandom(uint64_t seed, int max_attempts, std::vector<int32_t> *attempts)
+{
+    std::vector<int32_t> ok(W_), att(W_);
+    check(gpvmc_init_random(ctx_, seed, max_attempts, ok.data(), att.data()));
+    if (attempts) *attempts = att;
+    return ok;
+}
+
 void SlaterDeterminant::Refresh() { check(gpvmc_refresh(ctx_, nullptr)); }
 
 std::vector<LogAmp> SlaterDeterminant::Amp() const

@@ -202,20 +202,11 @@ int run_flags(int argc, char **argv)
         for (int x : ok)
             if (!x) throw gpvmc::Error("the Neel configuration has a zero amplitude");
     } else {
-        // vmc.cpp:222-241: up to 10 RanInit draws per chain (draw t on Philox attempt t)
-        spins = gpvmc::RanInit(L, W, wav.GetNpt(0), seed, 0);
-        ok = amp.Init(spins);
-        for (int tries = 1;; ++tries) {
-            bool all = true;
-            for (int x : ok) all = all && x;
-            if (all) break;
-            if (tries == 10) throw gpvmc::Error("could not find a non-singular starting configuration.");
-            const std::vector<int32_t> fresh = gpvmc::RanInit(L, W, wav.GetNpt(0), seed, tries);
-            for (int w = 0; w < W; ++w)
-                if (!ok[w]) std::copy(fresh.begin() + (size_t)w * L * L, fresh.begin() + (size_t)(w + 1) * L * L,
-                                      spins.begin() + (size_t)w * L * L);
-            ok = amp.Init(spins);
-        }
+        // vmc.cpp:222-241: up to 10 RanInit draws per chain, drawn and initialised on the
+        // device (draw t on Philox attempt t, the draws of gpvmc::RanInit)
+        ok = amp.InitRandom(seed, 10);
+        for (int x : ok)
+            if (!x) throw gpvmc::Error("could not find a non-singular starting configuration.");
     }
     gpvmc::LatticeStepper stepper(amp);
     gpvmc::MetroMC mc(stepper);

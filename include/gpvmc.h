@@ -15,6 +15,8 @@
  *                         WaveFunction.h:61-66), restricted to occupied orbitals
  *   gpvmc_init_walkers    LatticeState::FockInit + SlaterDeterminant::Init
  *                         (LatticeState.cpp:84-113, SlaterDeterminant.cpp:60-92)
+ *   gpvmc_init_random     the random start with redraws of vmc.cpp:221-242
+ *                         (LatticeState::RanInit, LatticeState.cpp:37-82)
  *   gpvmc_walk            MetroMC::Walk (MetroMC.cpp:45-88) driving MetroMC::Step,
  *                         LatticeStepper::trystep/step (LatticeStepper.cpp:36-147),
  *                         SlaterDeterminant::VirtUpdate/Update (SlaterDeterminant.cpp:96-326)
@@ -83,6 +85,16 @@ int gpvmc_set_groups(gpvmc_ctx *ctx, const int32_t *group_of_walker);
  * (SlaterDeterminant::m_amp_ok), else 0.  `ok` may be NULL.  Resets the accumulators and
  * the Philox counters. */
 int gpvmc_init_walkers(gpvmc_ctx *ctx, const int32_t *spins, int32_t *ok);
+
+/* The random start of vmc.cpp:221-242 on the device: every walker draws its configuration
+ * by LatticeState::RanInit (LatticeState.cpp:37-82: flavour 0 takes site floor(u Ns) of each
+ * uniform until its particles sit on distinct sites, flavour 1 the rest), is initialised as
+ * by gpvmc_init_walkers, and the walkers whose amplitude is zero draw again, up to
+ * max_attempts draws in all.  Uniform i of walker w's draw t is Philox4x32-10 with key seed
+ * and counter (i, t, w, 2), u = (x0 + 1/2) 2^-32 (the host's gpvmc::RanInit makes the same
+ * draws).  ok[w] = 1 for a walker with a non-zero amplitude; attempts[w] = the draws it
+ * used.  Either may be NULL.  Channel contexts draw their channel's particle numbers. */
+int gpvmc_init_random(gpvmc_ctx *ctx, uint64_t seed, int32_t max_attempts, int32_t *ok, int32_t *attempts);
 
 /* Sweep kernel variant of a ground-state context.  GPVMC_SWEEP_ONE_MOVE updates the ratio
  * tables after each accepted move.  GPVMC_SWEEP_WINDOWS runs delayed-update windows where the

@@ -117,6 +117,10 @@ class SlaterDeterminant {
 
     /* SlaterDeterminant::Init on every walker's configuration; returns m_amp_ok per walker */
     std::vector<int32_t> Init(const std::vector<int32_t> &spins);
+    /* the random start of vmc.cpp:221-242 drawn on the device (gpvmc_init_random): RanInit
+     * draws with redraws of zero-amplitude walkers, up to max_attempts draws; returns
+     * m_amp_ok per walker and, when asked, the draws each used */
+    std::vector<int32_t> InitRandom(uint64_t seed, int max_attempts, std::vector<int32_t> *attempts = nullptr);
     /* FP64 recomputation of the inverses at the current configurations */
     void Refresh();
     /* Amp() of every walker */

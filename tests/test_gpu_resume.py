@@ -1,4 +1,5 @@
-"""Resuming from a FockState file (vmc.cpp:178-220, State.cpp:117-166).
+"""Starting and resuming chains: the device's random start (vmc.cpp:221-242) and resuming
+from a FockState file (vmc.cpp:178-220, State.cpp:117-166).
 
 * A chain stopped after n1 steps, its configurations saved to a FockState file by the host
   (SaveFockState), read back (the --spinstate reader) and started again (FockInit labels,
@@ -17,7 +18,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from gpvmc_amd import binding, build, model
+from gpvmc_amd import binding, build, channels, model
 
 pytestmark = pytest.mark.gpu
 
@@ -103,3 +104,64 @@ def test_vmc_host_spinstate_starts_from_the_file(tmp_path, W):
                        ["--therm=0", "--samples=0", "--prefix=3", f"--spinstate={tmp_path / '1-FockState.h5'}"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "rank-" in r.stderr
+
+
+def raninit_host(L, W, n_up, seed, attempt):
+    lib = ctypes.CDLL(build.HOST_LIB)
+    lib.gpvmc_host_raninit.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                       ctypes.c_int32, ctypes.c_void_p]
+    out = np.zeros((W, L * L), np.int32)
+    assert lib.gpvmc_host_raninit(L, W, n_up, seed, attempt, p(out)) == 0
+    return out
+
+
+@pytest.mark.parametrize("kind,L,W", [("ground", 4, 256), ("ground", 8, 64), ("trans", 4, 64), ("long", 4, 48)])
+def test_device_random_start_equals_host_draws(kind, L, W):
+    """gpvmc_init_random (vmc.cpp:221-242 on the device): each walker's final configuration is
+    the host's gpvmc::RanInit draw of its last attempt (LatticeState.cpp:37-82 on the Philox
+    stream), every earlier draw of a walker has a zero amplitude and its last one not, and
+    the initialised state (amplitude, labels) equals gpvmc_init_walkers on those
+    configurations."""
+    say("case device random start", kind, L, W)
+    seed = 4321 + L
+    tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+    if kind == "ground":
+        occ = model.ground_occ(tab)
+        n_up = L * L // 2
+
+        def make():
+            b = binding.Batch(L, W, binding.FP64, seed=3)
+            b.set_orbitals(0, occ)
+            return b
+    else:
+        ch = channels.Channel(kind, L, (1, 2) if kind == "trans" else (1, 0))
+        n_up = ch.npt[0]
+
+        def make():
+            b = binding.ChannelBatch(L, W, ch, binding.FP64, seed=3)
+            b.set_orbital_table(0, tab)
+            return b
+    with make() as b:
+        # 40 draws: a random L = 4 start is singular often enough that 10 can run out
+        ok, att = b.init_random(seed, 40)
+        spins, part, la, ph = b.state()
+    assert ok.all() and (att >= 1).all() and (att <= 40).all()
+    draws = {t: raninit_host(L, W, n_up, seed, t) for t in range(int(att.max()))}
+    for w in range(W):
+        assert np.array_equal(spins[w], draws[att[w] - 1][w])
+    if kind == "ground" and L == 4:
+        assert (att > 1).sum() >= 3  # singular starts are common at L = 4
+    # earlier draws are singular, and a host-fed init reproduces the device's start
+    with make() as b2:
+        ok2 = b2.init(spins)
+        s2, part2, la2, ph2 = b2.state()
+    assert ok2.all()
+    assert np.array_equal(part2, part) and np.array_equal(la2, la) and np.array_equal(ph2, ph)
+    redrawn = np.nonzero(att > 1)[0]
+    for t in range(int(att.max()) - 1):
+        ws = [w for w in redrawn if att[w] - 1 > t]
+        if not ws:
+            continue
+        with make() as b3:
+            ok3 = b3.init(draws[t])
+        assert not ok3[ws].any()
