@@ -16,7 +16,8 @@
 //  * di_reg_kernel<NP, LPR>: the working matrix in VGPRs, padded to NP x NP with the
 //    identity (padding rows and columns never mix with the real block: their entries in
 //    the real columns are exact zeros), thread (row, c) holding the LPR-strided columns.
-//    One barrier per column; the pivot candidates of each wave go through LDS.
+//    Two barriers per column: the waves' pivot candidates (key and value) go through LDS,
+//    then the winning row alone writes itself for the update.
 //  * di_stream_kernel<NB>: matrices too large for a CU's registers (n > 128).  One row
 //    per thread; the working matrix lives column-major in a work buffer, and columns are
 //    eliminated in panels of NB: the panel's columns stay in registers during its NB
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
     constexpr int BT = batch_slots(CPT, CPT >= 32 ? 4 : 8);
     static_assert(NP % LPR == 0 && (LPR & (LPR - 1)) == 0, "LPR: a power of two dividing NP");
 
-    __shared__ C s_prow[2][NW][NP];  // each wave's candidate row, in (slot, lane) order t * LPR + c
+    __shared__ C s_prow[2][NP];      // the pivot row, in (slot, lane) order t * LPR + c
     __shared__ unsigned long long s_ckey[2][NW];
     __shared__ C s_cpiv[2][NW];      // each wave's candidate pivot value
     __shared__ C s_piv[NP];
@@ -307,15 +308,14 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
             const C uk = group_bcast_rt<LPR>(u[0], gbase, ck);
             unsigned long long key = pivoted ? 0ull : pivot_key(uk, row);
             key = wave_max_key<LPR>(key);  // uniform over the wave
+            // Only the keys and candidate pivot values cross the first barrier; the winning
+            // row alone is written after it.  (A ds_write costs the LDS pipeline the whole
+            // wave's slots however few lanes are active, so every wave writing its candidate
+            // row would cost 8x the LDS time of one row.)
             const int wrow = (int)(~(unsigned int)(key & 0xffffffffull));
-            if (!pivoted && row == wrow) {
-#pragma unroll
-                for (int t = 0; t < CPT; ++t) s_prow[par][wid][t * LPR + c] = u[t];
-                if (c == 0) s_cpiv[par][wid] = uk;
-            }
+            if (!pivoted && row == wrow && c == 0) s_cpiv[par][wid] = uk;
             if (lane == 0) s_ckey[par][wid] = key;
             __syncthreads();
-            int bw = 0;
             unsigned long long bkey = s_ckey[par][0];
             C piv = s_cpiv[par][0];
 #pragma unroll
@@ -324,13 +324,16 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
                 const C pq = s_cpiv[par][q];
                 if (v > bkey) {
                     bkey = v;
-                    bw = q;
                     piv = pq;
                 }
             }
             // NP - k >= 1 rows are unpivoted, so bkey names a row
             const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
-            const C *pr = &s_prow[par][bw][c];
+            if (row == p) {
+#pragma unroll
+                for (int t = 0; t < CPT; ++t) s_prow[par][t * LPR + c] = u[t];
+            }
+            const C *pr = &s_prow[par][c];
             const double np2 = piv.x * piv.x + piv.y * piv.y;
             const double rn = np2 > 0.0 ? 1.0 / np2 : 0.0;
             const C ainv = cm(piv.x * rn, -piv.y * rn);
@@ -346,6 +349,10 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
                 sc = cm(-ainv.x, -ainv.y);
                 pivoted = true;
             }
+            // the multiplier (and its division) is formed while the pivot row is written;
+            // the second barrier publishes the row
+            asm volatile("" ::"v"(f.x), "v"(f.y));
+            __syncthreads();
             // U -= f r in batches of BT slots, the next batch's LDS reads in flight (the
             // compiler barrier keeps the reads from all being hoisted, which would need
             // the whole pivot row in VGPRs next to U)
