@@ -587,6 +587,10 @@ sweep_kernel(SweepArgs a)
                 const C q1 = cmk<C>(readlane_real(q1l.x, j), readlane_real(q1l.y, j));
                 const double n0 = (double)q0.x * q0.x + (double)q0.y * q0.y;
                 const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
+                // 1/q per flavour, one reciprocal each, formed while the moving rows are
+                // stashed (kept ahead of the barrier below)
+                const double rn0 = 1.0 / n0, rn1 = 1.0 / n1;
+                const C qi0 = cmk<C>(q0.x * rn0, -q0.y * rn0), qi1 = cmk<C>(q1.x * rn1, -q1.y * rn1);
                 // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
                 const int ks0 = cj_j / LPR, ls0 = cj_j % LPR, ks1 = ci_j / LPR, ls1 = ci_j % LPR;
                 C x0, x1;
@@ -639,11 +643,11 @@ sweep_kernel(SweepArgs a)
                         }
                     }
                 }
+                asm volatile("" ::"v"(qi0.x), "v"(qi0.y), "v"(qi1.x), "v"(qi1.y));
                 __syncthreads();
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const C q = fl ? q1 : q0;
-                    const double nq = fl ? n1 : n0;
+                    const C qi = fl ? qi1 : qi0;
                     const C x = fl ? x1 : x0;
                     const int ls = fl ? ls1 : ls0, rsel = fl ? cj_j : ci_j;
                     // u_i = G[i][cs] - delta(i, rs), held by lane ls of row i's group
@@ -651,7 +655,6 @@ sweep_kernel(SweepArgs a)
                     uu.x = __shfl(x.x, gbase + ls);
                     uu.y = __shfl(x.y, gbase + ls);
                     if (row == rsel) uu.x -= (T)1;
-                    const C qi = cmk<C>(q.x / nq, -q.y / nq);
                     const C coef = cmul(uu, qi);
                     const C nc = cmk<C>(-coef.x, -coef.y);
                     if (has_row) {
