@@ -321,7 +321,8 @@ def exciton_leg(seed, periods, k, W=8192, L=12, n_cmp=1024):
         "mc_steps_per_s": round(W * Ns * sweeps / tm, 1),
         "ms_per_sweep_incl_refresh": round(1e3 * tm / sweeps, 3),
         "sweeps": sweeps, "refreshes": periods,
-        "kernel_ms_per_sweep": round(kt["sweep_ms"] / max(kt["sweep_launches"], 1), 3),
+        "kernel_ms_per_sweep": round(kt["sweep_ms"] / sweeps, 3),
+        "walk_launches": kt["sweep_launches"],
         "refresh_ms": round(kt["refresh_ms"] / max(kt["refresh_launches"], 1), 3),
         "measure_ms": round(1e3 * t_meas, 3),
         "p_accept_move": round(float(acc[:, 8].sum() / acc[:, 7].sum()), 4),
