@@ -45,7 +45,8 @@ __device__ __forceinline__ C cm(double x, double y)
     return r;
 }
 
-__device__ __forceinline__ C cmul(C a, C b) { return cm(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// explicit multiply-adds, so that every kernel (and every inlining of it) rounds alike
+__device__ __forceinline__ C cmul(C a, C b) { return cm(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)); }
 
 // acc - f r
 __device__ __forceinline__ C cfms(C f, C r, C acc)
@@ -204,9 +205,10 @@ template <int FROM> __device__ __forceinline__ unsigned long long wave_max(unsig
 }
 
 // ln|det|, arg det and the singular flag from the pivots and the permutation, written by
-// thread 0; s_red needs 2 * nw doubles
+// thread 0; s_red needs 3 * nw doubles.  The permutation's sign is the parity of its
+// inversions (pairs i < j with perm[i] > perm[j]), counted by all threads.
 template <int NT> __device__ void finish_det(const gv::DIArgs &a, const Blk &b, const C *s_piv, const int *s_perm,
-                                             int np, double *s_red, int *s_seen)
+                                             int np, double *s_red)
 {
     constexpr int NW = NT / 64;
     double lg = 0.0, ar = 0.0;
@@ -215,37 +217,33 @@ template <int NT> __device__ void finish_det(const gv::DIArgs &a, const Blk &b, 
         lg += log(hypot(p.x, p.y));
         ar += atan2(p.y, p.x);
     }
+    unsigned int inv = 0;
+    for (int e = threadIdx.x; e < np * np; e += NT) {
+        const int i = e / np, j = e - i * np;
+        inv += (unsigned int)(j > i && s_perm[j] < s_perm[i]);
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         lg += __shfl_xor(lg, o);
         ar += __shfl_xor(ar, o);
     }
+    const unsigned int odd = (unsigned int)__popcll(__ballot(inv & 1u)) & 1u;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
         s_red[wid] = lg;
         s_red[NW + wid] = ar;
+        s_red[2 * NW + wid] = (double)odd;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double la = 0.0, ph = 0.0;
+        int odd_all = 0;
         for (int q = 0; q < NW; ++q) {
             la += s_red[q];
             ph += s_red[NW + q];
+            odd_all ^= (int)s_red[2 * NW + q];
         }
-        // sign of the permutation k -> perm[k] by cycle counting
-        int sgn = 1;
-        for (int i = 0; i < np; ++i) s_seen[i] = 0;
-        for (int i = 0; i < np; ++i) {
-            if (s_seen[i]) continue;
-            int len = 0, j = i;
-            while (!s_seen[j]) {
-                s_seen[j] = 1;
-                j = s_perm[j];
-                ++len;
-            }
-            if ((len & 1) == 0) sgn = -sgn;
-        }
-        if (sgn < 0) ph += M_PI;
+        if (odd_all) ph += M_PI;
         a.logabs[b.o] = la;
         a.phase[b.o] = isfinite(ph) ? remainder(ph, 2 * M_PI) : 0.0;
         a.status[b.o] = !(2.0 * la >= -20.72326583694641);  // |det|^2 < 1e-9 (ln 1e-9), NaN included
@@ -276,7 +274,7 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
     __shared__ int s_perm[NP];       // pivot row of column k
     __shared__ int s_invp[NP];       // column at which row i was pivoted
     __shared__ int s_row[NP], s_col[NP];
-    __shared__ double s_red[2 * NW];
+    __shared__ double s_red[3 * NW];
 
     const Blk b = blk_of(a);
     const int n = b.n;
@@ -334,7 +332,7 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
                 for (int t = 0; t < CPT; ++t) s_prow[par][t * LPR + c] = u[t];
             }
             const C *pr = &s_prow[par][c];
-            const double np2 = piv.x * piv.x + piv.y * piv.y;
+            const double np2 = fma(piv.x, piv.x, piv.y * piv.y);
             const double rn = np2 > 0.0 ? 1.0 / np2 : 0.0;
             const C ainv = cm(piv.x * rn, -piv.y * rn);
             if (tid == 0) {
@@ -394,7 +392,184 @@ __global__ void __launch_bounds__(((NP * LPR + 63) / 64) * 64) di_reg_kernel(gv:
             }
         }
     }
-    finish_det<NT>(a, b, s_piv, s_perm, NP, s_red, s_row);
+    finish_det<NT>(a, b, s_piv, s_perm, NP, s_red);
+}
+
+// a double of lane l (l uniform), into scalar registers
+__device__ __forceinline__ double rdl(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Panel-blocked form of di_reg_kernel's elimination (same pivots, same multipliers and the
+// same multiply-adds in the same order per element, so the same bits).  Wave w holds the
+// PW columns [PW w, PW w + PW) of all rows, lane l the rows l + 64 m.  Columns are eliminated
+// in panels of PW, panel P by wave P alone: its pivot search is a wave reduction, the pivot
+// row's values reach its lanes through scalar registers (readlane), and each step's
+// multipliers go to LDS.  After one barrier every other wave applies the panel's PW steps
+// to its own columns in order: the pivot row of step k is row perm[k] of its own registers
+// (already updated through step k - 1), again broadcast by readlane, and the multipliers
+// come from LDS.  One barrier per panel instead of two per column.
+// LDS buffers alternate by panel parity: panel P + 1's multipliers are written while panel
+// P's are read, and panel P - 1's readers have passed the barrier that ends its panel.
+template <int NP, int PW>
+__global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
+{
+    constexpr int NW = NP / PW;
+    constexpr int NT = NW * 64;
+    constexpr int RPL = (NP + 63) / 64;
+    static_assert(NP % PW == 0 && NT <= 1024, "NP: a multiple of PW, at most 16 waves");
+
+    __shared__ C s_F[2][PW][RPL * 64];  // multipliers of a panel's steps, by row
+    __shared__ C s_piv[NP];
+    __shared__ C s_sc[RPL * 64];        // -1/pivot of each pivot row
+    __shared__ int s_perm[NP];          // pivot row of column k
+    __shared__ int s_invp[RPL * 64];    // column at which row i was pivoted
+    __shared__ int s_done[RPL * 64];    // row pivoted (rows past NP count as pivoted)
+    __shared__ int s_row[NP], s_col[NP];
+    __shared__ double s_red[3 * NW];
+
+    const Blk b = blk_of(a);
+    const int n = b.n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    stage_indices(a, b, s_row, s_col, NP, NT);
+    for (int i = tid; i < RPL * 64; i += NT) {
+        s_done[i] = i >= NP;
+        s_sc[i] = cm(1.0, 0.0);
+        s_invp[i] = 0;
+    }
+    __syncthreads();
+
+    // rows and columns past n are the identity, rows past NP zero
+    C u[RPL][PW];
+#pragma unroll
+    for (int m = 0; m < RPL; ++m) {
+        const int row = m * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < PW; ++j) {
+            const int col = wv * PW + j;
+            u[m][j] = cm(row == col ? 1.0 : 0.0, 0.0);
+            if (row < n && col < n) u[m][j] = elem(a, b, s_row, s_col, row, col);
+        }
+    }
+    // the pivot row's values of this wave's columns: scalar registers
+    auto prow = [&](int p, C (&r)[PW]) __attribute__((always_inline)) {
+        const int pl = p & 63, pm = p >> 6;
+#pragma unroll
+        for (int m = 0; m < RPL; ++m)
+            if (m == pm) {
+#pragma unroll
+                for (int j = 0; j < PW; ++j) r[j] = cm(rdl(u[m][j].x, pl), rdl(u[m][j].y, pl));
+            }
+    };
+    // factorisation of panel P by wave P: its PW steps on its own columns
+    auto factor = [&](int P) __attribute__((always_inline)) {
+        const int par = P & 1;
+        bool pv[RPL];
+#pragma unroll
+        for (int m = 0; m < RPL; ++m) pv[m] = s_done[m * 64 + lane] != 0;
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const int k = P * PW + q;
+            unsigned long long key = 0ull;
+#pragma unroll
+            for (int m = 0; m < RPL; ++m)
+                if (!pv[m]) key = umax64(key, pivot_key(u[m][q], m * 64 + lane));
+            key = wave_max_key<1>(key);
+            // NP - k >= 1 rows are unpivoted, so the key names a row
+            const int p = (int)(~(unsigned int)(key & 0xffffffffull));
+            C r[PW];
+            prow(p, r);
+            const C piv = r[q];
+            const double np2 = fma(piv.x, piv.x, piv.y * piv.y);
+            const double rn = np2 > 0.0 ? 1.0 / np2 : 0.0;
+            const C ainv = cm(piv.x * rn, -piv.y * rn);
+            C f[RPL];
+            bool me[RPL];
+#pragma unroll
+            for (int m = 0; m < RPL; ++m) {
+                me[m] = m * 64 + lane == p;
+                f[m] = me[m] ? cm(0.0, 0.0) : cmul(u[m][q], ainv);
+                pv[m] = pv[m] || me[m];
+            }
+#pragma unroll
+            for (int m = 0; m < RPL; ++m) {
+#pragma unroll
+                for (int j = 0; j < PW; ++j) u[m][j] = cfms(f[m], r[j], u[m][j]);
+                u[m][q] = me[m] ? cm(-1.0, 0.0) : f[m];
+                s_F[par][q][m * 64 + lane] = f[m];
+            }
+            if (lane == 0) {
+                s_perm[k] = p;
+                s_invp[p] = k;
+                s_piv[k] = piv;
+                s_sc[p] = cm(-ainv.x, -ainv.y);
+                s_done[p] = 1;
+            }
+        }
+    };
+    // panel P's PW steps on this wave's columns, in order
+    auto apply = [&](int P) __attribute__((always_inline)) {
+        const int par = P & 1;
+        int pk[PW];
+#pragma unroll
+        for (int q = 0; q < PW; ++q) pk[q] = __builtin_amdgcn_readfirstlane(s_perm[P * PW + q]);
+        C fa[RPL], fb[RPL];
+#pragma unroll
+        for (int m = 0; m < RPL; ++m) fa[m] = s_F[par][0][m * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            if (q + 1 < PW) {
+#pragma unroll
+                for (int m = 0; m < RPL; ++m) fb[m] = s_F[par][q + 1 < PW ? q + 1 : q][m * 64 + lane];
+            }
+            C r[PW];
+            prow(pk[q], r);
+#pragma unroll
+            for (int m = 0; m < RPL; ++m)
+#pragma unroll
+                for (int j = 0; j < PW; ++j) u[m][j] = cfms(fa[m], r[j], u[m][j]);
+#pragma unroll
+            for (int m = 0; m < RPL; ++m) fa[m] = fb[m];
+        }
+    };
+    // Lookahead: while the other waves apply panel P, wave P + 1 applies it first at a
+    // raised issue priority and then factors panel P + 1, so the factorisation's serial
+    // latency overlaps the others' multiply-adds.  One barrier per panel.
+    if (wv == 0) factor(0);
+    __syncthreads();
+#pragma unroll 1
+    for (int P = 0; P < NW; ++P) {
+        if (wv == P + 1) __builtin_amdgcn_s_setprio(2);
+        if (wv != P) apply(P);
+        if (wv == P + 1) {
+            factor(P + 1);
+            __builtin_amdgcn_s_setprio(0);
+        }
+        __syncthreads();
+    }
+    // inverse: (A^-1)[invp[i]][perm[j]] = sc_i U[i][j]
+#pragma unroll
+    for (int m = 0; m < RPL; ++m) {
+        const int row = m * 64 + lane;
+        if (row < n) {
+            const int x = s_invp[row];
+            const C sc = s_sc[row];
+#pragma unroll
+            for (int j = 0; j < PW; ++j) {
+                const int col = wv * PW + j;
+                if (col < n) {
+                    const int y = s_perm[col];
+                    if (x < n && y < n) put(a, b, x, y, cmul(sc, u[m][j]));
+                }
+            }
+        }
+    }
+    finish_det<NT>(a, b, s_piv, s_perm, NP, s_red);
 }
 
 // One row per thread (n <= 256), panels of NB columns; work: column-major, leading
@@ -409,7 +584,7 @@ __global__ void __launch_bounds__(256) di_stream_kernel(gv::DIArgs a)
     __shared__ C s_piv[NT];
     __shared__ int s_perm[NT], s_invp[NT];
     __shared__ int s_row[NT], s_col[NT];
-    __shared__ double s_red[2 * NW];
+    __shared__ double s_red[3 * NW];
 
     const Blk b = blk_of(a);
     const int n = b.n;
@@ -463,7 +638,7 @@ __global__ void __launch_bounds__(256) di_stream_kernel(gv::DIArgs a)
                 const int p = (int)(~(unsigned int)(bkey & 0xffffffffull));
                 const C *cand = s_cand[par][bw];
                 const C piv = cand[m];
-                const double np2 = piv.x * piv.x + piv.y * piv.y;
+                const double np2 = fma(piv.x, piv.x, piv.y * piv.y);
                 const C ainv = np2 > 0.0 ? cm(piv.x / np2, -piv.y / np2) : cm(0.0, 0.0);
                 if (tid == 0) {
                     s_perm[k] = p;
@@ -512,7 +687,7 @@ __global__ void __launch_bounds__(256) di_stream_kernel(gv::DIArgs a)
         __threadfence_block();
         __syncthreads();
     }
-    finish_det<NT>(a, b, s_piv, s_perm, n, s_red, s_row);
+    finish_det<NT>(a, b, s_piv, s_perm, n, s_red);
     if (has_row) {
         const int x = s_invp[row];
         for (int j = 0; j < n; ++j) {
@@ -529,6 +704,12 @@ template <int NP, int LPR> static hipError_t launch_reg(const gv::DIArgs &a, int
     return hipGetLastError();
 }
 
+template <int NP, int PW> static hipError_t launch_pan(const gv::DIArgs &a, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL((di_pan_kernel<NP, PW>), dim3(nblocks), dim3(NP / PW * 64), 0, st, a);
+    return hipGetLastError();
+}
+
 struct RegShape {
     int np;
     hipError_t (*fn)(const gv::DIArgs &, int, hipStream_t);
@@ -540,7 +721,7 @@ struct RegShape {
 static const RegShape kReg[] = {
     {8, launch_reg<8, 8>},    {12, launch_reg<12, 4>},  {18, launch_reg<18, 2>},  {20, launch_reg<20, 4>},
     {32, launch_reg<32, 4>},  {36, launch_reg<36, 4>},  {50, launch_reg<50, 2>},  {52, launch_reg<52, 4>},
-    {72, launch_reg<72, 4>},  {76, launch_reg<76, 4>},  {128, launch_reg<128, 4>},
+    {72, launch_reg<72, 4>},  {76, launch_reg<76, 4>},  {128, launch_pan<128, 8>},
 };
 
 constexpr int kStreamNB = 8;

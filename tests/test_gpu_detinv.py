@@ -1,7 +1,7 @@
 """gpvmc_det_inv (batched linalg::DetInv, linalg.cpp:9-44) on device memory against the
 oracle's or_detinv (LU with partial pivoting, pinned against numpy in test_oracle.py) and
-numpy.  Sizes cover the register-resident kernel (n <= 128, padded) and the streamed one
-(n > 128, with its workspace).
+numpy.  Sizes cover the register-resident kernels (n <= 128, padded; n in 77 .. 128 is the
+panel-blocked one) and the streamed one (n > 128, with its workspace).
 
 Bars: inverse within 1e-10 of the largest oracle entry (times the matrix's condition
 estimate |A| |A^-1| / n, which is 1e1 .. 1e3 for these random matrices), ln|det| within
@@ -43,7 +43,7 @@ def random_mats(n, batch, seed):
     return (rng.normal(size=(batch, n, n)) + 1j * rng.normal(size=(batch, n, n))) / np.sqrt(2)
 
 
-@pytest.mark.parametrize("n,batch", [(4, 9), (8, 16), (9, 7), (33, 12), (50, 8), (73, 8), (128, 6),
+@pytest.mark.parametrize("n,batch", [(4, 9), (8, 16), (9, 7), (33, 12), (50, 8), (73, 8), (100, 5), (128, 6),
                                      (129, 5), (200, 4), (256, 3)])
 def test_det_inv_matches_oracle(n, batch):
     say("case det_inv n=%d batch=%d" % (n, batch))
