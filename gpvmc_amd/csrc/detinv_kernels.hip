@@ -409,27 +409,38 @@ __device__ __forceinline__ double rdl(double v, int l)
 // PW columns [PW w, PW w + PW) of all rows, lane l the rows l + 64 m.  Columns are eliminated
 // in panels of PW, panel P by wave P alone: its pivot search is a wave reduction, the pivot
 // row's values reach its lanes through scalar registers (readlane), and each step's
-// multipliers go to LDS.  After one barrier every other wave applies the panel's PW steps
-// to its own columns in order: the pivot row of step k is row perm[k] of its own registers
-// (already updated through step k - 1), again broadcast by readlane, and the multipliers
-// come from LDS.  One barrier per panel instead of two per column.
-// LDS buffers alternate by panel parity: panel P + 1's multipliers are written while panel
-// P's are read, and panel P - 1's readers have passed the barrier that ends its panel.
+// multipliers go to LDS.  Every other wave applies the panel's steps to its own columns in
+// order: the pivot row of step k is row perm[k] of its own registers (already updated
+// through step k - 1), again broadcast by readlane, and the multipliers come from LDS.
+//
+// No barriers between the first and the last column: the producing wave publishes each
+// step with a release store of the step count, and the other waves follow step by step
+// with acquire loads, so that wave P + 1 (raised issue priority) starts factoring panel
+// P + 1 one step after panel P's last step.  The multipliers live in a ring of NB panels;
+// a producer first waits until every wave is done with the panel it overwrites.  Every
+// wait is bounded: past kSpin polls (about 4 ms; the longest wait in a correct run is a few
+// panels' work, under 0.1 ms) the wave stops waiting for the rest of the matrix and the
+// matrix is flagged singular (status 1), so that no logic error can keep a workgroup from
+// finishing.
+constexpr int kSpin = 1 << 16;
+
 template <int NP, int PW>
 __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
 {
     constexpr int NW = NP / PW;
     constexpr int NT = NW * 64;
     constexpr int RPL = (NP + 63) / 64;
-    static_assert(NP % PW == 0 && NT <= 1024, "NP: a multiple of PW, at most 16 waves");
+    constexpr int NB = 4;
+    static_assert(NP % PW == 0 && NT <= 1024 && NW <= 64, "NP: a multiple of PW, at most 16 waves");
 
-    __shared__ C s_F[2][PW][RPL * 64];  // multipliers of a panel's steps, by row
+    __shared__ C s_F[NB][PW][RPL * 64];  // multipliers of the ring's panels, by row
     __shared__ C s_piv[NP];
-    __shared__ C s_sc[RPL * 64];        // -1/pivot of each pivot row
-    __shared__ int s_perm[NP];          // pivot row of column k
-    __shared__ int s_invp[RPL * 64];    // column at which row i was pivoted
-    __shared__ int s_done[RPL * 64];    // row pivoted (rows past NP count as pivoted)
+    __shared__ C s_sc[RPL * 64];         // -1/pivot of each pivot row
+    __shared__ int s_perm[NP];           // pivot row of column k
+    __shared__ int s_invp[RPL * 64];     // column at which row i was pivoted
+    __shared__ int s_done[RPL * 64];     // row pivoted (rows past NP count as pivoted)
     __shared__ int s_row[NP], s_col[NP];
+    __shared__ int s_sync[2 + NW];       // steps published, wait failure, panels done per wave
     __shared__ double s_red[3 * NW];
 
     const Blk b = blk_of(a);
@@ -442,6 +453,7 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
         s_sc[i] = cm(1.0, 0.0);
         s_invp[i] = 0;
     }
+    for (int i = tid; i < 2 + NW; i += NT) s_sync[i] = 0;
     __syncthreads();
 
     // rows and columns past n are the identity, rows past NP zero
@@ -456,6 +468,35 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
             if (row < n && col < n) u[m][j] = elem(a, b, s_row, s_col, row, col);
         }
     }
+    auto ld_acq = [&](int i) __attribute__((always_inline)) {
+        return __hip_atomic_load(&s_sync[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto st_rel = [&](int i, int v) __attribute__((always_inline)) {
+        __hip_atomic_store(&s_sync[i], v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    bool failed = false;
+    int seen = 0;  // steps known to be published
+    auto wait_pub = [&](int need) __attribute__((always_inline)) {
+        if (seen >= need || failed) return;
+        for (int it = 0; it < kSpin; ++it) {
+            seen = __builtin_amdgcn_readfirstlane(ld_acq(0));
+            if (seen >= need) return;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        failed = true;
+        seen = need;
+    };
+    // before panel P's multipliers overwrite ring slot P % NB: every wave done with P - NB
+    auto wait_ring = [&](int P) __attribute__((always_inline)) {
+        const int need = P - NB + 1;
+        if (need <= 0 || failed) return;
+        for (int it = 0; it < kSpin; ++it) {
+            const int v = lane < NW ? ld_acq(2 + lane) : need;
+            if (__ballot(v < need) == 0ull) return;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        failed = true;
+    };
     // the pivot row's values of this wave's columns: scalar registers
     auto prow = [&](int p, C (&r)[PW]) __attribute__((always_inline)) {
         const int pl = p & 63, pm = p >> 6;
@@ -466,9 +507,9 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
                 for (int j = 0; j < PW; ++j) r[j] = cm(rdl(u[m][j].x, pl), rdl(u[m][j].y, pl));
             }
     };
-    // factorisation of panel P by wave P: its PW steps on its own columns
+    // factorisation of panel P by wave P: its PW steps on its own columns, each published
     auto factor = [&](int P) __attribute__((always_inline)) {
-        const int par = P & 1;
+        const int slot = P % NB;
         bool pv[RPL];
 #pragma unroll
         for (int m = 0; m < RPL; ++m) pv[m] = s_done[m * 64 + lane] != 0;
@@ -495,13 +536,7 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
                 me[m] = m * 64 + lane == p;
                 f[m] = me[m] ? cm(0.0, 0.0) : cmul(u[m][q], ainv);
                 pv[m] = pv[m] || me[m];
-            }
-#pragma unroll
-            for (int m = 0; m < RPL; ++m) {
-#pragma unroll
-                for (int j = 0; j < PW; ++j) u[m][j] = cfms(f[m], r[j], u[m][j]);
-                u[m][q] = me[m] ? cm(-1.0, 0.0) : f[m];
-                s_F[par][q][m * 64 + lane] = f[m];
+                s_F[slot][q][m * 64 + lane] = f[m];
             }
             if (lane == 0) {
                 s_perm[k] = p;
@@ -509,49 +544,48 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
                 s_piv[k] = piv;
                 s_sc[p] = cm(-ainv.x, -ainv.y);
                 s_done[p] = 1;
+                st_rel(0, k + 1);
+            }
+#pragma unroll
+            for (int m = 0; m < RPL; ++m) {
+#pragma unroll
+                for (int j = 0; j < PW; ++j) u[m][j] = cfms(f[m], r[j], u[m][j]);
+                u[m][q] = me[m] ? cm(-1.0, 0.0) : f[m];
             }
         }
     };
-    // panel P's PW steps on this wave's columns, in order
+    // panel P's PW steps on this wave's columns, in order, each once it is published
     auto apply = [&](int P) __attribute__((always_inline)) {
-        const int par = P & 1;
-        int pk[PW];
-#pragma unroll
-        for (int q = 0; q < PW; ++q) pk[q] = __builtin_amdgcn_readfirstlane(s_perm[P * PW + q]);
-        C fa[RPL], fb[RPL];
-#pragma unroll
-        for (int m = 0; m < RPL; ++m) fa[m] = s_F[par][0][m * 64 + lane];
-#pragma unroll
+        const int slot = P % NB;
+#pragma unroll 1
         for (int q = 0; q < PW; ++q) {
-            if (q + 1 < PW) {
+            const int k = P * PW + q;
+            wait_pub(k + 1);
+            const int p = __builtin_amdgcn_readfirstlane(s_perm[k]);
+            C f[RPL];
 #pragma unroll
-                for (int m = 0; m < RPL; ++m) fb[m] = s_F[par][q + 1 < PW ? q + 1 : q][m * 64 + lane];
-            }
+            for (int m = 0; m < RPL; ++m) f[m] = s_F[slot][q][m * 64 + lane];
             C r[PW];
-            prow(pk[q], r);
+            prow(p, r);
 #pragma unroll
             for (int m = 0; m < RPL; ++m)
 #pragma unroll
-                for (int j = 0; j < PW; ++j) u[m][j] = cfms(fa[m], r[j], u[m][j]);
-#pragma unroll
-            for (int m = 0; m < RPL; ++m) fa[m] = fb[m];
+                for (int j = 0; j < PW; ++j) u[m][j] = cfms(f[m], r[j], u[m][j]);
         }
     };
-    // Lookahead: while the other waves apply panel P, wave P + 1 applies it first at a
-    // raised issue priority and then factors panel P + 1, so the factorisation's serial
-    // latency overlaps the others' multiply-adds.  One barrier per panel.
-    if (wv == 0) factor(0);
-    __syncthreads();
 #pragma unroll 1
     for (int P = 0; P < NW; ++P) {
         if (wv == P + 1) __builtin_amdgcn_s_setprio(2);
         if (wv != P) apply(P);
-        if (wv == P + 1) {
-            factor(P + 1);
+        if (wv == P) {
+            wait_ring(P);
+            factor(P);
             __builtin_amdgcn_s_setprio(0);
         }
-        __syncthreads();
+        if (lane == 0) st_rel(2 + wv, P + 1);
     }
+    if (failed && lane == 0) s_sync[1] = 1;
+    __syncthreads();
     // inverse: (A^-1)[invp[i]][perm[j]] = sc_i U[i][j]
 #pragma unroll
     for (int m = 0; m < RPL; ++m) {
@@ -570,6 +604,7 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
         }
     }
     finish_det<NT>(a, b, s_piv, s_perm, NP, s_red);
+    if (tid == 0 && s_sync[1]) a.status[b.o] = 1;
 }
 
 // One row per thread (n <= 256), panels of NB columns; work: column-major, leading
