@@ -735,7 +735,8 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         int kk;
         reduce_alpha(S0, kk);
         for (int k = tid; k < nk; k += NT) s_a0[k] = s_alpha[k];
-        store_regs();
+        // (the bond sums below read [Ai; Q] from M in memory: a measurement launch loaded
+        // its registers from there and changes nothing)
         // Z_fl = M_fl x Phi_fl(current orbitals, sites of the other flavour's particles),
         // 16 targets per tile, on MFMA: Zg[target][row]
         C2 *Ct = reinterpret_cast<C2 *>(s_un);
@@ -1022,15 +1023,17 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             reduce_alpha(S, kmax);
         }
     }
-    // write back
+    // write back (a measurement launch changes neither [Ai; Q] nor the configuration)
     __syncthreads();
-    store_regs();
-    for (int i = tid; i < Ns; i += NT) a.site[(size_t)w * Ns + i] = s_site[i];
-    for (int i = tid; i < 2 * nmax; i += NT) {
-        a.part[(size_t)w * 2 * nmax + i] = s_part[i];
-        a.cur[(size_t)w * 2 * nmax + i] = s_cur[i];
+    if constexpr (MK == 0) {
+        store_regs();
+        for (int i = tid; i < Ns; i += NT) a.site[(size_t)w * Ns + i] = s_site[i];
+        for (int i = tid; i < 2 * nmax; i += NT) {
+            a.part[(size_t)w * 2 * nmax + i] = s_part[i];
+            a.cur[(size_t)w * 2 * nmax + i] = s_cur[i];
+        }
+        for (int i = tid; i < 2 * nqm; i += NT) a.qorb[(size_t)w * 2 * nqm + i] = s_qorb[i];
     }
-    for (int i = tid; i < 2 * nqm; i += NT) a.qorb[(size_t)w * 2 * nqm + i] = s_qorb[i];
     if (tid == 0) {
         if (MK == 0) {
             // ln|det| and arg det per flavour += the launch's product of ratios
