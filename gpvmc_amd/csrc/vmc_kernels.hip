@@ -1222,6 +1222,123 @@ gbuild_kernel(GBArgs a)
     }
 }
 
+// The same tables on the FP64 matrix cores (v_mfma_f64_16x16x4f64), N a multiple of 16:
+// wave w forms the 16-row band [16 w, 16 w + 16) of G over all N / 16 column tiles, the
+// complex product as four real products per tile (re += Ar Br - Ai Bi, im += Ar Bi + Ai Br).
+// Orbitals go through LDS in chunks of KC; the next chunk's global loads are in flight
+// while the current one is multiplied.  Operand layouts: A lane l = row l % 16, k l / 16;
+// B lane l = k l / 16, column l % 16; result element v of lane l = row l / 16 + 4 v,
+// column l % 16.
+template <typename OUT, int N>
+__global__ void __launch_bounds__(N * 4) gbuild_mfma_kernel(GBArgs a)
+{
+    using C = double2;
+    using CO = typename cpx<OUT>::t;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    constexpr int NW = N / 16, NT = 64 * NW, NCB = N / 16;
+    constexpr int KC = 16;                       // orbitals per LDS chunk
+    constexpr int PER = N * KC / NT;             // chunk elements per thread, per operand
+    static_assert(N % 16 == 0 && N % KC == 0 && (N * KC) % NT == 0, "N: a multiple of 16");
+
+    __shared__ int s_other[N];
+    __shared__ C s_a[N][KC + 1];                 // Ai chunk, [row][orbital]
+    __shared__ C s_bm[KC][N + 1];                // Phi chunk, [orbital][particle]
+    __shared__ double s_red[NW][2];
+
+    const int w = blockIdx.x >> 1, fl = blockIdx.x & 1;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int li = lane & 15, kq = lane >> 4;
+    for (int i = tid; i < N; i += NT) s_other[i] = a.part[(size_t)w * 2 * N + (size_t)(1 - fl) * N + i];
+    const C *ai = a.Ai + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    const C *occ = a.occ + (size_t)a.group[w] * 2 * 2 * N * N + (size_t)fl * 2 * N * N;
+    CO *out = reinterpret_cast<CO *>(a.G) + (size_t)w * 2 * N * N + (size_t)fl * N * N;
+    __syncthreads();
+
+    // chunk element e = tid + NT i of each operand: row (or particle) e / KC, orbital e % KC
+    // (plain doubles: the vector type would keep the in-flight chunk in scratch)
+    double par[PER], pai[PER], pbr[PER], pbi[PER];
+    auto fetch = [&](int f0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + NT * i, r = e / KC, ff = e % KC;
+            const C x = ai[(size_t)r * N + f0 + ff];
+            const C y = occ[(size_t)s_other[r] * N + f0 + ff];
+            par[i] = x.x;
+            pai[i] = x.y;
+            pbr[i] = y.x;
+            pbi[i] = y.y;
+        }
+    };
+    fetch(0);
+    v4 cr[NCB], cim[NCB];
+#pragma unroll
+    for (int t = 0; t < NCB; ++t) {
+        cr[t] = v4{0.0, 0.0, 0.0, 0.0};
+        cim[t] = v4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 1
+    for (int f0 = 0; f0 < N; f0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + NT * i, r = e / KC, ff = e % KC;
+            s_a[r][ff] = make_double2(par[i], pai[i]);
+            s_bm[ff][r] = make_double2(pbr[i], pbi[i]);
+        }
+        __syncthreads();
+        if (f0 + KC < N) fetch(f0 + KC);
+#pragma unroll
+        for (int kk = 0; kk < KC / 4; ++kk) {
+            const C av = s_a[wid * 16 + li][kk * 4 + kq];
+            const double nai = -av.y;
+            C bv[NCB];
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) bv[t] = s_bm[kk * 4 + kq][t * 16 + li];
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv[t].x, cr[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv[t].y, cim[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bv[t].y, cr[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv[t].x, cim[t], 0, 0, 0);
+        }
+    }
+    double dmax = 0.0, nmax = 0.0;
+#pragma unroll
+    for (int t = 0; t < NCB; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = wid * 16 + kq + 4 * v, col = t * 16 + li;
+            const size_t o = (size_t)row * N + col;
+            const double gx = cr[t][v], gy = cim[t][v];
+            if (a.drift) {
+                const CO old = out[o];
+                const double dx = (double)old.x - gx, dy = (double)old.y - gy;
+                dmax = fmax(dmax, sqrt(dx * dx + dy * dy));
+                nmax = fmax(nmax, sqrt(gx * gx + gy * gy));
+            }
+            CO g;
+            g.x = (OUT)gx;
+            g.y = (OUT)gy;
+            out[o] = g;
+        }
+    if (a.drift) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, o));
+            nmax = fmax(nmax, __shfl_xor(nmax, o));
+        }
+        if (lane == 0) { s_red[wid][0] = dmax; s_red[wid][1] = nmax; }
+        __syncthreads();
+        if (tid == 0) {
+            double d = 0.0, n = 0.0;
+            for (int q = 0; q < NW; ++q) { d = fmax(d, s_red[q][0]); n = fmax(n, s_red[q][1]); }
+            a.drift[2 * w + fl] = n > 0.0 ? d / n : 0.0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Sum of per-walker StatSpinStruct blocks over the walkers of each parameter group:
 // blockIdx.y = group, one thread per entry, walkers read in order (coalesced over e).
@@ -1270,6 +1387,13 @@ static hipError_t launch_gb_t(const GBArgs &a, int nwalkers, hipStream_t st)
     return hipGetLastError();
 }
 
+template <typename OUT, int N>
+static hipError_t launch_gb_mfma(const GBArgs &a, int nwalkers, hipStream_t st)
+{
+    hipLaunchKernelGGL((gbuild_mfma_kernel<OUT, N>), dim3(2 * nwalkers), dim3(N * 4), 0, st, a);
+    return hipGetLastError();
+}
+
 struct SweepEntry {
     int prec, N, lpr, res;
     hipError_t (*fn)(const SweepArgs &, int, hipStream_t);
@@ -1310,17 +1434,17 @@ static const SweepWduEntry kSweepWdu[] = {
 static const GBEntry kGB[] = {
     {0, 8, 8, launch_gb_t<double, 8, 8>},
     {0, 18, 2, launch_gb_t<double, 18, 2>},
-    {0, 32, 4, launch_gb_t<double, 32, 4>},
+    {0, 32, 4, launch_gb_mfma<double, 32>},
     {0, 50, 2, launch_gb_t<double, 50, 2>},
     {0, 72, 4, launch_gb_t<double, 72, 4>},
-    {0, 128, 8, launch_gb_t<double, 128, 8>},
+    {0, 128, 8, launch_gb_mfma<double, 128>},
     {0, 200, 2, launch_gb_t<double, 200, 2>},
     {1, 8, 8, launch_gb_t<float, 8, 8>},
     {1, 18, 2, launch_gb_t<float, 18, 2>},
-    {1, 32, 4, launch_gb_t<float, 32, 4>},
+    {1, 32, 4, launch_gb_mfma<float, 32>},
     {1, 50, 2, launch_gb_t<float, 50, 2>},
     {1, 72, 4, launch_gb_t<float, 72, 4>},
-    {1, 128, 8, launch_gb_t<float, 128, 8>},
+    {1, 128, 8, launch_gb_mfma<float, 128>},
     {1, 200, 2, launch_gb_t<float, 200, 2>},
 };
 
