@@ -794,6 +794,10 @@ static gv::SweepArgs sweep_args(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uin
     return a;
 }
 
+// steps of one ground-state sweep launch at most (the kernel counts them in 32 bits);
+// longer untraced walks are split into launches of whole measurement intervals
+static constexpr int64_t kMaxLaunchSteps = int64_t(1) << 30;
+
 static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
                      const double *uniforms, int8_t *noop, int8_t *accept, double *ratio,
                      double *eloc, double *mz, double *logw, int measure_only)
@@ -807,6 +811,18 @@ static int walk_impl(gpvmc_ctx *c, int64_t n_steps, int64_t meas, uint32_t mask,
     if (n_steps == 0 && !measure_only) return GPVMC_OK;
     if (c->channel) return walk_channel(c, n_steps, meas, mask, uniforms, noop, accept, ratio, eloc, mz, logw,
                                         measure_only);
+    if (meas > kMaxLaunchSteps) return fail(GPVMC_ERR_ARG, "meas above 2^30");
+    if (n_steps > kMaxLaunchSteps) {
+        if (uniforms || noop || accept || ratio || eloc || mz || logw)
+            return fail(GPVMC_ERR_ARG, "a replayed or traced walk takes at most 2^30 steps per call");
+        const int64_t chunk = meas > 0 ? (kMaxLaunchSteps / meas) * meas : kMaxLaunchSteps;
+        for (int64_t done = 0; done < n_steps; done += chunk) {
+            const int64_t n = n_steps - done < chunk ? n_steps - done : chunk;
+            int rc = walk_impl(c, n, meas, mask, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+            if (rc) return rc;
+        }
+        return GPVMC_OK;
+    }
     const size_t W = c->W;
     const int64_t nmeas = measure_only ? 1
                           : meas > 0 ? (meas == 1 ? n_steps : (n_steps + meas - 1) / meas) : 0;
@@ -1372,6 +1388,8 @@ int gpvmc_batch_add(gpvmc_batch *b, gpvmc_ctx *c, int64_t n_steps, int64_t meas,
     if (c->cfg.device != b->device) return fail(GPVMC_ERR_ARG, "context on another device");
     if (!c->initialised) return fail(GPVMC_ERR_STATE, "walkers not initialised");
     if (n_steps < 1 || meas < 0) return fail(GPVMC_ERR_ARG, "n_steps must be >= 1 and meas >= 0");
+    if (!c->channel && (n_steps > kMaxLaunchSteps || meas > kMaxLaunchSteps))
+        return fail(GPVMC_ERR_ARG, "a ground-state batch member takes at most 2^30 steps per sweep");
     if (mask & ~(uint32_t)(GPVMC_MEAS_PROJHEIS | GPVMC_MEAS_STAGMAGNZ | GPVMC_MEAS_OVERLAP | GPVMC_MEAS_STATSPIN))
         return fail(GPVMC_ERR_ARG, "unknown estimator bit in mask");
     for (const auto &mb : b->m)

@@ -41,11 +41,12 @@ template <typename C> __device__ __forceinline__ C cfma(C a, C b, C acc)
     return acc;
 }
 
+// complex product with a fixed FMA order (the same rounding wherever it is inlined)
 template <typename C> __device__ __forceinline__ C cmul(C a, C b)
 {
     C r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
+    r.x = fma(a.x, b.x, -(a.y * b.y));
+    r.y = fma(a.x, b.y, a.y * b.x);
     return r;
 }
 
@@ -229,10 +230,22 @@ constexpr int stream_chunk(int cpt)
     return 1;
 }
 
-template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false>
+template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false, bool TR = true>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
 {
+    // TR = false: no replay stream and no traces (the production launch); the kernel then
+    // carries none of their pointers or tests
+    const double *const uniforms = TR ? a.uniforms : nullptr;
+    int8_t *const tr_noop = TR ? a.tr_noop : nullptr;
+    int8_t *const tr_accept = TR ? a.tr_accept : nullptr;
+    double *const tr_ratio = TR ? a.tr_ratio : nullptr;
+    double *const tr_eloc = TR ? a.tr_eloc : nullptr;
+    double *const tr_mz = TR ? a.tr_mz : nullptr;
+    double *const tr_logw = TR ? a.tr_logw : nullptr;
+    // steps of this launch (the host splits longer walks) and the measurement interval
+    const int n_steps = (int)a.n_steps;
+    const int meas = (int)a.meas;
     using C = typename cpx<T>::t;
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
     constexpr int CPT = N / LPR;
@@ -246,14 +259,29 @@ sweep_kernel(SweepArgs a)
     static_assert(N % LPR == 0, "LPR must divide N");
     static_assert(64 % LPR == 0, "row groups must not straddle wavefronts");
     static_assert(NT <= 1024, "workgroup too large");
+    // Bond ratio cache (register tables, one-move windows): for every lattice bond b (edge
+    // b of a.edge) whose two sites hold an up particle i and a down particle j, s_bq[b]
+    // holds (G_up[i][j], G_dn[j][i]), the two single-hop ratios of swapping them.  It is
+    // updated with the table's own rank-1 arithmetic after every accepted move, so each
+    // entry equals the table entry bit for bit; proposals and ProjHeis read it instead of
+    // the register tables.
+    constexpr bool BQ = RES && !WDU && !MONLY;
+    constexpr int NB = 2 * NS;                       // bonds of the periodic square lattice
+    constexpr int NBT = BQ ? (NB + NT - 1) / NT : 1; // bonds per thread
+    static_assert(!BQ || NS <= 1024, "bond cache packs v1, v2 in 10 bits each");
 
     __shared__ int s_site[NS];        // (flavour << 16) | label
     __shared__ int s_part[2][N];      // label -> site
-    __shared__ int s_nbr[NS * 4];
-    __shared__ C s_q[2][64][2];       // [parity][lane of the step][flavour] single-hop ratios
+    __shared__ int s_nbr[NS * 4];     // neighbour site; BQ: | (bond index << 16)
+    __shared__ C s_q[BQ ? 1 : 2][64][2];  // [parity][lane of the step][flavour] single-hop ratios
     // [flavour][lane of the row group][re, im][slot]: the moving row, e_cs added
     __shared__ __align__(16) T s_row[2][LPR][2][RP];
-    __shared__ C s_qb[NS * 4];        // measurement: G entry of the particle at site s for neighbour d
+    __shared__ C s_qb[BQ ? 1 : NS * 4];   // measurement: G entry of the particle at site s for neighbour d
+    __shared__ __align__(16) C s_bq[BQ ? NB : 1][2];
+    // BQ, accepted move: per flavour and row, the update coefficient -(G[row][cs] - delta)/q
+    // and the column entry G[row][cs] - delta (cs = cj for the up table, ci for the down one)
+    __shared__ C s_nc[BQ ? 2 : 1][BQ ? N : 1];
+    __shared__ C s_ux[BQ ? 2 : 1][BQ ? N : 1];
     __shared__ double s_red[NT / 64][4];
     __shared__ double s_acc[ACC_STRIDE];
     // thread 0: amplitude change per flavour as complex mantissa [0..3] and binary
@@ -282,7 +310,7 @@ sweep_kernel(SweepArgs a)
         s_amp[0] = 1.0; s_amp[1] = 0.0; s_amp[2] = 1.0; s_amp[3] = 0.0;
         s_amp[4] = 0.0; s_amp[5] = 0.0;
     }
-    int64_t n_noop = 0, n_acc = 0, n_rej = 0;
+    int n_noop = 0, n_acc = 0, n_rej = 0;
 
     C *gG = reinterpret_cast<C *>(a.G) + (size_t)w * 2 * N * N;
     auto gat = [&](int fl, int i, int j) -> C & { return gG[(size_t)fl * N * N + (size_t)i * N + j]; };
@@ -302,8 +330,57 @@ sweep_kernel(SweepArgs a)
         }
     }
     const uint64_t ctr0 = a.counter[w];
-    int64_t mslot = 0;
+    // ln|det| of both flavours at the launch's start (the launch adds its ratios at the end)
+    const double la0 = a.logabs[2 * w] + a.logabs[2 * w + 1];
+    int mslot = 0;
     __syncthreads();
+
+    // BQ: bond b = tid + m NT of this thread, its end sites (a | t << 16) and their site
+    // words, kept current move by move
+    int bend[NBT], binfo[NBT][2];
+    if constexpr (BQ) {
+#pragma unroll
+        for (int m = 0; m < NBT; ++m) {
+            const int b = tid + m * NT;
+            bend[m] = 0;
+            binfo[m][0] = binfo[m][1] = 0;
+            if (b < NB) {
+                const int4 eg = a.edge[b];
+                s_nbr[eg.x * 4 + eg.y] |= b << 16;
+                s_nbr[eg.z * 4 + eg.w] |= b << 16;
+                bend[m] = eg.x | (eg.z << 16);
+                binfo[m][0] = s_site[eg.x];
+                binfo[m][1] = s_site[eg.z];
+            }
+        }
+        __syncthreads();
+        // the owner of row i of G_fl writes G_fl[i][j] for every neighbour site holding a
+        // particle j of the other flavour
+        if (has_row) {
+#pragma unroll
+            for (int fl = 0; fl < 2; ++fl) {
+                const int sh = s_part[fl][row];
+                for (int d = 0; d < 4; ++d) {
+                    const int nb = s_nbr[sh * 4 + d];
+                    const int info = s_site[nb & 0xFFFF];
+                    if ((info >> 16) == fl) continue;  // parallel bond
+                    const int j = info & 0xFFFF;
+                    if (c == j % LPR) {
+                        const int kj = j / LPR;
+                        C val = cmk<C>(0, 0);
+#pragma unroll UNR
+                        for (int k = 0; k < CPT; ++k) {
+                            const T m = (k == kj) ? (T)1 : (T)0;
+                            val.x = fma(m, gr[fl][k], val.x);
+                            val.y = fma(m, gi[fl][k], val.y);
+                        }
+                        s_bq[nb >> 16][fl] = val;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
 
     // StatSpinStruct::measure (StatSpinStruct.cpp:16-135) into this walker's FP64 sums:
     // szz[x][y] += sz_x sz_y (1/4 each); spm or smp diagonal += 1 for up or down at x; for
@@ -369,16 +446,35 @@ sweep_kernel(SweepArgs a)
     };
 
     // Quantity::measure (MetroMC.cpp:61-68); s < 0 is the measurement-only pass
-    auto measure = [&](int64_t s) {
+    auto measure = [&](int s) {
         __syncthreads();
         double ev[2] = {0.0, 0.0};
-        if (a.mask & MEAS_PROJHEIS) {
+        if (BQ && (a.mask & MEAS_PROJHEIS)) {
+            // ProjHeis.cpp:131-166 over the bond cache; qa is the ratio of the particle at
+            // the edge's first site
+#pragma unroll
+            for (int m = 0; m < NBT; ++m) {
+                if (tid + m * NT < NB) {
+                    const int ia = binfo[m][0], it = binfo[m][1];
+                    if (((ia ^ it) >> 16) == 0) {
+                        ev[0] += 0.25;
+                    } else {
+                        const int fa = ia >> 16;
+                        const C qa = s_bq[tid + m * NT][fa], qb = s_bq[tid + m * NT][fa ^ 1];
+                        const double pr = (double)qa.x * qb.x - (double)qa.y * qb.y;
+                        const double pi = (double)qa.x * qb.y + (double)qa.y * qb.x;
+                        ev[0] += -0.25 - 0.5 * pr;
+                        ev[1] += -0.5 * pi;
+                    }
+                }
+            }
+        } else if (a.mask & MEAS_PROJHEIS) {
             if (has_row) {
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     const int sh = s_part[fl][row];
                     for (int d = 0; d < 4; ++d) {
-                        const int t = s_nbr[sh * 4 + d];
+                        const int t = s_nbr[sh * 4 + d] & 0xFFFF;
                         const int info = s_site[t];
                         if ((info >> 16) == fl) continue;  // parallel bond
                         const int j = info & 0xFFFF;
@@ -431,8 +527,7 @@ sweep_kernel(SweepArgs a)
         block_sum<NT, 3>(red, s_red);
         const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
         if (tid == 0) {
-            const double la = a.logabs[2 * w] + a.logabs[2 * w + 1] +
-                              log(hypot(s_amp[0], s_amp[1])) + log(hypot(s_amp[2], s_amp[3])) +
+            const double la = la0 + log(hypot(s_amp[0], s_amp[1])) + log(hypot(s_amp[2], s_amp[3])) +
                               (s_amp[4] + s_amp[5]) * 0.69314718055994531;
             const double lw = 2.0 * la;
             if (s >= 0) {
@@ -449,9 +544,9 @@ sweep_kernel(SweepArgs a)
                 s_acc[3] += 1;
             }
             const size_t mi = (size_t)w * a.n_meas_slots + mslot;
-            if (a.tr_eloc) { a.tr_eloc[2 * mi] = E_re; a.tr_eloc[2 * mi + 1] = E_im; }
-            if (a.tr_mz) { a.tr_mz[2 * mi] = mz; a.tr_mz[2 * mi + 1] = 0.0; }
-            if (a.tr_logw) a.tr_logw[mi] = lw;
+            if (tr_eloc) { tr_eloc[2 * mi] = E_re; tr_eloc[2 * mi + 1] = E_im; }
+            if (tr_mz) { tr_mz[2 * mi] = mz; tr_mz[2 * mi + 1] = 0.0; }
+            if (tr_logw) tr_logw[mi] = lw;
         }
         ++mslot;
         __syncthreads();
@@ -459,13 +554,13 @@ sweep_kernel(SweepArgs a)
 
     // uniforms of the batch of 64 steps starting at s0 (s0 % 64 == 0): lane l decodes step
     // s0 + l into bpv = v1 | (v2 << 16) and its acceptance uniform bu3
-    auto draw_batch = [&](int64_t s0, int &bpv, double &bu3) {
-        const int64_t sl = s0 + lane;
+    auto draw_batch = [&](int s0, int &bpv, double &bu3) {
+        const int sl = s0 + lane;
         int v1 = 0, ed = 0;
         double u3 = 1.0;
-        if (a.uniforms) {
-            if (sl < a.n_steps) {
-                const double *up = a.uniforms + ((size_t)w * a.n_steps + sl) * 4;
+        if (uniforms) {
+            if (sl < n_steps) {
+                const double *up = uniforms + ((size_t)w * n_steps + sl) * 4;
                 v1 = (int)(NS * up[0]);
                 v1 = v1 < NS ? v1 : NS - 1;
                 ed = (int)(4 * up[1]);
@@ -475,14 +570,20 @@ sweep_kernel(SweepArgs a)
         } else {
             const uint64_t st = ctr0 + (uint64_t)sl;
             uint32_t cc[4] = {(uint32_t)st, (uint32_t)(st >> 32), (uint32_t)w, 0u};
-            philox4x32_10_dev(cc, a.key0, a.key1);
+            // the key is made opaque here so that its ten round keys are formed per draw
+            // rather than held in scalar registers through the whole walk
+            uint32_t k0 = a.key0, k1 = a.key1;
+            asm volatile("" : "+s"(k0), "+s"(k1));
+            philox4x32_10_dev(cc, k0, k1);
             // floor(NS u0) and floor(4 u1) of u = (x + 1/2) 2^-32, in integers
             // (exact: NS (2x + 1) < 2^42); cc[2] is the internal-state draw, unused
             v1 = (int)(((uint64_t)NS * (2ull * cc[0] + 1ull)) >> 33);
             ed = (int)(cc[1] >> 30);
             u3 = u32_to_open01(cc[3]);
         }
-        bpv = v1 | (s_nbr[v1 * 4 + ed] << 16);
+        const int nb = s_nbr[v1 * 4 + ed];
+        // BQ: v1 | v2 << 10 | bond << 20; otherwise v1 | v2 << 16
+        bpv = BQ ? (v1 | ((nb & 0xFFFF) << 10) | ((nb >> 16) << 20)) : (v1 | (nb << 16));
         bu3 = u3;
     };
 
@@ -492,30 +593,45 @@ sweep_kernel(SweepArgs a)
         int bpv = 0;          // batch: v1 | (v2 << 16) of step (s0 & ~63) + lane
         double bu3 = 1.0;     // batch: acceptance uniform of that step
         int par = 0;
-        int64_t s0 = 0;
+        int s0 = 0;
         // the next measured step, meas > 0: measurements follow step s when meas == 1 or
         // s % meas == 0, and no window runs past one
-        int64_t next_meas = 0;
-        while (s0 < a.n_steps) {
+        int next_meas = 0;
+        while (s0 < n_steps) {
             const int b0 = (int)(s0 & 63);
             if (b0 == 0) draw_batch(s0, bpv, bu3);
             // window: up to KW steps, within the batch, ending at the next measured step
-            int64_t lim = a.n_steps - s0;
+            int lim = n_steps - s0;
             if (lim > 64 - b0) lim = 64 - b0;
-            if (a.meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
+            if (meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
             const int len = (int)(lim < KW ? lim : KW);
             // lane b0 + i holds step s0 + i of the window, as it holds it in the batch
             const bool inw = lane >= b0 && lane < b0 + len;
-            const int v1 = bpv & 0xFFFF, v2 = bpv >> 16;
+            const int v1 = BQ ? (bpv & 0x3FF) : (bpv & 0xFFFF);
+            const int v2 = BQ ? ((bpv >> 10) & 0x3FF) : (bpv >> 16);
             const double u3 = bu3;
-            const int i1 = s_site[v1], i2 = s_site[v2];
+            // the window's lanes read the two site words (BQ: and the bond's cached ratios);
+            // the other lanes see a trivial step, which no count or decision takes
+            int i1 = 0, i2 = 0;
+            C q0l = cmk<C>(0, 0), q1l = cmk<C>(0, 0);
+            if (inw) {
+                i1 = s_site[v1];
+                i2 = s_site[v2];
+                if constexpr (BQ) {
+                    q0l = s_bq[bpv >> 20][0];
+                    q1l = s_bq[bpv >> 20][1];
+                }
+            }
             const bool noop = ((i1 ^ i2) >> 16) == 0;
             // up particle ci on site sa, down particle cj on site sb
             const bool up1 = (i1 >> 16) == 0;
             const int ci = (up1 ? i1 : i2) & 0xFFFF, cj = (up1 ? i2 : i1) & 0xFFFF;
             const int sa = up1 ? v1 : v2, sb = up1 ? v2 : v1;
-            C q0l = cmk<C>(0, 0), q1l = cmk<C>(0, 0);
-            if constexpr (RES) {
+            if constexpr (BQ) {
+                // the bond's cached ratios were read with the site words (used only on
+                // antiparallel bonds); the cache is current for this window: the last accepted
+                // move ended with a barrier after its update
+            } else if constexpr (RES) {
                 // each wavefront publishes the ratios whose rows it owns: q_up of step i
                 // sits in row ci of G_up, q_dn in row cj of G_dn
                 uint64_t own = __ballot(inw && !noop && ci / RPW == wid);
@@ -569,11 +685,11 @@ sweep_kernel(SweepArgs a)
             n_noop += nnoop;
             n_acc += nmove;
             n_rej += nproc - nnoop - nmove;
-            if (wid == 0 && ((pmask >> lane) & 1ull)) {
-                const size_t ti = (size_t)w * a.n_steps + s0 + (lane - b0);
-                if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
-                if (a.tr_accept) a.tr_accept[ti] = accl ? 1 : 0;
-                if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+            if (TR && wid == 0 && ((pmask >> lane) & 1ull)) {
+                const size_t ti = (size_t)w * n_steps + s0 + (lane - b0);
+                if (tr_noop) tr_noop[ti] = noop ? 1 : 0;
+                if (tr_accept) tr_accept[ti] = accl ? 1 : 0;
+                if (tr_ratio) tr_ratio[ti] = ratio;
             }
             // The accepted move is wave-uniform; taken through an opaque copy it is
             // compiled as an exec-masked region (skipped by s_cbranch_execz like a scalar
@@ -589,7 +705,20 @@ sweep_kernel(SweepArgs a)
                 const double n1 = (double)q1.x * q1.x + (double)q1.y * q1.y;
                 // 1/q per flavour, one reciprocal each, formed while the moving rows are
                 // stashed (kept ahead of the barrier below)
-                const double rn0 = 1.0 / n0, rn1 = 1.0 / n1;
+                double rn0, rn1;
+                if constexpr (sizeof(T) == 4) {
+                    // the coefficients are rounded to FP32: FP64 reciprocal estimate and two
+                    // Newton steps (full FP64 accuracy before that rounding)
+                    rn0 = __builtin_amdgcn_rcp(n0);
+                    rn1 = __builtin_amdgcn_rcp(n1);
+                    rn0 = fma(rn0, fma(-n0, rn0, 1.0), rn0);
+                    rn1 = fma(rn1, fma(-n1, rn1, 1.0), rn1);
+                    rn0 = fma(rn0, fma(-n0, rn0, 1.0), rn0);
+                    rn1 = fma(rn1, fma(-n1, rn1, 1.0), rn1);
+                } else {
+                    rn0 = 1.0 / n0;
+                    rn1 = 1.0 / n1;
+                }
                 const C qi0 = cmk<C>(q0.x * rn0, -q0.y * rn0), qi1 = cmk<C>(q1.x * rn1, -q1.y * rn1);
                 // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
                 const int ks0 = cj_j / LPR, ls0 = cj_j % LPR, ks1 = ci_j / LPR, ls1 = ci_j % LPR;
@@ -600,6 +729,32 @@ sweep_kernel(SweepArgs a)
                 } else {
                     x0 = has_row ? gat(0, row, cj_j) : cmk<C>(0, 0);
                     x1 = has_row ? gat(1, row, ci_j) : cmk<C>(0, 0);
+                }
+                // u_i = G[i][cs] - delta(i, rs), held by lane ls of row i's group, and the
+                // row's update coefficient -u_i / q
+                auto coefficient = [&](int fl, C &uu) -> C {
+                    const C qi = fl ? qi1 : qi0;
+                    const C x = fl ? x1 : x0;
+                    const int ls = fl ? ls1 : ls0, rsel = fl ? cj_j : ci_j;
+                    uu.x = __shfl(x.x, gbase + ls);
+                    uu.y = __shfl(x.y, gbase + ls);
+                    if (row == rsel) uu.x -= (T)1;
+                    const C coef = cmul(uu, qi);
+                    return cmk<C>(-coef.x, -coef.y);
+                };
+                // BQ: both coefficients before the barrier, published with u for the bond
+                // cache update
+                C ncb[2];
+                if constexpr (BQ) {
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        C uu;
+                        ncb[fl] = coefficient(fl, uu);
+                        if (has_row && c == 0) {
+                            s_nc[fl][row] = ncb[fl];
+                            s_ux[fl][row] = uu;
+                        }
+                    }
                 }
                 // the moving rows G_up[ci,:] + e_cj and G_dn[cj,:] + e_ci go to LDS
 #pragma unroll
@@ -643,20 +798,44 @@ sweep_kernel(SweepArgs a)
                         }
                     }
                 }
+                // State::Hop (State.cpp:73-84) and amplitude *= q per flavour
+                // (SlaterDeterminant.cpp:321)
+                auto hop = [&]() {
+                    s_site[sb_j] = ci_j;
+                    s_site[sa_j] = (1 << 16) | cj_j;
+                    s_part[0][ci_j] = sb_j;
+                    s_part[1][cj_j] = sa_j;
+                };
+                auto amplitude = [&]() {
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        const C q = fl ? q1 : q0;
+                        double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1], ex = s_amp[4 + fl];
+                        amp_mul(mr, mi, ex, (double)q.x, (double)q.y);
+                        s_amp[2 * fl] = mr;
+                        s_amp[2 * fl + 1] = mi;
+                        s_amp[4 + fl] = ex;
+                    }
+                };
+                // BQ: one wavefront that stashes no row updates the amplitude before the
+                // barrier and the site map after it (every wavefront reads the window's sites
+                // for itself, so the map changes only once all of them are past the barrier;
+                // the next reads follow the barrier that ends the move)
+                constexpr int NW = NT / 64;
+                const int wu = ci_j / RPW, wd = cj_j / RPW;
+                const int wb = NW < 3 ? 0 : ((wu != 0 && wd != 0) ? 0 : ((wu != 1 && wd != 1) ? 1 : 2));
+                if constexpr (BQ) {
+                    if (wid == wb && lane == 0) amplitude();
+                }
                 asm volatile("" ::"v"(qi0.x), "v"(qi0.y), "v"(qi1.x), "v"(qi1.y));
                 __syncthreads();
+                if constexpr (BQ) {
+                    if (wid == wb && lane == 0) hop();
+                }
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const C qi = fl ? qi1 : qi0;
-                    const C x = fl ? x1 : x0;
-                    const int ls = fl ? ls1 : ls0, rsel = fl ? cj_j : ci_j;
-                    // u_i = G[i][cs] - delta(i, rs), held by lane ls of row i's group
                     C uu;
-                    uu.x = __shfl(x.x, gbase + ls);
-                    uu.y = __shfl(x.y, gbase + ls);
-                    if (row == rsel) uu.x -= (T)1;
-                    const C coef = cmul(uu, qi);
-                    const C nc = cmk<C>(-coef.x, -coef.y);
+                    const C nc = BQ ? ncb[fl] : coefficient(fl, uu);
                     if (has_row) {
                         const T *sr = &s_row[fl][c][0][0], *si = &s_row[fl][c][1][0];
                         if constexpr (RES) {
@@ -699,35 +878,67 @@ sweep_kernel(SweepArgs a)
                         }
                     }
                 }
-                // State::Hop (State.cpp:73-84): every wavefront writes the same values,
-                // so each one reads back its own writes in the next window
-                if (lane == 0) {
-                    s_site[sb_j] = ci_j;
-                    s_site[sa_j] = (1 << 16) | cj_j;
-                    s_part[0][ci_j] = sb_j;
-                    s_part[1][cj_j] = sa_j;
-                }
-                if (tid == 0) {
-                    // amplitude *= q per flavour (SlaterDeterminant.cpp:321)
+                if constexpr (BQ) {
+                    // bond cache: G'[i][j] = G[i][j] + nc_i (G[rs][j] + delta(j, cs)) in the
+                    // table update's operation order.  A bond that holds an antiparallel pair
+                    // only after this move holds ci or cj: its old entries are the published
+                    // column (other label cj / ci) or the stashed row (own label ci / cj).
 #pragma unroll
-                    for (int fl = 0; fl < 2; ++fl) {
-                        const C q = fl ? q1 : q0;
-                        double mr = s_amp[2 * fl], mi = s_amp[2 * fl + 1], ex = s_amp[4 + fl];
-                        amp_mul(mr, mi, ex, (double)q.x, (double)q.y);
-                        s_amp[2 * fl] = mr;
-                        s_amp[2 * fl + 1] = mi;
-                        s_amp[4 + fl] = ex;
+                    for (int m = 0; m < NBT; ++m) {
+                        const int b = tid + m * NT;
+                        if (b < NB) {
+                            const int ea = bend[m] & 0xFFFF, et = bend[m] >> 16;
+                            const int wdn = (1 << 16) | cj_j;
+                            const int ia = ea == sa_j ? wdn : (ea == sb_j ? ci_j : binfo[m][0]);
+                            const int it = et == sa_j ? wdn : (et == sb_j ? ci_j : binfo[m][1]);
+                            binfo[m][0] = ia;
+                            binfo[m][1] = it;
+                            // every operand is loaded in one batch (each label indexes valid
+                                // rows also on a parallel bond); only antiparallel bonds store
+                            {
+                                const bool aup = (ia >> 16) == 0;
+                                const int i = (aup ? ia : it) & 0xFFFF, jd = (aup ? it : ia) & 0xFFFF;
+                                // G_up[ci][jd] + delta(jd, cj) and G_dn[cj][i] + delta(i, ci)
+                                const C v0 = cmk<C>(s_row[0][jd % LPR][0][jd / LPR], s_row[0][jd % LPR][1][jd / LPR]);
+                                const C v1 = cmk<C>(s_row[1][i % LPR][0][i / LPR], s_row[1][i % LPR][1][i / LPR]);
+                                const bool oi = i == ci_j, oj = jd == cj_j;
+                                const C c0 = s_bq[b][0], c1 = s_bq[b][1];
+                                const C x0 = s_ux[0][i], x1 = s_ux[1][jd];
+                                const C n0 = s_nc[0][i], n1 = s_nc[1][jd];
+                                const C o0 = (oj && !oi) ? x0 : ((oi && !oj) ? v0 : c0);
+                                const C o1 = (oi && !oj) ? x1 : ((oj && !oi) ? v1 : c1);
+                                C r0, r1;
+                                r0.x = fma(n0.x, v0.x, o0.x);
+                                r0.x = fma(-n0.y, v0.y, r0.x);
+                                r0.y = fma(n0.x, v0.y, o0.y);
+                                r0.y = fma(n0.y, v0.x, r0.y);
+                                r1.x = fma(n1.x, v1.x, o1.x);
+                                r1.x = fma(-n1.y, v1.y, r1.x);
+                                r1.y = fma(n1.x, v1.y, o1.y);
+                                r1.y = fma(n1.y, v1.x, r1.y);
+                                if ((ia ^ it) >> 16) {
+                                    s_bq[b][0] = r0;
+                                    s_bq[b][1] = r1;
+                                }
+                            }
+                        }
                     }
+                } else {
+                    // every wavefront writes the same values, so each one reads back its own
+                    // writes in the next window
+                    if (lane == 0) hop();
+                    if (tid == 0) amplitude();
                 }
-                // streamed tables: every row is rewritten before the next window reads them
-                if constexpr (!RES) __syncthreads();
+                // streamed tables: every row is rewritten before the next window reads them;
+                // BQ: the bond cache is current before the next window reads it
+                if constexpr (!RES || BQ) __syncthreads();
             }
             s0 += nproc;
             par ^= 1;
-            const int64_t last = s0 - 1;
-            if (a.meas > 0 && last == next_meas) {
+            const int last = s0 - 1;
+            if (meas > 0 && last == next_meas) {
                 measure(last);
-                next_meas = last + a.meas;
+                next_meas = last + meas;
             }
         }
     } else {
@@ -757,14 +968,14 @@ sweep_kernel(SweepArgs a)
         int bpv = 0;
         double bu3 = 1.0;
         int par = 0;
-        int64_t s0 = 0;
-        int64_t next_meas = 0;  // the next measured step (meas > 0)
-        while (s0 < a.n_steps) {
+        int s0 = 0;
+        int next_meas = 0;  // the next measured step (meas > 0)
+        while (s0 < n_steps) {
             const int b0 = (int)(s0 & 63);
             if (b0 == 0) draw_batch(s0, bpv, bu3);
-            int64_t lim = a.n_steps - s0;
+            int lim = n_steps - s0;
             if (lim > 64 - b0) lim = 64 - b0;
-            if (a.meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
+            if (meas > 0 && next_meas - s0 + 1 < lim) lim = next_meas - s0 + 1;
             const int len = (int)(lim < WD_K ? lim : WD_K);
             const bool inw = lane >= b0 && lane < b0 + len;
             const int v1 = bpv & 0xFFFF, v2 = bpv >> 16;
@@ -883,10 +1094,10 @@ sweep_kernel(SweepArgs a)
                     const bool fin = act && lane <= j;
                     noop_m |= __ballot(fin && noop);
                     if (fin) {
-                        const size_t ti = (size_t)w * a.n_steps + s0 + (lane - b0);
-                        if (a.tr_noop) a.tr_noop[ti] = noop ? 1 : 0;
-                        if (a.tr_accept) a.tr_accept[ti] = accl ? 1 : 0;
-                        if (a.tr_ratio) a.tr_ratio[ti] = ratio;
+                        const size_t ti = (size_t)w * n_steps + s0 + (lane - b0);
+                        if (tr_noop) tr_noop[ti] = noop ? 1 : 0;
+                        if (tr_accept) tr_accept[ti] = accl ? 1 : 0;
+                        if (tr_ratio) tr_ratio[ti] = ratio;
                     }
                     if (!moved) break;
                     const int ci_j = __builtin_amdgcn_readlane(ci, j), cj_j = __builtin_amdgcn_readlane(cj, j);
@@ -1093,10 +1304,10 @@ sweep_kernel(SweepArgs a)
             }
             s0 += nproc;
             par ^= 1;
-            const int64_t last = s0 - 1;
-            if (a.meas > 0 && last == next_meas) {
+            const int last = s0 - 1;
+            if (meas > 0 && last == next_meas) {
                 measure(last);
-                next_meas = last + a.meas;
+                next_meas = last + meas;
             }
         }
     }
@@ -1361,10 +1572,15 @@ static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st
     constexpr int NT = ((N * LPR + 63) / 64) * 64;
     // the measurement-only pass (gpvmc_measure) is its own kernel symbol, so profiles of
     // the sweep report sweeps only
+    const bool traced = a.uniforms || a.tr_noop || a.tr_accept || a.tr_ratio || a.tr_eloc || a.tr_mz ||
+                        a.tr_logw;
     if (a.measure_only)
         hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, true>), dim3(nblocks), dim3(NT), 0, st, a);
-    else
+    else if (traced)
         hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false>), dim3(nblocks), dim3(NT), 0, st, a);
+    else
+        hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false, false, false>), dim3(nblocks), dim3(NT), 0, st,
+                           a);
     return hipGetLastError();
 }
 

@@ -129,7 +129,9 @@ int gpvmc_refresh(gpvmc_ctx *ctx, double *drift);
  * counter = (step, walker)) when `uniforms` is NULL, else the replay stream
  * uniforms[w][s][4] (host array, values in [0,1)).  Optional traces (host arrays, NULL to
  * skip): noop/accept[w][s] (int8), ratio[w][s] (double), and per measurement
- * eloc[w][m] (complex), mz[w][m] (complex), logw[w][m] (double, ln |amp|^2). */
+ * eloc[w][m] (complex), mz[w][m] (complex), logw[w][m] (double, ln |amp|^2).
+ * Ground state: meas <= 2^30; a replayed or traced walk takes n_steps <= 2^30, a longer
+ * untraced walk runs as launches of whole measurement intervals. */
 int gpvmc_walk(gpvmc_ctx *ctx, int64_t n_steps, int64_t meas, uint32_t mask,
                const double *uniforms, int8_t *noop, int8_t *accept, double *ratio,
                double *eloc, double *mz, double *logw);
@@ -214,7 +216,7 @@ int gpvmc_read_amplitudes(gpvmc_ctx *ctx, double *amp);
  * run concurrently; later calls replay it.  Adding a member drops the graph.  Members
  * must not be walked through other calls while a sweep runs; refresh (gpvmc_refresh)
  * between sweeps as usual.  gpvmc_batch_sweep returns after the sweeps complete and
- * reports their device time. */
+ * reports their device time.  A ground-state member takes n_steps, meas <= 2^30. */
 typedef struct gpvmc_batch gpvmc_batch;
 int gpvmc_batch_create(int32_t device, gpvmc_batch **out);
 int gpvmc_batch_add(gpvmc_batch *b, gpvmc_ctx *ctx, int64_t n_steps, int64_t meas, uint32_t mask);

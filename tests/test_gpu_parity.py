@@ -349,6 +349,33 @@ def test_errors_are_reported():
         bad[1, 3, 0] = 1.0
         with pytest.raises(binding.GpvmcError, match=r"\[0, 1\)"):
             b.walk(5, uniforms=bad)
+        with pytest.raises(binding.GpvmcError, match="2\\^30"):
+            b.walk(10, meas=(1 << 30) + 1)
+
+
+@pytest.mark.parametrize("prec,L,lpr,W", [(binding.MIXED, 16, 0, 64), (binding.MIXED, 10, 0, 64),
+                                          (binding.MIXED, 8, 2, 64), (binding.FP64, 8, 4, 32),
+                                          (binding.FP64, 4, 8, 32), (binding.FP64, 16, 0, 8)])
+def test_untraced_walk_equals_traced(prec, L, lpr, W):
+    """A walk with no replay stream and no traces runs its own kernel instantiation (the
+    production launch); from the same start and Philox stream it leaves bit-identical
+    configurations, log-amplitudes, ratio tables and accumulators."""
+    say("case untraced vs traced L=%d precision=%d lanes_per_row=%d W=%d" % (L, prec, lpr, W))
+    occ = occ_for(L, 0.25, 0.1)
+    spins = model.random_spins(L, W, 321)
+    Ns = L * L
+    res = []
+    for traced in (True, False):
+        with binding.Batch(L, W, prec, seed=99, lanes_per_row=lpr) as b:
+            b.set_orbitals(0, occ)
+            b.init(spins)
+            b.walk(4 * Ns, meas=Ns, trace=traced)
+            b.walk(Ns + 7, meas=5, trace=traced)
+            sp, part, la, ph = b.state()
+            tab = np.stack([b.ratio_table(w) for w in range(4)])
+            res.append((sp, part, la, ph, b.accumulators(), tab))
+    for x, y in zip(*res):
+        assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8))
 
 
 # ----------------------------------------------------------------------------- mixed FP32
