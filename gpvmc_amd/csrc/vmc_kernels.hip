@@ -92,6 +92,36 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*s_red)[4])
     }
 }
 
+// one DPP move of a double (two 32-bit halves); lanes of rows outside ROW_MASK read 0
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// sum over the 64 lanes of a wavefront, left in lane 63 (every lane active): quad
+// permutations, half-row and row mirrors, then the row broadcasts 15 and 31
+__device__ __forceinline__ double wave_sum_to_lane63(double v)
+{
+    v += dpp_f64<0xB1>(v);        // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);        // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);       // row_half_mirror
+    v += dpp_f64<0x140>(v);       // row_mirror: every lane holds its row's sum
+    v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+constexpr int isqrt_c(int n)
+{
+    int r = 0;
+    while ((r + 1) * (r + 1) <= n) ++r;
+    return r;
+}
+
 // ---------------------------------------------------------------------------------
 // Wave-uniform helpers of the sweep kernel.
 
@@ -230,7 +260,7 @@ constexpr int stream_chunk(int cpt)
     return 1;
 }
 
-template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false, bool TR = true>
+template <typename T, int N, int LPR, bool RES, bool MONLY, bool WDU = false, bool TR = true, bool MF = false>
 __global__ void __launch_bounds__(((N * LPR + 63) / 64) * 64)
 sweep_kernel(SweepArgs a)
 {
@@ -266,6 +296,15 @@ sweep_kernel(SweepArgs a)
     // entry equals the table entry bit for bit; proposals and ProjHeis read it instead of
     // the register tables.
     constexpr bool BQ = RES && !WDU && !MONLY;
+    // MF (FP32, N = 128, 512 threads): the tables live in the accumulator layout of
+    // v_mfma_f32_32x32x2_f32 and each accepted move's rank-1 update is one K = 2 matrix
+    // operation per 32 x 32 tile.  A complex rank-1 update is a real rank-2 update of each
+    // real part: Re G += [Re nc, -Im nc] [Re v; Im v], Im G += [Im nc, Re nc] [Re v; Im v],
+    // with nc the row's coefficient and v the moving row.  Wavefront w holds row block
+    // w / 2 and column blocks 2 (w % 2) and 2 (w % 2) + 1 of both parts of both tables; lane
+    // l of a tile holds column l % 32 and, in register r, row 8 (r / 4) + 4 (l / 32) + r % 4.
+    // The bond cache follows the tables to FP32 rounding (its update is the VALU sequence).
+    static_assert(!MF || (N == 128 && LPR == 4 && sizeof(T) == 4 && BQ), "matrix-core tables: FP32, N = 128");
     constexpr int NB = 2 * NS;                       // bonds of the periodic square lattice
     constexpr int NBT = BQ ? (NB + NT - 1) / NT : 1; // bonds per thread
     static_assert(!BQ || NS <= 1024, "bond cache packs v1, v2 in 10 bits each");
@@ -284,6 +323,13 @@ sweep_kernel(SweepArgs a)
     __shared__ C s_ux[BQ ? 2 : 1][BQ ? N : 1];
     __shared__ double s_red[NT / 64][4];
     __shared__ double s_acc[ACC_STRIDE];
+    // amplitude (s_amp) at each measurement since the last flush, its trace slot and
+    // whether it is accumulated; wavefront 0 forms their log-weights AR at a time
+    constexpr int AR = 64;
+    __shared__ double s_snap[AR][6];
+    __shared__ int s_snap_slot[AR];
+    constexpr int LS = isqrt_c(NS);  // lattice side
+    static_assert(LS * LS == NS, "square lattice");
     // thread 0: amplitude change per flavour as complex mantissa [0..3] and binary
     // exponent [4..5], so no logarithm is taken per step
     __shared__ double s_amp[6];
@@ -317,7 +363,21 @@ sweep_kernel(SweepArgs a)
 
     // RES: real and imaginary parts of the thread's CPT slots of each table
     regvec<T, CPT> gr[2], gi[2];
-    if constexpr (RES) {
+    // MF: tiles [flavour][column block of the pair] of the real and imaginary parts
+    regvec<float, 16> mr[2][2], mi[2][2];
+    const int mrb = wid >> 1, mcp = wid & 1, mh = lane >> 5, ml = lane & 31;
+    if constexpr (MF) {
+#pragma unroll
+        for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const C e = gat(fl, mrb * 32 + (r >> 2) * 8 + mh * 4 + (r & 3), (2 * mcp + cb) * 32 + ml);
+                    mr[fl][cb][r] = e.x;
+                    mi[fl][cb][r] = e.y;
+                }
+    } else if constexpr (RES) {
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
@@ -355,8 +415,20 @@ sweep_kernel(SweepArgs a)
         }
         __syncthreads();
         // the owner of row i of G_fl writes G_fl[i][j] for every neighbour site holding a
-        // particle j of the other flavour
-        if (has_row) {
+        // particle j of the other flavour (MF: every bond thread reads its two entries from
+        // the tables in global memory)
+        if constexpr (MF) {
+#pragma unroll
+            for (int m = 0; m < NBT; ++m) {
+                const int ia = binfo[m][0], it = binfo[m][1];
+                if (tid + m * NT < NB && ((ia ^ it) >> 16)) {
+                    const bool aup = (ia >> 16) == 0;
+                    const int i = (aup ? ia : it) & 0xFFFF, jd = (aup ? it : ia) & 0xFFFF;
+                    s_bq[tid + m * NT][0] = gat(0, i, jd);
+                    s_bq[tid + m * NT][1] = gat(1, jd, i);
+                }
+            }
+        } else if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl) {
                 const int sh = s_part[fl][row];
@@ -445,6 +517,30 @@ sweep_kernel(SweepArgs a)
         }
     };
 
+    // log-weights of the snapshots, ln |amp|^2 = 2 (ln|det_up| + ln|det_dn|), one per lane of
+    // wavefront 0, into the log-weight trace and (in measurement order) s_acc[11]
+    int nsnap = 0;
+    auto flush_snaps = [&]() {
+        if (wid == 0 && nsnap > 0) {
+            double lw = 0.0;
+            if (lane < nsnap) {
+                const double *sp = s_snap[lane];
+                const double la = la0 + log(hypot(sp[0], sp[1])) + log(hypot(sp[2], sp[3])) +
+                                  (sp[4] + sp[5]) * 0.69314718055994531;
+                lw = 2.0 * la;
+                const int slot = s_snap_slot[lane] & 0x7FFFFFFF;
+                if (tr_logw) tr_logw[(size_t)w * a.n_meas_slots + slot] = lw;
+            }
+            double t = s_acc[11];
+            for (int i = 0; i < nsnap; ++i) {
+                const double v = readlane_real(lw, i);
+                if (s_snap_slot[i] >= 0) t += v;
+            }
+            if (lane == 0) s_acc[11] = t;
+        }
+        nsnap = 0;
+    };
+
     // Quantity::measure (MetroMC.cpp:61-68); s < 0 is the measurement-only pass
     auto measure = [&](int s) {
         __syncthreads();
@@ -518,38 +614,55 @@ sweep_kernel(SweepArgs a)
         if (a.mask & MEAS_STAGMAGNZ) {
             // StagMagnZ.cpp:12-43: (1/Ns) sum_v (-1)^(x+y) (2 up_v - 1)
             for (int v = tid; v < NS; v += NT) {
-                const int x = v % a.L, y = v / a.L;
+                const int x = v % LS, y = v / LS;
                 const double sg = ((x + y) & 1) ? -1.0 : 1.0;
                 mv2 += sg * (((s_site[v] >> 16) == 0) ? 1.0 : -1.0);
             }
         }
-        double red[3] = {ev[0], ev[1], mv2};
-        block_sum<NT, 3>(red, s_red);
-        const double E_re = red[0] / NS, E_im = red[1] / NS, mz = red[2] / NS;
+        // the amplitude now (read before the next barrier: an accepted move after this
+        // measurement rewrites s_amp only behind it)
         if (tid == 0) {
-            const double la = la0 + log(hypot(s_amp[0], s_amp[1])) + log(hypot(s_amp[2], s_amp[3])) +
-                              (s_amp[4] + s_amp[5]) * 0.69314718055994531;
-            const double lw = 2.0 * la;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s_snap[nsnap][i] = s_amp[i];
+            s_snap_slot[nsnap] = mslot | (s >= 0 ? 0 : (int)0x80000000);
+        }
+        // per-wavefront sums (DPP, lane 63), then one barrier; thread 0 adds the partials
+        double red[3] = {ev[0], ev[1], mv2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) red[i] = wave_sum_to_lane63(red[i]);
+        if (lane == 63) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) s_red[wid][i] = red[i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NT / 64; ++q)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) t[i] += s_red[q][i];
+            const double E_re = t[0] / NS, E_im = t[1] / NS, mz = t[2] / NS;
             if (s >= 0) {
+                // explicit FMAs: the same rounding in every instantiation of the kernel
                 if (a.mask & MEAS_PROJHEIS) {
                     s_acc[0] += E_re;
                     s_acc[1] += E_im;
-                    s_acc[2] += E_re * E_re + E_im * E_im;
+                    s_acc[2] += fma(E_re, E_re, E_im * E_im);
                 }
                 if (a.mask & MEAS_STAGMAGNZ) {
                     s_acc[4] += mz;
-                    s_acc[6] += mz * mz;
+                    s_acc[6] = fma(mz, mz, s_acc[6]);
                 }
-                s_acc[11] += lw;
                 s_acc[3] += 1;
             }
             const size_t mi = (size_t)w * a.n_meas_slots + mslot;
             if (tr_eloc) { tr_eloc[2 * mi] = E_re; tr_eloc[2 * mi + 1] = E_im; }
             if (tr_mz) { tr_mz[2 * mi] = mz; tr_mz[2 * mi + 1] = 0.0; }
-            if (tr_logw) tr_logw[mi] = lw;
         }
         ++mslot;
-        __syncthreads();
+        if (++nsnap == AR) flush_snaps();
+        // no closing barrier: thread 0 alone touches s_acc, s_red and the snapshots until
+        // the next measurement's barriers
     };
 
     // uniforms of the batch of 64 steps starting at s0 (s0 % 64 == 0): lane l decodes step
@@ -722,8 +835,9 @@ sweep_kernel(SweepArgs a)
                 const C qi0 = cmk<C>(q0.x * rn0, -q0.y * rn0), qi1 = cmk<C>(q1.x * rn1, -q1.y * rn1);
                 // column cj of G_up and column ci of G_dn: slot ks, lane ls of each row group
                 const int ks0 = cj_j / LPR, ls0 = cj_j % LPR, ks1 = ci_j / LPR, ls1 = ci_j % LPR;
-                C x0, x1;
-                if constexpr (RES) {
+                C x0 = cmk<C>(0, 0), x1 = cmk<C>(0, 0);
+                if constexpr (MF) {
+                } else if constexpr (RES) {
                     x0 = cmk<C>(gr[0][ks0], gi[0][ks0]);
                     x1 = cmk<C>(gr[1][ks1], gi[1][ks1]);
                 } else {
@@ -745,7 +859,57 @@ sweep_kernel(SweepArgs a)
                 // BQ: both coefficients before the barrier, published with u for the bond
                 // cache update
                 C ncb[2];
-                if constexpr (BQ) {
+                // MF: the moving columns G_up[:, cj] and G_dn[:, ci] (raw entries, re | im by row)
+                // and the moving rows G_up[ci, :] + e_cj and G_dn[cj, :] + e_ci (re | im by column)
+                T *const mux = reinterpret_cast<T *>(&s_ux[0][0]);
+                T *const mrow = &s_row[0][0][0][0];
+                // MF: the row coefficient -(G[row][cs] - delta) / q from the published column
+                // (formed after the barrier by the table update and the bond cache alike)
+                auto mf_coefficient = [&](int fl, int i) -> C {
+                    C uu = cmk<C>(mux[(fl * 2) * N + i], mux[(fl * 2 + 1) * N + i]);
+                    if (i == (fl ? cj_j : ci_j)) uu.x -= (T)1;
+                    const C coef = cmul(uu, fl ? qi1 : qi0);
+                    return cmk<C>(-coef.x, -coef.y);
+                };
+                if constexpr (MF) {
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        const int cs = fl ? ci_j : cj_j, rs = fl ? cj_j : ci_j;
+                        // column: the lanes of column cs % 32 in the wavefronts holding its block
+                        const int cbc = cs >> 5;
+                        if ((cbc >> 1) == mcp && ml == (cs & 31)) {
+#pragma unroll
+                            for (int cb = 0; cb < 2; ++cb) {
+                                if ((cbc & 1) == cb) {
+#pragma unroll
+                                    for (int q = 0; q < 4; ++q) {
+                                        const int r0 = mrb * 32 + q * 8 + mh * 4;
+                                        *reinterpret_cast<float4 *>(mux + (fl * 2) * N + r0) =
+                                            make_float4(mr[fl][cb][4 * q], mr[fl][cb][4 * q + 1],
+                                                        mr[fl][cb][4 * q + 2], mr[fl][cb][4 * q + 3]);
+                                        *reinterpret_cast<float4 *>(mux + (fl * 2 + 1) * N + r0) =
+                                            make_float4(mi[fl][cb][4 * q], mi[fl][cb][4 * q + 1],
+                                                        mi[fl][cb][4 * q + 2], mi[fl][cb][4 * q + 3]);
+                                    }
+                                }
+                            }
+                        }
+                        // row: register 4 (rr / 8) + rr % 4 of the lanes in half (rr / 4) % 2
+                        const int rr = rs & 31;
+                        if (mrb == (rs >> 5) && mh == ((rr >> 2) & 1)) {
+                            const int reg = (rr >> 3) * 4 + (rr & 3);
+#pragma unroll
+                            for (int cb = 0; cb < 2; ++cb) {
+                                const int col = (2 * mcp + cb) * 32 + ml;
+                                T vr = mr[fl][cb][reg];
+                                const T vi = mi[fl][cb][reg];
+                                if (col == cs) vr += (T)1;
+                                mrow[(fl * 2) * N + col] = vr;
+                                mrow[(fl * 2 + 1) * N + col] = vi;
+                            }
+                        }
+                    }
+                } else if constexpr (BQ) {
 #pragma unroll
                     for (int fl = 0; fl < 2; ++fl) {
                         C uu;
@@ -760,7 +924,7 @@ sweep_kernel(SweepArgs a)
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     const int rsel = fl ? cj_j : ci_j;
-                    if (row == rsel) {
+                    if (!MF && row == rsel) {
                         T *dr = &s_row[fl][c][0][0], *di = &s_row[fl][c][1][0];
                         if constexpr (RES) {
 #pragma unroll
@@ -832,11 +996,74 @@ sweep_kernel(SweepArgs a)
                 if constexpr (BQ) {
                     if (wid == wb && lane == 0) hop();
                 }
+                auto bond_update = [&]() {
+                    // bond cache: G'[i][j] = G[i][j] + nc_i (G[rs][j] + delta(j, cs)) in the
+                    // table update's operation order.  A bond that holds an antiparallel pair
+                    // only after this move holds ci or cj: its old entries are the published
+                    // column (other label cj / ci) or the stashed row (own label ci / cj).
+#pragma unroll
+                    for (int m = 0; m < NBT; ++m) {
+                        const int b = tid + m * NT;
+                        if (b < NB) {
+                            const int ea = bend[m] & 0xFFFF, et = bend[m] >> 16;
+                            const int wdn = (1 << 16) | cj_j;
+                            const int ia = ea == sa_j ? wdn : (ea == sb_j ? ci_j : binfo[m][0]);
+                            const int it = et == sa_j ? wdn : (et == sb_j ? ci_j : binfo[m][1]);
+                            binfo[m][0] = ia;
+                            binfo[m][1] = it;
+                            // every operand is loaded in one batch (each label indexes valid
+                                // rows also on a parallel bond); only antiparallel bonds store
+                            {
+                                const bool aup = (ia >> 16) == 0;
+                                const int i = (aup ? ia : it) & 0xFFFF, jd = (aup ? it : ia) & 0xFFFF;
+                                // G_up[ci][jd] + delta(jd, cj) and G_dn[cj][i] + delta(i, ci)
+                                const C v0 = MF ? cmk<C>(mrow[jd], mrow[N + jd])
+                                                : cmk<C>(s_row[0][jd % LPR][0][jd / LPR], s_row[0][jd % LPR][1][jd / LPR]);
+                                const C v1 = MF ? cmk<C>(mrow[2 * N + i], mrow[3 * N + i])
+                                                : cmk<C>(s_row[1][i % LPR][0][i / LPR], s_row[1][i % LPR][1][i / LPR]);
+                                const bool oi = i == ci_j, oj = jd == cj_j;
+                                const C c0 = s_bq[b][0], c1 = s_bq[b][1];
+                                const C x0 = MF ? cmk<C>(mux[i], mux[N + i]) : s_ux[0][i];
+                                const C x1 = MF ? cmk<C>(mux[2 * N + jd], mux[3 * N + jd]) : s_ux[1][jd];
+                                const C n0 = MF ? mf_coefficient(0, i) : s_nc[0][i];
+                                const C n1 = MF ? mf_coefficient(1, jd) : s_nc[1][jd];
+                                const C o0 = (oj && !oi) ? x0 : ((oi && !oj) ? v0 : c0);
+                                const C o1 = (oi && !oj) ? x1 : ((oj && !oi) ? v1 : c1);
+                                C r0, r1;
+                                r0.x = fma(n0.x, v0.x, o0.x);
+                                r0.x = fma(-n0.y, v0.y, r0.x);
+                                r0.y = fma(n0.x, v0.y, o0.y);
+                                r0.y = fma(n0.y, v0.x, r0.y);
+                                r1.x = fma(n1.x, v1.x, o1.x);
+                                r1.x = fma(-n1.y, v1.y, r1.x);
+                                r1.y = fma(n1.x, v1.y, o1.y);
+                                r1.y = fma(n1.y, v1.x, r1.y);
+                                if ((ia ^ it) >> 16) {
+                                    s_bq[b][0] = r0;
+                                    s_bq[b][1] = r1;
+                                }
+                            }
+                        }
+                    }
+                };
+                if constexpr (MF) {
+#pragma unroll
+                    for (int fl = 0; fl < 2; ++fl) {
+                        const C nc = mf_coefficient(fl, mrb * 32 + ml);
+                        const float ar = mh ? -nc.y : nc.x, ai = mh ? nc.x : nc.y;
+#pragma unroll
+                        for (int cb = 0; cb < 2; ++cb) {
+                            const float bv = mrow[(fl * 2 + mh) * N + (2 * mcp + cb) * 32 + ml];
+                            mr[fl][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bv, mr[fl][cb], 0, 0, 0);
+                            mi[fl][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bv, mi[fl][cb], 0, 0, 0);
+                        }
+                    }
+                }
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     C uu;
-                    const C nc = BQ ? ncb[fl] : coefficient(fl, uu);
-                    if (has_row) {
+                    const C nc = MF ? cmk<C>(0, 0) : (BQ ? ncb[fl] : coefficient(fl, uu));
+                    if (!MF && has_row) {
                         const T *sr = &s_row[fl][c][0][0], *si = &s_row[fl][c][1][0];
                         if constexpr (RES) {
 #pragma unroll
@@ -879,50 +1106,7 @@ sweep_kernel(SweepArgs a)
                     }
                 }
                 if constexpr (BQ) {
-                    // bond cache: G'[i][j] = G[i][j] + nc_i (G[rs][j] + delta(j, cs)) in the
-                    // table update's operation order.  A bond that holds an antiparallel pair
-                    // only after this move holds ci or cj: its old entries are the published
-                    // column (other label cj / ci) or the stashed row (own label ci / cj).
-#pragma unroll
-                    for (int m = 0; m < NBT; ++m) {
-                        const int b = tid + m * NT;
-                        if (b < NB) {
-                            const int ea = bend[m] & 0xFFFF, et = bend[m] >> 16;
-                            const int wdn = (1 << 16) | cj_j;
-                            const int ia = ea == sa_j ? wdn : (ea == sb_j ? ci_j : binfo[m][0]);
-                            const int it = et == sa_j ? wdn : (et == sb_j ? ci_j : binfo[m][1]);
-                            binfo[m][0] = ia;
-                            binfo[m][1] = it;
-                            // every operand is loaded in one batch (each label indexes valid
-                                // rows also on a parallel bond); only antiparallel bonds store
-                            {
-                                const bool aup = (ia >> 16) == 0;
-                                const int i = (aup ? ia : it) & 0xFFFF, jd = (aup ? it : ia) & 0xFFFF;
-                                // G_up[ci][jd] + delta(jd, cj) and G_dn[cj][i] + delta(i, ci)
-                                const C v0 = cmk<C>(s_row[0][jd % LPR][0][jd / LPR], s_row[0][jd % LPR][1][jd / LPR]);
-                                const C v1 = cmk<C>(s_row[1][i % LPR][0][i / LPR], s_row[1][i % LPR][1][i / LPR]);
-                                const bool oi = i == ci_j, oj = jd == cj_j;
-                                const C c0 = s_bq[b][0], c1 = s_bq[b][1];
-                                const C x0 = s_ux[0][i], x1 = s_ux[1][jd];
-                                const C n0 = s_nc[0][i], n1 = s_nc[1][jd];
-                                const C o0 = (oj && !oi) ? x0 : ((oi && !oj) ? v0 : c0);
-                                const C o1 = (oi && !oj) ? x1 : ((oj && !oi) ? v1 : c1);
-                                C r0, r1;
-                                r0.x = fma(n0.x, v0.x, o0.x);
-                                r0.x = fma(-n0.y, v0.y, r0.x);
-                                r0.y = fma(n0.x, v0.y, o0.y);
-                                r0.y = fma(n0.y, v0.x, r0.y);
-                                r1.x = fma(n1.x, v1.x, o1.x);
-                                r1.x = fma(-n1.y, v1.y, r1.x);
-                                r1.y = fma(n1.x, v1.y, o1.y);
-                                r1.y = fma(n1.y, v1.x, r1.y);
-                                if ((ia ^ it) >> 16) {
-                                    s_bq[b][0] = r0;
-                                    s_bq[b][1] = r1;
-                                }
-                            }
-                        }
-                    }
+                    bond_update();
                 } else {
                     // every wavefront writes the same values, so each one reads back its own
                     // writes in the next window
@@ -1312,8 +1496,18 @@ sweep_kernel(SweepArgs a)
         }
     }
 
+    flush_snaps();
     __syncthreads();
-    if constexpr (RES) {
+    if constexpr (MF) {
+#pragma unroll
+        for (int fl = 0; fl < 2; ++fl)
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    gat(fl, mrb * 32 + (r >> 2) * 8 + mh * 4 + (r & 3), (2 * mcp + cb) * 32 + ml) =
+                        cmk<C>(mr[fl][cb][r], mi[fl][cb][r]);
+    } else if constexpr (RES) {
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
@@ -1574,7 +1768,17 @@ static hipError_t launch_sweep_t(const SweepArgs &a, int nblocks, hipStream_t st
     // the sweep report sweeps only
     const bool traced = a.uniforms || a.tr_noop || a.tr_accept || a.tr_ratio || a.tr_eloc || a.tr_mz ||
                         a.tr_logw;
-    if (a.measure_only)
+    // FP32 tables of N = 128 on the matrix cores, except for StatSpinStruct, which reads the
+    // tables in their row layout
+    constexpr bool MFOK = RES && N == 128 && LPR == 4 && sizeof(T) == 4;
+    if (MFOK && !a.measure_only && !(a.mask & MEAS_STATSPIN)) {
+        if (traced)
+            hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false, false, true, MFOK>), dim3(nblocks), dim3(NT), 0,
+                               st, a);
+        else
+            hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false, false, false, MFOK>), dim3(nblocks), dim3(NT), 0,
+                               st, a);
+    } else if (a.measure_only)
         hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, true>), dim3(nblocks), dim3(NT), 0, st, a);
     else if (traced)
         hipLaunchKernelGGL((sweep_kernel<T, N, LPR, RES, false>), dim3(nblocks), dim3(NT), 0, st, a);
