@@ -34,8 +34,7 @@ METRIC = "MC steps/s + ProjHeis E_loc samples/s at L=16; mixed-FP32 vs plain FP6
 PHIS = (0.10, 0.1667, 0.2333, 0.30)
 NEELS = (0.00, 0.10)
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector peak (dual-issue / packed FP32)
-FP32_VALU_TFLOPS = 78.6       # the same without packed FP32 (this build disables it)
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 matrix (and packed-vector) peak
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -575,13 +574,16 @@ def main():
     b64.close()
     fp64_rate = W * Ns * args.fp64_steps / t64
 
-    # rooflines.  Sweep kernel: the ratio tables stay in VGPRs for a whole launch, so the
-    # bound is FP32 arithmetic on the VALU: per MC step 16 p_ap N + 32 a_ap N^2 flops, per
-    # E_loc sample 16 n_ap N (n_ap ~ p_ap 2 Ns antiparallel bonds); bytes of the same model
+    # rooflines.  Sweep kernel: the ratio tables stay in registers for a whole launch, in the
+    # accumulator layout of the matrix cores at L = 16.  Per MC step the model counts 16 p_ap N
+    # flops of ratios and 32 a_ap N^2 of the rank-1 table update, per E_loc sample 16 n_ap N
+    # (n_ap ~ p_ap 2 Ns antiparallel bonds); the update dominates and runs as
+    # v_mfma_f32_32x32x2_f32 (FP32 matrix peak 157.3 TFLOP/s).  Bytes of the same model
     # (p_ap 4 N s + a_ap 4 N^2 s per step, 2 N^2 s per sample, s = 8) are register/LDS-level.
     s = 8
     spl = W * Ns * k  # one launch = one period of k sweeps
     bytes_launch = spl * (p_ap * 4 * N * s + a_ap * 4 * N * N * s) + k * W * 2 * N * N * s
+    flops_update = spl * 32 * a_ap * N * N
     flops_launch = spl * (16 * p_ap * N + 32 * a_ap * N * N) + k * W * 16 * (p_ap * 2 * Ns) * N
     avg_ms = kt["sweep_ms"] / max(kt["sweep_launches"], 1)
     ach_tf = flops_launch / (avg_ms * 1e-3) / 1e12
@@ -591,22 +593,26 @@ def main():
         for ln in open(tf):
             if ln.startswith("hbm_bytes_per_launch:"):
                 traffic = float(ln.split(":")[1])
-    roofline = {"bound": "valu", "achieved": round(ach_tf, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach_tf / FP32_PEAK_TFLOPS, 5), "traffic": traffic,
+    mf = L == 16
+    roofline = {"bound": "mfma" if mf else "valu", "achieved": round(ach_tf, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach_tf / FP32_PEAK_TFLOPS, 5), "traffic": traffic,
                 "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel at this workload, scaled by a "
                                   "plain copy's readings (profiles/sweep_L16_mixed_traffic.md); not measured in "
                                   "this run",
-                "peak_without_packed_fp32": FP32_VALU_TFLOPS,
-                "frac_of_peak_without_packed_fp32": round(ach_tf / FP32_VALU_TFLOPS, 5),
-                "kernel": "sweep_kernel<float,128,4,RES>", "avg_launch_ms": round(avg_ms, 3),
-                "sweeps_per_launch": k,
-                "note": "FP32 VALU arithmetic of the per-step model (16 p_ap N + 32 a_ap N^2 flops per step, "
-                        "16 n_ap N per E_loc sample) per launch over the launch's average duration (HIP events "
-                        "on the context's stream); the build disables packed FP32, whose rate the 157.3 peak "
-                        "assumes",
+                "kernel": "sweep_kernel<float,128,4,RES,MF> (ratio tables in matrix-core tiles)" if mf
+                          else "sweep_kernel<float,N,LPR,RES>",
+                "avg_launch_ms": round(avg_ms, 3), "sweeps_per_launch": k,
+                "update_share_of_flops": round(flops_update / flops_launch, 4),
+                "note": "FP32 flops of the per-step model (16 p_ap N + 32 a_ap N^2 per step, 16 n_ap N per E_loc "
+                        "sample) per launch over the launch's average duration (HIP events on the context's "
+                        "stream).  The rank-1 table update (update_share_of_flops of the model) runs as "
+                        "v_mfma_f32_32x32x2_f32, one K = 2 product per 32 x 32 tile (a complex rank-1 update is a "
+                        "real rank-2 update of each part); peak = the dense FP32 matrix rate.  Ratios, ProjHeis "
+                        "and the bond cache run on the VALU" if mf else
+                        "FP32 VALU arithmetic of the per-step model per launch over its average duration",
                 "algorithmic_bytes": {"achieved": round(bytes_launch / (avg_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
                                       "note": "register/LDS-level bytes of the per-step model; not HBM traffic "
-                                              "(the tables stay in VGPRs)"}}
+                                              "(the tables stay in registers)"}}
     ref_ms = kt["refresh_ms"] / max(kt["refresh_launches"], 1)
     ref_flops = 2 * W * 2 * 8.0 * N ** 3  # Gauss-Jordan 8 N^3 and G = Ai Phi 8 N^3 per matrix
     refresh_roof = {"bound": "fp64", "achieved": round(ref_flops / (ref_ms * 1e-3) / 1e12, 3),

@@ -1499,14 +1499,18 @@ sweep_kernel(SweepArgs a)
     flush_snaps();
     __syncthreads();
     if constexpr (MF) {
+        // the element addresses are formed anew here (an opaque base), so none of the load's
+        // addresses stays live through the walk
+        C *gS = gG;
+        asm volatile("" : "+s"(gS));
 #pragma unroll
         for (int fl = 0; fl < 2; ++fl)
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    gat(fl, mrb * 32 + (r >> 2) * 8 + mh * 4 + (r & 3), (2 * mcp + cb) * 32 + ml) =
-                        cmk<C>(mr[fl][cb][r], mi[fl][cb][r]);
+                    gS[(size_t)fl * N * N + (size_t)(mrb * 32 + (r >> 2) * 8 + mh * 4 + (r & 3)) * N +
+                       (2 * mcp + cb) * 32 + ml] = cmk<C>(mr[fl][cb][r], mi[fl][cb][r]);
     } else if constexpr (RES) {
         if (has_row) {
 #pragma unroll
