@@ -361,7 +361,8 @@ sweep_kernel(SweepArgs a)
     C *gG = reinterpret_cast<C *>(a.G) + (size_t)w * 2 * N * N;
     auto gat = [&](int fl, int i, int j) -> C & { return gG[(size_t)fl * N * N + (size_t)i * N + j]; };
 
-    // RES: real and imaginary parts of the thread's CPT slots of each table
+    // RES: real and imaginary parts of the thread's CPT slots of each table (the
+    // measurement-only pass reads just the bond entries it needs from HBM instead)
     regvec<T, CPT> gr[2], gi[2];
     // MF: tiles [flavour][column block of the pair] of the real and imaginary parts
     regvec<float, 16> mr[2][2], mi[2][2];
@@ -377,7 +378,7 @@ sweep_kernel(SweepArgs a)
                     mr[fl][cb][r] = e.x;
                     mi[fl][cb][r] = e.y;
                 }
-    } else if constexpr (RES) {
+    } else if constexpr (RES && !MONLY) {
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)
@@ -575,7 +576,7 @@ sweep_kernel(SweepArgs a)
                         if ((info >> 16) == fl) continue;  // parallel bond
                         const int j = info & 0xFFFF;
                         if (c == j % LPR) {
-                            if constexpr (RES) {
+                            if constexpr (RES && !MONLY) {
                                 // masked sum over the slots (the slot index differs
                                 // between lanes): no run-time register index
                                 const int kj = j / LPR;
@@ -1511,7 +1512,7 @@ sweep_kernel(SweepArgs a)
                 for (int r = 0; r < 16; ++r)
                     gS[(size_t)fl * N * N + (size_t)(mrb * 32 + (r >> 2) * 8 + mh * 4 + (r & 3)) * N +
                        (2 * mcp + cb) * 32 + ml] = cmk<C>(mr[fl][cb][r], mi[fl][cb][r]);
-    } else if constexpr (RES) {
+    } else if constexpr (RES && !MONLY) {
         if (has_row) {
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl)

@@ -424,6 +424,52 @@ def test_mixed_ratios_track_oracle(L, W, nsweeps):
     assert agree_steps > 0.5 * W * nsteps
 
 
+def test_mixed_matrix_core_tables_match_row_layout():
+    """At L = 16 the FP32 ratio tables live in matrix-core tiles and each accepted move's
+    rank-1 update is v_mfma_f32_32x32x2_f32; a walk that measures StatSpinStruct runs the
+    row-layout kernel (VALU update) instead.  From the same start and Philox stream, with the
+    production's FP64 refresh after every sweep, the two take the same decisions (a flip only
+    on a near-tie), their ratios agree to FP32 rounding (median relative difference 1e-5,
+    the mixed path's bar against the oracle) and the final tables agree to 1e-3 of their
+    largest entry."""
+    L, W, nsw = 16, 32, 4
+    Ns = L * L
+    say("case L=16 mixed: matrix-core tables vs row layout, W=%d" % W)
+    occ = occ_for(L, 0.25, 0.1)
+    spins = model.random_spins(L, W, 2718)
+    outs, tabs = [], []
+    for mask in (binding.MEAS_PROJHEIS, binding.MEAS_PROJHEIS | binding.MEAS_STATSPIN):
+        with binding.Batch(L, W, binding.MIXED, seed=11) as b:
+            b.set_orbitals(0, occ)
+            ok = b.init(spins)
+            parts = []
+            for t in range(nsw):
+                parts.append(b.walk(Ns, meas=Ns, mask=mask, trace=True))
+                tab = np.stack([b.ratio_table(w) for w in range(W)])
+                b.refresh()
+            outs.append({k: np.concatenate([o[k] for o in parts], axis=1) for k in ("accept", "ratio", "noop")})
+            tabs.append(tab)
+    compared = 0
+    for w in range(W):
+        if not ok[w]:
+            continue
+        a0, a1 = outs[0]["accept"][w], outs[1]["accept"][w]
+        diff = np.nonzero(a0 != a1)[0]
+        upto = diff[0] if len(diff) else len(a0)
+        if len(diff):
+            s = diff[0]
+            r = min(outs[0]["ratio"][w][s], outs[1]["ratio"][w][s], 1.0)
+            assert abs(outs[0]["ratio"][w][s] - outs[1]["ratio"][w][s]) <= 1e-3 * max(r, 1e-30) + 1e-6
+            continue
+        live = outs[0]["noop"][w][:upto] == 0
+        r0, r1 = outs[0]["ratio"][w][:upto][live], outs[1]["ratio"][w][:upto][live]
+        assert np.median(np.abs(r0 - r1) / np.maximum(r1, 1e-300)) <= 1e-5
+        scale = np.abs(tabs[1][w]).max()
+        assert np.abs(tabs[0][w] - tabs[1][w]).max() <= 1e-3 * scale
+        compared += 1
+    assert compared >= W // 2
+
+
 @pytest.mark.parametrize("L,W", [(8, 8), (16, 4), (10, 4), (12, 4), (20, 2)])
 def test_mixed_refresh_and_energy_on_identical_configurations(L, W):
     say("case mixed refresh", L, W)
