@@ -1022,11 +1022,13 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             eval0();
             reduce_alpha(S, kmax);
         }
+        // only a walk changes [Ai; Q]: storing it on this branch alone ends the registers'
+        // live range here, so the other branches' work does not add to the walk's peak
+        store_regs();
     }
-    // write back (a measurement launch changes neither [Ai; Q] nor the configuration)
+    // write back the configuration (a measurement launch changes neither it nor [Ai; Q])
     __syncthreads();
     if constexpr (MK == 0) {
-        store_regs();
         for (int i = tid; i < Ns; i += NT) a.site[(size_t)w * Ns + i] = s_site[i];
         for (int i = tid; i < 2 * nmax; i += NT) {
             a.part[(size_t)w * 2 * nmax + i] = s_part[i];
