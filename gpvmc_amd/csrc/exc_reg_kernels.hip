@@ -86,7 +86,7 @@ __host__ __device__ __forceinline__ Sizes make_sizes(int L, int n0, int n1, int 
 // LDS of one walker (byte offsets); ts = bytes of one complex element of the working
 // precision
 struct Off {
-    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, total;
+    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, mz, total;
 };
 
 __host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
@@ -96,7 +96,15 @@ __host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
     return o;
 }
 
-__host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts)
+// measurement with its tiles in LDS: the Q rows' products Zq_fl[target][q] of both
+// flavours, the inverse-row entries y0[fl][target][bond direction] and the packed neighbour
+// map nbt[fl][target]
+__host__ __device__ __forceinline__ size_t mz_bytes(const Sizes &z, size_t ts)
+{
+    return ts * z.zt * (size_t)(z.nq0 + z.nq1) + ts * 8 * (size_t)z.nmax + 4 * 2 * (size_t)z.nmax;
+}
+
+__host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts, bool lz = false)
 {
     Off o;
     size_t off = 0;
@@ -120,6 +128,7 @@ __host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts)
     o.hop = take(off, 4 * 19 * (size_t)z.nk);
     const size_t un = (size_t)z.nqm * z.nc > (size_t)z.nc * 16 ? (size_t)z.nqm * z.nc : (size_t)z.nc * 16;
     o.un = take(off, ts * un);
+    o.mz = take(off, lz ? mz_bytes(z, ts) : 0);
     o.total = off;
     return o;
 }
@@ -135,7 +144,8 @@ template <typename C2> __device__ __forceinline__ C2 mk2(double x, double y)
 // MK = 0, mode 0: walk n_steps (no measurement: the host launches the measurements
 // between walk segments); 2: initial state choice (largest |a_k|^2, vmc.cpp:243-254) and
 // weight; 3: weight only; 4: amplitudes into amp_out.
-// MK = 1: one ProjHeis measurement, into the sums (mode 5) or only the traces (mode 1).
+// MK = 1: one ProjHeis measurement, into the sums (mode 5) or only the traces (mode 1);
+// MK = 2: the same with the contraction's tiles in LDS instead of the HBM scratch zscr.
 template <typename T, int CB, int RBW, int MK>
 __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
 {
@@ -154,7 +164,8 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     const int RBT = z.rb0 + z.rb1, RS = 16 * RBT;
     auto NF = [&](int fl) { return fl ? z.n1 : z.n0; };
     auto NQ = [&](int fl) { return fl ? z.nq1 : z.nq0; };
-    const Off lo = carve(z, sizeof(C2));
+    constexpr bool LZ = MK == 2;
+    const Off lo = carve(z, sizeof(C2), LZ);
     int *const s_site = reinterpret_cast<int *>(smem + lo.site);
     int *const s_part = reinterpret_cast<int *>(smem + lo.part);
     int *const s_cur = reinterpret_cast<int *>(smem + lo.cur);
@@ -175,9 +186,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // [0..3] and binary exponent [4..5], folded into logabs / phase when the launch ends (in
     // the spare bytes past the carved layout)
     double *const s_amp = reinterpret_cast<double *>(smem + lo.total);
-    // the current state's row of the hop graph, resolved to Q rows and slots:
-    // hn[k][fl] hops, hto/hfr[k][fl][m] orbitals, hq[k][fl][m] Q row of hto, hs[k][fl][m]
-    // slot of hfr, hsg[k] HopSign
+    // the current state's row of the hop graph, resolved to Q rows and slots, state index
+    // innermost (the lanes of a wave are consecutive states): hn[fl][k] hops,
+    // hto/hfr[fl][m][k] orbitals, hq[fl][m][k] Q row of hto, hs[fl][m][k] slot of hfr,
+    // hsg[k] HopSign
     int *const s_hn = reinterpret_cast<int *>(smem + lo.hop);
     int *const s_hto = s_hn + 2 * nk;
     int *const s_hfr = s_hto + 4 * nk;
@@ -265,7 +277,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     };
     auto cache_hops = [&]() __attribute__((always_inline)) {
         for (int e = tid; e < nk * 2; e += NT) {
-            const int k = e >> 1, fl = e & 1;
+            const int fl = e >= nk ? 1 : 0, k = e - fl * nk;
             const int n = hop_n(cs, k, fl);
             s_hn[e] = n;
             if (fl == 0) s_hsg[k] = hsign(cs, k);
@@ -277,15 +289,15 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     q = s_qrow[fl * Ns + o2];
                     sl = s_slot[fl * Ns + o1];
                 }
-                s_hfr[e * 2 + m] = o1;
-                s_hto[e * 2 + m] = o2;
-                s_hq[e * 2 + m] = q;
-                s_hs[e * 2 + m] = sl;
+                s_hfr[(fl * 2 + m) * nk + k] = o1;
+                s_hto[(fl * 2 + m) * nk + k] = o2;
+                s_hq[(fl * 2 + m) * nk + k] = q;
+                s_hs[(fl * 2 + m) * nk + k] = sl;
             }
         }
         __syncthreads();
     };
-    // RA[k][fl][m][n] = Q_fl[row of hop_to(c, k, fl, m)][slot of hop_from(c, k, fl, n)]
+    // RA[fl][m][n][k] = Q_fl[row of hop_to(c, k, fl, m)][slot of hop_from(c, k, fl, n)]
     auto build_ra = [&]() __attribute__((always_inline)) {
         C2 *Qd = reinterpret_cast<C2 *>(s_un);
         for (int fl = 0; fl < 2; ++fl) {
@@ -302,12 +314,12 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             }
             __syncthreads();
             for (int e = tid; e < nk * 4; e += NT) {
-                const int k = e >> 2, m = (e >> 1) & 1, q = e & 1;
-                const int R = s_hn[k * 2 + fl];
+                const int mq = e / nk, k = e - mq * nk, m = mq >> 1, q = mq & 1;
+                const int R = s_hn[fl * nk + k];
                 zc v = zmk(0.0, 0.0);
                 if (m < R && q < R)
-                    v = toz(Qd[s_hq[(k * 2 + fl) * 2 + m] * NC + s_hs[(k * 2 + fl) * 2 + q]]);
-                s_ra[((k * 2 + fl) * 2 + m) * 2 + q] = v;
+                    v = toz(Qd[s_hq[(fl * 2 + m) * nk + k] * NC + s_hs[(fl * 2 + q) * nk + k]]);
+                s_ra[((fl * 2 + m) * 2 + q) * nk + k] = v;
             }
             __syncthreads();
         }
@@ -366,14 +378,14 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
             for (int fl = 0; fl < 2; ++fl) {
-                const int R = s_hn[k * 2 + fl];
+                const int R = s_hn[fl * nk + k];
                 zc B[9];
                 eye3(B);
 #pragma unroll
                 for (int m = 0; m < 2; ++m)
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
-                        if (m < R && q < R) B[m * 3 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+                        if (m < R && q < R) B[m * 3 + q] = s_ra[((fl * 2 + m) * 2 + q) * nk + k];
                 const zc d = det3(B);
                 s_dets[k * 2 + fl] = d;
                 tot = zmul(tot, d);
@@ -385,20 +397,20 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // capacitance of state k for the move (ci -> sb, cj -> sa), flavour fl, from the
     // step's LDS vectors
     auto cap_move = [&](int k, int fl, int ci, int sa, int sb, int cj, zc *B) __attribute__((always_inline)) {
-        const int R = s_hn[k * 2 + fl], r = fl ? cj : ci, snew = fl ? sa : sb;
+        const int R = s_hn[fl * nk + k], r = fl ? cj : ci, snew = fl ? sa : sb;
         eye3(B);
         B[0] = toz(s_z[fl * r16 + r]);
 #pragma unroll
         for (int q = 0; q < 2; ++q)
-            if (q < R) B[1 + q] = toz(s_row[fl * NC + s_hs[(k * 2 + fl) * 2 + q]]);
+            if (q < R) B[1 + q] = toz(s_row[fl * NC + s_hs[(fl * 2 + q) * nk + k]]);
 #pragma unroll
         for (int m = 0; m < 2; ++m)
             if (m < R) {
-                const int o2 = s_hto[(k * 2 + fl) * 2 + m];
-                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_hq[(k * 2 + fl) * 2 + m]]), phi(fl, o2, snew));
+                const int o2 = s_hto[(fl * 2 + m) * nk + k];
+                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_hq[(fl * 2 + m) * nk + k]]), phi(fl, o2, snew));
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
-                    if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+                    if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((fl * 2 + m) * 2 + q) * nk + k];
             }
     };
     // ratios of every state for the swap moving up particle ci to sb and down particle
@@ -480,7 +492,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // move to state kn after eval0 (mv false) or eval_mv (mv true)
     auto apply = [&](bool mv, int ci, int sa, int sb, int cj, int kn) __attribute__((always_inline)) {
         const int m0 = mv ? 1 : 0;
-        const int Rk0 = __builtin_amdgcn_readfirstlane(s_hn[kn * 2]), Rk1 = __builtin_amdgcn_readfirstlane(s_hn[kn * 2 + 1]);
+        const int Rk0 = __builtin_amdgcn_readfirstlane(s_hn[kn]), Rk1 = __builtin_amdgcn_readfirstlane(s_hn[nk + kn]);
         auto RK = [&](int fl) { return fl ? Rk1 : Rk0; };
         // columns f_m of M and the Q rows of the filled orbitals, before any change
 #pragma unroll
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
                     if (m >= RK(fl)) break;
-                    const int fm = s_hs[(kn * 2 + fl) * 2 + m];
+                    const int fm = s_hs[(fl * 2 + m) * nk + kn];
                     if (kq == (fm & 3)) {
                         T vr = 0, vi = 0;
 #pragma unroll
@@ -501,7 +513,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             }
                         s_colb[(fl * 2 + m) * r16 + row] = mk2<C2>(vr, vi);
                     }
-                    if (row == NF(fl) + s_hq[(kn * 2 + fl) * 2 + m]) {
+                    if (row == NF(fl) + s_hq[(fl * 2 + m) * nk + kn]) {
 #pragma unroll
                         for (int cb = 0; cb < CB; ++cb)
                             s_vq[(fl * 2 + m) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
@@ -522,7 +534,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     for (int m = 0; m < 2; ++m)
 #pragma unroll
                         for (int q = 0; q < 2; ++q)
-                            if (m < R && q < R) B[m * 3 + q] = s_ra[((kn * 2 + fl) * 2 + m) * 2 + q];
+                            if (m < R && q < R) B[m * 3 + q] = s_ra[((fl * 2 + m) * 2 + q) * nk + kn];
                 }
                 inv3(B, s_binv + fl * 9);
             }
@@ -550,9 +562,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     int ms = -1;
 #pragma unroll
                     for (int m = 0; m < 2; ++m)
-                        if (m < R && s_hto[(kn * 2 + fl) * 2 + m] == o) ms = m;
+                        if (m < R && s_hto[(fl * 2 + m) * nk + kn] == o) ms = m;
                     if (ms >= 0) {
-                        const int o1 = s_hfr[(kn * 2 + fl) * 2 + ms];
+                        const int o1 = s_hfr[(fl * 2 + ms) * nk + kn];
                         rp = s_slot[fl * Ns + o1];
                         if (mv) {
                             const zc eps = zsub(phi(fl, o1, snew), phi(fl, o1, sold));
@@ -562,7 +574,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             for (int m = 0; m < 2; ++m)
                                 if (m < R)
                                     kp[1 + m] = zadd(zmk(m == ms ? 1.0 : 0.0, 0.0),
-                                                     zmul(eps, toz(s_row[fl * NC + s_hs[(kn * 2 + fl) * 2 + m]])));
+                                                     zmul(eps, toz(s_row[fl * NC + s_hs[(fl * 2 + m) * nk + kn]])));
                         } else {
 #pragma unroll
                             for (int m = 0; m < 2; ++m)
@@ -576,7 +588,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                         for (int m = 0; m < 2; ++m)
                             if (m < R)
                                 kp[1 + m] = zadd(toz(s_colb[(fl * 2 + m) * r16 + i]),
-                                                 zmul(dq, toz(s_row[fl * NC + s_hs[(kn * 2 + fl) * 2 + m]])));
+                                                 zmul(dq, toz(s_row[fl * NC + s_hs[(fl * 2 + m) * nk + kn]])));
                     } else {
 #pragma unroll
                         for (int m = 0; m < 2; ++m)
@@ -600,7 +612,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 for (int m = 0; m < 2; ++m)
                     if (m < R) {
                         zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
-                        if (f == s_hs[(kn * 2 + fl) * 2 + m]) v.x -= 1.0;
+                        if (f == s_hs[(fl * 2 + m) * nk + kn]) v.x -= 1.0;
                         V[1 + m] = v;
                     }
             } else {
@@ -608,7 +620,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 for (int m = 0; m < 2; ++m)
                     if (m < R) {
                         zc v = toz(s_vq[(fl * 2 + m) * NC + f]);
-                        if (f == s_hs[(kn * 2 + fl) * 2 + m]) v.x -= 1.0;
+                        if (f == s_hs[(fl * 2 + m) * nk + kn]) v.x -= 1.0;
                         V[m] = v;
                     }
             }
@@ -683,7 +695,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             }
             for (int fl = 0; fl < 2; ++fl)
                 for (int m = 0; m < RK(fl); ++m) {
-                    const int o1 = s_hfr[(kn * 2 + fl) * 2 + m], o2 = s_hto[(kn * 2 + fl) * 2 + m];
+                    const int o1 = s_hfr[(fl * 2 + m) * nk + kn], o2 = s_hto[(fl * 2 + m) * nk + kn];
                     const int fm = s_slot[fl * Ns + o1];
                     s_slot[fl * Ns + o1] = -1;
                     s_slot[fl * Ns + o2] = fm;
@@ -738,8 +750,34 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         // (the bond sums below read [Ai; Q] from M in memory: a measurement launch loaded
         // its registers from there and changes nothing)
         // Z_fl = M_fl x Phi_fl(current orbitals, sites of the other flavour's particles),
-        // 16 targets per tile, on MFMA: Zg[target][row]
+        // 16 targets per tile, on MFMA: Zg[target][row] (MK = 1), or in LDS (MK = 2) only the
+        // entries the bond sums read: the Q rows, Zq_fl[target][q], and of the inverse rows
+        // the entry y0[fl][t][d] of target t and the flavour-fl particle on its bond d
         C2 *Ct = reinterpret_cast<C2 *>(s_un);
+        C2 *const s_zq = reinterpret_cast<C2 *>(smem + lo.mz);
+        C2 *const s_y0 = s_zq + (size_t)z.zt * (z.nq0 + z.nq1);
+        int *const s_nbt = reinterpret_cast<int *>(s_y0 + 8 * nmax);
+        auto zq_at = [&](int fl, int tg, int q) __attribute__((always_inline)) {
+            return s_zq + (fl ? (size_t)z.zt * z.nq0 : 0) + (size_t)tg * NQ(fl) + q;
+        };
+        if constexpr (LZ) {
+            // nbt[fl][t]: byte d holds the flavour-fl particle on bond d of target t (a
+            // particle of the other flavour), 0xFF where that site holds flavour 1 - fl
+            // (published by the barrier after the first tile is staged)
+            for (int i = tid; i < 2 * nmax; i += NT) {
+                const int fl = i >= nmax ? 1 : 0, t = i - fl * nmax;
+                unsigned pk = 0xFFFFFFFFu;
+                if (t < NF(1 - fl)) {
+                    const int st = s_part[(1 - fl) * nmax + t];
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        const int v = s_site[a.nbr[st * 4 + d]];
+                        if ((v >> 16) == fl) pk = (pk & ~(0xFFu << (8 * d))) | ((unsigned)(v & 0xFF) << (8 * d));
+                    }
+                }
+                s_nbt[i] = (int)pk;
+            }
+        }
         for (int fl = 0; fl < 2; ++fl) {
             const int no = NF(1 - fl);
             for (int t0 = 0; t0 < no; t0 += 16) {
@@ -765,29 +803,57 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             zi = MF::mma(mi[j][cb], b.x, zi);
                         }
                         const int rbg = wv + j * NW;
+                        if constexpr (LZ) {
+                            const int tg = t0 + li, rb_f = rbg - (fl ? RB0 : 0);
+                            const unsigned nb = tg < no ? (unsigned)s_nbt[fl * nmax + tg] : 0xFFFFFFFFu;
 #pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            Zg[(size_t)(t0 + li) * RS + rbg * 16 + MF::orow(lane, v)] = mk2<C2>(zr[v], zi[v]);
+                            for (int v = 0; v < 4; ++v) {
+                                const int rw = rb_f * 16 + MF::orow(lane, v);
+                                const C2 val = mk2<C2>(zr[v], zi[v]);
+                                if (tg < no) {
+                                    if (rw < NF(fl)) {
+#pragma unroll
+                                        for (int d = 0; d < 4; ++d)
+                                            if (((nb >> (8 * d)) & 0xFFu) == (unsigned)rw)
+                                                s_y0[(fl * nmax + tg) * 4 + d] = val;
+                                    } else if (rw < NF(fl) + NQ(fl)) {
+                                        *zq_at(fl, tg, rw - NF(fl)) = val;
+                                    }
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                Zg[(size_t)(t0 + li) * RS + rbg * 16 + MF::orow(lane, v)] = mk2<C2>(zr[v], zi[v]);
+                        }
                     }
                 }
                 __syncthreads();
             }
         }
         // antiparallel bonds in bond order (a ballot prefix sum per round of NT bonds) in the
-        // area the contraction tiles used: bl[2 b] = ci | cj << 16, bl[2 b + 1] = sa | sb << 16
-        // (up particle ci at site sa, down particle cj at site sb)
+        // area the contraction tiles used: bl[3 b] = ci | cj << 16, bl[3 b + 1] = sa | sb << 16,
+        // bl[3 b + 2] = d0 | d1 << 8 (up particle ci at site sa, down particle cj at site sb,
+        // sb = nbr[sa][d0], sa = nbr[sb][d1])
         int *const bl = reinterpret_cast<int *>(s_un);
         int nb = 0;
         for (int e0 = 0; e0 < 2 * Ns; e0 += NT) {
             const int e = e0 + tid;
             bool ap = false;
-            int v0 = 0, v1 = 0;
+            int v0 = 0, v1 = 0, v2 = 0;
             if (e < 2 * Ns) {
                 const int4 eg = a.edge[e];
                 const int ii = s_site[eg.x], ij = s_site[eg.z];
                 ap = (ii >> 16) != (ij >> 16);
-                if ((ii >> 16) == 0) { v0 = (ii & 0xFFFF) | ((ij & 0xFFFF) << 16); v1 = eg.x | (eg.z << 16); }
-                else { v0 = (ij & 0xFFFF) | ((ii & 0xFFFF) << 16); v1 = eg.z | (eg.x << 16); }
+                if ((ii >> 16) == 0) {
+                    v0 = (ii & 0xFFFF) | ((ij & 0xFFFF) << 16);
+                    v1 = eg.x | (eg.z << 16);
+                    v2 = eg.y | (eg.w << 8);
+                } else {
+                    v0 = (ij & 0xFFFF) | ((ii & 0xFFFF) << 16);
+                    v1 = eg.z | (eg.x << 16);
+                    v2 = eg.w | (eg.y << 8);
+                }
             }
             const unsigned long long bm = __ballot(ap);
             const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
@@ -799,8 +865,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 tot += s_ired[q];
             }
             if (ap) {
-                bl[2 * (off + pre)] = v0;
-                bl[2 * (off + pre) + 1] = v1;
+                bl[3 * (off + pre)] = v0;
+                bl[3 * (off + pre) + 1] = v1;
+                bl[3 * (off + pre) + 2] = v2;
             }
             nb += tot;
             __syncthreads();
@@ -817,20 +884,26 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             };
             auto bld = [&](int b, BondIn &in) __attribute__((always_inline)) {
                 const int bc = b < nb ? b : nb - 1;
-                const int v0 = bl[2 * bc], v1 = bl[2 * bc + 1];
+                const int v0 = bl[3 * bc], v1 = bl[3 * bc + 1], v2 = bl[3 * bc + 2];
                 const int ci = v0 & 0xFFFF, cj = v0 >> 16, sa = v1 & 0xFFFF, sb = v1 >> 16;
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
                     const int r = fl ? cj : ci, tg = fl ? ci : cj;
                     const int snew = fl ? sa : sb, base = fl ? RB0 * 16 : 0;
                     const C2 *zcol = Zg + (size_t)tg * RS + base;
-                    in.y0[fl] = zcol[r];
+                    if constexpr (LZ)
+                        in.y0[fl] = s_y0[(fl * nmax + tg) * 4 + ((v2 >> (fl ? 0 : 8)) & 3)];
+                    else
+                        in.y0[fl] = zcol[r];
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
-                        in.aq[fl][m] = Mg[(size_t)(base + r) * NC + s_hs[(k * 2 + fl) * 2 + m]];
-                        const int o2 = s_hto[(k * 2 + fl) * 2 + m];
+                        in.aq[fl][m] = Mg[(size_t)(base + r) * NC + s_hs[(fl * 2 + m) * nk + k]];
+                        const int o2 = s_hto[(fl * 2 + m) * nk + k];
                         in.p2s[fl][m] = phi(fl, o2 >= 0 ? o2 : 0, snew);
-                        in.kq[fl][m] = zcol[NF(fl) + s_hq[(k * 2 + fl) * 2 + m]];
+                        if constexpr (LZ)
+                            in.kq[fl][m] = *zq_at(fl, tg, s_hq[(fl * 2 + m) * nk + k]);
+                        else
+                            in.kq[fl][m] = zcol[NF(fl) + s_hq[(fl * 2 + m) * nk + k]];
                     }
                 }
             };
@@ -839,7 +912,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 zc tot = zmk((double)s_hsg[k], 0.0);
 #pragma unroll
                 for (int fl = 0; fl < 2; ++fl) {
-                    const int R = s_hn[k * 2 + fl];
+                    const int R = s_hn[fl * nk + k];
                     zc B[9];
                     eye3(B);
                     B[0] = toz(in.y0[fl]);
@@ -852,7 +925,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                             B[(1 + m) * 3] = zsub(toz(in.kq[fl][m]), in.p2s[fl][m]);
 #pragma unroll
                             for (int q = 0; q < 2; ++q)
-                                if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((k * 2 + fl) * 2 + m) * 2 + q];
+                                if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((fl * 2 + m) * 2 + q) * nk + k];
                         }
                     tot = zmul(tot, det3(B));
                 }
@@ -892,45 +965,15 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         __syncthreads();
         const double lw = 2.0 * (a.logabs[2 * w] + a.logabs[2 * w + 1]) + log(S0);
         if (acc_on) {
-            double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
-            // in batches of four elements per thread, the next batch's loads issued before this
-            // batch's stores (elements past the end repeat the last one and are not stored)
-            const int ne = nk * nk;
-            constexpr int HB = 4;
-            auto hld = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) __attribute__((always_inline)) {
-#pragma unroll
-                for (int j = 0; j < HB; ++j) {
-                    const int e = e0 + j * NT < ne ? e0 + j * NT : ne - 1;
-                    hs[j] = HO[e];
-                    os[j] = HO[(size_t)nk * nk + e];
-                }
-            };
-            auto hst = [&](int e0, double2 (&hs)[HB], double2 (&os)[HB]) __attribute__((always_inline)) {
-#pragma unroll
-                for (int j = 0; j < HB; ++j) {
-                    const int e = e0 + j * NT;
-                    if (e >= ne) continue;
-                    const int k1 = e / nk, k2 = e - k1 * nk;
-                    const zc a1 = s_a0[k1], h2 = s_alpha[k2], a2 = s_a0[k2];
-                    const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
-                    hs[j].x += hv.x / S0;
-                    hs[j].y += hv.y / S0;
-                    os[j].x += ov.x / S0;
-                    os[j].y += ov.y / S0;
-                    HO[e] = hs[j];
-                    HO[(size_t)nk * nk + e] = os[j];
-                }
-            };
-            double2 ha[HB], oa[HB], hb[HB], ob[HB];
-            constexpr int stride = HB * NT;
-            hld(tid, ha, oa);
-            for (int e0 = tid; e0 < ne; e0 += 2 * stride) {
-                hld(e0 + stride, hb, ob);
-                hst(e0, ha, oa);
-                hld(e0 + 2 * stride, ha, oa);
-                hst(e0 + stride, hb, ob);
+            // the record of this measurement; excr_fold_kernel adds it to the H and O sums
+            double2 *hb = a.hbuf + ((size_t)w * a.hb_cap + a.hb_slot) * (2 * nk + 1);
+            for (int k = tid; k < nk; k += NT) {
+                const zc a0 = s_a0[k], hk = s_alpha[k];
+                hb[k] = make_double2(a0.x, a0.y);
+                hb[nk + k] = make_double2(hk.x, hk.y);
             }
             if (tid == 0) {
+                hb[2 * nk] = make_double2(S0, 0.0);
                 sacc[0] += 1;
                 sacc[1] += lw;
             }
@@ -940,7 +983,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         __syncthreads();
     };
 
-    if constexpr (MK == 1) {
+    if constexpr (MK != 0) {
         measure(a.mode == 5);
     } else if (a.mode == 4) {
         const double la = a.logabs[2 * w] + a.logabs[2 * w + 1], ph = a.phase[2 * w] + a.phase[2 * w + 1];
@@ -1048,7 +1091,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         }
         a.cstate[w] = cs;
         a.weight[w] = S;
-        if (MK == 1 && a.mode == 5) {
+        if (MK != 0 && a.mode == 5) {
             double *ac = a.acc + (size_t)w * gv::ACC_STRIDE;
             ac[3] += sacc[0];
             ac[11] += sacc[1];
@@ -1108,13 +1151,46 @@ template <typename T> __global__ void __launch_bounds__(256) excr_mbuild_kernel(
         a.qorb[((size_t)w * 2 + fl) * nqm + q] = (q < nq && q < nnew) ? no[q] : -1;
 }
 
+// ProjHeis H and O sums (ProjHeis.cpp:140-153) of the buffered measurements, in measurement
+// order: H[k1][k2] += conj(a_k1) h_k2 / S, O[k1][k2] += conj(a_k1) a_k2 / S.  One block per
+// walker; the records are staged in LDS and each sum is loaded and stored once per fold
+// instead of once per measurement.
+__global__ void __launch_bounds__(256) excr_fold_kernel(ExcArgs a, int cnt)
+{
+    extern __shared__ double2 s_rec[];
+    const int w = blockIdx.x, nk = a.nk, per = 2 * nk + 1;
+    const double2 *hb = a.hbuf + (size_t)w * a.hb_cap * per;
+    for (int i = threadIdx.x; i < cnt * per; i += blockDim.x) s_rec[i] = hb[i];
+    __syncthreads();
+    double2 *HO = reinterpret_cast<double2 *>(a.ho) + (size_t)w * 2 * nk * nk;
+    const int ne = nk * nk;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        const int k1 = e / nk, k2 = e - k1 * nk;
+        double2 hs = HO[e], os = HO[ne + e];
+        for (int m = 0; m < cnt; ++m) {
+            const double2 *r = s_rec + m * per;
+            const double S0 = r[2 * nk].x;
+            const zc a1 = zmk(r[k1].x, r[k1].y), h2 = zmk(r[nk + k2].x, r[nk + k2].y), a2 = zmk(r[k2].x, r[k2].y);
+            const zc hv = zconjmul(a1, h2), ov = zconjmul(a1, a2);  // conj(a_k1) h_k2, conj(a_k1) a_k2
+            hs.x += hv.x / S0;
+            hs.y += hv.y / S0;
+            os.x += ov.x / S0;
+            os.y += ov.y / S0;
+        }
+        HO[e] = hs;
+        HO[ne + e] = os;
+    }
+}
+
 struct Shape {
     int prec, cb, rbw;
     void (*kern)(ExcArgs);
     void (*meas)(ExcArgs);
+    void (*meas_lds)(ExcArgs);
 };
 
-#define GR_SHAPE(P, TT, C, R) {P, C, R, excr_kernel<TT, C, R, 0>, excr_kernel<TT, C, R, 1>}
+#define GR_SHAPE(P, TT, C, R) \
+    {P, C, R, excr_kernel<TT, C, R, 0>, excr_kernel<TT, C, R, 1>, excr_kernel<TT, C, R, 2>}
 const Shape kShapes[] = {
     GR_SHAPE(1, float, 3, 1),  GR_SHAPE(1, float, 5, 1),   GR_SHAPE(1, float, 9, 2),
     GR_SHAPE(1, float, 13, 2), GR_SHAPE(1, float, 19, 2),  GR_SHAPE(1, float, 19, 3),
@@ -1150,13 +1226,41 @@ extern "C" size_t gv_excr_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0
     return gr::carve(z, prec == 0 ? 16 : 8).total + 64;
 }
 
+extern "C" size_t gv_excr_meas_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0, int nq1, int rb0, int rb1,
+                                         int cb, int nk)
+{
+    const gr::Sizes z = gr::make_sizes(L, n_up, n_dn, nq0, nq1, rb0, rb1, cb, nk);
+    return gr::carve(z, prec == 0 ? 16 : 8, true).total + 64;
+}
+
 extern "C" hipError_t gv_launch_excr(int prec, int rbw, const gv::ExcArgs *a, int nblocks, hipStream_t st)
 {
     const gr::Shape *sh = gr::find_shape(prec, a->cb, rbw);
     if (!sh || (a->rb0 + a->rb1) > rbw * gr::NW) return hipErrorInvalidValue;
     const size_t lds = gv_excr_lds_bytes(prec, a->L, a->n_up, a->n_dn, a->nq0, a->nq1, a->rb0, a->rb1, a->cb, a->nk);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((a->mode == 1 || a->mode == 5) ? sh->meas : sh->kern, dim3(nblocks), dim3(gr::NT), lds, st, *a);
+    if (a->mode == 1 || a->mode == 5) {
+        // measurement: tiles in LDS where they fit, else in the HBM scratch
+        const size_t lz =
+            gv_excr_meas_lds_bytes(prec, a->L, a->n_up, a->n_dn, a->nq0, a->nq1, a->rb0, a->rb1, a->cb, a->nk);
+        if (lz <= 160 * 1024) {
+            hipLaunchKernelGGL(sh->meas_lds, dim3(nblocks), dim3(gr::NT), lz, st, *a);
+        } else {
+            if (!a->zscr) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(sh->meas, dim3(nblocks), dim3(gr::NT), lds, st, *a);
+        }
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(sh->kern, dim3(nblocks), dim3(gr::NT), lds, st, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t gv_launch_excr_fold(const gv::ExcArgs *a, int cnt, int nwalkers, hipStream_t st)
+{
+    if (cnt < 1 || cnt > a->hb_cap || !a->hbuf) return hipErrorInvalidValue;
+    const size_t lds = (size_t)cnt * (2 * a->nk + 1) * sizeof(double2);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gr::excr_fold_kernel, dim3(nwalkers), dim3(256), lds, st, *a, cnt);
     return hipGetLastError();
 }
 

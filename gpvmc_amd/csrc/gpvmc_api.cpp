@@ -120,6 +120,9 @@ struct gpvmc_ctx {
     bool regk = false;
     int rk_cb = 0, rk_rbw = 0, nq0 = 0, nq1 = 0, rb0 = 0, rb1 = 0;
     void *d_M = nullptr, *d_tabT = nullptr, *d_zscr = nullptr;
+    // measurements not yet folded into d_ho: hb_cap records of 2 nk + 1 double2 per walker
+    double2 *d_hbuf = nullptr;
+    int hb_cap = 0;
     void *d_Q = nullptr;  // streamed channel kernel: Q rows [W][2][nqm][N]
     void *d_Y = nullptr;  // streamed channel kernel measurement: [W][2][N + nqm][N]
     int32_t *d_qorb = nullptr;
@@ -134,7 +137,7 @@ static void free_ctx(gpvmc_ctx *c)
                     c->d_cur, c->d_cstate, c->d_weight, c->d_ho, c->d_tab64, c->d_tab32, c->d_Ai32,
                     c->d_work, c->d_hop_n, c->d_hop_orb, c->d_hop_new, c->d_new_off, c->d_new_orb,
                     c->d_hopsign, c->d_chan, c->d_M, c->d_tabT, c->d_zscr, c->d_qorb, c->d_Q, c->d_Y,
-                    c->d_flag};
+                    c->d_flag, c->d_hbuf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -222,6 +225,8 @@ static gv::ExcArgs exc_args(gpvmc_ctx *c, int mode)
     a.qorb = c->d_qorb;
     a.tabT = c->d_tabT;
     a.zscr = c->d_zscr;
+    a.hbuf = c->d_hbuf;
+    a.hb_cap = c->hb_cap;
     a.nq0 = c->nq0;
     a.nq1 = c->nq1;
     a.rb0 = c->rb0;
@@ -246,10 +251,13 @@ static hipError_t launch_exc(gpvmc_ctx *c, const gv::ExcArgs &a, hipStream_t st 
 
 // The register-resident channel kernel walks the segments between measured steps (steps
 // 0, meas, 2 meas, ... are followed by a measurement) and measures in a launch of its own.
+// Each measurement leaves a record (a_k, h_k, S) per walker; every hb_cap measurements, and
+// at the end of the call, the fold kernel adds the records to the H and O sums in order.
 static hipError_t launch_exc_walk(gpvmc_ctx *c, gv::ExcArgs a, int64_t n_steps, int64_t meas, hipStream_t st)
 {
     a.n_total = n_steps;
     int64_t s0 = 0, slot = 0;
+    int pend = 0;
     while (s0 < n_steps) {
         int64_t last = n_steps - 1;
         bool m_after = false;
@@ -268,11 +276,18 @@ static hipError_t launch_exc_walk(gpvmc_ctx *c, gv::ExcArgs a, int64_t n_steps, 
         if (m_after) {
             a.mode = 5;
             a.meas_slot = slot++;
+            a.hb_slot = pend;
             e = launch_exc(c, a, st);
             if (e != hipSuccess) return e;
+            if (++pend == c->hb_cap) {
+                e = gv_launch_excr_fold(&a, pend, c->W, st);
+                if (e != hipSuccess) return e;
+                pend = 0;
+            }
         }
         s0 = last + 1;
     }
+    if (pend > 0) return gv_launch_excr_fold(&a, pend, c->W, st);
     return hipSuccess;
 }
 
@@ -1209,7 +1224,12 @@ int gpvmc_create_channels(const gpvmc_config *cfg, int32_t n_channels, const gpv
         const size_t nqm = (size_t)std::max(1, std::max(nqf[0], nqf[1]));
         CK(hipMalloc(&c->d_M, W * rows * ncol * ts));
         CK(hipMemset(c->d_M, 0, W * rows * ncol * ts));
-        CK(hipMalloc(&c->d_zscr, W * zt * rows * ts));
+        // the measurement's HBM tiles, only where its LDS tiles do not fit
+        if (gv_excr_meas_lds_bytes(cfg->precision, L, nu, nd, nqf[0], nqf[1], rb0, rb1, kcb, nk) > 160 * 1024)
+            CK(hipMalloc(&c->d_zscr, W * zt * rows * ts));
+        const size_t rec = (size_t)(2 * nk + 1) * sizeof(double2);
+        c->hb_cap = (int)std::max<size_t>(1, std::min<size_t>(16, 64 * 1024 / rec));
+        CK(hipMalloc(&c->d_hbuf, W * (size_t)c->hb_cap * rec));
         CK(hipMalloc(&c->d_qorb, W * 2 * nqm * sizeof(int32_t)));
         CK(hipMemset(c->d_qorb, 0xff, W * 2 * nqm * sizeof(int32_t)));
     } else {

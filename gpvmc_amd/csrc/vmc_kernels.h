@@ -97,6 +97,11 @@ struct ExcArgs {
                               // [Ai; Q] x Phi(current orbitals, sites of the other flavour's particles)
     const void *tabT;         // T2 [G][2][Ns sites][Ns orbitals] transposed orbital table
     void *zscr;               // T2 [W][16 * ceil(nmax / 16) targets][(rb0 + rb1) * 16] measurement tiles
+    // register-resident kernel: measurements not yet folded into ho, per walker hb_cap
+    // records of 2 nk + 1 values (a_k, then h_k, then the weight S in .x); a measurement
+    // launch writes record hb_slot, gv_launch_excr_fold adds records 0 .. cnt-1 to ho
+    double2 *hbuf;
+    int32_t hb_cap, hb_slot;
     int32_t nq0, nq1;         // Q rows per flavour
     int32_t rb0, rb1, cb;     // row blocks of 16 per flavour, column blocks of 4
     int64_t n_steps;
@@ -202,8 +207,10 @@ hipError_t gv_launch_detinv(const gv::DIArgs *a, int nblocks, hipStream_t st);
 // row-block slots >= row_blocks) of a precision (1 = found)
 int gv_excr_shape(int prec, int cb_min, int row_blocks, int *cb, int *rbw);
 size_t gv_excr_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0, int nq1, int rb0, int rb1, int cb, int nk);
+size_t gv_excr_meas_lds_bytes(int prec, int L, int n_up, int n_dn, int nq0, int nq1, int rb0, int rb1, int cb, int nk);
 hipError_t gv_launch_excr(int prec, int rbw, const gv::ExcArgs *a, int nblocks, hipStream_t st);
 hipError_t gv_launch_excr_mbuild(int prec, const gv::ExcArgs *a, const double2 *Ai64, int nwalkers, hipStream_t st);
+hipError_t gv_launch_excr_fold(const gv::ExcArgs *a, int cnt, int nwalkers, hipStream_t st);
 hipError_t gv_launch_ss_reduce(const double *ss, const int32_t *idx, const int32_t *off, int G,
                                size_t per, double *out, hipStream_t st);
 }
