@@ -546,10 +546,17 @@ __global__ void __launch_bounds__(NP / PW * 64) di_pan_kernel(gv::DIArgs a)
                 s_done[p] = 1;
                 st_rel(0, k + 1);
             }
+            // the next step's pivot column first: its pivot search can then start while the
+            // other columns are updated
+            if (q + 1 < PW) {
+#pragma unroll
+                for (int m = 0; m < RPL; ++m) u[m][q + 1] = cfms(f[m], r[q + 1], u[m][q + 1]);
+            }
 #pragma unroll
             for (int m = 0; m < RPL; ++m) {
 #pragma unroll
-                for (int j = 0; j < PW; ++j) u[m][j] = cfms(f[m], r[j], u[m][j]);
+                for (int j = 0; j < PW; ++j)
+                    if (j != q + 1) u[m][j] = cfms(f[m], r[j], u[m][j]);
                 u[m][q] = me[m] ? cm(-1.0, 0.0) : f[m];
             }
         }
