@@ -26,6 +26,72 @@ __device__ __forceinline__ zc zdiv(zc a, zc b)
 }
 template <typename C> __device__ __forceinline__ zc toz(C v) { return zmk((double)v.x, (double)v.y); }
 
+// v_l + v_(l xor 16) and v_l + v_(l xor 32) in every lane (every lane active), by the gfx950
+// lane swaps: v_permlane16_swap(v, v) returns {v_l, v_(l xor 16)} in some order, and IEEE
+// addition is commutative, so the sums equal those of a ds_bpermute butterfly
+__device__ __forceinline__ float sum_xor16(float v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum_xor32(float v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ double sum_xor16(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0])) +
+           __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ double sum_xor32(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0])) +
+           __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+// one DPP move of a 64-bit value (two 32-bit halves); lanes of rows outside ROW_MASK keep old
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v, unsigned long long old)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)old, (int)(unsigned)v, CTRL, ROW_MASK, 0xF, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp((int)(unsigned)(old >> 32), (int)(unsigned)(v >> 32), CTRL, ROW_MASK, 0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// one step of the wave reduction below: sum s; largest v with the smallest index k on ties
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ void sum_argmax_step(double &s, double &v, int &k)
+{
+    s += __longlong_as_double((long long)dpp_u64<CTRL, ROW_MASK>((unsigned long long)__double_as_longlong(s), 0ull));
+    const double v2 = __longlong_as_double((long long)dpp_u64<CTRL, ROW_MASK>(
+        (unsigned long long)__double_as_longlong(v), (unsigned long long)__double_as_longlong(v)));
+    const int k2 = __builtin_amdgcn_update_dpp(k, k, CTRL, ROW_MASK, 0xF, false);
+    if (v2 > v || (v2 == v && k2 < k)) {
+        v = v2;
+        k = k2;
+    }
+}
+
+// the sum and the argmax over the 64 lanes of a wavefront, left in lane 63 (every lane
+// active): quad permutations, half-row and row mirrors, then the row broadcasts 15 and 31
+__device__ __forceinline__ void wave_sum_argmax_to_lane63(double &s, double &v, int &k)
+{
+    sum_argmax_step<0xB1>(s, v, k);       // quad_perm [1, 0, 3, 2]
+    sum_argmax_step<0x4E>(s, v, k);       // quad_perm [2, 3, 0, 1]
+    sum_argmax_step<0x141>(s, v, k);      // row_half_mirror
+    sum_argmax_step<0x140>(s, v, k);      // row_mirror: every lane holds its row's result
+    sum_argmax_step<0x142, 0xA>(s, v, k); // row_bcast:15 into rows 1 and 3
+    sum_argmax_step<0x143, 0xC>(s, v, k); // row_bcast:31 into rows 2 and 3
+}
+
 // running product of complex factors as mantissa (mr, mi) times 2^ex: the mantissa is
 // rescaled by a power of two only when its exponent leaves [-256, 256], so that a walk needs
 // no logarithm per accepted move

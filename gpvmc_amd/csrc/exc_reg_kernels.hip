@@ -86,7 +86,8 @@ __host__ __device__ __forceinline__ Sizes make_sizes(int L, int n0, int n1, int 
 // LDS of one walker (byte offsets); ts = bytes of one complex element of the working
 // precision
 struct Off {
-    size_t site, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, mz, total;
+    size_t site, nbr, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, mz,
+        total;
 };
 
 __host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
@@ -109,6 +110,7 @@ __host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts, bool lz
     Off o;
     size_t off = 0;
     o.site = take(off, 4 * (size_t)z.Ns);
+    o.nbr = take(off, 16 * (size_t)z.Ns);
     o.part = take(off, 8 * (size_t)z.nmax);
     o.cur = take(off, 8 * (size_t)z.nmax);
     o.slot = take(off, 8 * (size_t)z.Ns);
@@ -181,6 +183,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     zc *const s_binv = reinterpret_cast<zc *>(smem + lo.binv);
     zc *const s_red = reinterpret_cast<zc *>(smem + lo.red);
     int *const s_ired = reinterpret_cast<int *>(smem + lo.ired);
+    int *const s_nbr = reinterpret_cast<int *>(smem + lo.nbr);  // the lattice's neighbour table
     char *const s_un = smem + lo.un;
     // thread 0: the product of the accepted moves' determinant ratios per flavour, mantissa
     // [0..3] and binary exponent [4..5], folded into logabs / phase when the launch ends (in
@@ -216,6 +219,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     C2 *Mg = reinterpret_cast<C2 *>(a.M) + (size_t)w * RS * NC;
 
     // ---- walker state into LDS and registers
+    for (int i = tid; i < 4 * Ns; i += NT) s_nbr[i] = a.nbr[i];
     for (int i = tid; i < Ns; i += NT) {
         s_site[i] = a.site[(size_t)w * Ns + i];
         s_slot[i] = -1;
@@ -325,24 +329,17 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         }
     };
     // sum_k |alpha_k|^2 and the first k of largest |alpha_k|^2 (std::max_element) from each
-    // thread's partials: wave butterflies, then one barrier and a pass over the waves'
-    // results.  The results alternate between two halves of s_red, so the next reduction
-    // writes the other half; the one after it comes after this one's barrier in every thread.
+    // thread's partials: DPP wave reductions into lane 63, then one barrier and a pass over
+    // the waves' results.  The results alternate between two halves of s_red, so the next
+    // reduction writes the other half; the one after it comes after this one's barrier in
+    // every thread.
     int rpar = 0;
     auto reduce_partials = [&](double ls, double bv, int bk, double &sum, int &kmax) __attribute__((always_inline)) {
-        for (int o = 32; o > 0; o >>= 1) {
-            ls += __shfl_xor(ls, o);
-            const double ov = __shfl_xor(bv, o);
-            const int ok = __shfl_xor(bk, o);
-            if (ov > bv || (ov == bv && ok < bk)) {
-                bv = ov;
-                bk = ok;
-            }
-        }
+        wave_sum_argmax_to_lane63(ls, bv, bk);
         zc *const rd = s_red + rpar * NW;
         int *const ri = s_ired + rpar * NW;
         rpar ^= 1;
-        if (lane == 0) {
+        if (lane == 63) {
             rd[wv] = zmk(ls, bv);
             ri[wv] = bk;
         }
@@ -396,7 +393,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     };
     // capacitance of state k for the move (ci -> sb, cj -> sa), flavour fl, from the
     // step's LDS vectors
-    auto cap_move = [&](int k, int fl, int ci, int sa, int sb, int cj, zc *B) __attribute__((always_inline)) {
+    // (with usepf: pf0, pf1 = Phi(hop_to(m), snew) of state k's hops, loaded ahead by the caller)
+    auto cap_move = [&](int k, int fl, int ci, int sa, int sb, int cj, zc *B, bool usepf = false,
+                        C2 pf0 = C2(), C2 pf1 = C2()) __attribute__((always_inline)) {
         const int R = s_hn[fl * nk + k], r = fl ? cj : ci, snew = fl ? sa : sb;
         eye3(B);
         B[0] = toz(s_z[fl * r16 + r]);
@@ -407,7 +406,8 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         for (int m = 0; m < 2; ++m)
             if (m < R) {
                 const int o2 = s_hto[(fl * 2 + m) * nk + k];
-                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_hq[(fl * 2 + m) * nk + k]]), phi(fl, o2, snew));
+                B[(1 + m) * 3] = zsub(toz(s_z[fl * r16 + NF(fl) + s_hq[(fl * 2 + m) * nk + k]]),
+                                      usepf ? toz(m ? pf1 : pf0) : phi(fl, o2, snew));
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
                     if (q < R) B[(1 + m) * 3 + 1 + q] = s_ra[((fl * 2 + m) * 2 + q) * nk + k];
@@ -416,6 +416,20 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // ratios of every state for the swap moving up particle ci to sb and down particle
     // cj to sa: c vectors, one GEMV per flavour, the moving rows, the capacitances
     auto eval_mv = [&](int ci, int sa, int sb, int cj, double &S2, int &kn) __attribute__((always_inline)) {
+        // Phi(hop_to(m), snew) of the hops of thread tid's (state, flavour) = (tid / 2, tid % 2)
+        // (the first pass of the capacitance loop), loaded now so that their latency passes
+        // under the c vectors and the GEMV (where the matrix leaves the registers for it: up
+        // to 38 f32 or 19 f64 tiles per lane)
+        constexpr bool PF = CB * RBW * (int)sizeof(T) <= 152;
+        C2 pf[2];
+        if constexpr (PF) {
+            const int e = tid < 2 * nk ? tid : 0, kp = e >> 1, fp = e & 1;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const int o2 = s_hto[(fp * 2 + m) * nk + kp];
+                pf[m] = tabT[((size_t)fp * Ns + (fp ? sa : sb)) * Ns + (o2 >= 0 ? o2 : 0)];
+            }
+        }
         for (int t = tid; t < 2 * NC; t += NT) {
             const int fl = t >= NC ? 1 : 0, f = t - fl * NC;
             C2 v = mk2<C2>(0.0, 0.0);
@@ -453,11 +467,9 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
             for (int j = 0; j < RBW; ++j)
                 if (okj[j]) {
                     const int fl = flj[j], row = rowj[j];
-                    T xr = ar[j], xi = ai[j];
-                    xr += __shfl_xor(xr, 16);
-                    xi += __shfl_xor(xi, 16);
-                    xr += __shfl_xor(xr, 32);
-                    xi += __shfl_xor(xi, 32);
+                    T xr = sum_xor16(ar[j]), xi = sum_xor16(ai[j]);
+                    xr = sum_xor32(xr);
+                    xi = sum_xor32(xi);
                     if (kq == 0) s_z[fl * r16 + row] = mk2<C2>(xr, xi);
                     if (row == (fl ? cj : ci)) {
 #pragma unroll
@@ -466,25 +478,32 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 }
         }
         __syncthreads();
-        // with the reduction's partials (its barrier also publishes alpha and the dets)
+        // with the reduction's partials (its barrier also publishes alpha and the dets): one
+        // thread per (state, flavour), the two flavours of state k in lanes 2k and 2k + 1;
+        // the even lane takes its partner's determinant by DPP and forms the amplitude
         double ls = 0.0, bv = -1.0;
         int bk = nk;
-        for (int k = tid; k < nk; k += NT) {
-            zc tot = zmk((double)s_hsg[k], 0.0);
-#pragma unroll
-            for (int fl = 0; fl < 2; ++fl) {
+        for (int e0 = 0; e0 < 2 * nk; e0 += NT) {
+            const int e = e0 + tid, k = e >> 1, fl = e & 1;
+            const bool on = e < 2 * nk;
+            zc d = zmk(0.0, 0.0);
+            if (on) {
                 zc B[9];
-                cap_move(k, fl, ci, sa, sb, cj, B);
-                const zc d = det3(B);
-                s_dets[k * 2 + fl] = d;
-                tot = zmul(tot, d);
+                cap_move(k, fl, ci, sa, sb, cj, B, PF && e0 == 0, pf[0], pf[1]);
+                d = det3(B);
+                s_dets[e] = d;
             }
-            s_alpha[k] = tot;
-            const double v = znorm(tot);
-            ls += v;
-            if (v > bv) {
-                bv = v;
-                bk = k;
+            const zc d1 = zmk(__longlong_as_double((long long)dpp_u64<0xB1>((unsigned long long)__double_as_longlong(d.x), 0ull)),
+                              __longlong_as_double((long long)dpp_u64<0xB1>((unsigned long long)__double_as_longlong(d.y), 0ull)));
+            if (on && fl == 0) {
+                const zc tot = zmul(zmul(zmk((double)s_hsg[k], 0.0), d), d1);
+                s_alpha[k] = tot;
+                const double v = znorm(tot);
+                ls += v;
+                if (v > bv) {
+                    bv = v;
+                    bk = k;
+                }
             }
         }
         reduce_partials(ls, bv, bk, S2, kn);
@@ -771,7 +790,7 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     const int st = s_part[(1 - fl) * nmax + t];
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
-                        const int v = s_site[a.nbr[st * 4 + d]];
+                        const int v = s_site[s_nbr[st * 4 + d]];
                         if ((v >> 16) == fl) pk = (pk & ~(0xFFu << (8 * d))) | ((unsigned)(v & 0xFF) << (8 * d));
                     }
                 }
@@ -995,6 +1014,10 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
         }
     } else if (MK == 0 && (a.mode == 0 || hop0)) {
         const int64_t n_walk = a.mode == 0 ? a.n_steps : 0;
+        // Philox draws of 64 steps at a time, lane l holding step (st & ~63) + l: the site,
+        // the bond and the acceptance word
+        int bv1 = 0, bed = 0;
+        uint32_t bc3 = 0;
         for (int64_t st = hop0 ? -1 : 0; st < n_walk; ++st) {
             asm volatile("" : "+v"(tid), "+v"(lane), "+v"(li), "+v"(kq));
             bool mv = st >= 0, acc = true, noop = false;
@@ -1011,16 +1034,22 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                     ed = ed < 4 ? ed : 3;
                     u3 = up[3];
                 } else {
-                    const uint64_t sc = ctr0 + (uint64_t)st;
-                    uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
-                    gv::philox4x32_10_dev(cc, a.key0, a.key1);
-                    v1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
-                    ed = (int)(cc[1] >> 30);
-                    u3 = ((double)cc[3] + 0.5) * 2.3283064365386963e-10;
+                    if ((st & 63) == 0) {
+                        const uint64_t sc = ctr0 + (uint64_t)(st + lane);
+                        uint32_t cc[4] = {(uint32_t)sc, (uint32_t)(sc >> 32), (uint32_t)w, 0u};
+                        gv::philox4x32_10_dev(cc, a.key0, a.key1);
+                        bv1 = (int)(((uint64_t)Ns * (2ull * cc[0] + 1ull)) >> 33);
+                        bed = (int)(cc[1] >> 30);
+                        bc3 = cc[3];
+                    }
+                    const int l = (int)(st & 63);
+                    v1 = __builtin_amdgcn_readlane(bv1, l);
+                    ed = __builtin_amdgcn_readlane(bed, l);
+                    u3 = ((double)(uint32_t)__builtin_amdgcn_readlane((int)bc3, l) + 0.5) * 2.3283064365386963e-10;
                 }
                 v1 = __builtin_amdgcn_readfirstlane(v1);
                 ed = __builtin_amdgcn_readfirstlane(ed);
-                const int v2 = __builtin_amdgcn_readfirstlane(a.nbr[v1 * 4 + ed]);
+                const int v2 = __builtin_amdgcn_readfirstlane(s_nbr[v1 * 4 + ed]);
                 const int i1 = __builtin_amdgcn_readfirstlane(s_site[v1]), i2 = __builtin_amdgcn_readfirstlane(s_site[v2]);
                 noop = (i1 >> 16) == (i2 >> 16);
                 if (noop) {
