@@ -1039,14 +1039,17 @@ sweep_kernel(SweepArgs a)
                                 r1.x = fma(-n1.y, v1.y, r1.x);
                                 r1.y = fma(n1.x, v1.y, o1.y);
                                 r1.y = fma(n1.y, v1.x, r1.y);
-                                if ((ia ^ it) >> 16) {
-                                    s_bq[b][0] = r0;
-                                    s_bq[b][1] = r1;
-                                }
+                                // (a parallel bond keeps its entries: no branch around the store)
+                                const bool apb = ((ia ^ it) >> 16) != 0;
+                                s_bq[b][0] = apb ? r0 : c0;
+                                s_bq[b][1] = apb ? r1 : c1;
                             }
                         }
                     }
                 };
+                // matrix-core tables: the bond update first, in the same block as the MFMAs, so
+                // that its loads and arithmetic issue while the matrix pipe works through them
+                if constexpr (MF && BQ) bond_update();
                 if constexpr (MF) {
 #pragma unroll
                     for (int fl = 0; fl < 2; ++fl) {
@@ -1107,7 +1110,7 @@ sweep_kernel(SweepArgs a)
                     }
                 }
                 if constexpr (BQ) {
-                    bond_update();
+                    if constexpr (!MF) bond_update();
                 } else {
                     // every wavefront writes the same values, so each one reads back its own
                     // writes in the next window
