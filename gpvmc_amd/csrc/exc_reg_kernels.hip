@@ -797,17 +797,34 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 s_nbt[i] = (int)pk;
             }
         }
+        // the tiles (flavour, first target) in sequence; each tile's Phi is loaded into
+        // registers while the previous tile's MFMAs run, then stored to LDS
+        constexpr int SPT = (NC * 16 + NT - 1) / NT;
+        C2 sv[SPT];
+        auto stage_ld = [&](int fl, int t0) __attribute__((always_inline)) {
+            const int no = NF(1 - fl);
+#pragma unroll
+            for (int i = 0; i < SPT; ++i) {
+                const int e = tid + i * NT, f = e >> 4, tg = t0 + (e & 15);
+                C2 v = mk2<C2>(0.0, 0.0);
+                if (e < NC * 16 && f < NF(fl) && tg < no)
+                    v = tabT[((size_t)fl * Ns + s_part[(1 - fl) * nmax + tg]) * Ns + s_cur[fl * nmax + f]];
+                sv[i] = v;
+            }
+        };
+        stage_ld(0, 0);
         for (int fl = 0; fl < 2; ++fl) {
             const int no = NF(1 - fl);
             for (int t0 = 0; t0 < no; t0 += 16) {
-                for (int e = tid; e < NC * 16; e += NT) {
-                    const int f = e >> 4, tg = t0 + (e & 15);
-                    C2 v = mk2<C2>(0.0, 0.0);
-                    if (f < NF(fl) && tg < no)
-                        v = tabT[((size_t)fl * Ns + s_part[(1 - fl) * nmax + tg]) * Ns + s_cur[fl * nmax + f]];
-                    Ct[e] = v;
-                }
+#pragma unroll
+                for (int i = 0; i < SPT; ++i)
+                    if (tid + i * NT < NC * 16) Ct[tid + i * NT] = sv[i];
                 __syncthreads();
+                {
+                    const bool last = t0 + 16 >= no;
+                    if (!last) stage_ld(fl, t0 + 16);
+                    else if (fl == 0) stage_ld(1, 0);
+                }
 #pragma unroll
                 for (int j = 0; j < RBW; ++j) {
                     int f_, row;
