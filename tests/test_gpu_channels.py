@@ -79,6 +79,39 @@ def test_fp64_channel_walk_matches_oracle(name, L, q, W, sweeps, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+def test_channel_sums_over_many_measurements_match_oracle(kernel):
+    """40 measurements in one walk call (a measurement after every step): the register
+    kernel's buffered H/O records are folded after 16, 32 and at the end of the call; the
+    sums must equal the oracle's sequential ProjHeis accumulation (ProjHeis.cpp:140-153)."""
+    name, L, q, W, nsteps = "trans", 4, (1, 2), 4, 40
+    say("case channel many measurements", name, L, q, W, kernel)
+    ch = channels.Channel(name, L, q)
+    tab = model.orbital_table(L, 0.25 * np.pi, 0.1)
+    seed = 611
+    spins = rand_spins(L, ch.npt[0], W, seed)
+    with binding.ChannelBatch(L, W, ch, binding.FP64, seed=seed, kernel=kernel) as b:
+        assert b.register_resident == (kernel == "auto")
+        b.set_orbital_table(0, tab)
+        ok = b.init(spins)
+        out = b.walk(nsteps, meas=1, trace=True)
+        ho, _, _ = b.channel_sums()
+        acc = b.accumulators()
+    u = model.philox_uniforms(seed, range(W), 0, nsteps)
+    checked = 0
+    for w in range(W):
+        if not ok[w]:
+            continue
+        wo = O.ExcWalker(L, tab, ch, spins[w])
+        o = wo.walk(u[w].reshape(-1), meas=1)
+        assert np.array_equal(o["accept"], out["accept"][w])
+        assert o["n_meas"] == nsteps and acc[w, 3] == nsteps
+        scale = max(1.0, np.abs(o["HO"]).max())
+        np.testing.assert_allclose(ho[w], o["HO"], rtol=0, atol=1e-8 * scale)
+        checked += 1
+    assert checked > 0
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,L,q,W", [("trans", 6, (1, 2), 4), ("long", 8, (2, 2), 2)])
 def test_mixed_channel_tracks_oracle(name, L, q, W, kernel):
     say("case channel mixed", name, L, q, W, kernel)
