@@ -88,6 +88,7 @@ __host__ __device__ __forceinline__ Sizes make_sizes(int L, int n0, int n1, int 
 struct Off {
     size_t site, nbr, part, cur, slot, qrow, qorb, rep, c2, ra, alpha, dets, a0, hpart, binv, red, ired, hop, un, mz,
         total;
+    bool qboth;  // the shared area holds both flavours' Q rows at once (one RA build for both)
 };
 
 __host__ __device__ __forceinline__ size_t take(size_t &off, size_t bytes)
@@ -128,10 +129,20 @@ __host__ __device__ __forceinline__ Off carve(const Sizes &z, size_t ts, bool lz
     o.red = take(off, 16 * 16);
     o.ired = take(off, 4 * 16);
     o.hop = take(off, 4 * 19 * (size_t)z.nk);
-    const size_t un = (size_t)z.nqm * z.nc > (size_t)z.nc * 16 ? (size_t)z.nqm * z.nc : (size_t)z.nc * 16;
-    o.un = take(off, ts * un);
-    o.mz = take(off, lz ? mz_bytes(z, ts) : 0);
-    o.total = off;
+    // the shared area: the dump of the Q rows for the RA blocks (both flavours at once where
+    // that still fits in 160 KB with the rest, else one flavour at a time), the measurement's
+    // Phi tiles and bond list
+    const size_t base = off;
+    auto umax = [](size_t x, size_t y) { return x > y ? x : y; };
+    for (int both = 1; both >= 0; --both) {
+        off = base;
+        const size_t rows = both ? (size_t)(z.nq0 + z.nq1) : (size_t)z.nqm;
+        o.un = take(off, ts * umax(rows * z.nc, (size_t)z.nc * 16));
+        o.mz = take(off, lz ? mz_bytes(z, ts) : 0);
+        o.total = off;
+        o.qboth = both != 0;
+        if (!both || o.total + 64 <= 160 * 1024) break;
+    }
     return o;
 }
 
@@ -304,6 +315,31 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
     // RA[fl][m][n][k] = Q_fl[row of hop_to(c, k, fl, m)][slot of hop_from(c, k, fl, n)]
     auto build_ra = [&]() __attribute__((always_inline)) {
         C2 *Qd = reinterpret_cast<C2 *>(s_un);
+        if (lo.qboth) {
+            // both flavours' Q rows (flavour 1 after flavour 0's nq0 rows), one barrier, then
+            // every RA entry of both flavours
+#pragma unroll
+            for (int j = 0; j < RBW; ++j) {
+                int fl, row;
+                if (rinfo(j, fl, row) && row >= NF(fl) && row < NF(fl) + NQ(fl)) {
+                    C2 *qd = Qd + (fl ? (size_t)z.nq0 * NC : 0);
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb)
+                        qd[(row - NF(fl)) * NC + 4 * cb + kq] = mk2<C2>(mr[j][cb], mi[j][cb]);
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < nk * 8; e += NT) {
+                const int fmq = e / nk, k = e - fmq * nk, fl = fmq >> 2, m = (fmq >> 1) & 1, q = fmq & 1;
+                const int R = s_hn[fl * nk + k];
+                const C2 *qd = Qd + (fl ? (size_t)z.nq0 * NC : 0);
+                zc v = zmk(0.0, 0.0);
+                if (m < R && q < R) v = toz(qd[s_hq[(fl * 2 + m) * nk + k] * NC + s_hs[(fl * 2 + q) * nk + k]]);
+                s_ra[((fl * 2 + m) * 2 + q) * nk + k] = v;
+            }
+            __syncthreads();
+            return;
+        }
         for (int fl = 0; fl < 2; ++fl) {
             if (NQ(fl) > 0) {
 #pragma unroll
