@@ -742,30 +742,34 @@ __global__ void __launch_bounds__(NT) excr_kernel(ExcArgs a)
                 asm volatile("" ::: "memory");
             }
         }
-        // amplitude and slot / Q-row bookkeeping
-        if (tid == 0) {
-            for (int fl = 0; fl < 2; ++fl) {
-                const zc d = s_dets[kn * 2 + fl];
-                zamp_mul(s_amp[2 * fl], s_amp[2 * fl + 1], s_amp[4 + fl], d.x, d.y);
+        // amplitude and slot / Q-row bookkeeping, the independent parts side by side: one lane
+        // per hop (fl, m) in wavefront 0 (a state's hops empty and fill distinct orbitals, so
+        // they touch distinct slots and Q rows), one per flavour's amplitude in wavefront 1,
+        // the site map in wavefront 2
+        if (wv == 0 && lane < 4) {
+            const int fl = lane >> 1, m = lane & 1;
+            if (m < RK(fl)) {
+                const int o1 = s_hfr[(fl * 2 + m) * nk + kn], o2 = s_hto[(fl * 2 + m) * nk + kn];
+                const int fm = s_slot[fl * Ns + o1];
+                s_slot[fl * Ns + o1] = -1;
+                s_slot[fl * Ns + o2] = fm;
+                s_cur[fl * nmax + fm] = o2;
+                const int q = s_qrow[fl * Ns + o2];
+                s_qrow[fl * Ns + o2] = -1;
+                s_qrow[fl * Ns + o1] = q;
+                s_qorb[fl * nqm + q] = o1;
             }
-            for (int fl = 0; fl < 2; ++fl)
-                for (int m = 0; m < RK(fl); ++m) {
-                    const int o1 = s_hfr[(fl * 2 + m) * nk + kn], o2 = s_hto[(fl * 2 + m) * nk + kn];
-                    const int fm = s_slot[fl * Ns + o1];
-                    s_slot[fl * Ns + o1] = -1;
-                    s_slot[fl * Ns + o2] = fm;
-                    s_cur[fl * nmax + fm] = o2;
-                    const int q = s_qrow[fl * Ns + o2];
-                    s_qrow[fl * Ns + o2] = -1;
-                    s_qrow[fl * Ns + o1] = q;
-                    s_qorb[fl * nqm + q] = o1;
-                }
-            if (mv) {
-                s_site[sb] = ci;
-                s_site[sa] = (1 << 16) | cj;
-                s_part[ci] = sb;
-                s_part[nmax + cj] = sa;
-            }
+        }
+        if (wv == (NW > 1 ? 1 : 0) && lane >= 4 && lane < 6) {
+            const int fl = lane - 4;
+            const zc d = s_dets[kn * 2 + fl];
+            zamp_mul(s_amp[2 * fl], s_amp[2 * fl + 1], s_amp[4 + fl], d.x, d.y);
+        }
+        if (mv && wv == (NW > 2 ? 2 : 0) && lane == 6) {
+            s_site[sb] = ci;
+            s_site[sa] = (1 << 16) | cj;
+            s_part[ci] = sb;
+            s_part[nmax + cj] = sa;
         }
         // the hop graph's row of the current state changes only with the state (the slot
         // and Q-row bookkeeping above moves nothing when kn is the current state)
