@@ -653,7 +653,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "L=16 staggered-flux groundstate, N=128, %d walkers, 8-point (phi,neel) grid; "
                                "1 bench step = 1 FP64-refresh period = %d sweeps (256 steps + 1 ProjHeis/"
-                               "StagMagnZ measurement each, one walk launch) + the FP64 LU refresh; therm %d "
+                               "StagMagnZ measurement each, one walk launch) + the FP64 Gauss-Jordan refresh; therm %d "
                                "sweeps untimed" % (W, k, args.therm),
                    "L": L, "walkers": W, "sweeps_per_step": k, "parallelism": "1 GPU, 1 workgroup per walker"},
         "mc_steps_per_s": round(value, 1),

@@ -1,7 +1,7 @@
 """Autotuning of the mixed-precision path.
 
 The FP32 ratio tables drift from their FP64 values by rounding in every accepted update;
-gpvmc_refresh recomputes them from an FP64 LU and reports the table drift
+gpvmc_refresh recomputes them from an FP64 Gauss-Jordan inverse and reports the table drift
 max|G_old - G_new| / max|G_new| per walker.  What the drift costs is an error in the
 estimator: the local energy measured on drifted tables differs from the one measured on
 fresh tables at the same configuration.  `tune_refresh` finds the longest refresh
